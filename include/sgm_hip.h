@@ -1,0 +1,235 @@
+/*
+ * libsgm_hip -- MI355X (gfx950) semi-global-matching dense-stereo engine, C ABI.
+ *
+ * Drop-in compute library behind the reference's GPU-SGM backend
+ * (`--gpu-sgm resident`), replacing the OpenCL runtime the reference reaches
+ * through the JDK FFM API (src/main/java/sgm/GpuSgm.java:30-54).  Plain
+ * pointers and sizes only; every entry point returns 0 on success or a
+ * negative SGM_ERR_* code and records a message for sgm_last_error().
+ *
+ * Ownership: the library owns all device memory it allocates; the caller owns
+ * every host pointer and every device pointer it passes in.  A context is bound
+ * to one device and one HIP stream; calls on one context are serialised by the
+ * library (the reference serialises all OpenCL calls, GpuSgm.java:30).
+ */
+#ifndef SGM_HIP_H
+#define SGM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGM_OK 0
+#define SGM_ERR_INVALID_ARGUMENT (-1)
+#define SGM_ERR_NO_DEVICE (-2)
+#define SGM_ERR_OUT_OF_MEMORY (-3)
+#define SGM_ERR_NO_PLAN (-4)        /* no device memory plan fits (GpuStereo.java:231-232) */
+#define SGM_ERR_HIP (-5)            /* a HIP runtime call failed */
+#define SGM_ERR_REJECTED (-6)       /* serving admission refused the request */
+#define SGM_ERR_TIMEOUT (-7)
+#define SGM_ERR_NOT_FOUND (-8)
+
+/* Matching costs: 0..3 are StereoCost.code (StereoCost.java:7-15); 3 is the
+ * reference's exact integer/FP64 ZNCC.  4 is the FP32 ZNCC variant with cost
+ * round((1 - ncc) * 1024), invalid cells marked separately. */
+#define SGM_COST_CENSUS 0
+#define SGM_COST_AD_CENSUS 1
+#define SGM_COST_CENSUS_GRADIENT 2
+#define SGM_COST_ZNCC 3
+#define SGM_COST_ZNCC_FP32 4
+
+/* SGM options subset (Options.java:17-40,67).  Penalties are raw scaled ints,
+ * exactly as the reference's kernels receive them (GpuStereo.java:394-396);
+ * the library applies P2 = max(P1 + 1, P2) (StereoCost.java:21-23). */
+typedef struct sgm_params {
+  int32_t window_w, window_h;   /* census window; the reference uses census x census */
+  int32_t disparities;          /* D, 3..4096 (GpuStereo.java:43-44) */
+  int32_t min_disparity;        /* signed */
+  int32_t cost;                 /* SGM_COST_* */
+  int32_t p1, p2;               /* raw penalties */
+  int32_t edge_scale;           /* sgmEdgeScale (StereoCost.java:44-48); 0 disables */
+  int32_t directions;           /* 4 or 8 (Options.java:932) */
+  int32_t aggregate_bytes;      /* 0 auto, 2 or 4 (StereoCost.java:35-41) */
+  int32_t census_weight;        /* round(censusWeight * 1000) (GpuStereo.java:612) */
+  int32_t ad_truncation;
+  int32_t gradient_truncation;
+  int32_t reserved;
+  double uniqueness;            /* Options.uniqueness */
+  double lr_threshold;          /* Options.lrThreshold */
+} sgm_params;
+
+/* CensusSgm.Stats counters (CensusSgm.java:683-694). */
+typedef struct sgm_stats {
+  int64_t lower_boundary_rejected;
+  int64_t full_interval_pixels;
+  int64_t boundary_rejected;
+  int64_t upper_boundary_rejected;
+  int64_t ambiguity_rejected;
+  int64_t selected;
+  int64_t lr_rejected;
+  int64_t post_median_lr_rejected;
+  int64_t photometric_rejected;
+  int64_t speckle_rejected;
+  int64_t valid;                /* finite pixels of the left-right-validated map */
+  int64_t reserved[5];
+} sgm_stats;
+
+/* GpuSgm.Device (GpuSgm.java:11-18). */
+typedef struct sgm_device_info {
+  char name[128];
+  char arch[64];
+  int64_t memory;               /* global memory bytes */
+  int64_t allocation;           /* largest single allocation */
+  int64_t local_memory;         /* LDS bytes per workgroup */
+  int32_t group_size;           /* max threads per workgroup */
+  int32_t compute_units;
+} sgm_device_info;
+
+/* GpuStereo.Plan (GpuStereo.java:12-20) for the resident mode. */
+typedef struct sgm_plan {
+  int32_t resident;             /* 1 if the resident plan fits the budget */
+  int32_t aggregate_bytes;      /* 2 or 4 */
+  int32_t padded_disparities;   /* per-pixel stride of the volumes */
+  int32_t lanes_per_disparity_group; /* disparities held per lane in the path sweeps */
+  int64_t device_bytes;         /* footprint of one matching direction */
+  int64_t pair_bytes;           /* footprint of a left+right pair (both volumes resident) */
+  int64_t budget_bytes;
+  int64_t aggregate_bound;      /* StereoCost.aggregateBound */
+} sgm_plan;
+
+typedef struct sgm_context sgm_context;
+
+/* GpuSgm.devices (GpuSgm.java:79-141). */
+int sgm_device_count(int *count);
+int sgm_device_info_get(int index, sgm_device_info *out);
+
+/* GpuSgm session (GpuSgm.java:163-221); budget_bytes 0 = 70% of device
+ * memory, as GpuSgm.budget (GpuSgm.java:629-633). */
+int sgm_open(int device, int64_t budget_bytes, sgm_context **out);
+int sgm_close(sgm_context *ctx);
+
+/* GpuStereo.plan resident branch (GpuStereo.java:33-54). */
+int sgm_plan_get(sgm_context *ctx, const sgm_params *params, int width, int height,
+                 sgm_plan *out);
+/* GpuSgm.disparityLimit / GpuStereo.limit (GpuSgm.java:237-245, GpuStereo.java:86-94):
+ * the largest D whose resident plan fits, 0 if none. */
+int sgm_disparity_limit(sgm_context *ctx, const sgm_params *params, int width, int height,
+                        int *limit);
+
+/* GpuSgm.match (GpuSgm.java:267-303): one matching direction.  gray_a/gray_b
+ * are the reference/target rasters, mask_a/mask_b the texture-and-valid masks
+ * (0/1), reverse selects target x + d.  disp receives the selected disparity
+ * (NaN = invalid); stats are ADDED to *stats; reasons/boundary (may be NULL)
+ * are updated in place exactly where the reference's select writes them. */
+int sgm_match(sgm_context *ctx, const sgm_params *params, const uint8_t *gray_a,
+              const uint8_t *gray_b, const uint8_t *mask_a, const uint8_t *mask_b, int width,
+              int height, int reverse, float *disp, sgm_stats *stats, uint8_t *reasons,
+              uint8_t *boundary);
+
+/* CensusSgm.computeRecorded up to 02_left_right_validated (CensusSgm.java:45-202):
+ * census + texture masks on device, both matching directions, device L-R check.
+ * valid_left/valid_right, dl, dr, reasons, boundary may be NULL. *stats is
+ * overwritten. */
+int sgm_match_pair(sgm_context *ctx, const sgm_params *params, const uint8_t *left,
+                   const uint8_t *right, const uint8_t *valid_left, const uint8_t *valid_right,
+                   int width, int height, float *checked, float *dl, float *dr, sgm_stats *stats,
+                   uint8_t *reasons, uint8_t *boundary);
+
+/* Same as sgm_match_pair with every buffer in device memory (no transfers, no
+ * host synchronisation).  d_stats (device, sgm_stats) may be NULL.  Work is
+ * enqueued on the context's stream; call sgm_synchronize before reading. */
+int sgm_match_pair_device(sgm_context *ctx, const sgm_params *params, const void *d_left,
+                          const void *d_right, const void *d_valid_left,
+                          const void *d_valid_right, int width, int height, void *d_checked,
+                          void *d_dl, void *d_dr, void *d_stats, void *d_reasons,
+                          void *d_boundary);
+
+/* Copy the last call's cost (uint16, n*D, 512 = invalid) and aggregate
+ * (widened to uint32, n*D) volumes of one matching direction (0 = left
+ * reference, 1 = right reference) for parity checks.  Volumes are only kept
+ * when keep_volumes was set. */
+int sgm_set_keep_volumes(sgm_context *ctx, int keep);
+int sgm_read_volumes(sgm_context *ctx, int direction, uint16_t *cost, uint32_t *aggregate);
+
+int sgm_synchronize(sgm_context *ctx);
+/* Kernel timing of the last calls (GpuSgm.Compute.profile, GpuSgm.java:592-604). */
+int sgm_set_profiling(sgm_context *ctx, int enable);
+int sgm_profile_json(sgm_context *ctx, char *buffer, size_t length);
+/* HIP stream handle of the context (hipStream_t as void*). */
+void *sgm_stream(sgm_context *ctx);
+
+/* Host-forceable failure hook for tests of the fallback path: the next
+ * `count` device allocations fail as if out of memory. */
+int sgm_debug_fail_allocations(sgm_context *ctx, int count);
+
+/* Message of the last failing call on this thread. */
+const char *sgm_last_error(void);
+
+/* ---- batched mode: many uneven pairs in one persistent launch ---------------- */
+
+/* One stereo pair of a batch.  Sizes and range are per pair; the window,
+ * cost and penalties come from the batch's sgm_params.  With SGM_BATCH_DEVICE
+ * all pointers are device pointers, otherwise host pointers. */
+typedef struct sgm_pair {
+  int32_t width, height;
+  int32_t disparities, min_disparity;
+  const uint8_t *left, *right;
+  const uint8_t *valid_left, *valid_right;   /* may be NULL */
+  float *checked;                            /* required */
+  float *dl, *dr;                            /* may be NULL */
+  uint8_t *reasons, *boundary;               /* may be NULL */
+  sgm_stats *stats;                          /* host (or device with SGM_BATCH_DEVICE); may be NULL */
+} sgm_pair;
+
+#define SGM_BATCH_DEVICE 1     /* sgm_pair pointers are device pointers */
+#define SGM_BATCH_GRAPH 2      /* capture the launch into a hipGraph and replay it */
+#define SGM_BATCH_NO_SYNC 4    /* do not wait for completion (device mode only) */
+
+typedef struct sgm_batch sgm_batch;
+
+/* Plan a batch: allocate per-pair volumes and the work queue, choose tile
+ * shape / waves per path (autotuned table), optionally build a hipGraph. */
+int sgm_batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pairs,
+                     int count, int flags, sgm_batch **out);
+int sgm_batch_run(sgm_batch *batch);
+int sgm_batch_destroy(sgm_batch *batch);
+int64_t sgm_batch_device_bytes(sgm_batch *batch);
+
+/* Autotuner: time the candidate path-sweep configurations on this device for
+ * the given params and store the winners in the context (and, if path is not
+ * NULL, in a JSON file that sgm_autotune_load reads back). */
+int sgm_autotune(sgm_context *ctx, const sgm_params *params, const char *path);
+int sgm_autotune_load(sgm_context *ctx, const char *path);
+
+/* ---- serving mode: continuous batching with HBM-footprint admission --------- */
+
+typedef struct sgm_server sgm_server;
+
+typedef struct sgm_request_info {
+  double arrival_s, admitted_s, started_s, finished_s;  /* seconds on the server clock */
+  int64_t device_bytes;
+  int32_t status;
+  int32_t batch_id;
+} sgm_request_info;
+
+/* budget_bytes bounds the device footprint of admitted requests (cost +
+ * aggregate volumes and per-pair buffers); max_batch caps the pairs launched
+ * together; streams is the number of batches that may run concurrently. */
+int sgm_server_create(sgm_context *ctx, const sgm_params *params, int64_t budget_bytes,
+                      int max_batch, int streams, sgm_server **out);
+/* Submit one pair (host buffers, owned by the caller until the request
+ * completes).  Returns a ticket; fails with SGM_ERR_REJECTED if the pair can
+ * never fit the budget. */
+int sgm_server_submit(sgm_server *srv, const sgm_pair *pair, uint64_t *ticket);
+/* Wait for a ticket (timeout_s < 0 waits forever). */
+int sgm_server_wait(sgm_server *srv, uint64_t ticket, double timeout_s, sgm_request_info *info);
+double sgm_server_now(sgm_server *srv);
+int sgm_server_destroy(sgm_server *srv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,152 @@
+"""ctypes wrapper of the C oracle (liboracle.so) -- test infrastructure only.
+
+Every function mirrors one function of oracle/sgm_oracle.c, which restates the
+reference's CensusSgm / StereoCost CPU path (file:line citations live there).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+CENSUS, AD_CENSUS, CENSUS_GRADIENT, ZNCC, ZNCC1024 = 0, 1, 2, 3, 4
+
+
+class OracleParams(ctypes.Structure):
+    _fields_ = [
+        ("window_w", ctypes.c_int32), ("window_h", ctypes.c_int32),
+        ("disparities", ctypes.c_int32), ("min_disparity", ctypes.c_int32),
+        ("cost", ctypes.c_int32), ("p1", ctypes.c_int32), ("p2", ctypes.c_int32),
+        ("edge_scale", ctypes.c_int32), ("directions", ctypes.c_int32),
+        ("aggregate_bytes", ctypes.c_int32), ("census_weight", ctypes.c_int32),
+        ("ad_truncation", ctypes.c_int32), ("gradient_truncation", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("uniqueness", ctypes.c_double), ("lr_threshold", ctypes.c_double),
+    ]
+
+
+STAT_FIELDS = [
+    "lower_boundary_rejected", "full_interval_pixels", "boundary_rejected",
+    "upper_boundary_rejected", "ambiguity_rejected", "selected", "lr_rejected",
+    "post_median_lr_rejected", "photometric_rejected", "speckle_rejected", "valid",
+]
+
+
+class OracleStats(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_int64) for f in STAT_FIELDS] + [("reserved", ctypes.c_int64 * 5)]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f in STAT_FIELDS}
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        L.oracle_census.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
+        L.oracle_texture.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
+        L.oracle_match.argtypes = [ctypes.POINTER(OracleParams), P, P, P, P, P, P, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, P, ctypes.POINTER(OracleStats),
+                                   P, P, P, P]
+        L.oracle_compute.argtypes = [ctypes.POINTER(OracleParams), P, P, P, P, ctypes.c_int,
+                                     ctypes.c_int, P, P, P, ctypes.POINTER(OracleStats), P, P]
+        L.oracle_round_float.argtypes = [ctypes.c_float]
+        L.oracle_round_float.restype = ctypes.c_int32
+        L.oracle_round_double.argtypes = [ctypes.c_double]
+        L.oracle_round_double.restype = ctypes.c_int64
+        L.oracle_effective_p2.argtypes = [ctypes.POINTER(OracleParams)]
+        L.oracle_aggregate_bytes.argtypes = [ctypes.POINTER(OracleParams)]
+        L.oracle_aggregate_bound.argtypes = [ctypes.POINTER(OracleParams)]
+        L.oracle_aggregate_bound.restype = ctypes.c_int64
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def params(**kw):
+    """Oracle parameters; defaults are the reference's Options defaults for a 5x5 census
+    (Options.java:17-40,67) with penalties already scaled (4/24)."""
+    d = dict(window_w=5, window_h=5, disparities=128, min_disparity=0, cost=CENSUS, p1=4, p2=24,
+             edge_scale=32, directions=8, aggregate_bytes=0, census_weight=500,
+             ad_truncation=30, gradient_truncation=60, reserved=0, uniqueness=0.03,
+             lr_threshold=0.75)
+    d.update(kw)
+    return OracleParams(**d)
+
+
+def census(gray, w, h, ww, wh):
+    words = (ww * wh - 1 + 63) // 64
+    out = np.zeros(words * w * h, dtype=np.uint64)
+    lib().oracle_census(_ptr(np.ascontiguousarray(gray, np.uint8)), w, h, ww, wh, _ptr(out))
+    return out.reshape(words, h * w)
+
+
+def texture(gray, w, h, ww, wh, valid=None):
+    out = np.zeros(w * h, dtype=np.uint8)
+    v = None if valid is None else np.ascontiguousarray(valid, np.uint8)
+    lib().oracle_texture(_ptr(np.ascontiguousarray(gray, np.uint8)), _ptr(v), w, h, ww, wh, _ptr(out))
+    return out
+
+
+def match(p, da, db, ga, gb, ta, tb, w, h, reverse, volumes=False):
+    """One matching direction. Returns (disp, stats, reasons, boundary[, cost, agg])."""
+    n = w * h
+    disp = np.empty(n, np.float32)
+    reasons = np.ones(n, np.uint8)
+    boundary = np.zeros(n, np.uint8)
+    st = OracleStats()
+    cost = np.empty(n * p.disparities, np.uint16) if volumes else None
+    agg = np.empty(n * p.disparities, np.uint32) if volumes else None
+    rc = lib().oracle_match(ctypes.byref(p), _ptr(np.ascontiguousarray(da)), _ptr(np.ascontiguousarray(db)),
+                            _ptr(ga), _ptr(gb), _ptr(ta), _ptr(tb), w, h, int(reverse), _ptr(disp),
+                            ctypes.byref(st), _ptr(reasons), _ptr(boundary), _ptr(cost), _ptr(agg))
+    if rc != 0:
+        raise ValueError("oracle_match failed: %d" % rc)
+    out = (disp, st.as_dict(), reasons, boundary)
+    return out + (cost, agg) if volumes else out
+
+
+def compute(p, left, right, w, h, valid_left=None, valid_right=None):
+    """Both directions + L-R check. Returns dict of checked, dl, dr, stats, reasons, boundary."""
+    n = w * h
+    left = np.ascontiguousarray(left, np.uint8).reshape(-1)
+    right = np.ascontiguousarray(right, np.uint8).reshape(-1)
+    vl = None if valid_left is None else np.ascontiguousarray(valid_left, np.uint8).reshape(-1)
+    vr = None if valid_right is None else np.ascontiguousarray(valid_right, np.uint8).reshape(-1)
+    checked = np.empty(n, np.float32)
+    dl = np.empty(n, np.float32)
+    dr = np.empty(n, np.float32)
+    reasons = np.empty(n, np.uint8)
+    boundary = np.empty(n, np.uint8)
+    st = OracleStats()
+    rc = lib().oracle_compute(ctypes.byref(p), _ptr(left), _ptr(right), _ptr(vl), _ptr(vr), w, h,
+                              _ptr(checked), _ptr(dl), _ptr(dr), ctypes.byref(st), _ptr(reasons),
+                              _ptr(boundary))
+    if rc != 0:
+        raise ValueError("oracle_compute failed: %d" % rc)
+    return dict(checked=checked, dl=dl, dr=dr, stats=st.as_dict(), reasons=reasons,
+                boundary=boundary)
+
+
+def round_float(v):
+    return lib().oracle_round_float(v)
+
+
+def round_double(v):
+    return lib().oracle_round_double(v)

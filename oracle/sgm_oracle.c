@@ -1,0 +1,373 @@
+/*
+ * CPU oracle for the SGM hot path -- TEST INFRASTRUCTURE ONLY (see sgm_oracle.h).
+ *
+ * Literal restatement of the reference's CPU SGM in plain C, compiled with
+ * -O2 -ffp-contract=off and without fast-math so every floating-point step is
+ * the IEEE operation the Java source performs (Java >= 17 is strict IEEE):
+ *   census / texture      CensusSgm.java:519-555
+ *   cost value            StereoCost.java:50-90, cost loop CensusSgm.java:405-420
+ *   penalties / bounds    StereoCost.java:17-48
+ *   path recurrence       CensusSgm.java:421-459
+ *   WTA / subpixel        CensusSgm.java:460-515
+ *   left-right check      CensusSgm.java:181-201
+ *   16-bit storage        PagedVolume.java:160-214 ((short) store, & 65535 load)
+ * Generalisation: the reference uses one square census window (radius
+ * census/2 on both axes); here the window is ww x wh with rx = ww/2,
+ * ry = wh/2, and a square window reduces to the reference exactly.
+ */
+#include "sgm_oracle.h"
+
+#include <math.h>
+#include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define INVALID_COST 512
+
+int32_t oracle_round_float(float v) {
+  /* Math.round(float): closest int, ties toward +inf (exact, no v+0.5f). */
+  float f = floorf(v);
+  float frac = v - f; /* exact for |v| < 2^24 */
+  return (int32_t)f + (frac >= 0.5f ? 1 : 0);
+}
+
+int64_t oracle_round_double(double v) {
+  double f = floor(v);
+  double frac = v - f; /* exact for |v| < 2^52 */
+  return (int64_t)f + (frac >= 0.5 ? 1 : 0);
+}
+
+int oracle_words(int ww, int wh) { return (ww * wh - 1 + 63) / 64; }
+
+int oracle_effective_p2(const oracle_params *o) {
+  /* StereoCost.p2: max(P1 + 1, P2) (StereoCost.java:21-23). */
+  return o->p2 > o->p1 + 1 ? o->p2 : o->p1 + 1;
+}
+
+static int max_cost_value(const oracle_params *o) {
+  return o->cost == OR_ZNCC1024 ? 2048 : INVALID_COST;
+}
+
+int64_t oracle_aggregate_bound(const oracle_params *o) {
+  /* StereoCost.aggregateBound: directions * (512 + P2) (StereoCost.java:31-33). */
+  return (int64_t)o->directions * (max_cost_value(o) + (int64_t)oracle_effective_p2(o));
+}
+
+int oracle_aggregate_bytes(const oracle_params *o) {
+  /* StereoCost.aggregateBytes (StereoCost.java:35-41). */
+  int fits = oracle_aggregate_bound(o) <= 65535;
+  if (o->aggregate_bytes == 2) return fits ? 2 : -1;
+  if (o->aggregate_bytes == 4) return 4;
+  return fits ? 2 : 4;
+}
+
+void oracle_census(const uint8_t *in, int w, int h, int ww, int wh, uint64_t *desc) {
+  int rx = ww / 2, ry = wh / 2, words = oracle_words(ww, wh);
+  size_t n = (size_t)w * h;
+  memset(desc, 0, sizeof(uint64_t) * n * words);
+  for (int y = ry; y < h - ry; y++)
+    for (int x = rx; x < w - rx; x++) {
+      size_t p = (size_t)y * w + x;
+      int bit = 0, center = in[p];
+      for (int yy = -ry; yy <= ry; yy++)
+        for (int xx = -rx; xx <= rx; xx++) {
+          if (xx == 0 && yy == 0) continue;
+          if (in[(size_t)(y + yy) * w + x + xx] < center)
+            desc[(size_t)(bit / 64) * n + p] |= 1ULL << (bit % 64);
+          bit++;
+        }
+    }
+}
+
+void oracle_texture(const uint8_t *in, const uint8_t *valid, int w, int h, int ww, int wh,
+                    uint8_t *mask) {
+  int rx = ww / 2, ry = wh / 2;
+  memset(mask, 0, (size_t)w * h);
+  for (int y = ry; y < h - ry; y++)
+    for (int x = rx; x < w - rx; x++) {
+      int low = 255, high = 0;
+      for (int yy = -ry; yy <= ry; yy++)
+        for (int xx = -rx; xx <= rx; xx++) {
+          int v = in[(size_t)(y + yy) * w + x + xx];
+          if (v < low) low = v;
+          if (v > high) high = v;
+        }
+      size_t p = (size_t)y * w + x;
+      mask[p] = (high - low >= 4) && (valid == NULL || valid[p] != 0);
+    }
+}
+
+static int popcount64(uint64_t v) { return __builtin_popcountll(v); }
+
+/* StereoCost.value (StereoCost.java:50-90) plus the FP64 ZNCC-1024 oracle. */
+static int cost_value(const oracle_params *o, const uint64_t *a, const uint64_t *b,
+                      const uint8_t *ga, const uint8_t *gb, size_t p, size_t q, int w,
+                      size_t n, int words) {
+  int bits = o->window_w * o->window_h - 1;
+  int mode = o->cost;
+  if (mode == OR_CENSUS || mode == OR_AD_CENSUS || mode == OR_CENSUS_GRADIENT) {
+    int hamming = 0;
+    for (int word = 0; word < words; word++)
+      hamming += popcount64(a[(size_t)word * n + p] ^ b[(size_t)word * n + q]);
+    if (mode == OR_CENSUS) return hamming;
+    int difference, cap;
+    if (mode == OR_AD_CENSUS) {
+      difference = abs((int)ga[p] - (int)gb[q]);
+      cap = o->ad_truncation;
+    } else {
+      int ax = (int)ga[p + 1] - (int)ga[p - 1], ay = (int)ga[p + w] - (int)ga[p - w];
+      int bx = (int)gb[q + 1] - (int)gb[q - 1], by = (int)gb[q + w] - (int)gb[q - w];
+      difference = abs(ax - bx) + abs(ay - by);
+      cap = o->gradient_truncation;
+    }
+    int weight = o->census_weight, denominator = 1000 * cap;
+    return (weight * hamming * cap + (1000 - weight) * (difference < cap ? difference : cap) * bits +
+            denominator / 2) /
+           denominator;
+  }
+  int rx = o->window_w / 2, ry = o->window_h / 2;
+  int64_t count = (int64_t)o->window_w * o->window_h;
+  int64_t sa = 0, sb = 0, saa = 0, sbb = 0, sab = 0;
+  for (int dy = -ry; dy <= ry; dy++)
+    for (int dx = -rx; dx <= rx; dx++) {
+      int64_t x = ga[p + (ptrdiff_t)dy * w + dx], y = gb[q + (ptrdiff_t)dy * w + dx];
+      sa += x;
+      sb += y;
+      saa += x * x;
+      sbb += y * y;
+      sab += x * y;
+    }
+  int64_t va = count * saa - sa * sa, vb = count * sbb - sb * sb;
+  if (mode == OR_ZNCC) {
+    if (va <= 0 || vb <= 0) return INVALID_COST;
+    double correlation = (double)(count * sab - sa * sb) / sqrt((double)va * (double)vb);
+    double c = correlation < 1 ? correlation : 1; /* Math.min(1, c) */
+    c = c > -1 ? c : -1;                          /* Math.max(-1, .) */
+    return (int)oracle_round_double((1 - c) * .5 * bits);
+  }
+  /* OR_ZNCC1024: FP64 oracle of the FP32 variant. */
+  if (va <= 0 || vb <= 0) return 0x8800;
+  double correlation = (double)(count * sab - sa * sb) / sqrt((double)va * (double)vb);
+  double c = correlation < 1 ? correlation : 1;
+  c = c > -1 ? c : -1;
+  return (int)oracle_round_double((1 - c) * 1024.0);
+}
+
+static int edge_p2(int p1, int p2, int scale, const uint8_t *gray, size_t current, size_t previous) {
+  /* StereoCost.pathP2 (StereoCost.java:44-48). */
+  if (scale == 0) return p2;
+  int difference = abs((int)gray[current] - (int)gray[previous]);
+  int v = (int)((int64_t)p2 * scale / (scale + difference));
+  return p1 + 1 > v ? p1 + 1 : v;
+}
+
+int oracle_match(const oracle_params *o, const uint64_t *a, const uint64_t *b,
+                 const uint8_t *grayA, const uint8_t *grayB, const uint8_t *texturedA,
+                 const uint8_t *texturedB, int width, int height, int reverse, float *result,
+                 oracle_stats *stats, uint8_t *reasons, uint8_t *searchBoundary,
+                 uint16_t *cost_out, uint32_t *agg_out) {
+  int nd = o->disparities, rx = o->window_w / 2, ry = o->window_h / 2;
+  int words = oracle_words(o->window_w, o->window_h);
+  size_t count = (size_t)width * height, cells = count * (size_t)nd;
+  int sum_bytes = oracle_aggregate_bytes(o);
+  if (sum_bytes < 0 || nd < 1 || o->directions < 1 || o->directions > 8) return -1;
+  int invalid = o->cost == OR_ZNCC1024 ? 0x8800 : INVALID_COST;
+  int value_mask = o->cost == OR_ZNCC1024 ? 0x7FFF : 0xFFFF;
+  uint16_t *costs = cost_out ? cost_out : (uint16_t *)malloc(cells * sizeof(uint16_t));
+  /* Aggregate storage: 16-bit wraps like PagedVolume ((short) v, & 65535). */
+  uint16_t *agg16 = NULL;
+  int32_t *agg32 = NULL;
+  if (sum_bytes == 2) agg16 = (uint16_t *)calloc(cells, sizeof(uint16_t));
+  else agg32 = (int32_t *)calloc(cells, sizeof(int32_t));
+  int *previous = (int *)malloc(sizeof(int) * (size_t)width * nd);
+  int *row = (int *)malloc(sizeof(int) * (size_t)width * nd);
+  if (!costs || (!agg16 && !agg32) || !previous || !row) return -2;
+  for (size_t i = 0; i < cells; i++) costs[i] = (uint16_t)invalid;
+  /* Hot loop 1: cost fill (CensusSgm.java:405-420). */
+  for (int y = ry; y < height - ry; y++)
+    for (int x = rx; x < width - rx; x++) {
+      size_t p = (size_t)y * width + x, base = p * nd;
+      if (!texturedA[p]) continue;
+      for (int k = 0; k < nd; k++) {
+        int disparity = o->min_disparity + k;
+        int64_t target = (int64_t)x + (reverse ? disparity : -(int64_t)disparity);
+        if (target < rx || target >= width - rx) continue;
+        size_t q = (size_t)y * width + (size_t)target;
+        if (!texturedB[q]) continue;
+        costs[base + k] = (uint16_t)cost_value(o, a, b, grayA, grayB, p, q, width, count, words);
+      }
+    }
+  /* Hot loop 2: path recurrence (CensusSgm.java:421-459). */
+  static const int paths[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1},
+                                  {1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
+  int p1 = o->p1, p2 = oracle_effective_p2(o);
+  for (int path = 0; path < o->directions; path++) {
+    int dx = paths[path][0], dy = paths[path][1];
+    int yStart = dy < 0 ? height - 1 : 0, yStep = dy < 0 ? -1 : 1;
+    int xStart = dx < 0 ? width - 1 : 0, xStep = dx < 0 ? -1 : 1;
+    for (int yi = 0, y = yStart; yi < height; yi++, y += yStep) {
+      for (int xi = 0, x = xStart; xi < width; xi++, x += xStep) {
+        int px = x - dx, py = y - dy;
+        size_t dest = (size_t)x * nd, base = ((size_t)y * width + x) * nd;
+        int hasPrevious = px >= 0 && px < width && py >= 0 && py < height &&
+                          texturedA[(size_t)py * width + px];
+        int *prev = dy == 0 ? row : previous;
+        size_t prevBase = hasPrevious ? (size_t)px * nd : 0;
+        int minPrev = 0x7FFFFFFF;
+        if (hasPrevious)
+          for (int k = 0; k < nd; k++)
+            if (prev[prevBase + k] < minPrev) minPrev = prev[prevBase + k];
+        int localP2 = hasPrevious ? edge_p2(p1, p2, o->edge_scale, grayA, (size_t)y * width + x,
+                                            (size_t)py * width + px)
+                                  : p2;
+        for (int k = 0; k < nd; k++) {
+          int value = costs[base + k] & value_mask;
+          if (hasPrevious) {
+            int transition = prev[prevBase + k];
+            if (minPrev + localP2 < transition) transition = minPrev + localP2;
+            if (k > 0 && prev[prevBase + k - 1] + p1 < transition)
+              transition = prev[prevBase + k - 1] + p1;
+            if (k + 1 < nd && prev[prevBase + k + 1] + p1 < transition)
+              transition = prev[prevBase + k + 1] + p1;
+            value += transition - minPrev;
+          }
+          row[dest + k] = value;
+          if (agg16) agg16[base + k] = (uint16_t)(agg16[base + k] + (uint32_t)value);
+          else agg32[base + k] = (int32_t)((uint32_t)agg32[base + k] + (uint32_t)value);
+        }
+      }
+      int *swap = previous;
+      previous = row;
+      row = swap;
+    }
+  }
+#define AGG(i) (agg16 ? (int)agg16[i] : (int)agg32[i])
+  /* Hot loop 3: WTA, boundary, uniqueness, subpixel (CensusSgm.java:460-515). */
+  for (size_t i = 0; i < count; i++) result[i] = NAN;
+  for (int y = ry; y < height - ry; y++)
+    for (int x = rx; x < width - rx; x++) {
+      size_t pixel = (size_t)y * width + x, base = pixel * nd;
+      int best = -1, bestCost = 0x7FFFFFFF;
+      for (int k = 0; k < nd; k++)
+        if (costs[base + k] != invalid && AGG(base + k) < bestCost) {
+          best = k;
+          bestCost = AGG(base + k);
+        }
+      if (best < 0) continue;
+      int full = costs[base] != invalid && costs[base + nd - 1] != invalid;
+      if (full) {
+        stats->full_interval_pixels++;
+        if (searchBoundary) searchBoundary[pixel] = 4;
+      }
+      if (best == 0 || best == nd - 1) {
+        stats->boundary_rejected++;
+        if (reasons) reasons[pixel] = 2;
+        if (full && best == 0) {
+          stats->lower_boundary_rejected++;
+          if (searchBoundary) searchBoundary[pixel] |= 1;
+        }
+        if (full && best == nd - 1) {
+          stats->upper_boundary_rejected++;
+          if (searchBoundary) searchBoundary[pixel] |= 2;
+        }
+        continue;
+      }
+      int second = 0x7FFFFFFF;
+      for (int k = 0; k < nd; k++)
+        if (abs(k - best) > 1 && costs[base + k] != invalid && AGG(base + k) < second)
+          second = AGG(base + k);
+      /* second - bestCost <= Math.max(1, bestCost * uniqueness): int vs double. */
+      double allowed = (double)bestCost * o->uniqueness;
+      if (!(allowed >= 1.0)) allowed = 1.0;
+      if (second != 0x7FFFFFFF && (double)(second - bestCost) <= allowed) {
+        stats->ambiguity_rejected++;
+        if (reasons) reasons[pixel] = 3;
+        continue;
+      }
+      double offset = 0;
+      if (best > 0 && best + 1 < nd && costs[base + best - 1] != invalid &&
+          costs[base + best + 1] != invalid) {
+        double before = AGG(base + best - 1), after = AGG(base + best + 1);
+        double curvature = before + after - 2.0 * bestCost;
+        if (curvature > 0) {
+          double t = .5 * (before - after) / curvature;
+          t = t < .5 ? t : .5;
+          offset = t > -.5 ? t : -.5;
+        }
+      }
+      result[pixel] = (float)((double)(o->min_disparity + best) + offset);
+      stats->selected++;
+      if (reasons) reasons[pixel] = 0;
+    }
+  if (agg_out)
+    for (size_t i = 0; i < cells; i++) agg_out[i] = (uint32_t)AGG(i);
+#undef AGG
+  if (!cost_out) free(costs);
+  free(agg16);
+  free(agg32);
+  free(previous);
+  free(row);
+  return 0;
+}
+
+static int isfinite_f(float v) { return isfinite(v); }
+
+int oracle_compute(const oracle_params *o, const uint8_t *left, const uint8_t *right,
+                   const uint8_t *valid_left, const uint8_t *valid_right, int width, int height,
+                   float *checked, float *dl_out, float *dr_out, oracle_stats *stats,
+                   uint8_t *reasons, uint8_t *searchBoundary) {
+  /* CensusSgm.computeRecorded (CensusSgm.java:56-60 dimension check). */
+  if (width < o->window_w + 2 || height < o->window_h + 2) return -1;
+  size_t n = (size_t)width * height;
+  int words = oracle_words(o->window_w, o->window_h);
+  uint64_t *a = (uint64_t *)malloc(sizeof(uint64_t) * n * words);
+  uint64_t *b = (uint64_t *)malloc(sizeof(uint64_t) * n * words);
+  uint8_t *ta = (uint8_t *)malloc(n), *tb = (uint8_t *)malloc(n);
+  float *dl = dl_out ? dl_out : (float *)malloc(sizeof(float) * n);
+  float *dr = dr_out ? dr_out : (float *)malloc(sizeof(float) * n);
+  if (!a || !b || !ta || !tb || !dl || !dr) return -2;
+  if (reasons) memset(reasons, 1, n);
+  if (searchBoundary) memset(searchBoundary, 0, n);
+  oracle_census(left, width, height, o->window_w, o->window_h, a);
+  oracle_census(right, width, height, o->window_w, o->window_h, b);
+  oracle_texture(left, valid_left, width, height, o->window_w, o->window_h, ta);
+  oracle_texture(right, valid_right, width, height, o->window_w, o->window_h, tb);
+  oracle_stats discarded;
+  memset(&discarded, 0, sizeof discarded);
+  int rc = oracle_match(o, a, b, left, right, ta, tb, width, height, 0, dl, stats, reasons,
+                        searchBoundary, NULL, NULL);
+  if (rc == 0)
+    rc = oracle_match(o, b, a, right, left, tb, ta, width, height, 1, dr, &discarded, NULL, NULL,
+                      NULL, NULL);
+  if (rc == 0) {
+    /* Left-right check (CensusSgm.java:181-201). */
+    int rx = o->window_w / 2, ry = o->window_h / 2;
+    for (size_t i = 0; i < n; i++) checked[i] = NAN;
+    for (int y = ry; y < height - ry; y++)
+      for (int x = rx; x < width - rx; x++) {
+        size_t p = (size_t)y * width + x;
+        if (!isfinite_f(dl[p])) continue;
+        int xr = oracle_round_float((float)x - dl[p]);
+        if (xr < rx || xr >= width - rx) {
+          if (reasons) reasons[p] = 4;
+          continue;
+        }
+        float other = dr[(size_t)y * width + xr];
+        if (isfinite_f(other) && (double)fabsf(dl[p] - other) <= o->lr_threshold)
+          checked[p] = dl[p];
+        else {
+          stats->lr_rejected++;
+          if (reasons) reasons[p] = 4;
+        }
+      }
+    for (size_t i = 0; i < n; i++)
+      if (isfinite_f(checked[i])) stats->valid++;
+  }
+  free(a);
+  free(b);
+  free(ta);
+  free(tb);
+  if (!dl_out) free(dl);
+  if (!dr_out) free(dr);
+  return rc;
+}

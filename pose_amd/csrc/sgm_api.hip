@@ -1,0 +1,854 @@
+// libsgm_hip: C-ABI host side of the MI355X SGM engine (plain per-pair path).
+//
+// Replaces the reference's OpenCL session / Compute runtime
+// (src/main/java/sgm/GpuSgm.java:163-619) and the resident branch of
+// GpuStereo.executeTargets (src/main/java/sgm/GpuStereo.java:212-479) with one
+// HIP stream per context, flat 64-bit-indexed volumes instead of 8 banks, and
+// device-side census, selection and left-right check.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/sgm_hip.h"
+#include "sgm_internal.h"
+
+using namespace sgm;
+
+namespace sgm {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  return fail(e == hipErrorOutOfMemory ? SGM_ERR_OUT_OF_MEMORY : SGM_ERR_HIP, "%s failed (HIP %d: %s)",
+              what, (int)e, hipGetErrorString(e));
+}
+
+int context_alloc(sgm_context *ctx, void **ptr, size_t bytes, const char *what) {
+  if (ctx->fail_allocations > 0) {
+    ctx->fail_allocations--;
+    *ptr = nullptr;
+    return fail(SGM_ERR_OUT_OF_MEMORY, "GPU buffer allocation (%zu MiB) for %s failed (simulated)",
+                bytes >> 20, what);
+  }
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 8);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *ptr = nullptr;
+    return fail(e == hipErrorOutOfMemory ? SGM_ERR_OUT_OF_MEMORY : SGM_ERR_HIP,
+                "GPU buffer allocation (%zu MiB) for %s failed (HIP %d)", bytes >> 20, what, (int)e);
+  }
+  ctx->allocated += bytes;
+  return SGM_OK;
+}
+
+int Buffer::ensure(sgm_context *ctx, size_t bytes, const char *what) {
+  if (bytes <= cap && ptr) return SGM_OK;
+  release(ctx);
+  int rc = context_alloc(ctx, &ptr, bytes, what);
+  if (rc == SGM_OK) cap = bytes;
+  return rc;
+}
+
+void Buffer::release(sgm_context *ctx) {
+  if (ptr) {
+    (void)hipFree(ptr);
+    ctx->allocated -= cap;
+  }
+  ptr = nullptr;
+  cap = 0;
+}
+
+int validate_params(const sgm_params *p, int w, int h) {
+  if (!p) return fail(SGM_ERR_INVALID_ARGUMENT, "null params");
+  if (p->window_w < 3 || p->window_w > 11 || p->window_w % 2 == 0 || p->window_h < 3 ||
+      p->window_h > 11 || p->window_h % 2 == 0)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "census window must be odd, 3..11 (got %dx%d)",
+                p->window_w, p->window_h);
+  if (p->disparities < 3 || p->disparities > 4096)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "disparities must be 3..4096 (got %d)", p->disparities);
+  if (p->cost < 0 || p->cost > 4) return fail(SGM_ERR_INVALID_ARGUMENT, "unknown matching cost %d", p->cost);
+  if (p->directions != 4 && p->directions != 8)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "directions must be 4 or 8 (got %d)", p->directions);
+  if (p->p1 < 0 || p->p2 < 0) return fail(SGM_ERR_INVALID_ARGUMENT, "penalties must be >= 0");
+  if (p->edge_scale < 0 || p->edge_scale > 255)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "edge scale must be 0..255");
+  if ((p->cost == 1 && p->ad_truncation < 1) || (p->cost == 2 && p->gradient_truncation < 1))
+    return fail(SGM_ERR_INVALID_ARGUMENT, "truncation must be >= 1");
+  if (p->census_weight < 0 || p->census_weight > 1000)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "census weight must be 0..1000");
+  if (!(p->uniqueness >= 0 && p->uniqueness < 1))
+    return fail(SGM_ERR_INVALID_ARGUMENT, "uniqueness must be in [0, 1)");
+  if (!(p->lr_threshold > 0)) return fail(SGM_ERR_INVALID_ARGUMENT, "lr threshold must be > 0");
+  if (aggregate_bytes(p) < 0)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "16-bit SGM aggregate cannot hold proved bound %lld",
+                (long long)aggregate_bound(p));
+  if (aggregate_bound(p) > 0x7FFFFFFFLL)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "aggregate bound %lld exceeds 32 bits",
+                (long long)aggregate_bound(p));
+  // CensusSgm.java:56-60
+  if (w < p->window_w + 2 || h < p->window_h + 2)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "Invalid stereo image dimensions %dx%d", w, h);
+  return SGM_OK;
+}
+
+int effective_p2(const sgm_params *p) { return std::max(p->p1 + 1, p->p2); }
+
+int64_t aggregate_bound(const sgm_params *p) {
+  const int64_t maxc = p->cost == SGM_COST_ZNCC_FP32 ? 2048 : 512;
+  return (int64_t)p->directions * (maxc + effective_p2(p));
+}
+
+int aggregate_bytes(const sgm_params *p) {
+  const bool fits = aggregate_bound(p) <= 65535;
+  if (p->aggregate_bytes == 2) return fits ? 2 : -1;
+  if (p->aggregate_bytes == 4) return 4;
+  return fits ? 2 : 4;
+}
+
+int lanes_group(int D) {
+  int R = 1;
+  while (R * 64 < D) R <<= 1;
+  return R;
+}
+
+int padded_disparities(int D) {
+  const int R = lanes_group(D), a = std::max(R, 4);
+  return (D + a - 1) / a * a;
+}
+
+int words_of(const sgm_params *p) { return (p->window_w * p->window_h - 1 + 63) / 64; }
+
+int64_t direction_bytes(const sgm_params *p, int w, int h) {
+  const int64_t n = (int64_t)w * h, Dp = padded_disparities(p->disparities);
+  const int sb = std::max(aggregate_bytes(p), 2);
+  const bool zncc = p->cost == SGM_COST_ZNCC || p->cost == SGM_COST_ZNCC_FP32;
+  // cost + aggregate volumes, descriptors, mask, gray copy, disparity, window sums
+  return n * Dp * (2 + sb) + n * (8 * words_of(p) + 1 + 1 + 4 + (zncc ? 8 : 0));
+}
+
+int64_t pair_bytes(const sgm_params *p, int w, int h) {
+  const int64_t n = (int64_t)w * h;
+  // both directions + input copies, validity, reasons/boundary scratch, checked, stats
+  return 2 * direction_bytes(p, w, h) + n * (2 + 2 + 2 + 4) + 256;
+}
+
+}  // namespace sgm
+
+// ---------------------------------------------------------------------------
+extern "C" const char *sgm_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int sgm_device_count(int *count) {
+  if (!count) return fail(SGM_ERR_INVALID_ARGUMENT, "null count");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *count = 0;
+    return SGM_OK;  // no runtime / no device: empty list, as GpuSgm.devices (GpuSgm.java:79-86)
+  }
+  *count = n;
+  return SGM_OK;
+}
+
+extern "C" int sgm_device_info_get(int index, sgm_device_info *out) {
+  if (!out) return fail(SGM_ERR_INVALID_ARGUMENT, "null output");
+  int n = 0;
+  sgm_device_count(&n);
+  if (index < 0 || index >= n) return fail(SGM_ERR_NO_DEVICE, "No GPU at index %d", index);
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDeviceProperties(&prop, index);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+  memset(out, 0, sizeof *out);
+  snprintf(out->name, sizeof out->name, "%s", prop.name);
+  snprintf(out->arch, sizeof out->arch, "%s", prop.gcnArchName);
+  out->memory = (int64_t)prop.totalGlobalMem;
+  out->allocation = (int64_t)prop.totalGlobalMem;
+  out->local_memory = (int64_t)prop.sharedMemPerBlock;
+  out->group_size = prop.maxThreadsPerBlock;
+  out->compute_units = prop.multiProcessorCount;
+  return SGM_OK;
+}
+
+extern "C" int sgm_open(int device, int64_t budget_bytes, sgm_context **out) {
+  if (!out) return fail(SGM_ERR_INVALID_ARGUMENT, "null output");
+  *out = nullptr;
+  sgm_device_info info;
+  int rc = sgm_device_info_get(device, &info);
+  if (rc) return rc;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  sgm_context *ctx = new sgm_context();
+  ctx->device = device;
+  ctx->info = info;
+  // GpuSgm.budget (GpuSgm.java:629-633): 70% of VRAM by default, at most 95%.
+  ctx->budget = budget_bytes > 0 ? std::min(budget_bytes, info.memory * 95 / 100) : info.memory * 7 / 10;
+  e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  *out = ctx;
+  return SGM_OK;
+}
+
+extern "C" int sgm_close(sgm_context *ctx) {
+  if (!ctx) return SGM_OK;
+  std::lock_guard<std::mutex> g(ctx->lock);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &kv : ctx->buffers) kv.second.release(ctx);
+  for (auto &pe : ctx->pending_events) {
+    (void)hipEventDestroy(pe.start);
+    (void)hipEventDestroy(pe.stop);
+  }
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SGM_OK;
+}
+
+extern "C" void *sgm_stream(sgm_context *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+extern "C" int sgm_plan_get(sgm_context *ctx, const sgm_params *params, int w, int h, sgm_plan *out) {
+  if (!ctx || !out) return fail(SGM_ERR_INVALID_ARGUMENT, "null argument");
+  int rc = validate_params(params, w, h);
+  if (rc) return rc;
+  memset(out, 0, sizeof *out);
+  out->aggregate_bytes = aggregate_bytes(params);
+  out->padded_disparities = padded_disparities(params->disparities);
+  out->lanes_per_disparity_group = lanes_group(params->disparities);
+  out->device_bytes = direction_bytes(params, w, h);
+  out->pair_bytes = pair_bytes(params, w, h);
+  out->budget_bytes = ctx->budget;
+  out->aggregate_bound = aggregate_bound(params);
+  out->resident = out->pair_bytes <= ctx->budget;
+  return SGM_OK;
+}
+
+extern "C" int sgm_disparity_limit(sgm_context *ctx, const sgm_params *params, int w, int h,
+                                   int *limit) {
+  // GpuStereo.limit (GpuStereo.java:86-94): binary search over the plan.
+  if (!ctx || !params || !limit) return fail(SGM_ERR_INVALID_ARGUMENT, "null argument");
+  sgm_params q = *params;
+  int low = 2, high = 4096;
+  while (low < high) {
+    int mid = (low + high + 1) / 2;
+    q.disparities = mid;
+    sgm_plan plan;
+    bool ok = mid >= 3 && sgm_plan_get(ctx, &q, w, h, &plan) == SGM_OK && plan.resident;
+    if (ok) low = mid;
+    else high = mid - 1;
+  }
+  *limit = low == 2 ? 0 : low;
+  return SGM_OK;
+}
+
+extern "C" int sgm_set_keep_volumes(sgm_context *ctx, int keep) {
+  if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
+  ctx->keep_volumes = keep != 0;
+  return SGM_OK;
+}
+
+extern "C" int sgm_set_profiling(sgm_context *ctx, int enable) {
+  if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
+  ctx->profiling = enable != 0;
+  return SGM_OK;
+}
+
+extern "C" int sgm_debug_fail_allocations(sgm_context *ctx, int count) {
+  if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
+  ctx->fail_allocations = count;
+  return SGM_OK;
+}
+
+extern "C" int sgm_synchronize(sgm_context *ctx) {
+  if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  collect_profile(ctx);
+  return SGM_OK;
+}
+
+namespace sgm {
+
+void collect_profile(sgm_context *ctx) {
+  for (auto &pe : ctx->pending_events) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, pe.start, pe.stop) == hipSuccess) {
+      auto &k = ctx->kernel_stats[pe.name];
+      k.first += ms;
+      k.second += 1;
+    }
+    (void)hipEventDestroy(pe.start);
+    (void)hipEventDestroy(pe.stop);
+  }
+  ctx->pending_events.clear();
+}
+
+ProfileScope::ProfileScope(sgm_context *c, const char *n) : ctx(c), name(n) {
+  if (ctx->profiling) {
+    (void)hipEventCreate(&ev.start);
+    (void)hipEventCreate(&ev.stop);
+    ev.name = n;
+    (void)hipEventRecord(ev.start, ctx->stream);
+  }
+}
+ProfileScope::~ProfileScope() {
+  if (ctx->profiling) {
+    (void)hipEventRecord(ev.stop, ctx->stream);
+    ctx->pending_events.push_back(ev);
+  }
+}
+
+}  // namespace sgm
+
+extern "C" int sgm_profile_json(sgm_context *ctx, char *buffer, size_t length) {
+  if (!ctx || !buffer) return fail(SGM_ERR_INVALID_ARGUMENT, "null argument");
+  std::string s = "{\"device_kernel_milliseconds\":{";
+  bool first = true;
+  for (auto &kv : ctx->kernel_stats) {
+    char b[256];
+    snprintf(b, sizeof b, "%s\"%s\":{\"total_ms\":%.6f,\"launches\":%lld}", first ? "" : ",",
+             kv.first.c_str(), kv.second.first, (long long)kv.second.second);
+    s += b;
+    first = false;
+  }
+  char b[256];
+  snprintf(b, sizeof b, "},\"allocated_bytes\":%lld,\"budget_bytes\":%lld}", (long long)ctx->allocated,
+           (long long)ctx->budget);
+  s += b;
+  if (s.size() + 1 > length) return fail(SGM_ERR_INVALID_ARGUMENT, "buffer too small (%zu)", s.size() + 1);
+  memcpy(buffer, s.c_str(), s.size() + 1);
+  ctx->kernel_stats.clear();
+  return SGM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+namespace sgm {
+
+static int launch_check(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, what);
+  return SGM_OK;
+}
+
+int launch_fill_u8(sgm_context *ctx, uint8_t *p, uint8_t v, size_t n) {
+  if (!n) return SGM_OK;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  fill_u8_kernel<<<blocks, 256, 0, ctx->stream>>>(p, v, n);
+  return launch_check("fill kernel");
+}
+
+int launch_fill_u64(sgm_context *ctx, unsigned long long *p, unsigned long long v, size_t n) {
+  if (!n) return SGM_OK;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  fill_u64_kernel<<<blocks, 256, 0, ctx->stream>>>(p, v, n);
+  return launch_check("fill kernel");
+}
+
+int launch_census(sgm_context *ctx, const CensusJobs &jobs, int njobs, int w, int h,
+                  const sgm_params *p) {
+  ProfileScope ps(ctx, "census");
+  dim3 grid((w + kCensusTx - 1) / kCensusTx, (h + kCensusTy - 1) / kCensusTy, njobs);
+  dim3 block(kCensusTx, kCensusTy);
+  if (words_of(p) == 1)
+    census_kernel<1><<<grid, block, 0, ctx->stream>>>(jobs, p->window_w, p->window_h);
+  else
+    census_kernel<2><<<grid, block, 0, ctx->stream>>>(jobs, p->window_w, p->window_h);
+  return launch_check("census kernel");
+}
+
+CostArgs cost_args(const sgm_params *p) {
+  CostArgs A;
+  A.mode = p->cost;
+  A.ww = p->window_w;
+  A.wh = p->window_h;
+  A.words = words_of(p);
+  A.weight = p->census_weight;
+  A.adcap = p->ad_truncation;
+  A.gradcap = p->gradient_truncation;
+  A.inv_code = invalid_code(p);
+  return A;
+}
+
+int launch_cost(sgm_context *ctx, const CostJobs &jobs, int njobs, int w, int h, int Dp,
+                const sgm_params *p) {
+  ProfileScope ps(ctx, "cost");
+  const int quads = w * (Dp / 4);
+  dim3 grid((quads + 255) / 256, h, njobs);
+  cost_kernel<<<grid, 256, 0, ctx->stream>>>(jobs, cost_args(p));
+  return launch_check("cost kernel");
+}
+
+PathArgs path_args(const sgm_params *p, int dir, int njobs, int total_lines, int keep_sum) {
+  static const int dirs[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
+  PathArgs A;
+  A.dx = dirs[dir][0];
+  A.dy = dirs[dir][1];
+  A.cls = A.dy == 0 ? 0 : A.dx == 0 ? 1 : 2;
+  A.p1 = p->p1;
+  A.p2 = effective_p2(p);
+  A.edge = p->edge_scale;
+  A.inv_code = invalid_code(p);
+  A.val_mask = p->cost == SGM_COST_ZNCC_FP32 ? 0x7FFF : 0xFFFF;
+  A.keep_sum = keep_sum;
+  A.njobs = njobs;
+  A.total_lines = total_lines;
+  A.uniq = p->uniqueness;
+  return A;
+}
+
+int lines_of(int cls, int w, int h) { return cls == 0 ? h : cls == 1 ? w : w + h - 1; }
+
+template <int R, typename SumT>
+static void launch_path_t(sgm_context *ctx, const PathJobs &jobs, const PathArgs &A, int phase,
+                          int waves) {
+  const int threads = waves * 64;
+  const unsigned blocks = (unsigned)((A.total_lines + waves - 1) / waves);
+  if (phase == PHASE_FIRST)
+    path_kernel<R, SumT, PHASE_FIRST><<<blocks, threads, 0, ctx->stream>>>(jobs, A);
+  else if (phase == PHASE_MID)
+    path_kernel<R, SumT, PHASE_MID><<<blocks, threads, 0, ctx->stream>>>(jobs, A);
+  else
+    path_kernel<R, SumT, PHASE_LAST><<<blocks, threads, 0, ctx->stream>>>(jobs, A);
+}
+
+template <typename SumT>
+static int launch_path_s(sgm_context *ctx, int R, const PathJobs &jobs, const PathArgs &A,
+                         int phase, int waves) {
+  switch (R) {
+    case 1: launch_path_t<1, SumT>(ctx, jobs, A, phase, waves); break;
+    case 2: launch_path_t<2, SumT>(ctx, jobs, A, phase, waves); break;
+    case 4: launch_path_t<4, SumT>(ctx, jobs, A, phase, waves); break;
+    case 8: launch_path_t<8, SumT>(ctx, jobs, A, phase, waves); break;
+    case 16: launch_path_t<16, SumT>(ctx, jobs, A, phase, waves); break;
+    case 32: launch_path_t<32, SumT>(ctx, jobs, A, phase, waves); break;
+    case 64: launch_path_t<64, SumT>(ctx, jobs, A, phase, waves); break;
+    default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
+  }
+  return launch_check("path kernel");
+}
+
+int launch_path(sgm_context *ctx, int R, int sum_bytes, const PathJobs &jobs, const PathArgs &A,
+                int phase, int waves) {
+  ProfileScope ps(ctx, "path");
+  if (A.total_lines <= 0) return SGM_OK;
+  return sum_bytes == 2 ? launch_path_s<uint16_t>(ctx, R, jobs, A, phase, waves)
+                        : launch_path_s<int32_t>(ctx, R, jobs, A, phase, waves);
+}
+
+int launch_lr(sgm_context *ctx, const LrArgs &A) {
+  ProfileScope ps(ctx, "lr_check");
+  const size_t n = (size_t)A.w * A.h;
+  lr_kernel<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(A);
+  return launch_check("lr kernel");
+}
+
+int invalid_code(const sgm_params *p) { return p->cost == SGM_COST_ZNCC_FP32 ? 0x8800 : 512; }
+
+int path_waves(sgm_context *ctx, int R) {
+  auto it = ctx->tuned_waves.find(R);
+  return it != ctx->tuned_waves.end() ? it->second : 4;
+}
+
+}  // namespace sgm
+
+// ---------------------------------------------------------------------------
+// Plain per-pair execution.
+namespace {
+
+struct PairBuffers {
+  uint8_t *gray[2], *valid[2], *mask[2];
+  uint64_t *desc[2];
+  int32_t *wsum[2], *wsum2[2];
+  uint16_t *cost[2];
+  void *sum[2];
+  float *disp[2], *checked;
+  uint8_t *reasons, *boundary, *reasons_out, *boundary_out;
+  unsigned long long *stats;
+};
+
+int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuffers &B) {
+  const size_t n = (size_t)w * h, Dp = padded_disparities(p->disparities);
+  const int sb = aggregate_bytes(p), words = words_of(p);
+  const bool zncc = p->cost == SGM_COST_ZNCC || p->cost == SGM_COST_ZNCC_FP32;
+  int rc;
+#define ENSURE(name, bytes)                                                   \
+  if ((rc = ctx->buffers[name].ensure(ctx, (bytes), name)) != SGM_OK) return rc;
+  for (int d = 0; d < 2; d++) {
+    std::string s = std::to_string(d);
+    ENSURE(("gray" + s).c_str(), n);
+    ENSURE(("valid" + s).c_str(), n);
+    ENSURE(("mask" + s).c_str(), n);
+    ENSURE(("desc" + s).c_str(), n * 8 * words);
+    if (zncc) {
+      ENSURE(("wsum" + s).c_str(), n * 4);
+      ENSURE(("wsum2" + s).c_str(), n * 4);
+    }
+    ENSURE(("cost" + s).c_str(), n * Dp * 2);
+    ENSURE(("sum" + s).c_str(), n * Dp * sb);
+    ENSURE(("disp" + s).c_str(), n * 4);
+    B.gray[d] = (uint8_t *)ctx->buffers["gray" + s].ptr;
+    B.valid[d] = (uint8_t *)ctx->buffers["valid" + s].ptr;
+    B.mask[d] = (uint8_t *)ctx->buffers["mask" + s].ptr;
+    B.desc[d] = (uint64_t *)ctx->buffers["desc" + s].ptr;
+    B.wsum[d] = zncc ? (int32_t *)ctx->buffers["wsum" + s].ptr : nullptr;
+    B.wsum2[d] = zncc ? (int32_t *)ctx->buffers["wsum2" + s].ptr : nullptr;
+    B.cost[d] = (uint16_t *)ctx->buffers["cost" + s].ptr;
+    B.sum[d] = ctx->buffers["sum" + s].ptr;
+    B.disp[d] = (float *)ctx->buffers["disp" + s].ptr;
+  }
+  ENSURE("checked", n * 4);
+  ENSURE("reasons", n);
+  ENSURE("boundary", n);
+  ENSURE("reasons_out", n);
+  ENSURE("boundary_out", n);
+  ENSURE("stats", 16 * 8);
+#undef ENSURE
+  B.checked = (float *)ctx->buffers["checked"].ptr;
+  B.reasons = (uint8_t *)ctx->buffers["reasons"].ptr;
+  B.boundary = (uint8_t *)ctx->buffers["boundary"].ptr;
+  B.reasons_out = (uint8_t *)ctx->buffers["reasons_out"].ptr;
+  B.boundary_out = (uint8_t *)ctx->buffers["boundary_out"].ptr;
+  B.stats = (unsigned long long *)ctx->buffers["stats"].ptr;
+  return SGM_OK;
+}
+
+// Cost + path sweeps + fused selection for `nd` matching directions
+// (d = 0: left reference, target x - d; d = 1: right reference, target x + d).
+int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuffers &B,
+                   const int *slots, int nd, int out_slot, unsigned long long *stats,
+                   uint8_t *reasons, uint8_t *boundary) {
+  int rc;
+  const int D = p->disparities, Dp = padded_disparities(D), R = lanes_group(D);
+  const int sb = aggregate_bytes(p);
+  CostJobs cj;
+  memset(&cj, 0, sizeof cj);
+  for (int i = 0; i < nd; i++) {
+    const int d = slots[i], o = 1 - d;
+    CostJob &J = cj.job[i];
+    J.desc_a = B.desc[d];
+    J.desc_b = B.desc[o];
+    J.mask_a = B.mask[d];
+    J.mask_b = B.mask[o];
+    J.gray_a = B.gray[d];
+    J.gray_b = B.gray[o];
+    J.sum_a = B.wsum[d];
+    J.sum2_a = B.wsum2[d];
+    J.sum_b = B.wsum[o];
+    J.sum2_b = B.wsum2[o];
+    J.cost = B.cost[d];
+    J.w = w;
+    J.h = h;
+    J.D = D;
+    J.Dp = Dp;
+    J.minD = p->min_disparity;
+    J.reverse = d;
+  }
+  if ((rc = launch_cost(ctx, cj, nd, w, h, Dp, p))) return rc;
+  PathJobs pj;
+  memset(&pj, 0, sizeof pj);
+  pj.table = nullptr;
+  for (int i = 0; i < nd; i++) {
+    const int d = slots[i];
+    PathJob &J = pj.local[i];
+    J.cost = B.cost[d];
+    J.sum = B.sum[d];
+    J.mask = B.mask[d];
+    J.gray = B.gray[d];
+    J.disp = B.disp[d];
+    J.reasons = d == out_slot ? reasons : nullptr;
+    J.boundary = d == out_slot ? boundary : nullptr;
+    J.stats = d == out_slot ? stats : nullptr;
+    J.w = w;
+    J.h = h;
+    J.D = D;
+    J.Dp = Dp;
+    J.minD = p->min_disparity;
+    J.rx = p->window_w / 2;
+    J.ry = p->window_h / 2;
+  }
+  const int waves = path_waves(ctx, R);
+  for (int dir = 0; dir < p->directions; dir++) {
+    const int phase = dir == 0 ? PHASE_FIRST : dir == p->directions - 1 ? PHASE_LAST : PHASE_MID;
+    PathArgs A = path_args(p, dir, nd, 0, ctx->keep_volumes ? 1 : 0);
+    A.total_lines = nd * lines_of(A.cls, w, h);
+    A.split = lines_of(A.cls, w, h);
+    if ((rc = launch_path(ctx, R, sb, pj, A, phase, waves))) return rc;
+  }
+  return SGM_OK;
+}
+
+int run_pair(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuffers &B,
+             const uint8_t *left, const uint8_t *right, const uint8_t *vl, const uint8_t *vr,
+             float *checked, uint8_t *reasons_out, uint8_t *boundary_out,
+             unsigned long long *stats) {
+  int rc;
+  const size_t n = (size_t)w * h;
+  if ((rc = launch_fill_u64(ctx, stats, 0ULL, 16))) return rc;
+  if ((rc = launch_fill_u8(ctx, B.reasons, 1, n))) return rc;
+  if ((rc = launch_fill_u8(ctx, B.boundary, 0, n))) return rc;
+  CensusJobs cj;
+  memset(&cj, 0, sizeof cj);
+  const uint8_t *src[2] = {left, right}, *val[2] = {vl, vr};
+  for (int d = 0; d < 2; d++) {
+    CensusJob &J = cj.job[d];
+    J.gray = src[d];
+    J.valid = val[d];
+    J.mask_in = nullptr;
+    J.gray_copy = src[d] == B.gray[d] ? nullptr : B.gray[d];
+    J.desc = B.desc[d];
+    J.mask = B.mask[d];
+    J.wsum = B.wsum[d];
+    J.wsum2 = B.wsum2[d];
+    J.w = w;
+    J.h = h;
+  }
+  if ((rc = launch_census(ctx, cj, 2, w, h, p))) return rc;
+  static const int both[2] = {0, 1};
+  if ((rc = run_directions(ctx, p, w, h, B, both, 2, 0, stats, B.reasons, B.boundary))) return rc;
+  LrArgs L;
+  L.dl = B.disp[0];
+  L.dr = B.disp[1];
+  L.reasons_in = B.reasons;
+  L.boundary_in = B.boundary;
+  L.checked = checked;
+  L.reasons_out = reasons_out;
+  L.boundary_out = boundary_out;
+  L.stats = stats;
+  L.w = w;
+  L.h = h;
+  L.rx = p->window_w / 2;
+  L.ry = p->window_h / 2;
+  L.threshold = p->lr_threshold;
+  return launch_lr(ctx, L);
+}
+
+void stats_from_device(const unsigned long long *c, sgm_stats *s) {
+  memset(s, 0, sizeof *s);
+  s->lower_boundary_rejected = (int64_t)c[0];
+  s->full_interval_pixels = (int64_t)c[1];
+  s->boundary_rejected = (int64_t)c[2];
+  s->upper_boundary_rejected = (int64_t)c[3];
+  s->ambiguity_rejected = (int64_t)c[4];
+  s->selected = (int64_t)c[5];
+  s->lr_rejected = (int64_t)c[6];
+  s->valid = (int64_t)c[10];
+}
+
+int copy_h2d(sgm_context *ctx, void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+  return e == hipSuccess ? SGM_OK : hip_fail(e, "host to device copy");
+}
+
+int copy_d2h(sgm_context *ctx, void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+  return e == hipSuccess ? SGM_OK : hip_fail(e, "device to host copy");
+}
+
+int finish(sgm_context *ctx) {
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  collect_profile(ctx);
+  return SGM_OK;
+}
+
+}  // namespace
+
+extern "C" int sgm_match_pair(sgm_context *ctx, const sgm_params *params, const uint8_t *left,
+                              const uint8_t *right, const uint8_t *valid_left,
+                              const uint8_t *valid_right, int w, int h, float *checked, float *dl,
+                              float *dr, sgm_stats *stats, uint8_t *reasons, uint8_t *boundary) {
+  if (!ctx || !left || !right || !checked) return fail(SGM_ERR_INVALID_ARGUMENT, "null argument");
+  if ((valid_left == nullptr) != (valid_right == nullptr))
+    return fail(SGM_ERR_INVALID_ARGUMENT, "Invalid stereo mask dimensions");
+  std::lock_guard<std::mutex> g(ctx->lock);
+  int rc = validate_params(params, w, h);
+  if (rc) return rc;
+  if (pair_bytes(params, w, h) > ctx->budget)
+    return fail(SGM_ERR_NO_PLAN, "No GPU SGM memory plan fits the requested range (%lld > %lld bytes)",
+                (long long)pair_bytes(params, w, h), (long long)ctx->budget);
+  (void)hipSetDevice(ctx->device);
+  PairBuffers B;
+  if ((rc = ensure_pair_buffers(ctx, params, w, h, B))) return rc;
+  const size_t n = (size_t)w * h;
+  if ((rc = copy_h2d(ctx, B.gray[0], left, n))) return rc;
+  if ((rc = copy_h2d(ctx, B.gray[1], right, n))) return rc;
+  const uint8_t *vl = nullptr, *vr = nullptr;
+  if (valid_left) {
+    if ((rc = copy_h2d(ctx, B.valid[0], valid_left, n))) return rc;
+    if ((rc = copy_h2d(ctx, B.valid[1], valid_right, n))) return rc;
+    vl = B.valid[0];
+    vr = B.valid[1];
+  }
+  if ((rc = run_pair(ctx, params, w, h, B, B.gray[0], B.gray[1], vl, vr, B.checked, B.reasons_out,
+                     B.boundary_out, B.stats)))
+    return rc;
+  unsigned long long counters[16];
+  if ((rc = copy_d2h(ctx, checked, B.checked, n * 4))) return rc;
+  if (dl && (rc = copy_d2h(ctx, dl, B.disp[0], n * 4))) return rc;
+  if (dr && (rc = copy_d2h(ctx, dr, B.disp[1], n * 4))) return rc;
+  if (reasons && (rc = copy_d2h(ctx, reasons, B.reasons_out, n))) return rc;
+  if (boundary && (rc = copy_d2h(ctx, boundary, B.boundary_out, n))) return rc;
+  if ((rc = copy_d2h(ctx, counters, B.stats, sizeof counters))) return rc;
+  if ((rc = finish(ctx))) return rc;
+  if (stats) stats_from_device(counters, stats);
+  ctx->last_w = w;
+  ctx->last_h = h;
+  ctx->last_params = *params;
+  return SGM_OK;
+}
+
+extern "C" int sgm_match_pair_device(sgm_context *ctx, const sgm_params *params, const void *d_left,
+                                     const void *d_right, const void *d_valid_left,
+                                     const void *d_valid_right, int w, int h, void *d_checked,
+                                     void *d_dl, void *d_dr, void *d_stats, void *d_reasons,
+                                     void *d_boundary) {
+  if (!ctx || !d_left || !d_right || !d_checked) return fail(SGM_ERR_INVALID_ARGUMENT, "null argument");
+  if ((d_valid_left == nullptr) != (d_valid_right == nullptr))
+    return fail(SGM_ERR_INVALID_ARGUMENT, "Invalid stereo mask dimensions");
+  std::lock_guard<std::mutex> g(ctx->lock);
+  int rc = validate_params(params, w, h);
+  if (rc) return rc;
+  if (pair_bytes(params, w, h) > ctx->budget)
+    return fail(SGM_ERR_NO_PLAN, "No GPU SGM memory plan fits the requested range");
+  (void)hipSetDevice(ctx->device);
+  PairBuffers B;
+  if ((rc = ensure_pair_buffers(ctx, params, w, h, B))) return rc;
+  const size_t n = (size_t)w * h;
+  unsigned long long *stats = d_stats ? (unsigned long long *)d_stats : B.stats;
+  if ((rc = run_pair(ctx, params, w, h, B, (const uint8_t *)d_left, (const uint8_t *)d_right,
+                     (const uint8_t *)d_valid_left, (const uint8_t *)d_valid_right,
+                     (float *)d_checked, d_reasons ? (uint8_t *)d_reasons : B.reasons_out,
+                     d_boundary ? (uint8_t *)d_boundary : B.boundary_out, stats)))
+    return rc;
+  if (d_dl) {
+    hipError_t e = hipMemcpyAsync(d_dl, B.disp[0], n * 4, hipMemcpyDeviceToDevice, ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "device copy");
+  }
+  if (d_dr) {
+    hipError_t e = hipMemcpyAsync(d_dr, B.disp[1], n * 4, hipMemcpyDeviceToDevice, ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "device copy");
+  }
+  ctx->last_w = w;
+  ctx->last_h = h;
+  ctx->last_params = *params;
+  return SGM_OK;
+}
+
+extern "C" int sgm_match(sgm_context *ctx, const sgm_params *params, const uint8_t *gray_a,
+                         const uint8_t *gray_b, const uint8_t *mask_a, const uint8_t *mask_b, int w,
+                         int h, int reverse, float *disp, sgm_stats *stats, uint8_t *reasons,
+                         uint8_t *boundary) {
+  // GpuSgm.match (GpuSgm.java:267-303) for one matching direction with the
+  // caller's texture&valid masks.  Internally the reference image is slot 0
+  // and the target slot 1 with the direction's `reverse` flag.
+  if (!ctx || !gray_a || !gray_b || !mask_a || !mask_b || !disp)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> g(ctx->lock);
+  int rc = validate_params(params, w, h);
+  if (rc) return rc;
+  if (pair_bytes(params, w, h) > ctx->budget)
+    return fail(SGM_ERR_NO_PLAN, "No GPU SGM memory plan fits the requested range");
+  (void)hipSetDevice(ctx->device);
+  PairBuffers B;
+  if ((rc = ensure_pair_buffers(ctx, params, w, h, B))) return rc;
+  const size_t n = (size_t)w * h;
+  // slot d = reverse: the reference image sits in the slot whose cost job has J.reverse = reverse.
+  const int ref = reverse ? 1 : 0, tgt = 1 - ref;
+  if ((rc = copy_h2d(ctx, B.gray[ref], gray_a, n))) return rc;
+  if ((rc = copy_h2d(ctx, B.gray[tgt], gray_b, n))) return rc;
+  if ((rc = copy_h2d(ctx, B.valid[ref], mask_a, n))) return rc;
+  if ((rc = copy_h2d(ctx, B.valid[tgt], mask_b, n))) return rc;
+  if ((rc = launch_fill_u64(ctx, B.stats, 0ULL, 16))) return rc;
+  if (reasons) {
+    if ((rc = copy_h2d(ctx, B.reasons, reasons, n))) return rc;
+  }
+  if (boundary) {
+    if ((rc = copy_h2d(ctx, B.boundary, boundary, n))) return rc;
+  }
+  CensusJobs cj;
+  memset(&cj, 0, sizeof cj);
+  for (int d = 0; d < 2; d++) {
+    CensusJob &J = cj.job[d];
+    J.gray = B.gray[d];
+    J.mask_in = B.valid[d];
+    J.desc = B.desc[d];
+    J.mask = B.mask[d];
+    J.wsum = B.wsum[d];
+    J.wsum2 = B.wsum2[d];
+    J.w = w;
+    J.h = h;
+  }
+  if ((rc = launch_census(ctx, cj, 2, w, h, params))) return rc;
+  rc = run_directions(ctx, params, w, h, B, &ref, 1, ref, B.stats, reasons ? B.reasons : nullptr,
+                      boundary ? B.boundary : nullptr);
+  if (rc) return rc;
+  unsigned long long counters[16];
+  if ((rc = copy_d2h(ctx, disp, B.disp[ref], n * 4))) return rc;
+  if (reasons && (rc = copy_d2h(ctx, reasons, B.reasons, n))) return rc;
+  if (boundary && (rc = copy_d2h(ctx, boundary, B.boundary, n))) return rc;
+  if ((rc = copy_d2h(ctx, counters, B.stats, sizeof counters))) return rc;
+  if ((rc = finish(ctx))) return rc;
+  if (stats) {
+    // GpuSgm.java:289-294: the completed counters are added to the caller's.
+    sgm_stats s;
+    stats_from_device(counters, &s);
+    stats->lower_boundary_rejected += s.lower_boundary_rejected;
+    stats->full_interval_pixels += s.full_interval_pixels;
+    stats->boundary_rejected += s.boundary_rejected;
+    stats->upper_boundary_rejected += s.upper_boundary_rejected;
+    stats->ambiguity_rejected += s.ambiguity_rejected;
+    stats->selected += s.selected;
+  }
+  ctx->last_w = w;
+  ctx->last_h = h;
+  ctx->last_params = *params;
+  return SGM_OK;
+}
+
+extern "C" int sgm_read_volumes(sgm_context *ctx, int direction, uint16_t *cost, uint32_t *aggregate) {
+  if (!ctx || direction < 0 || direction > 1) return fail(SGM_ERR_INVALID_ARGUMENT, "bad argument");
+  std::lock_guard<std::mutex> g(ctx->lock);
+  if (!ctx->keep_volumes || ctx->last_w == 0)
+    return fail(SGM_ERR_NOT_FOUND, "no retained volumes (enable sgm_set_keep_volumes first)");
+  const sgm_params *p = &ctx->last_params;
+  const size_t n = (size_t)ctx->last_w * ctx->last_h, D = p->disparities, Dp = padded_disparities(p->disparities);
+  const int sb = aggregate_bytes(p);
+  std::string s = std::to_string(direction);
+  const void *dc = ctx->buffers["cost" + s].ptr, *ds = ctx->buffers["sum" + s].ptr;
+  if (!dc || !ds) return fail(SGM_ERR_NOT_FOUND, "no volumes");
+  std::vector<uint16_t> hc(n * Dp);
+  std::vector<uint8_t> hs(n * Dp * sb);
+  int rc;
+  if ((rc = copy_d2h(ctx, hc.data(), dc, hc.size() * 2))) return rc;
+  if ((rc = copy_d2h(ctx, hs.data(), ds, hs.size()))) return rc;
+  if ((rc = finish(ctx))) return rc;
+  for (size_t i = 0; i < n; i++)
+    for (size_t k = 0; k < D; k++) {
+      if (cost) cost[i * D + k] = hc[i * Dp + k];
+      if (aggregate) {
+        if (sb == 2) aggregate[i * D + k] = ((const uint16_t *)hs.data())[i * Dp + k];
+        else aggregate[i * D + k] = (uint32_t)((const int32_t *)hs.data())[i * Dp + k];
+      }
+    }
+  return SGM_OK;
+}

@@ -1,0 +1,85 @@
+// Host-side internals shared by the translation units of libsgm_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/sgm_hip.h"
+#include "sgm_kernels.h"
+
+struct sgm_context;
+
+namespace sgm {
+
+struct Buffer {
+  void *ptr = nullptr;
+  size_t cap = 0;
+  int ensure(sgm_context *ctx, size_t bytes, const char *what);
+  void release(sgm_context *ctx);
+};
+
+struct PendingEvent {
+  hipEvent_t start, stop;
+  const char *name;
+};
+
+int fail(int code, const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+int context_alloc(sgm_context *ctx, void **ptr, size_t bytes, const char *what);
+int validate_params(const sgm_params *p, int w, int h);
+int effective_p2(const sgm_params *p);
+int64_t aggregate_bound(const sgm_params *p);
+int aggregate_bytes(const sgm_params *p);
+int lanes_group(int D);
+int padded_disparities(int D);
+int words_of(const sgm_params *p);
+int invalid_code(const sgm_params *p);
+int64_t direction_bytes(const sgm_params *p, int w, int h);
+int64_t pair_bytes(const sgm_params *p, int w, int h);
+int lines_of(int cls, int w, int h);
+void collect_profile(sgm_context *ctx);
+
+struct ProfileScope {
+  sgm_context *ctx;
+  const char *name;
+  PendingEvent ev{};
+  ProfileScope(sgm_context *c, const char *n);
+  ~ProfileScope();
+};
+
+int launch_fill_u8(sgm_context *ctx, uint8_t *p, uint8_t v, size_t n);
+int launch_fill_u64(sgm_context *ctx, unsigned long long *p, unsigned long long v, size_t n);
+int launch_census(sgm_context *ctx, const CensusJobs &jobs, int njobs, int w, int h,
+                  const sgm_params *p);
+int launch_cost(sgm_context *ctx, const CostJobs &jobs, int njobs, int w, int h, int Dp,
+                const sgm_params *p);
+CostArgs cost_args(const sgm_params *p);
+PathArgs path_args(const sgm_params *p, int dir, int njobs, int total_lines, int keep_sum);
+int launch_path(sgm_context *ctx, int R, int sum_bytes, const PathJobs &jobs, const PathArgs &A,
+                int phase, int waves);
+int launch_lr(sgm_context *ctx, const LrArgs &A);
+int path_waves(sgm_context *ctx, int R);
+
+}  // namespace sgm
+
+struct sgm_context {
+  int device = 0;
+  sgm_device_info info{};
+  int64_t budget = 0;
+  int64_t allocated = 0;
+  hipStream_t stream = nullptr;
+  std::mutex lock;
+  std::map<std::string, sgm::Buffer> buffers;
+  bool keep_volumes = false;
+  bool profiling = false;
+  int fail_allocations = 0;
+  std::vector<sgm::PendingEvent> pending_events;
+  std::map<std::string, std::pair<double, long long>> kernel_stats;
+  std::map<int, int> tuned_waves;   // lane group R -> waves per workgroup in path sweeps
+  int last_w = 0, last_h = 0;
+  sgm_params last_params{};
+};

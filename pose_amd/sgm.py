@@ -1,0 +1,554 @@
+"""ctypes binding of libsgm_hip and a Python mirror of the reference's GPU-SGM backend.
+
+The reference (Java) reaches its GPU through ``GpuSgm`` (src/main/java/sgm/GpuSgm.java)
+and ``GpuStereo`` (src/main/java/sgm/GpuStereo.java), called from
+``CensusSgm.match`` (src/main/java/sgm/CensusSgm.java:359-374).  This module keeps
+that interface -- ``Options`` field names, ``StereoCost`` unit conversions,
+``GpuSgm.use / disparityLimit / match`` and ``CensusSgm.compute`` -- on top of the
+C ABI declared in include/sgm_hip.h.  Every computation runs in the HIP library;
+there is no CPU fallback in this package: when the library is missing or fails,
+calls raise.
+"""
+import ctypes
+import math
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsgm_hip.so")
+
+SGM_OK = 0
+ERRORS = {
+    -1: "invalid argument", -2: "no device", -3: "out of memory", -4: "no plan",
+    -5: "HIP error", -6: "rejected", -7: "timeout", -8: "not found",
+}
+
+COSTS = {"census": 0, "ad-census": 1, "census-gradient": 2, "zncc": 3, "zncc-fp32": 4}
+
+
+class SgmParams(ctypes.Structure):
+    _fields_ = [
+        ("window_w", ctypes.c_int32), ("window_h", ctypes.c_int32),
+        ("disparities", ctypes.c_int32), ("min_disparity", ctypes.c_int32),
+        ("cost", ctypes.c_int32), ("p1", ctypes.c_int32), ("p2", ctypes.c_int32),
+        ("edge_scale", ctypes.c_int32), ("directions", ctypes.c_int32),
+        ("aggregate_bytes", ctypes.c_int32), ("census_weight", ctypes.c_int32),
+        ("ad_truncation", ctypes.c_int32), ("gradient_truncation", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("uniqueness", ctypes.c_double), ("lr_threshold", ctypes.c_double),
+    ]
+
+
+STAT_FIELDS = [
+    "lower_boundary_rejected", "full_interval_pixels", "boundary_rejected",
+    "upper_boundary_rejected", "ambiguity_rejected", "selected", "lr_rejected",
+    "post_median_lr_rejected", "photometric_rejected", "speckle_rejected", "valid",
+]
+
+
+class SgmStats(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_int64) for f in STAT_FIELDS] + [("reserved", ctypes.c_int64 * 5)]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f in STAT_FIELDS}
+
+
+class SgmDeviceInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 128), ("arch", ctypes.c_char * 64),
+                ("memory", ctypes.c_int64), ("allocation", ctypes.c_int64),
+                ("local_memory", ctypes.c_int64), ("group_size", ctypes.c_int32),
+                ("compute_units", ctypes.c_int32)]
+
+
+class SgmPlan(ctypes.Structure):
+    _fields_ = [("resident", ctypes.c_int32), ("aggregate_bytes", ctypes.c_int32),
+                ("padded_disparities", ctypes.c_int32),
+                ("lanes_per_disparity_group", ctypes.c_int32),
+                ("device_bytes", ctypes.c_int64), ("pair_bytes", ctypes.c_int64),
+                ("budget_bytes", ctypes.c_int64), ("aggregate_bound", ctypes.c_int64)]
+
+
+class SgmPair(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("disparities", ctypes.c_int32), ("min_disparity", ctypes.c_int32),
+                ("left", ctypes.c_void_p), ("right", ctypes.c_void_p),
+                ("valid_left", ctypes.c_void_p), ("valid_right", ctypes.c_void_p),
+                ("checked", ctypes.c_void_p), ("dl", ctypes.c_void_p), ("dr", ctypes.c_void_p),
+                ("reasons", ctypes.c_void_p), ("boundary", ctypes.c_void_p),
+                ("stats", ctypes.c_void_p)]
+
+
+class SgmRequestInfo(ctypes.Structure):
+    _fields_ = [("arrival_s", ctypes.c_double), ("admitted_s", ctypes.c_double),
+                ("started_s", ctypes.c_double), ("finished_s", ctypes.c_double),
+                ("device_bytes", ctypes.c_int64), ("status", ctypes.c_int32),
+                ("batch_id", ctypes.c_int32)]
+
+
+EXPORTS = [
+    "sgm_device_count", "sgm_device_info_get", "sgm_open", "sgm_close", "sgm_plan_get",
+    "sgm_disparity_limit", "sgm_match", "sgm_match_pair", "sgm_match_pair_device",
+    "sgm_set_keep_volumes", "sgm_read_volumes", "sgm_synchronize", "sgm_set_profiling",
+    "sgm_profile_json", "sgm_stream", "sgm_debug_fail_allocations", "sgm_last_error",
+    "sgm_batch_create", "sgm_batch_run", "sgm_batch_destroy", "sgm_batch_device_bytes",
+    "sgm_autotune", "sgm_autotune_load",
+    "sgm_server_create", "sgm_server_submit", "sgm_server_wait", "sgm_server_now",
+    "sgm_server_destroy",
+]
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+class SgmError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__("%s (%s, code %d)" % (message, ERRORS.get(code, "error"), code))
+        self.code = code
+
+
+def lib():
+    """Load libsgm_hip.so; raises if it has not been built (no silent fallback)."""
+    global _lib
+    with _lib_lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise OSError("libsgm_hip.so not built at %s (run __graft_entry__.build())" % LIB_PATH)
+            L = ctypes.CDLL(LIB_PATH)
+            P, I, I64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+            pp = ctypes.POINTER(SgmParams)
+            ps = ctypes.POINTER(SgmStats)
+            L.sgm_last_error.restype = ctypes.c_char_p
+            L.sgm_device_count.argtypes = [ctypes.POINTER(I)]
+            L.sgm_device_info_get.argtypes = [I, ctypes.POINTER(SgmDeviceInfo)]
+            L.sgm_open.argtypes = [I, I64, ctypes.POINTER(P)]
+            L.sgm_close.argtypes = [P]
+            L.sgm_plan_get.argtypes = [P, pp, I, I, ctypes.POINTER(SgmPlan)]
+            L.sgm_disparity_limit.argtypes = [P, pp, I, I, ctypes.POINTER(I)]
+            L.sgm_match.argtypes = [P, pp, P, P, P, P, I, I, I, P, ps, P, P]
+            L.sgm_match_pair.argtypes = [P, pp, P, P, P, P, I, I, P, P, P, ps, P, P]
+            L.sgm_match_pair_device.argtypes = [P, pp, P, P, P, P, I, I, P, P, P, P, P, P]
+            L.sgm_set_keep_volumes.argtypes = [P, I]
+            L.sgm_read_volumes.argtypes = [P, I, P, P]
+            L.sgm_synchronize.argtypes = [P]
+            L.sgm_set_profiling.argtypes = [P, I]
+            L.sgm_profile_json.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
+            L.sgm_stream.argtypes = [P]
+            L.sgm_stream.restype = P
+            L.sgm_debug_fail_allocations.argtypes = [P, I]
+            L.sgm_batch_create.argtypes = [P, pp, ctypes.POINTER(SgmPair), I, I, ctypes.POINTER(P)]
+            L.sgm_batch_run.argtypes = [P]
+            L.sgm_batch_destroy.argtypes = [P]
+            L.sgm_batch_device_bytes.argtypes = [P]
+            L.sgm_batch_device_bytes.restype = I64
+            L.sgm_autotune.argtypes = [P, pp, ctypes.c_char_p]
+            L.sgm_autotune_load.argtypes = [P, ctypes.c_char_p]
+            L.sgm_server_create.argtypes = [P, pp, I64, I, I, ctypes.POINTER(P)]
+            L.sgm_server_submit.argtypes = [P, ctypes.POINTER(SgmPair), ctypes.POINTER(ctypes.c_uint64)]
+            L.sgm_server_wait.argtypes = [P, ctypes.c_uint64, ctypes.c_double,
+                                          ctypes.POINTER(SgmRequestInfo)]
+            L.sgm_server_now.argtypes = [P]
+            L.sgm_server_now.restype = ctypes.c_double
+            L.sgm_server_destroy.argtypes = [P]
+            _lib = L
+        return _lib
+
+
+def check(rc):
+    if rc != SGM_OK:
+        raise SgmError(rc, lib().sgm_last_error().decode(errors="replace"))
+    return rc
+
+
+def _p(a):
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return a
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    assert a.flags["C_CONTIGUOUS"], "arrays must be C-contiguous"
+    return a.ctypes.data
+
+
+# ---------------------------------------------------------------------------
+# Reference-interface mirror: Options subset, StereoCost, Stats, Result.
+
+
+def java_round(v):
+    """Math.round(double): closest integer, ties toward +infinity."""
+    f = math.floor(v)
+    return int(f) + (1 if v - f >= 0.5 else 0)
+
+
+class Options:
+    """SGM subset of the reference's Options (Options.java:17-40,67), same field names.
+
+    ``censusWidth``/``censusHeight`` generalise ``census`` to rectangular windows
+    (0 = use ``census``)."""
+
+    def __init__(self, **kw):
+        self.census = 5
+        self.censusWidth = 0
+        self.censusHeight = 0
+        self.disparities = 128
+        self.minDisparity = 0
+        self.matchingCost = "census"
+        self.p1 = 4.0
+        self.p2 = 24.0
+        self.sgmEdgeScale = 32
+        self.directions = 8
+        self.uniqueness = 0.03
+        self.lrThreshold = 0.75
+        self.sgmAggregate = "auto"
+        self.censusWeight = 0.5
+        self.adTruncation = 30
+        self.gradientTruncation = 60
+        self.compute = "auto"
+        self.gpuDevice = 0
+        self.gpuMemoryMb = 0
+        self.rawPenalties = False  # p1/p2 already in raw scaled units
+        for k, v in kw.items():
+            if not hasattr(self, k):
+                raise AttributeError("unknown option " + k)
+            setattr(self, k, v)
+
+    def copy(self):
+        o = Options()
+        o.__dict__.update(self.__dict__)
+        return o
+
+    def window(self):
+        return (self.censusWidth or self.census, self.censusHeight or self.census)
+
+
+class StereoCost:
+    """StereoCost.java:7-48."""
+
+    @staticmethod
+    def code(o):
+        if o.matchingCost not in COSTS:
+            raise ValueError("Unknown matching cost")
+        return COSTS[o.matchingCost]
+
+    @staticmethod
+    def penalty(value, o):
+        ww, wh = o.window()
+        return int(java_round(value * (ww * wh - 1) / 24.0))
+
+    @staticmethod
+    def p1(o):
+        return int(o.p1) if o.rawPenalties else StereoCost.penalty(o.p1, o)
+
+    @staticmethod
+    def p2(o):
+        raw = int(o.p2) if o.rawPenalties else StereoCost.penalty(o.p2, o)
+        return max(StereoCost.p1(o) + 1, raw)
+
+    @staticmethod
+    def aggregateBound(o):
+        maxc = 2048 if o.matchingCost == "zncc-fp32" else 512
+        return o.directions * (maxc + StereoCost.p2(o))
+
+    @staticmethod
+    def aggregateBytes(o):
+        fits = StereoCost.aggregateBound(o) <= 65535
+        if o.sgmAggregate == "16" and not fits:
+            raise ValueError("16-bit SGM aggregate cannot hold proved bound %d"
+                             % StereoCost.aggregateBound(o))
+        return 2 if o.sgmAggregate != "32" and fits else 4
+
+
+def params_from_options(o):
+    ww, wh = o.window()
+    return SgmParams(
+        window_w=ww, window_h=wh, disparities=o.disparities, min_disparity=o.minDisparity,
+        cost=StereoCost.code(o), p1=StereoCost.p1(o), p2=StereoCost.p2(o),
+        edge_scale=o.sgmEdgeScale, directions=o.directions,
+        aggregate_bytes=StereoCost.aggregateBytes(o),
+        census_weight=int(java_round(o.censusWeight * 1000)),
+        ad_truncation=o.adTruncation, gradient_truncation=o.gradientTruncation, reserved=0,
+        uniqueness=o.uniqueness, lr_threshold=o.lrThreshold)
+
+
+class Stats:
+    """CensusSgm.Stats (CensusSgm.java:683-737)."""
+
+    def __init__(self):
+        for f in STAT_FIELDS:
+            setattr(self, f, 0)
+
+    def add_match(self, d):
+        # GpuSgm.java:289-294
+        for f in ("lower_boundary_rejected", "full_interval_pixels", "boundary_rejected",
+                  "upper_boundary_rejected", "ambiguity_rejected", "selected"):
+            setattr(self, f, getattr(self, f) + d[f])
+
+    def set_all(self, d):
+        for f in STAT_FIELDS:
+            setattr(self, f, d[f])
+
+    def csv(self):
+        return ",".join(str(getattr(self, f)) for f in (
+            "boundary_rejected", "upper_boundary_rejected", "ambiguity_rejected", "selected",
+            "lr_rejected", "post_median_lr_rejected", "photometric_rejected", "speckle_rejected",
+            "valid"))
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f in STAT_FIELDS}
+
+
+class Result:
+    """CensusSgm.Result (CensusSgm.java:739-767) at the left-right-validated checkpoint."""
+
+    def __init__(self, width, height, disparity, stats, reasons=None, boundary=None,
+                 left=None, right=None):
+        self.width, self.height = width, height
+        self.disparity = disparity
+        self.stats = stats
+        self.reasons = reasons
+        self.searchBoundary = boundary
+        self.left = left
+        self.right = right
+
+
+# ---------------------------------------------------------------------------
+# Context: one device session (GpuSgm session, GpuSgm.java:163-221).
+
+
+class Context:
+    def __init__(self, device=0, budget_bytes=0):
+        h = ctypes.c_void_p()
+        check(lib().sgm_open(device, int(budget_bytes), ctypes.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            lib().sgm_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def plan(self, params, w, h):
+        out = SgmPlan()
+        check(lib().sgm_plan_get(self.h, ctypes.byref(params), w, h, ctypes.byref(out)))
+        return {f: getattr(out, f) for f, _ in SgmPlan._fields_}
+
+    def disparity_limit(self, params, w, h):
+        v = ctypes.c_int()
+        check(lib().sgm_disparity_limit(self.h, ctypes.byref(params), w, h, ctypes.byref(v)))
+        return v.value
+
+    def match_pair(self, params, left, right, valid_left=None, valid_right=None):
+        """Both directions + L-R check on host arrays (h, w) uint8."""
+        left = np.ascontiguousarray(left, np.uint8)
+        right = np.ascontiguousarray(right, np.uint8)
+        h, w = left.shape
+        n = w * h
+        vl = None if valid_left is None else np.ascontiguousarray(valid_left, np.uint8)
+        vr = None if valid_right is None else np.ascontiguousarray(valid_right, np.uint8)
+        checked = np.empty(n, np.float32)
+        dl = np.empty(n, np.float32)
+        dr = np.empty(n, np.float32)
+        reasons = np.empty(n, np.uint8)
+        boundary = np.empty(n, np.uint8)
+        st = SgmStats()
+        check(lib().sgm_match_pair(self.h, ctypes.byref(params), _p(left), _p(right), _p(vl),
+                                   _p(vr), w, h, _p(checked), _p(dl), _p(dr), ctypes.byref(st),
+                                   _p(reasons), _p(boundary)))
+        return dict(checked=checked, dl=dl, dr=dr, stats=st.as_dict(), reasons=reasons,
+                    boundary=boundary)
+
+    def match(self, params, gray_a, gray_b, mask_a, mask_b, w, h, reverse, stats=None,
+              reasons=None, boundary=None):
+        disp = np.empty(w * h, np.float32)
+        st = SgmStats() if stats is None else stats
+        check(lib().sgm_match(self.h, ctypes.byref(params), _p(gray_a), _p(gray_b), _p(mask_a),
+                              _p(mask_b), w, h, int(reverse), _p(disp), ctypes.byref(st),
+                              _p(reasons), _p(boundary)))
+        return disp, st
+
+    def match_pair_device(self, params, d_left, d_right, w, h, d_checked, d_valid_left=None,
+                          d_valid_right=None, d_dl=None, d_dr=None, d_stats=None,
+                          d_reasons=None, d_boundary=None):
+        check(lib().sgm_match_pair_device(self.h, ctypes.byref(params), _p(d_left), _p(d_right),
+                                          _p(d_valid_left), _p(d_valid_right), w, h,
+                                          _p(d_checked), _p(d_dl), _p(d_dr), _p(d_stats),
+                                          _p(d_reasons), _p(d_boundary)))
+
+    def keep_volumes(self, keep=True):
+        check(lib().sgm_set_keep_volumes(self.h, int(keep)))
+
+    def read_volumes(self, direction, n, D):
+        cost = np.empty(n * D, np.uint16)
+        agg = np.empty(n * D, np.uint32)
+        check(lib().sgm_read_volumes(self.h, direction, _p(cost), _p(agg)))
+        return cost, agg
+
+    def synchronize(self):
+        check(lib().sgm_synchronize(self.h))
+
+    def set_profiling(self, on=True):
+        check(lib().sgm_set_profiling(self.h, int(on)))
+
+    def profile(self):
+        import json
+        buf = ctypes.create_string_buffer(1 << 16)
+        check(lib().sgm_profile_json(self.h, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def stream(self):
+        return lib().sgm_stream(self.h)
+
+    def fail_allocations(self, count):
+        check(lib().sgm_debug_fail_allocations(self.h, count))
+
+    def autotune(self, params, path=None):
+        check(lib().sgm_autotune(self.h, ctypes.byref(params),
+                                 None if path is None else path.encode()))
+
+    def autotune_load(self, path):
+        check(lib().sgm_autotune_load(self.h, path.encode()))
+
+
+def device_count():
+    n = ctypes.c_int()
+    check(lib().sgm_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def device_info(index):
+    d = SgmDeviceInfo()
+    check(lib().sgm_device_info_get(index, ctypes.byref(d)))
+    return dict(name=d.name.decode(), arch=d.arch.decode(), memory=d.memory,
+                allocation=d.allocation, local_memory=d.local_memory, group_size=d.group_size,
+                compute_units=d.compute_units)
+
+
+# ---------------------------------------------------------------------------
+# GpuSgm mirror (GpuSgm.java:237-303) -- one session per thread.
+
+_session = threading.local()
+_warned = threading.local()
+
+
+class GpuSgm:
+    @staticmethod
+    def session(o):
+        s = getattr(_session, "ctx", None)
+        if s is not None and s.device == o.gpuDevice:
+            return s
+        GpuSgm.closeSession()
+        if o.gpuDevice >= device_count():
+            raise RuntimeError("No GPU at index %d" % o.gpuDevice)
+        s = Context(o.gpuDevice, o.gpuMemoryMb * 1048576)
+        _session.ctx = s
+        return s
+
+    @staticmethod
+    def closeSession():
+        s = getattr(_session, "ctx", None)
+        _session.ctx = None
+        if s is not None:
+            s.close()
+        _warned.messages = set()
+
+    @staticmethod
+    def failure(o, e):
+        # GpuSgm.java:260-265
+        if o.compute == "gpu":
+            raise ValueError("GPU execution required: %s" % e) from e
+        msgs = getattr(_warned, "messages", None)
+        if msgs is None:
+            msgs = _warned.messages = set()
+        if str(e) not in msgs:
+            msgs.add(str(e))
+            print("GPU unavailable; using CPU: %s" % e)
+
+    @staticmethod
+    def disparityLimit(w, h, o):
+        if o.compute == "cpu":
+            return 0
+        try:
+            return GpuSgm.session(o).disparity_limit(params_from_options(o), w, h)
+        except (RuntimeError, OSError) as e:
+            GpuSgm.failure(o, e)
+            return 0
+
+    @staticmethod
+    def use(w, h, o):
+        # GpuSgm.java:247-258
+        if o.compute == "cpu" or (o.compute == "auto" and w * h * o.disparities < 1000000):
+            return False
+        limit = GpuSgm.disparityLimit(w, h, o)
+        if o.disparities <= limit:
+            return True
+        if o.compute == "gpu":
+            raise ValueError("GPU SGM search exceeds VRAM/allocation/local-memory budget; limit=%d"
+                             " disparities." % limit)
+        return False
+
+    @staticmethod
+    def match(o, a, b, grayA, grayB, ta, tb, w, h, reverse, stats, reasons, searchBoundary):
+        """GpuSgm.match: returns the disparity map or None (= caller's CPU fallback).
+
+        ``a``/``b`` (descriptors) are accepted for signature parity; the engine
+        recomputes census on the device from ``grayA``/``grayB``."""
+        if not GpuSgm.use(w, h, o):
+            return None
+        try:
+            ctx = GpuSgm.session(o)
+            st = SgmStats()
+            codes = None if reasons is None else np.array(reasons, np.uint8, copy=True)
+            bnd = None if searchBoundary is None else np.array(searchBoundary, np.uint8, copy=True)
+            disp, st = ctx.match(params_from_options(o), np.ascontiguousarray(grayA, np.uint8),
+                                 np.ascontiguousarray(grayB, np.uint8),
+                                 np.ascontiguousarray(ta, np.uint8),
+                                 np.ascontiguousarray(tb, np.uint8), w, h, reverse, st, codes, bnd)
+            stats.add_match(st.as_dict())
+            if codes is not None:
+                reasons[:] = codes
+            if bnd is not None:
+                searchBoundary[:] = bnd
+            return disp
+        except (RuntimeError, OSError) as e:
+            GpuSgm.closeSession()
+            GpuSgm.failure(o, e)
+            return None
+
+
+class CensusSgm:
+    """CensusSgm.compute (CensusSgm.java:18-280) through the device path, up to the
+    02_left_right_validated checkpoint.  Post-L-R CPU filters (median, photometric,
+    speckles) are outside this engine; options that enable them are rejected."""
+
+    def __init__(self, options):
+        self.options = options
+
+    def compute(self, left, right, width, height, validLeft=None, validRight=None):
+        o = self.options
+        ww, wh = o.window()
+        left = np.ascontiguousarray(left, np.uint8).reshape(height, width)
+        right = np.ascontiguousarray(right, np.uint8).reshape(height, width)
+        if width < ww + 2 or height < wh + 2 or left.size != width * height or right.size != left.size:
+            raise ValueError("Invalid stereo image dimensions")
+        if (validLeft is None) != (validRight is None):
+            raise ValueError("Invalid stereo mask dimensions")
+        if o.compute == "cpu":
+            raise RuntimeError("this engine has no CPU SGM path (compute=cpu)")
+        try:
+            ctx = GpuSgm.session(o)
+            r = ctx.match_pair(params_from_options(o), left, right, validLeft, validRight)
+        except (RuntimeError, OSError) as e:
+            GpuSgm.closeSession()
+            GpuSgm.failure(o, e)
+            raise RuntimeError("GPU SGM failed and this engine has no CPU path: %s" % e) from e
+        st = Stats()
+        st.set_all(r["stats"])
+        return Result(width, height, r["checked"], st, r["reasons"], r["boundary"], r["dl"], r["dr"])
