@@ -147,6 +147,14 @@ int sgm_match_pair_device(sgm_context *ctx, const sgm_params *params, const void
                           void *d_dl, void *d_dr, void *d_stats, void *d_reasons,
                           void *d_boundary);
 
+/* Device memory for the *_device entry points (callers without their own HIP
+ * allocator, e.g. a JVM through FFM).  sgm_memcpy is ordered on the context's
+ * stream and returns after the copy completes; kind: 0 host->device,
+ * 1 device->host, 2 device->device. */
+int sgm_device_alloc(sgm_context *ctx, size_t bytes, void **out);
+int sgm_device_free(sgm_context *ctx, void *ptr);
+int sgm_memcpy(sgm_context *ctx, void *dst, const void *src, size_t bytes, int kind);
+
 /* Copy the last call's cost (uint16, n*D, 512 = invalid) and aggregate
  * (widened to uint32, n*D) volumes of one matching direction (0 = left
  * reference, 1 = right reference) for parity checks.  Volumes are only kept

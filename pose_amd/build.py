@@ -13,7 +13,7 @@ HIPCC_FLAGS = [
     "-ffp-contract=off",
     # no packed-FP32 instructions in generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-result",
+    "-Wall", "-Wno-unused-result", "-Wno-unused-function",
 ]
 
 

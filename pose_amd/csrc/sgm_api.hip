@@ -852,3 +852,33 @@ extern "C" int sgm_read_volumes(sgm_context *ctx, int direction, uint16_t *cost,
     }
   return SGM_OK;
 }
+
+extern "C" int sgm_device_alloc(sgm_context *ctx, size_t bytes, void **out) {
+  if (!ctx || !out) return fail(SGM_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> g(ctx->lock);
+  (void)hipSetDevice(ctx->device);
+  return context_alloc(ctx, out, bytes, "caller buffer");
+}
+
+extern "C" int sgm_device_free(sgm_context *ctx, void *ptr) {
+  if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
+  if (!ptr) return SGM_OK;
+  std::lock_guard<std::mutex> g(ctx->lock);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  hipError_t e = hipFree(ptr);
+  return e == hipSuccess ? SGM_OK : hip_fail(e, "hipFree");
+}
+
+extern "C" int sgm_memcpy(sgm_context *ctx, void *dst, const void *src, size_t bytes, int kind) {
+  if (!ctx || (!dst && bytes) || (!src && bytes) || kind < 0 || kind > 2)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "bad copy arguments");
+  std::lock_guard<std::mutex> g(ctx->lock);
+  (void)hipSetDevice(ctx->device);
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                          : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, k, ctx->stream);
+  if (e != hipSuccess) return hip_fail(e, "sgm_memcpy");
+  e = hipStreamSynchronize(ctx->stream);
+  return e == hipSuccess ? SGM_OK : hip_fail(e, "sgm_memcpy synchronize");
+}

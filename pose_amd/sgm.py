@@ -92,6 +92,7 @@ EXPORTS = [
     "sgm_disparity_limit", "sgm_match", "sgm_match_pair", "sgm_match_pair_device",
     "sgm_set_keep_volumes", "sgm_read_volumes", "sgm_synchronize", "sgm_set_profiling",
     "sgm_profile_json", "sgm_stream", "sgm_debug_fail_allocations", "sgm_last_error",
+    "sgm_device_alloc", "sgm_device_free", "sgm_memcpy",
     "sgm_batch_create", "sgm_batch_run", "sgm_batch_destroy", "sgm_batch_device_bytes",
     "sgm_autotune", "sgm_autotune_load",
     "sgm_server_create", "sgm_server_submit", "sgm_server_wait", "sgm_server_now",
@@ -137,6 +138,9 @@ def lib():
             L.sgm_stream.argtypes = [P]
             L.sgm_stream.restype = P
             L.sgm_debug_fail_allocations.argtypes = [P, I]
+            L.sgm_device_alloc.argtypes = [P, ctypes.c_size_t, ctypes.POINTER(P)]
+            L.sgm_device_free.argtypes = [P, P]
+            L.sgm_memcpy.argtypes = [P, P, P, ctypes.c_size_t, I]
             L.sgm_batch_create.argtypes = [P, pp, ctypes.POINTER(SgmPair), I, I, ctypes.POINTER(P)]
             L.sgm_batch_run.argtypes = [P]
             L.sgm_batch_destroy.argtypes = [P]
@@ -406,6 +410,12 @@ class Context:
     def stream(self):
         return lib().sgm_stream(self.h)
 
+    # -- device buffers owned by the caller (plain device pointers) --
+    def alloc(self, nbytes):
+        ptr = ctypes.c_void_p()
+        check(lib().sgm_device_alloc(self.h, int(nbytes), ctypes.byref(ptr)))
+        return DeviceBuffer(self, ptr.value, int(nbytes))
+
     def fail_allocations(self, count):
         check(lib().sgm_debug_fail_allocations(self.h, count))
 
@@ -415,6 +425,39 @@ class Context:
 
     def autotune_load(self, path):
         check(lib().sgm_autotune_load(self.h, path.encode()))
+
+
+class DeviceBuffer:
+    """A device allocation made through the library (freed on close/garbage collection)."""
+
+    def __init__(self, ctx, ptr, nbytes):
+        self.ctx, self.ptr, self.nbytes = ctx, ptr, nbytes
+
+    def data_ptr(self):
+        return self.ptr
+
+    def upload(self, array):
+        a = np.ascontiguousarray(array)
+        assert a.nbytes <= self.nbytes
+        check(lib().sgm_memcpy(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes, 0))
+        return self
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib().sgm_memcpy(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes, 1))
+        return out
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            lib().sgm_device_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def device_count():

@@ -54,6 +54,8 @@ def disparity_planes(h, w, min_disparity, disparities, rng):
     """3-6 slanted planes on a Voronoi partition; values inside [min+2, min+D-3]."""
     count = int(rng.integers(3, 7))
     lo, hi = min_disparity + 2.0, min_disparity + disparities - 3.0
+    if hi < lo:  # tiny ranges: use the whole interval
+        lo, hi = float(min_disparity), float(min_disparity + disparities - 1)
     seeds = np.stack([rng.uniform(0, w, count), rng.uniform(0, h, count)], 1).astype(np.float32)
     base = rng.uniform(lo + 0.15 * (hi - lo), hi - 0.15 * (hi - lo), count)
     span = (hi - lo) * 0.15

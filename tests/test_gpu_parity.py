@@ -202,29 +202,30 @@ def test_plain_workload_1600x1067_d256(gpu_context):
 
 
 def test_device_entry_point_matches_host(gpu_context):
-    torch = pytest.importorskip("torch")
     left, right, _ = synth.stereo_pair(200, 120, 48, -4, seed=9)
     op = O.params(window_w=9, window_h=7, disparities=48, min_disparity=-4, p1=10, p2=120,
                   lr_threshold=1.0)
     p = to_sgm(op)
-    host = gpu_context.match_pair(p, left, right)
-    dev = torch.device("cuda:0")
-    dl_ = torch.from_numpy(left).to(dev)
-    dr_ = torch.from_numpy(right).to(dev)
+    ctx = gpu_context
+    host = ctx.match_pair(p, left, right)
     n = left.size
-    out = torch.empty(n, dtype=torch.float32, device=dev)
-    st = torch.zeros(16, dtype=torch.int64, device=dev)
-    reasons = torch.empty(n, dtype=torch.uint8, device=dev)
-    bnd = torch.empty(n, dtype=torch.uint8, device=dev)
-    torch.cuda.synchronize()
+    d_left = ctx.alloc(n).upload(left)
+    d_right = ctx.alloc(n).upload(right)
+    out = ctx.alloc(4 * n)
+    st = ctx.alloc(128).upload(np.zeros(16, np.int64))
+    reasons = ctx.alloc(n)
+    bnd = ctx.alloc(n)
     for _ in range(2):  # twice: buffers are reused
-        gpu_context.match_pair_device(p, dl_, dr_, 200, 120, out, d_stats=st, d_reasons=reasons,
-                                      d_boundary=bnd)
-        gpu_context.synchronize()
-        assert np.array_equal(bits(out.cpu().numpy()), bits(host["checked"]))
-        assert np.array_equal(reasons.cpu().numpy(), host["reasons"])
-        s = st.cpu().numpy()
+        ctx.match_pair_device(p, d_left, d_right, 200, 120, out, d_stats=st, d_reasons=reasons,
+                              d_boundary=bnd)
+        ctx.synchronize()
+        assert np.array_equal(bits(out.download(np.float32, n)), bits(host["checked"]))
+        assert np.array_equal(reasons.download(np.uint8, n), host["reasons"])
+        assert np.array_equal(bnd.download(np.uint8, n), host["boundary"])
+        s = st.download(np.int64, 16)
         assert s[5] == host["stats"]["selected"] and s[10] == host["stats"]["valid"]
+    for b in (d_left, d_right, out, st, reasons, bnd):
+        b.free()
 
 
 def test_repeatable_and_recovers_from_allocation_failure():
