@@ -418,51 +418,79 @@ PathArgs path_args(const sgm_params *p, int dir, int njobs, int total_lines, int
 
 int lines_of(int cls, int w, int h) { return cls == 0 ? h : cls == 1 ? w : w + h - 1; }
 
-template <int R, typename SumT>
+template <int R, typename SumT, int MODE>
 static void launch_path_t(sgm_context *ctx, const PathJobs &jobs, const PathArgs &A, int phase,
                           int waves) {
   const int threads = waves * 64;
   const unsigned blocks = (unsigned)((A.total_lines + waves - 1) / waves);
   if (phase == PHASE_FIRST)
-    path_kernel<R, SumT, PHASE_FIRST><<<blocks, threads, 0, ctx->stream>>>(jobs, A);
+    path_kernel<R, SumT, PHASE_FIRST, MODE><<<blocks, threads, 0, ctx->stream>>>(jobs, A);
   else if (phase == PHASE_MID)
-    path_kernel<R, SumT, PHASE_MID><<<blocks, threads, 0, ctx->stream>>>(jobs, A);
+    path_kernel<R, SumT, PHASE_MID, MODE><<<blocks, threads, 0, ctx->stream>>>(jobs, A);
   else
-    path_kernel<R, SumT, PHASE_LAST><<<blocks, threads, 0, ctx->stream>>>(jobs, A);
+    path_kernel<R, SumT, PHASE_LAST, MODE><<<blocks, threads, 0, ctx->stream>>>(jobs, A);
 }
 
-template <typename SumT>
-static int launch_path_s(sgm_context *ctx, int R, const PathJobs &jobs, const PathArgs &A,
-                         int phase, int waves) {
-  switch (R) {
-    case 1: launch_path_t<1, SumT>(ctx, jobs, A, phase, waves); break;
-    case 2: launch_path_t<2, SumT>(ctx, jobs, A, phase, waves); break;
-    case 4: launch_path_t<4, SumT>(ctx, jobs, A, phase, waves); break;
-    case 8: launch_path_t<8, SumT>(ctx, jobs, A, phase, waves); break;
-    case 16: launch_path_t<16, SumT>(ctx, jobs, A, phase, waves); break;
-    case 32: launch_path_t<32, SumT>(ctx, jobs, A, phase, waves); break;
-    case 64: launch_path_t<64, SumT>(ctx, jobs, A, phase, waves); break;
-    default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
+// MODE 0: generic 32-bit sweep (32-bit aggregates); 1: packed 16-bit; 2: packed
+// 16-bit first sweep with the census cost fused in.
+static int launch_path_any(sgm_context *ctx, int R, int sum_bytes, int mode, const PathJobs &jobs,
+                           const PathArgs &A, int phase, int waves) {
+#define CASE_WIDE(RR)                                                       \
+  case RR:                                                                  \
+    launch_path_t<RR, int32_t, 0>(ctx, jobs, A, phase, waves);              \
+    break;
+#define CASE_PK(RR)                                                         \
+  case RR:                                                                  \
+    if (mode == 2) launch_path_t<RR, uint16_t, 2>(ctx, jobs, A, phase, waves); \
+    else launch_path_t<RR, uint16_t, 1>(ctx, jobs, A, phase, waves);        \
+    break;
+  if (sum_bytes == 4) {
+    switch (R) {
+      CASE_WIDE(1) CASE_WIDE(2) CASE_WIDE(4) CASE_WIDE(8) CASE_WIDE(16) CASE_WIDE(32) CASE_WIDE(64)
+      default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
+    }
+  } else {
+    switch (R) {
+      CASE_PK(2) CASE_PK(4) CASE_PK(8)
+      case 16: launch_path_t<16, uint16_t, 1>(ctx, jobs, A, phase, waves); break;
+      case 32: launch_path_t<32, uint16_t, 1>(ctx, jobs, A, phase, waves); break;
+      case 64: launch_path_t<64, uint16_t, 1>(ctx, jobs, A, phase, waves); break;
+      default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
+    }
   }
+#undef CASE_WIDE
+#undef CASE_PK
   return launch_check("path kernel");
 }
 
-int launch_path(sgm_context *ctx, int R, int sum_bytes, const PathJobs &jobs, const PathArgs &A,
-                int phase, int waves) {
+int launch_path(sgm_context *ctx, int R, int sum_bytes, int mode, const PathJobs &jobs,
+                const PathArgs &A, int phase, int waves) {
   ProfileScope ps(ctx, "path");
   if (A.total_lines <= 0) return SGM_OK;
-  return sum_bytes == 2 ? launch_path_s<uint16_t>(ctx, R, jobs, A, phase, waves)
-                        : launch_path_s<int32_t>(ctx, R, jobs, A, phase, waves);
+  return launch_path_any(ctx, R, sum_bytes, mode, jobs, A, phase, waves);
 }
 
 int launch_lr(sgm_context *ctx, const LrArgs &A) {
-  ProfileScope ps(ctx, "lr_check");
+  ProfileScope ps(ctx, A.dr ? "lr_check" : "count");
   const size_t n = (size_t)A.w * A.h;
-  lr_kernel<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(A);
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 1024);
+  lr_kernel<<<blocks, 256, 0, ctx->stream>>>(A);
   return launch_check("lr kernel");
 }
 
 int invalid_code(const sgm_params *p) { return p->cost == SGM_COST_ZNCC_FP32 ? 0x8800 : 512; }
+
+int path_lanes(const sgm_params *p) {
+  // 16-bit aggregates run packed pairs: at least two disparities per lane.
+  const int R = lanes_group(p->disparities);
+  return aggregate_bytes(p) == 2 ? std::max(R, 2) : R;
+}
+
+bool fused_cost(const sgm_params *p) {
+  // census cost computed inside the first (left-to-right) sweep
+  return p->cost == SGM_COST_CENSUS && words_of(p) == 1 && aggregate_bytes(p) == 2 &&
+         path_lanes(p) <= 8;
+}
 
 int path_waves(sgm_context *ctx, int R) {
   auto it = ctx->tuned_waves.find(R);
@@ -477,12 +505,13 @@ namespace {
 
 struct PairBuffers {
   uint8_t *gray[2], *valid[2], *mask[2];
+  uint16_t *gm[2];
   uint64_t *desc[2];
   int32_t *wsum[2], *wsum2[2];
   uint16_t *cost[2];
   void *sum[2];
   float *disp[2], *checked;
-  uint8_t *reasons, *boundary, *reasons_out, *boundary_out;
+  uint8_t *reasons, *boundary, *reasons_out, *boundary_out, *code;
   unsigned long long *stats;
 };
 
@@ -498,6 +527,7 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     ENSURE(("gray" + s).c_str(), n);
     ENSURE(("valid" + s).c_str(), n);
     ENSURE(("mask" + s).c_str(), n);
+    ENSURE(("gm" + s).c_str(), n * 2);
     ENSURE(("desc" + s).c_str(), n * 8 * words);
     if (zncc) {
       ENSURE(("wsum" + s).c_str(), n * 4);
@@ -509,6 +539,7 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     B.gray[d] = (uint8_t *)ctx->buffers["gray" + s].ptr;
     B.valid[d] = (uint8_t *)ctx->buffers["valid" + s].ptr;
     B.mask[d] = (uint8_t *)ctx->buffers["mask" + s].ptr;
+    B.gm[d] = (uint16_t *)ctx->buffers["gm" + s].ptr;
     B.desc[d] = (uint64_t *)ctx->buffers["desc" + s].ptr;
     B.wsum[d] = zncc ? (int32_t *)ctx->buffers["wsum" + s].ptr : nullptr;
     B.wsum2[d] = zncc ? (int32_t *)ctx->buffers["wsum2" + s].ptr : nullptr;
@@ -522,6 +553,7 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
   ENSURE("reasons_out", n);
   ENSURE("boundary_out", n);
   ENSURE("stats", 16 * 8);
+  ENSURE("code", n);
 #undef ENSURE
   B.checked = (float *)ctx->buffers["checked"].ptr;
   B.reasons = (uint8_t *)ctx->buffers["reasons"].ptr;
@@ -529,55 +561,66 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
   B.reasons_out = (uint8_t *)ctx->buffers["reasons_out"].ptr;
   B.boundary_out = (uint8_t *)ctx->buffers["boundary_out"].ptr;
   B.stats = (unsigned long long *)ctx->buffers["stats"].ptr;
+  B.code = (uint8_t *)ctx->buffers["code"].ptr;
   return SGM_OK;
 }
 
 // Cost + path sweeps + fused selection for `nd` matching directions
 // (d = 0: left reference, target x - d; d = 1: right reference, target x + d).
+// The selection codes of slot `out_slot` go to B.code (counted later), its
+// reason / boundary updates to `reasons` / `boundary`.
 int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuffers &B,
-                   const int *slots, int nd, int out_slot, unsigned long long *stats,
-                   uint8_t *reasons, uint8_t *boundary) {
+                   const int *slots, int nd, int out_slot, uint8_t *reasons, uint8_t *boundary) {
   int rc;
-  const int D = p->disparities, Dp = padded_disparities(D), R = lanes_group(D);
+  const int D = p->disparities, Dp = padded_disparities(D), R = path_lanes(p);
   const int sb = aggregate_bytes(p);
-  CostJobs cj;
-  memset(&cj, 0, sizeof cj);
-  for (int i = 0; i < nd; i++) {
-    const int d = slots[i], o = 1 - d;
-    CostJob &J = cj.job[i];
-    J.desc_a = B.desc[d];
-    J.desc_b = B.desc[o];
-    J.mask_a = B.mask[d];
-    J.mask_b = B.mask[o];
-    J.gray_a = B.gray[d];
-    J.gray_b = B.gray[o];
-    J.sum_a = B.wsum[d];
-    J.sum2_a = B.wsum2[d];
-    J.sum_b = B.wsum[o];
-    J.sum2_b = B.wsum2[o];
-    J.cost = B.cost[d];
-    J.w = w;
-    J.h = h;
-    J.D = D;
-    J.Dp = Dp;
-    J.minD = p->min_disparity;
-    J.reverse = d;
+  const bool fused = fused_cost(p);
+  const int mode = sb == 2 ? 1 : 0;
+  if (!fused) {
+    CostJobs cj;
+    memset(&cj, 0, sizeof cj);
+    for (int i = 0; i < nd; i++) {
+      const int d = slots[i], o = 1 - d;
+      CostJob &J = cj.job[i];
+      J.desc_a = B.desc[d];
+      J.desc_b = B.desc[o];
+      J.mask_a = B.mask[d];
+      J.mask_b = B.mask[o];
+      J.gray_a = B.gray[d];
+      J.gray_b = B.gray[o];
+      J.sum_a = B.wsum[d];
+      J.sum2_a = B.wsum2[d];
+      J.sum_b = B.wsum[o];
+      J.sum2_b = B.wsum2[o];
+      J.cost = B.cost[d];
+      J.w = w;
+      J.h = h;
+      J.D = D;
+      J.Dp = Dp;
+      J.minD = p->min_disparity;
+      J.reverse = d;
+    }
+    if ((rc = launch_cost(ctx, cj, nd, w, h, Dp, p))) return rc;
   }
-  if ((rc = launch_cost(ctx, cj, nd, w, h, Dp, p))) return rc;
   PathJobs pj;
   memset(&pj, 0, sizeof pj);
   pj.table = nullptr;
   for (int i = 0; i < nd; i++) {
-    const int d = slots[i];
+    const int d = slots[i], o = 1 - d;
     PathJob &J = pj.local[i];
     J.cost = B.cost[d];
     J.sum = B.sum[d];
     J.mask = B.mask[d];
     J.gray = B.gray[d];
+    J.gm = B.gm[d];
+    J.desc_a = B.desc[d];
+    J.desc_b = B.desc[o];
+    J.mask_b = B.mask[o];
+    J.cost_out = B.cost[d];
     J.disp = B.disp[d];
     J.reasons = d == out_slot ? reasons : nullptr;
     J.boundary = d == out_slot ? boundary : nullptr;
-    J.stats = d == out_slot ? stats : nullptr;
+    J.code = d == out_slot ? B.code : nullptr;
     J.w = w;
     J.h = h;
     J.D = D;
@@ -585,6 +628,8 @@ int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuff
     J.minD = p->min_disparity;
     J.rx = p->window_w / 2;
     J.ry = p->window_h / 2;
+    J.reverse = d;
+    J.R = R;
   }
   const int waves = path_waves(ctx, R);
   for (int dir = 0; dir < p->directions; dir++) {
@@ -592,7 +637,8 @@ int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuff
     PathArgs A = path_args(p, dir, nd, 0, ctx->keep_volumes ? 1 : 0);
     A.total_lines = nd * lines_of(A.cls, w, h);
     A.split = lines_of(A.cls, w, h);
-    if ((rc = launch_path(ctx, R, sb, pj, A, phase, waves))) return rc;
+    const int m = (dir == 0 && fused) ? 2 : mode;
+    if ((rc = launch_path(ctx, R, sb, m, pj, A, phase, waves))) return rc;
   }
   return SGM_OK;
 }
@@ -617,6 +663,7 @@ int run_pair(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuffers &B
     J.gray_copy = src[d] == B.gray[d] ? nullptr : B.gray[d];
     J.desc = B.desc[d];
     J.mask = B.mask[d];
+    J.gm = B.gm[d];
     J.wsum = B.wsum[d];
     J.wsum2 = B.wsum2[d];
     J.w = w;
@@ -624,8 +671,9 @@ int run_pair(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuffers &B
   }
   if ((rc = launch_census(ctx, cj, 2, w, h, p))) return rc;
   static const int both[2] = {0, 1};
-  if ((rc = run_directions(ctx, p, w, h, B, both, 2, 0, stats, B.reasons, B.boundary))) return rc;
+  if ((rc = run_directions(ctx, p, w, h, B, both, 2, 0, B.reasons, B.boundary))) return rc;
   LrArgs L;
+  L.code = B.code;
   L.dl = B.disp[0];
   L.dr = B.disp[1];
   L.reasons_in = B.reasons;
@@ -793,14 +841,24 @@ extern "C" int sgm_match(sgm_context *ctx, const sgm_params *params, const uint8
     J.mask_in = B.valid[d];
     J.desc = B.desc[d];
     J.mask = B.mask[d];
+    J.gm = B.gm[d];
     J.wsum = B.wsum[d];
     J.wsum2 = B.wsum2[d];
     J.w = w;
     J.h = h;
   }
   if ((rc = launch_census(ctx, cj, 2, w, h, params))) return rc;
-  rc = run_directions(ctx, params, w, h, B, &ref, 1, ref, B.stats, reasons ? B.reasons : nullptr,
+  rc = run_directions(ctx, params, w, h, B, &ref, 1, ref, reasons ? B.reasons : nullptr,
                       boundary ? B.boundary : nullptr);
+  if (rc == SGM_OK) {
+    LrArgs L;
+    memset(&L, 0, sizeof L);
+    L.code = B.code;
+    L.stats = B.stats;
+    L.w = w;
+    L.h = h;
+    rc = launch_lr(ctx, L);
+  }
   if (rc) return rc;
   unsigned long long counters[16];
   if ((rc = copy_d2h(ctx, disp, B.disp[ref], n * 4))) return rc;

@@ -59,8 +59,10 @@ int launch_cost(sgm_context *ctx, const CostJobs &jobs, int njobs, int w, int h,
                 const sgm_params *p);
 CostArgs cost_args(const sgm_params *p);
 PathArgs path_args(const sgm_params *p, int dir, int njobs, int total_lines, int keep_sum);
-int launch_path(sgm_context *ctx, int R, int sum_bytes, const PathJobs &jobs, const PathArgs &A,
-                int phase, int waves);
+int launch_path(sgm_context *ctx, int R, int sum_bytes, int mode, const PathJobs &jobs,
+                const PathArgs &A, int phase, int waves);
+int path_lanes(const sgm_params *p);
+bool fused_cost(const sgm_params *p);
 int launch_lr(sgm_context *ctx, const LrArgs &A);
 int path_waves(sgm_context *ctx, int R);
 

@@ -1,0 +1,912 @@
+// Path sweeps of the SGM recurrence (Hirschmuller with edge-adaptive P2):
+// CPU CensusSgm.java:421-459, GPU `paths` GpuStereo.java:732-757, and the
+// winner-take-all of CensusSgm.java:460-515 / GpuStereo.java:631-687,758-769
+// fused into the last sweep.
+//
+// One wave64 sweeps one scanline.  Lane l holds the R consecutive disparities
+// k = l*R .. l*R+R-1 of the current pixel, so each pixel step moves one
+// contiguous Dp-vector of cost and aggregate per lane-vector load.  The
+// previous pixel's path costs stay in registers; k-1 / k+1 across lane edges
+// come from DPP wave shifts, min over k from a DPP reduction.
+//
+// With 16-bit aggregates (every sum <= 65535, StereoCost.java:25-41) the
+// recurrence runs on packed uint16 pairs (v_pk_min_u16 / v_pk_add_u16): all
+// intermediate values stay below 2^15 because L <= maxC + P2 and
+// minLp + P2 <= 2 * (maxC + P2) <= 65535 / 2.  Adding L to a packed 16-bit S
+// wraps exactly like the reference's (short) store.
+#pragma once
+
+namespace sgm {
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+struct PathJob {
+  const uint16_t *cost;     // cost volume (read by non-fused sweeps)
+  void *sum;                // aggregate volume
+  const uint8_t *mask;      // texture & valid of the reference image
+  const uint8_t *gray;      // reference image
+  const uint16_t *gm;       // gray | (texture & valid) << 8 of the reference image
+  // fused first sweep (census cost computed on the fly, written to cost_out)
+  const uint64_t *desc_a, *desc_b;
+  const uint8_t *mask_b;
+  uint16_t *cost_out;
+  // last sweep outputs
+  float *disp;
+  uint8_t *reasons;         // optional: 0/2/3 where the reference assigns them
+  uint8_t *boundary;        // optional: 4|1|2 where the reference assigns them
+  uint8_t *code;            // optional: per-pixel selection code for the counters
+  int w, h, D, Dp, minD, rx, ry, reverse;
+  int R;                    // disparities per lane of this job's sweeps
+};
+
+struct PathArgs {
+  int dx, dy, cls;         // cls: 0 horizontal, 1 vertical, 2 diagonal
+  int p1, p2, edge;
+  int inv_code, val_mask;
+  int keep_sum;            // last sweep: also store the final aggregate
+  int njobs, total_lines;
+  int split;               // two by-value jobs: first global line of job 1
+  double uniq;
+};
+
+enum { PHASE_FIRST = 0, PHASE_MID = 1, PHASE_LAST = 2 };
+
+// Selection code bits (one byte per pixel, counted by the L-R / count kernels).
+enum { CODE_SELECTED = 1, CODE_BOUNDARY = 2, CODE_AMBIGUOUS = 3, CODE_FULL = 4, CODE_LOWER = 8,
+       CODE_UPPER = 16 };
+
+// Geometry of line `line` of direction (dx, dy) in a w x h image
+// (line enumeration of GpuStereo.java:734-738, image-relative).
+__device__ __forceinline__ void line_start(int line, int dx, int dy, int w, int h, int &x, int &y,
+                                           int &len) {
+  if (dy == 0) {
+    y = line;
+    x = dx > 0 ? 0 : w - 1;
+    len = w;
+  } else if (dx == 0) {
+    x = line;
+    y = dy > 0 ? 0 : h - 1;
+    len = h;
+  } else {
+    if (line < w) {
+      x = line;
+      y = dy > 0 ? 0 : h - 1;
+    } else {
+      x = dx > 0 ? 0 : w - 1;
+      y = dy > 0 ? line - w + 1 : h - 2 - (line - w);
+    }
+    const int sx = dx > 0 ? w - x : x + 1;
+    const int sy = dy > 0 ? h - y : y + 1;
+    len = min(sx, sy);
+  }
+}
+
+// Element k (uniform) of a lane-distributed vector, or -1 where invalid.
+template <int R>
+__device__ __forceinline__ int lane_pick_valid(const int *S, const int *valid, int kbase, int k) {
+  int v = -1;
+#pragma unroll
+  for (int j = 0; j < R; j++) v = (kbase + j == k) ? (valid[j] ? S[j] : -1) : v;
+  return __builtin_amdgcn_readlane(v, k / R);
+}
+
+// Winner summary of one pixel (best, value, second, before, after, flags):
+// the reference GPU `winners` layout (GpuStereo.java:758-769).
+struct Winner {
+  int best, value, second, before, after, flags;
+};
+
+template <int R, bool WIDE>
+__device__ __forceinline__ Winner wta_reduce(const int *S, const int *valid, int kbase, int D) {
+  Winner r;
+  r.best = -1;
+  r.value = 0x7FFFFFFF;
+  if (!WIDE) {
+    unsigned key = 0xFFFFFFFFu;
+#pragma unroll
+    for (int j = 0; j < R; j++)
+      if (valid[j]) key = min(key, ((unsigned)S[j] << 12) | (unsigned)(kbase + j));
+    key = wave_min_u(key);
+    if (key != 0xFFFFFFFFu) {
+      r.best = (int)(key & 0xFFFu);
+      r.value = (int)(key >> 12);
+    }
+  } else {
+    int vm = 0x7FFFFFFF;
+#pragma unroll
+    for (int j = 0; j < R; j++)
+      if (valid[j]) vm = min(vm, S[j]);
+    vm = wave_min(vm);
+    int km = 0x7FFFFFFF;
+#pragma unroll
+    for (int j = 0; j < R; j++)
+      if (valid[j] && S[j] == vm) km = min(km, kbase + j);
+    km = wave_min(km);
+    if (km != 0x7FFFFFFF) {
+      r.best = km;
+      r.value = vm;
+    }
+  }
+  r.second = 0x7FFFFFFF;
+  r.before = -1;
+  r.after = -1;
+  r.flags = 0;
+  if (r.best >= 0) {
+    int sec = 0x7FFFFFFF;
+#pragma unroll
+    for (int j = 0; j < R; j++)
+      if (valid[j] && abs(kbase + j - r.best) > 1) sec = min(sec, S[j]);
+    r.second = wave_min(sec);
+    if (r.best > 0) r.before = lane_pick_valid<R>(S, valid, kbase, r.best - 1);
+    if (r.best + 1 < D) r.after = lane_pick_valid<R>(S, valid, kbase, r.best + 1);
+    const int v0 = __builtin_amdgcn_readlane(valid[0], 0);
+    const int vl = lane_pick_valid<R>(valid, valid, kbase, D - 1);  // valid[k] where valid
+    r.flags = (v0 ? 1 : 0) | (vl > 0 ? 2 : 0);
+  }
+  return r;
+}
+
+// Winners of up to 64 consecutive pixels of a line, one per lane (lane i holds
+// step base+i), are selected together: boundary rejection, uniqueness and the
+// double-precision parabolic fit of GpuStereo.select (GpuStereo.java:644-686).
+struct Deferred {
+  int best, value, second, before, after, flags;
+  __device__ __forceinline__ void put(const Winner &w, int slot) {
+    const bool mine = (int)(threadIdx.x & 63) == slot;
+    best = mine ? w.best : best;
+    value = mine ? w.value : value;
+    second = mine ? w.second : second;
+    before = mine ? w.before : before;
+    after = mine ? w.after : after;
+    flags = mine ? w.flags : flags;
+  }
+};
+
+__device__ __forceinline__ void select_flush(const PathJob &J, const PathArgs &A,
+                                             const Deferred &d, int count, int base, int x0,
+                                             int y0) {
+  const int lane = threadIdx.x & 63;
+  if (lane >= count) return;
+  const int s = base + lane;
+  const int px = x0 + s * A.dx, py = y0 + s * A.dy;
+  const size_t pp = (size_t)py * J.w + px;
+  const bool interior = px >= J.rx && px < J.w - J.rx && py >= J.ry && py < J.h - J.ry;
+  float out = __builtin_nanf("");
+  int reason = -1, bflag = -1, code = 0;
+  const int best = d.best, value = d.value;
+  if (interior && best >= 0) {
+    const bool full = d.flags == 3;
+    if (full) {
+      bflag = 4;
+      code |= CODE_FULL;
+    }
+    if (best == 0 || best == J.D - 1) {
+      reason = 2;
+      code |= CODE_BOUNDARY;
+      if (full && best == 0) {
+        bflag |= 1;
+        code |= CODE_LOWER;
+      }
+      if (full && best == J.D - 1) {
+        bflag |= 2;
+        code |= CODE_UPPER;
+      }
+    } else {
+      double allowed = (double)value * A.uniq;
+      if (!(allowed >= 1.0)) allowed = 1.0;
+      if (d.second != 0x7FFFFFFF && (double)(d.second - value) <= allowed) {
+        reason = 3;
+        code |= CODE_AMBIGUOUS;
+      } else {
+        double offset = 0.0;
+        if (d.before >= 0 && d.after >= 0) {
+          const double curvature = (double)d.before + (double)d.after - 2.0 * (double)value;
+          if (curvature > 0) {
+            double t = .5 * ((double)d.before - (double)d.after) / curvature;
+            t = t < .5 ? t : .5;
+            offset = t > -.5 ? t : -.5;
+          }
+        }
+        out = (float)((double)(J.minD + best) + offset);
+        reason = 0;
+        code |= CODE_SELECTED;
+      }
+    }
+  }
+  J.disp[pp] = out;
+  if (J.code) J.code[pp] = (uint8_t)code;
+  if (J.reasons && reason >= 0) J.reasons[pp] = (uint8_t)reason;
+  if (J.boundary && bflag >= 0) J.boundary[pp] = (uint8_t)bflag;
+}
+
+// Per-64-step chunk of a per-pixel value along the line, loaded one chunk ahead.
+struct LineChunk {
+  int cur = 0, next = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Generic sweep on 32-bit integers: any R, 16- or 32-bit aggregates.
+template <int R>
+struct CostVec {
+  static constexpr int kWords = (R + 1) / 2;
+  uint32_t w[kWords];
+  __device__ __forceinline__ void load(const uint16_t *ptr) {
+    if constexpr (R == 1) {
+      w[0] = *ptr;
+    } else if constexpr (R == 2) {
+      w[0] = *reinterpret_cast<const uint32_t *>(ptr);
+    } else if constexpr (R == 4) {
+      uint2 t = *reinterpret_cast<const uint2 *>(ptr);
+      w[0] = t.x;
+      w[1] = t.y;
+    } else {
+#pragma unroll
+      for (int i = 0; i < R / 8; i++) {
+        uint4 t = reinterpret_cast<const uint4 *>(ptr)[i];
+        w[4 * i] = t.x;
+        w[4 * i + 1] = t.y;
+        w[4 * i + 2] = t.z;
+        w[4 * i + 3] = t.w;
+      }
+    }
+  }
+  __device__ __forceinline__ int get(int j) const {
+    return (int)((w[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu);
+  }
+};
+
+template <int R, typename SumT>
+struct SumVec;
+
+template <int R>
+struct SumVec<R, uint16_t> {
+  CostVec<R> v;
+  __device__ __forceinline__ void load(const void *ptr) { v.load((const uint16_t *)ptr); }
+  __device__ __forceinline__ int get(int j) const { return v.get(j); }
+  __device__ __forceinline__ static void store(void *ptr, const int *vals) {
+    uint16_t *p = (uint16_t *)ptr;
+    if constexpr (R == 1) {
+      *p = (uint16_t)vals[0];
+    } else if constexpr (R == 2) {
+      *reinterpret_cast<uint32_t *>(p) = ((uint32_t)vals[0] & 0xFFFFu) | ((uint32_t)vals[1] << 16);
+    } else if constexpr (R == 4) {
+      uint2 t;
+      t.x = ((uint32_t)vals[0] & 0xFFFFu) | ((uint32_t)vals[1] << 16);
+      t.y = ((uint32_t)vals[2] & 0xFFFFu) | ((uint32_t)vals[3] << 16);
+      *reinterpret_cast<uint2 *>(p) = t;
+    } else {
+#pragma unroll
+      for (int i = 0; i < R / 8; i++) {
+        uint4 t;
+        t.x = ((uint32_t)vals[8 * i] & 0xFFFFu) | ((uint32_t)vals[8 * i + 1] << 16);
+        t.y = ((uint32_t)vals[8 * i + 2] & 0xFFFFu) | ((uint32_t)vals[8 * i + 3] << 16);
+        t.z = ((uint32_t)vals[8 * i + 4] & 0xFFFFu) | ((uint32_t)vals[8 * i + 5] << 16);
+        t.w = ((uint32_t)vals[8 * i + 6] & 0xFFFFu) | ((uint32_t)vals[8 * i + 7] << 16);
+        reinterpret_cast<uint4 *>(p)[i] = t;
+      }
+    }
+  }
+  __device__ __forceinline__ static int wrap(int v) { return v & 0xFFFF; }
+};
+
+template <int R>
+struct SumVec<R, int32_t> {
+  int w[R];
+  __device__ __forceinline__ void load(const void *ptr) {
+    const int32_t *p = (const int32_t *)ptr;
+    if constexpr (R == 1) {
+      w[0] = p[0];
+    } else if constexpr (R == 2) {
+      int2 t = *reinterpret_cast<const int2 *>(p);
+      w[0] = t.x;
+      w[1] = t.y;
+    } else {
+#pragma unroll
+      for (int i = 0; i < R / 4; i++) {
+        int4 t = reinterpret_cast<const int4 *>(p)[i];
+        w[4 * i] = t.x;
+        w[4 * i + 1] = t.y;
+        w[4 * i + 2] = t.z;
+        w[4 * i + 3] = t.w;
+      }
+    }
+  }
+  __device__ __forceinline__ int get(int j) const { return w[j]; }
+  __device__ __forceinline__ static void store(void *ptr, const int *vals) {
+    int32_t *p = (int32_t *)ptr;
+    if constexpr (R == 1) {
+      p[0] = vals[0];
+    } else if constexpr (R == 2) {
+      *reinterpret_cast<int2 *>(p) = make_int2(vals[0], vals[1]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < R / 4; i++)
+        reinterpret_cast<int4 *>(p)[i] =
+            make_int4(vals[4 * i], vals[4 * i + 1], vals[4 * i + 2], vals[4 * i + 3]);
+    }
+  }
+  __device__ __forceinline__ static int wrap(int v) { return v; }
+};
+
+template <int R>
+constexpr int prefetch_depth() {
+  return R <= 1 ? 8 : R <= 4 ? 4 : R <= 16 ? 2 : 1;
+}
+
+template <int R, typename SumT, int PHASE>
+__device__ __forceinline__ void sweep_line(const PathJob &J, const PathArgs &A, int line,
+                                           const int *p2_table) {
+  constexpr int PF = prefetch_depth<R>();
+  const int lane = threadIdx.x & 63;
+  const int kbase = lane * R;
+  const bool lane_mem = kbase < J.D;
+  int x, y, len;
+  line_start(line, A.dx, A.dy, J.w, J.h, x, y, len);
+  const ptrdiff_t step_off = (ptrdiff_t)A.dy * J.w + A.dx;
+  const ptrdiff_t p0 = (ptrdiff_t)y * J.w + x;
+  const uint16_t *cost = J.cost;
+  SumT *sum = (SumT *)J.sum;
+  const int Dp = J.Dp;
+
+  CostVec<R> cb[PF];
+  SumVec<R, SumT> sb[PF];
+#pragma unroll
+  for (int j = 0; j < PF; j++) {
+    cb[j] = CostVec<R>{};
+    sb[j] = SumVec<R, SumT>{};
+  }
+#pragma unroll
+  for (int j = 0; j < PF; j++) {
+    if (j < len && lane_mem) {
+      const ptrdiff_t pp = p0 + (ptrdiff_t)j * step_off;
+      cb[j].load(cost + pp * Dp + kbase);
+      if (PHASE != PHASE_FIRST) sb[j].load(sum + pp * Dp + kbase);
+    }
+  }
+
+  int prev[R];
+#pragma unroll
+  for (int j = 0; j < R; j++) prev[j] = kInf;
+  int minp = kInf;
+  int g_prev = 0, m_prev = 0;
+  LineChunk gch, mch;
+  if (lane < len) {
+    const ptrdiff_t pl = p0 + (ptrdiff_t)lane * step_off;
+    gch.next = J.gray[pl];
+    mch.next = J.mask[pl];
+  }
+  Deferred dfr{};
+
+  for (int s0 = 0; s0 < len; s0 += PF) {
+#pragma unroll
+    for (int jj = 0; jj < PF; jj++) {
+      const int s = s0 + jj;
+      if (s >= len) break;
+      const ptrdiff_t pp = p0 + (ptrdiff_t)s * step_off;
+      if ((s & 63) == 0) {
+        gch.cur = gch.next;
+        mch.cur = mch.next;
+        const int si = s + 64 + lane;
+        if (si < len) {
+          const ptrdiff_t pl = p0 + (ptrdiff_t)si * step_off;
+          gch.next = J.gray[pl];
+          mch.next = J.mask[pl];
+        }
+      }
+      const int g_cur = __builtin_amdgcn_readlane(gch.cur, s & 63);
+      const int m_cur = __builtin_amdgcn_readlane(mch.cur, s & 63);
+      const bool has = s > 0 && m_prev != 0;
+      const int P2l = has ? p2_table[abs(g_cur - g_prev)] : A.p2;
+      g_prev = g_cur;
+      m_prev = m_cur;
+
+      int c[R], L[R];
+#pragma unroll
+      for (int j = 0; j < R; j++) c[j] = cb[jj].get(j);
+      const int left_nb = dpp_shr1(prev[R - 1], kInf);
+      const int right_nb = dpp_shl1(prev[0], kInf);
+      int lmin = kInf;
+#pragma unroll
+      for (int j = 0; j < R; j++) {
+        const int cv = c[j] & A.val_mask;
+        int v = cv;
+        if (has) {
+          const int lft = j == 0 ? left_nb : prev[j - 1];
+          const int rgt = j == R - 1 ? right_nb : prev[j + 1];
+          int t = min(prev[j], minp + P2l);
+          t = min(t, lft + A.p1);
+          t = min(t, rgt + A.p1);
+          v = cv + t - minp;
+        }
+        if (kbase + j >= J.D) v = kInf;
+        L[j] = v;
+        lmin = min(lmin, v);
+      }
+      int S[R];
+      if (PHASE == PHASE_FIRST) {
+#pragma unroll
+        for (int j = 0; j < R; j++) S[j] = L[j];
+      } else {
+#pragma unroll
+        for (int j = 0; j < R; j++) S[j] = SumVec<R, SumT>::wrap(sb[jj].get(j) + L[j]);
+      }
+      if (lane_mem && (PHASE != PHASE_LAST || A.keep_sum))
+        SumVec<R, SumT>::store(sum + pp * Dp + kbase, S);
+      if (s + PF < len && lane_mem) {
+        const ptrdiff_t pn = pp + (ptrdiff_t)PF * step_off;
+        cb[jj].load(cost + pn * Dp + kbase);
+        if (PHASE != PHASE_FIRST) sb[jj].load(sum + pn * Dp + kbase);
+      }
+      minp = wave_min(lmin);
+#pragma unroll
+      for (int j = 0; j < R; j++) prev[j] = L[j];
+
+      if constexpr (PHASE == PHASE_LAST) {
+        int valid[R];
+#pragma unroll
+        for (int j = 0; j < R; j++) valid[j] = (kbase + j < J.D) && (c[j] != A.inv_code);
+        const Winner win = wta_reduce<R, sizeof(SumT) == 4>(S, valid, kbase, J.D);
+        dfr.put(win, s & 63);
+        if ((s & 63) == 63 || s == len - 1) select_flush(J, A, dfr, (s & 63) + 1, s & ~63, x, y);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Packed 16-bit sweep (R even), straight-line in the steady state: every load
+// and store goes through a buffer resource whose range drops the accesses of
+// lanes beyond D, so no branch splits the prefetch pipeline and the compiler
+// can keep PF pixel steps of cost/aggregate loads in flight.
+// FUSED: the census Hamming cost is computed inside the first (left-to-right)
+// sweep from a sliding window of target descriptors and written to the cost
+// volume for the remaining sweeps.
+__device__ __forceinline__ uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ u16x2 as_pk(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ uint32_t splat16(uint32_t s) { return (s & 0xFFFFu) | (s << 16); }
+__device__ __forceinline__ uint32_t pk_min(uint32_t a, uint32_t b) {
+  return as_u32(__builtin_elementwise_min(as_pk(a), as_pk(b)));
+}
+__device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) {
+  return as_u32(as_pk(a) + as_pk(b));
+}
+__device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) {
+  return as_u32(as_pk(a) - as_pk(b));
+}
+
+typedef unsigned int v2u32 __attribute__((ext_vector_type(2)));
+typedef unsigned int v4u32 __attribute__((ext_vector_type(4)));
+
+// Raw buffer resource over `bytes` bytes at `base` (gfx9 dword3).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
+}
+
+template <int NP>
+__device__ __forceinline__ void bload_pairs(uint32_t *dst, __amdgpu_buffer_rsrc_t r, int off) {
+  if constexpr (NP == 1) {
+    dst[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+  } else if constexpr (NP == 2) {
+    const v2u32 t = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    dst[0] = t.x;
+    dst[1] = t.y;
+  } else {
+#pragma unroll
+    for (int i = 0; i < NP / 4; i++) {
+      const v4u32 t = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16 * i, 0, 0);
+      dst[4 * i] = t.x;
+      dst[4 * i + 1] = t.y;
+      dst[4 * i + 2] = t.z;
+      dst[4 * i + 3] = t.w;
+    }
+  }
+}
+
+template <int NP>
+__device__ __forceinline__ void bstore_pairs(const uint32_t *src, __amdgpu_buffer_rsrc_t r, int off) {
+  if constexpr (NP == 1) {
+    __builtin_amdgcn_raw_buffer_store_b32(src[0], r, off, 0, 0);
+  } else if constexpr (NP == 2) {
+    v2u32 t;
+    t.x = src[0];
+    t.y = src[1];
+    __builtin_amdgcn_raw_buffer_store_b64(t, r, off, 0, 0);
+  } else {
+#pragma unroll
+    for (int i = 0; i < NP / 4; i++) {
+      v4u32 t;
+      t.x = src[4 * i];
+      t.y = src[4 * i + 1];
+      t.z = src[4 * i + 2];
+      t.w = src[4 * i + 3];
+      __builtin_amdgcn_raw_buffer_store_b128(t, r, off + 16 * i, 0, 0);
+    }
+  }
+}
+
+template <int R>
+constexpr int pk_prefetch_depth() {
+  return R <= 4 ? 8 : R <= 8 ? 4 : R <= 16 ? 2 : 1;
+}
+
+constexpr uint32_t kInfPk = 0x7FFF7FFFu;
+constexpr int kOutOfRange = 0x7FFFFF00;  // buffer offset that is always dropped
+
+// Target descriptor with bit 63 set when the target is usable (inside
+// [rx, w - rx) and textured): the validity test of the reference's `costs`
+// kernel (GpuStereo.java:711-713).  Branch-free: the load index is clamped.
+// rd / rm: buffer resources over the target row's descriptors / mask.
+__device__ __forceinline__ uint64_t fetch_target(const PathJob &J, __amdgpu_buffer_rsrc_t rd,
+                                                 __amdgpu_buffer_rsrc_t rm, int t) {
+  const bool ok = t >= J.rx && t < J.w - J.rx;
+  const int tc = min(max(t, 0), J.w - 1);
+  const v2u32 d = __builtin_amdgcn_raw_buffer_load_b64(rd, tc * 8, 0, 0);
+  const uint32_t m = __builtin_amdgcn_raw_buffer_load_b8(rm, tc, 0, 0);
+  const uint64_t dd = ((uint64_t)(d.y & 0x7FFFFFFFu) << 32) | d.x;
+  return (ok && m) ? (dd | 0x8000000000000000ull) : 0ull;
+}
+
+// Per-step metadata of a window of PF steps, one step per lane (lane i holds
+// step base + (i % PF)): the edge-adaptive P2 of the step, or -1 when the
+// predecessor is outside the image or untextured (no transition term).
+struct Meta {
+  int p2;       // >= 0: P2 to use; -1: L = C
+  int ok;       // fused cost: reference pixel interior and textured
+  uint32_t da_lo, da_hi;  // fused cost: reference descriptor
+  uint32_t e_lo, e_hi;    // fused cost: target descriptor entering after this step
+};
+
+// Buffer resources of one line: gray|mask of the reference, and for the
+// fused sweep the reference descriptors and the target row.
+struct LineRsrc {
+  __amdgpu_buffer_rsrc_t gm, da, tdesc, tmask;
+};
+
+template <int PF, bool FUSED>
+__device__ __forceinline__ void meta_load(const PathJob &J, const LineRsrc &rs, int base, int len,
+                                          ptrdiff_t p0, ptrdiff_t step_off, int x0, int &g_raw,
+                                          int &gp_raw, uint64_t &da, uint64_t &e) {
+  const int lane = threadIdx.x & 63;
+  const int s = min(base + (lane % PF), len - 1);
+  const int sp = max(s - 1, 0);
+  g_raw = __builtin_amdgcn_raw_buffer_load_b16(rs.gm, (int)((p0 + (ptrdiff_t)s * step_off) * 2), 0, 0);
+  gp_raw = __builtin_amdgcn_raw_buffer_load_b16(rs.gm, (int)((p0 + (ptrdiff_t)sp * step_off) * 2), 0, 0);
+  if constexpr (FUSED) {
+    const v2u32 d = __builtin_amdgcn_raw_buffer_load_b64(rs.da, (int)((p0 + s) * 8), 0, 0);
+    da = ((uint64_t)d.y << 32) | d.x;
+    const int t = J.reverse ? x0 + s + 1 + J.minD + 64 * J.R - 1 : x0 + s + 1 - J.minD;
+    e = fetch_target(J, rs.tdesc, rs.tmask, t);
+  }
+}
+
+template <int PF, bool FUSED>
+__device__ __forceinline__ Meta meta_finish(const PathJob &J, const PathArgs &A, int base,
+                                            int len, int x0, int y0, int g_raw, int gp_raw,
+                                            uint64_t da, uint64_t e, const int *p2_table) {
+  const int lane = threadIdx.x & 63;
+  const int s = min(base + (lane % PF), len - 1);
+  Meta m;
+  const int g = g_raw & 0xFF, gp = gp_raw & 0xFF;
+  const bool has = s > 0 && (gp_raw >> 8) != 0;
+  m.p2 = has ? p2_table[abs(g - gp)] : -1;
+  if constexpr (FUSED) {
+    const int x = x0 + s;  // the fused sweep runs along +x
+    m.ok = (y0 >= J.ry && y0 < J.h - J.ry && x >= J.rx && x < J.w - J.rx && (g_raw >> 8) != 0);
+    m.da_lo = (uint32_t)da;
+    m.da_hi = (uint32_t)(da >> 32);
+    m.e_lo = (uint32_t)e;
+    m.e_hi = (uint32_t)(e >> 32);
+  } else {
+    m.ok = 0;
+    m.da_lo = m.da_hi = m.e_lo = m.e_hi = 0;
+  }
+  return m;
+}
+
+// Branch-free winner summary (the branchy wta_reduce, with the dependent
+// reductions evaluated unconditionally; their results are ignored when no
+// disparity is valid).
+template <int R>
+__device__ __forceinline__ Winner wta_reduce_pk(const int *S, const int *valid, int kbase, int D) {
+  Winner r;
+  unsigned key = 0xFFFFFFFFu;
+#pragma unroll
+  for (int j = 0; j < R; j++)
+    key = valid[j] ? min(key, ((unsigned)S[j] << 12) | (unsigned)(kbase + j)) : key;
+  key = wave_min_u(key);
+  const bool any = key != 0xFFFFFFFFu;
+  const int best = any ? (int)(key & 0xFFFu) : -1;
+  r.best = best;
+  r.value = (int)(key >> 12);
+  int sec = 0x7FFFFFFF;
+#pragma unroll
+  for (int j = 0; j < R; j++)
+    sec = (valid[j] && abs(kbase + j - best) > 1) ? min(sec, S[j]) : sec;
+  r.second = wave_min(sec);
+  const int kb = max(best - 1, 0), ka = min(best + 1, D - 1);
+  int vb = -1, va = -1, vlast = -1;
+#pragma unroll
+  for (int j = 0; j < R; j++) {
+    const int k = kbase + j;
+    vb = (k == kb) ? (valid[j] ? S[j] : -1) : vb;
+    va = (k == ka) ? (valid[j] ? S[j] : -1) : va;
+    vlast = (k == D - 1) ? valid[j] : vlast;
+  }
+  r.before = best > 0 ? __builtin_amdgcn_readlane(vb, kb / R) : -1;
+  r.after = best + 1 < D ? __builtin_amdgcn_readlane(va, ka / R) : -1;
+  const int v0 = __builtin_amdgcn_readlane(valid[0], 0);
+  const int vl = __builtin_amdgcn_readlane(vlast, (D - 1) / R);
+  r.flags = (v0 ? 1 : 0) | (vl > 0 ? 2 : 0);
+  return r;
+}
+
+template <int R, int PHASE, bool FUSED>
+struct PkSweep {
+  static constexpr int NP = R / 2;
+  static constexpr int PF = pk_prefetch_depth<R>();
+
+  const PathJob &J;
+  const PathArgs &A;
+  const int *p2_table;
+  int lane, kbase, loff, x0, y0, len;
+  ptrdiff_t p0, step_off;
+  uint32_t row_bytes;
+  uint32_t keep[NP], infm[NP];
+  uint32_t vmask, p1pk;
+  uint32_t cb[PF][NP], sb[PF][NP];
+  uint64_t win[FUSED ? R : 1];
+  uint32_t prev[NP];
+  int minp;
+  LineRsrc rs;
+  Meta meta;
+  Deferred dfr;
+  int sum_store_bytes;
+
+  __device__ __forceinline__ PkSweep(const PathJob &J_, const PathArgs &A_, int line,
+                                     const int *tab)
+      : J(J_), A(A_), p2_table(tab) {
+    lane = threadIdx.x & 63;
+    kbase = lane * R;
+    loff = kbase < J.D ? kbase * 2 : kOutOfRange;
+    line_start(line, A.dx, A.dy, J.w, J.h, x0, y0, len);
+    step_off = (ptrdiff_t)A.dy * J.w + A.dx;
+    p0 = (ptrdiff_t)y0 * J.w + x0;
+    row_bytes = (uint32_t)J.Dp * 2u;
+    sum_store_bytes = (PHASE != PHASE_LAST || A.keep_sum) ? (int)row_bytes : 0;
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+      const bool lo = kbase + 2 * i < J.D, hi = kbase + 2 * i + 1 < J.D;
+      keep[i] = (lo ? 0xFFFFu : 0u) | (hi ? 0xFFFF0000u : 0u);
+      infm[i] = (lo ? 0u : 0x7FFFu) | (hi ? 0u : 0x7FFF0000u);
+      prev[i] = kInfPk;
+    }
+    vmask = A.val_mask == 0xFFFF ? 0xFFFFFFFFu : splat16((uint32_t)A.val_mask);
+    p1pk = splat16((uint32_t)A.p1);
+    minp = 0;
+    dfr = Deferred{};
+    const uint32_t n = (uint32_t)J.w * (uint32_t)J.h;
+    rs.gm = make_rsrc(J.gm, n * 2u);
+    if constexpr (FUSED) {
+      rs.da = make_rsrc(J.desc_a, n * 8u);
+      rs.tdesc = make_rsrc(J.desc_b + (size_t)y0 * J.w, (uint32_t)J.w * 8u);
+      rs.tmask = make_rsrc(J.mask_b + (size_t)y0 * J.w, (uint32_t)J.w);
+    } else {
+      rs.da = rs.tdesc = rs.tmask = rs.gm;
+    }
+  }
+
+  __device__ __forceinline__ ptrdiff_t pix(int s) const { return p0 + (ptrdiff_t)s * step_off; }
+
+  __device__ __forceinline__ void load_step(int slot, int s) {
+    const ptrdiff_t pp = pix(min(s, len - 1));
+    if constexpr (!FUSED) {
+      bload_pairs<NP>(cb[slot], make_rsrc(J.cost + pp * J.Dp, row_bytes), loff);
+      if constexpr (PHASE != PHASE_FIRST)
+        bload_pairs<NP>(sb[slot], make_rsrc((const uint16_t *)J.sum + pp * J.Dp, row_bytes), loff);
+    }
+  }
+
+  __device__ __forceinline__ void init_window() {
+    if constexpr (FUSED) {
+#pragma unroll
+      for (int j = 0; j < R; j++) {
+        const int k = kbase + j;
+        const int t = J.reverse ? x0 + J.minD + k : x0 - J.minD - k;
+        win[j] = fetch_target(J, rs.tdesc, rs.tmask, t);
+      }
+    }
+  }
+
+  // One pixel step with metadata lane `ml` (uniform, compile-time in the
+  // unrolled loop) and prefetch slot `slot`.
+  __device__ __forceinline__ void step(int s, int slot, int ml) {
+    const ptrdiff_t pp = pix(s);
+    const int p2h = __builtin_amdgcn_readlane(meta.p2, ml);
+    uint32_t c[NP];
+    if constexpr (FUSED) {
+      const uint32_t alo = (uint32_t)__builtin_amdgcn_readlane((int)meta.da_lo, ml);
+      const uint32_t ahi = (uint32_t)__builtin_amdgcn_readlane((int)meta.da_hi, ml);
+      const bool ok = __builtin_amdgcn_readlane(meta.ok, ml) != 0;
+      int cv[R];
+#pragma unroll
+      for (int j = 0; j < R; j++) {
+        const uint32_t wl = (uint32_t)win[j], wh = (uint32_t)(win[j] >> 32);
+        const int ham = __builtin_popcount(wl ^ alo) + __builtin_popcount((wh ^ ahi) & 0x7FFFFFFFu);
+        cv[j] = (ok && (wh >> 31) && kbase + j < J.D) ? ham : A.inv_code;
+      }
+#pragma unroll
+      for (int i = 0; i < NP; i++) c[i] = (uint32_t)cv[2 * i] | ((uint32_t)cv[2 * i + 1] << 16);
+      bstore_pairs<NP>(c, make_rsrc(J.cost_out + pp * J.Dp, row_bytes), loff);
+      const uint32_t elo = (uint32_t)__builtin_amdgcn_readlane((int)meta.e_lo, ml);
+      const uint32_t ehi = (uint32_t)__builtin_amdgcn_readlane((int)meta.e_hi, ml);
+      if (J.reverse) {
+        const uint32_t ilo = (uint32_t)dpp_shl1((int)(uint32_t)win[0], (int)elo);
+        const uint32_t ihi = (uint32_t)dpp_shl1((int)(uint32_t)(win[0] >> 32), (int)ehi);
+#pragma unroll
+        for (int j = 0; j < R - 1; j++) win[j] = win[j + 1];
+        win[R - 1] = ((uint64_t)ihi << 32) | ilo;
+      } else {
+        const uint32_t ilo = (uint32_t)dpp_shr1((int)(uint32_t)win[R - 1], (int)elo);
+        const uint32_t ihi = (uint32_t)dpp_shr1((int)(uint32_t)(win[R - 1] >> 32), (int)ehi);
+#pragma unroll
+        for (int j = R - 1; j > 0; j--) win[j] = win[j - 1];
+        win[0] = ((uint64_t)ihi << 32) | ilo;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NP; i++) c[i] = cb[slot][i];
+    }
+    // Recurrence: with no usable predecessor (p2h < 0) the offsets below are
+    // zero and t = min(prev, 0, ...) = 0, so L = C.
+    const bool has = p2h >= 0;
+    const uint32_t mp2 = splat16(has ? (uint32_t)(minp + p2h) : 0u);
+    const uint32_t mpk = splat16(has ? (uint32_t)minp : 0u);
+    const uint32_t lft = (uint32_t)dpp_shr1((int)prev[NP - 1], (int)kInfPk);
+    const uint32_t rgt = (uint32_t)dpp_shl1((int)prev[0], (int)kInfPk);
+    uint32_t nb[NP + 1];
+    nb[0] = __builtin_amdgcn_alignbit(prev[0], lft, 16);
+#pragma unroll
+    for (int i = 1; i < NP; i++) nb[i] = __builtin_amdgcn_alignbit(prev[i], prev[i - 1], 16);
+    nb[NP] = __builtin_amdgcn_alignbit(rgt, prev[NP - 1], 16);
+    uint32_t L[NP], lm = 0xFFFFFFFFu;
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+      uint32_t t = pk_min(prev[i], mp2);
+      t = pk_min(t, pk_add(nb[i], p1pk));
+      t = pk_min(t, pk_add(nb[i + 1], p1pk));
+      const uint32_t v = pk_sub(pk_add(c[i] & vmask, t), mpk);
+      L[i] = (v & keep[i]) | infm[i];
+      lm = pk_min(lm, L[i]);
+    }
+    const int lmin = (int)min(lm & 0xFFFFu, lm >> 16);
+    uint32_t S[NP];
+#pragma unroll
+    for (int i = 0; i < NP; i++) S[i] = PHASE == PHASE_FIRST ? L[i] : pk_add(sb[slot][i], L[i]);
+    bstore_pairs<NP>(S, make_rsrc((uint16_t *)J.sum + pp * J.Dp, (uint32_t)sum_store_bytes), loff);
+    minp = wave_min(lmin);
+#pragma unroll
+    for (int i = 0; i < NP; i++) prev[i] = L[i];
+    if constexpr (PHASE == PHASE_LAST) {
+      int Si[R], valid[R];
+#pragma unroll
+      for (int i = 0; i < NP; i++) {
+        Si[2 * i] = (int)(S[i] & 0xFFFFu);
+        Si[2 * i + 1] = (int)(S[i] >> 16);
+        valid[2 * i] = (kbase + 2 * i < J.D) && ((c[i] & 0xFFFFu) != (uint32_t)A.inv_code);
+        valid[2 * i + 1] = (kbase + 2 * i + 1 < J.D) && ((c[i] >> 16) != (uint32_t)A.inv_code);
+      }
+      dfr.put(wta_reduce_pk<R>(Si, valid, kbase, J.D), s & 63);
+    }
+  }
+
+  __device__ __forceinline__ void maybe_flush(int s_done) {
+    // after step s_done - 1: flush when a 64-step chunk is complete or the line ends
+    if constexpr (PHASE == PHASE_LAST) {
+      if ((s_done & 63) == 0 || s_done == len) {
+        const int base = (s_done - 1) & ~63;
+        select_flush(J, A, dfr, s_done - base, base, x0, y0);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void run() {
+    init_window();
+    // prologue: metadata of window 0, loads of steps 0 .. PF-1
+    int g_raw, gp_raw;
+    uint64_t da = 0, e = 0;
+    meta_load<PF, FUSED>(J, rs, 0, len, p0, step_off, x0, g_raw, gp_raw, da, e);
+    meta = meta_finish<PF, FUSED>(J, A, 0, len, x0, y0, g_raw, gp_raw, da, e, p2_table);
+#pragma unroll
+    for (int j = 0; j < PF; j++) load_step(j, j);
+    int s0 = 0;
+    for (; s0 + PF <= len; s0 += PF) {
+      // metadata of the next window, consumed next iteration
+      meta_load<PF, FUSED>(J, rs, s0 + PF, len, p0, step_off, x0, g_raw, gp_raw, da, e);
+#pragma unroll
+      for (int jj = 0; jj < PF; jj++) {
+        step(s0 + jj, jj, jj);
+        load_step(jj, s0 + jj + PF);
+      }
+      meta = meta_finish<PF, FUSED>(J, A, s0 + PF, len, x0, y0, g_raw, gp_raw, da, e, p2_table);
+      maybe_flush(s0 + PF);
+    }
+    // tail: fewer than PF steps; their loads are already in the slots
+#pragma unroll
+    for (int jj = 0; jj < PF - 1; jj++) {
+      if (s0 + jj < len) {
+        step(s0 + jj, jj, jj);
+        maybe_flush(s0 + jj + 1);
+      }
+    }
+  }
+};
+
+template <int R, int PHASE, bool FUSED>
+__device__ __forceinline__ void sweep_line_pk(const PathJob &J, const PathArgs &A, int line,
+                                              const int *p2_table) {
+  static_assert(R % 2 == 0, "packed sweep needs an even lane group");
+  PkSweep<R, PHASE, FUSED> sw(J, A, line, p2_table);
+  sw.run();
+}
+
+__device__ __forceinline__ void build_p2_table(int *tab, const PathArgs &A) {
+  // StereoCost.pathP2 (StereoCost.java:44-48) for every |dI| in 0..255.
+  for (int d = threadIdx.x; d < 256; d += blockDim.x) {
+    int v = A.p2;
+    if (A.edge != 0) {
+      v = (int)((long long)A.p2 * A.edge / (A.edge + d));
+      v = max(A.p1 + 1, v);
+    }
+    tab[d] = v;
+  }
+}
+
+__device__ __forceinline__ int find_job(const int *line_begin, int njobs, int gline) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (line_begin[mid] <= gline) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Job list of one launch: a device table (batches) or up to two jobs passed
+// by value in the kernel arguments (the plain left+right pair).
+struct PathJobs {
+  const PathJob *table;     // device table (batches), or nullptr
+  const int *line_begin;    // device: first global line of each table job for this direction
+  PathJob local[2];
+};
+
+// MODE: 0 generic int sweep, 1 packed u16 sweep, 2 packed u16 sweep with fused census cost
+template <int R, typename SumT, int PHASE, int MODE>
+__device__ __forceinline__ void sweep_dispatch(const PathJob &J, const PathArgs &A, int line,
+                                               const int *p2_table) {
+  if constexpr (MODE == 0) sweep_line<R, SumT, PHASE>(J, A, line, p2_table);
+  else sweep_line_pk<R, PHASE, MODE == 2>(J, A, line, p2_table);
+}
+
+template <int R, typename SumT, int PHASE, int MODE>
+__global__ __launch_bounds__(256) void path_kernel(PathJobs jobs, PathArgs A) {
+  __shared__ int p2_table[256];
+  build_p2_table(p2_table, A);
+  __syncthreads();
+  const int waves = blockDim.x >> 6;
+  const int gline =
+      __builtin_amdgcn_readfirstlane(blockIdx.x * waves + (int)(threadIdx.x >> 6));
+  if (gline >= A.total_lines) return;
+  if (jobs.table) {
+    const int ji = __builtin_amdgcn_readfirstlane(find_job(jobs.line_begin, A.njobs, gline));
+    const PathJob J = jobs.table[ji];
+    sweep_dispatch<R, SumT, PHASE, MODE>(J, A, gline - jobs.line_begin[ji], p2_table);
+  } else if (A.njobs > 1 && gline >= A.split) {
+    const PathJob J = jobs.local[1];
+    sweep_dispatch<R, SumT, PHASE, MODE>(J, A, gline - A.split, p2_table);
+  } else {
+    const PathJob J = jobs.local[0];
+    sweep_dispatch<R, SumT, PHASE, MODE>(J, A, gline, p2_table);
+  }
+}
+
+}  // namespace sgm
