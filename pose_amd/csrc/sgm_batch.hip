@@ -1,33 +1,581 @@
-// Batched and served modes (work in progress).
+// Batched mode: many uneven stereo pairs in one persistent-kernel launch,
+// optionally captured into a hipGraph, with tile shape and waves per path
+// chosen by the autotuner (sgm_tune.hip).  The reference has no batching:
+// it matches pairs sequentially (DenseReconstruction.java:64-72).
 #include <hip/hip_runtime.h>
 
-#include "sgm_internal.h"
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "sgm_batch_internal.h"
 
 using namespace sgm;
 
-extern "C" int sgm_batch_create(sgm_context *, const sgm_params *, const sgm_pair *, int, int,
-                                sgm_batch **out) {
-  if (out) *out = nullptr;
-  return fail(SGM_ERR_INVALID_ARGUMENT, "batched mode not built yet");
+namespace {
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Carves one device arena into 256-byte aligned sub-buffers.
+struct Carver {
+  size_t off = 0;
+  size_t take(size_t bytes) {
+    const size_t o = off;
+    off = align_up(off + std::max<size_t>(bytes, 8), 256);
+    return o;
+  }
+};
+
+}  // namespace
+
+namespace sgm {
+
+int batch_lines_per_item(sgm_context *ctx) { return ctx->batch_lines_per_item; }
+
+int64_t batch_pair_bytes(const sgm_params *p, int w, int h, int D, bool host_io) {
+  sgm_params q = *p;
+  q.disparities = D;
+  const int64_t n = (int64_t)w * h;
+  // two directions (volumes, descriptors, masks, disparities) + scratch
+  int64_t b = 2 * direction_bytes(&q, w, h) + n * 4;
+  if (host_io) b += n * (2 + 2 + 4 + 2);
+  return b + 4096;
 }
-extern "C" int sgm_batch_run(sgm_batch *) { return fail(SGM_ERR_INVALID_ARGUMENT, "batched mode not built yet"); }
-extern "C" int sgm_batch_destroy(sgm_batch *) { return SGM_OK; }
-extern "C" int64_t sgm_batch_device_bytes(sgm_batch *) { return 0; }
-extern "C" int sgm_autotune(sgm_context *, const sgm_params *, const char *) {
-  return fail(SGM_ERR_INVALID_ARGUMENT, "autotuner not built yet");
+
+}  // namespace sgm
+
+static int validate_pair(const sgm_params *p, const sgm_pair &pr, int flags) {
+  sgm_params q = *p;
+  q.disparities = pr.disparities;
+  q.min_disparity = pr.min_disparity;
+  int rc = validate_params(&q, pr.width, pr.height);
+  if (rc) return rc;
+  if (!pr.left || !pr.right || !pr.checked)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "pair needs left, right and checked buffers");
+  if ((pr.valid_left == nullptr) != (pr.valid_right == nullptr))
+    return fail(SGM_ERR_INVALID_ARGUMENT, "Invalid stereo mask dimensions");
+  if (lanes_group(pr.disparities) > 8)
+    return fail(SGM_ERR_INVALID_ARGUMENT,
+                "batched mode supports up to 512 disparities per pair (got %d)", pr.disparities);
+  if ((int64_t)pr.width * pr.height * 8 >= (int64_t)1 << 31)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "pair too large for the batched mode");
+  (void)flags;
+  return SGM_OK;
 }
-extern "C" int sgm_autotune_load(sgm_context *, const char *) {
-  return fail(SGM_ERR_INVALID_ARGUMENT, "autotuner not built yet");
+
+static int batch_enqueue(sgm_batch *b) {
+  sgm_context *ctx = b->ctx;
+  hipStream_t st = b->stream;
+  int rc;
+  if (!(b->flags & SGM_BATCH_DEVICE)) {
+    for (size_t i = 0; i < b->pairs.size(); i++) {
+      const sgm_pair &pr = b->pairs[i];
+      const size_t n = (size_t)pr.width * pr.height;
+      const PairSlots &s = b->slots[i];
+      hipError_t e = hipMemcpyAsync(s.gray[0], pr.left, n, hipMemcpyHostToDevice, st);
+      if (e == hipSuccess) e = hipMemcpyAsync(s.gray[1], pr.right, n, hipMemcpyHostToDevice, st);
+      if (e == hipSuccess && pr.valid_left) {
+        e = hipMemcpyAsync(s.valid[0], pr.valid_left, n, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(s.valid[1], pr.valid_right, n, hipMemcpyHostToDevice, st);
+      }
+      if (e != hipSuccess) return hip_fail(e, "batch upload");
+    }
+  }
+  {
+    const size_t words = b->control_words;
+    fill_u32_kernel<<<(unsigned)((words + 255) / 256), 256, 0, st>>>(b->d_control, 0u, words);
+    if (hipGetLastError() != hipSuccess) return fail(SGM_ERR_HIP, "batch reset launch failed");
+  }
+  const int threads = b->waves_per_block * 64;
+  const unsigned blocks = (unsigned)b->blocks;
+  {
+    ProfileScope ps(ctx, "batch");
+    if (b->sum_bytes == 2) {
+      if (b->words == 1) batch_kernel<uint16_t, 1><<<blocks, threads, 0, st>>>(b->args);
+      else batch_kernel<uint16_t, 2><<<blocks, threads, 0, st>>>(b->args);
+    } else {
+      if (b->words == 1) batch_kernel<int32_t, 1><<<blocks, threads, 0, st>>>(b->args);
+      else batch_kernel<int32_t, 2><<<blocks, threads, 0, st>>>(b->args);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
+  }
+  if (!(b->flags & SGM_BATCH_DEVICE)) {
+    for (size_t i = 0; i < b->pairs.size(); i++) {
+      const sgm_pair &pr = b->pairs[i];
+      const size_t n = (size_t)pr.width * pr.height;
+      const PairSlots &s = b->slots[i];
+      hipError_t e = hipMemcpyAsync(pr.checked, s.checked, n * 4, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess && pr.dl) e = hipMemcpyAsync(pr.dl, s.disp[0], n * 4, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess && pr.dr) e = hipMemcpyAsync(pr.dr, s.disp[1], n * 4, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess && pr.reasons)
+        e = hipMemcpyAsync(pr.reasons, s.reasons, n, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess && pr.boundary)
+        e = hipMemcpyAsync(pr.boundary, s.boundary, n, hipMemcpyDeviceToHost, st);
+      if (e != hipSuccess) return hip_fail(e, "batch download");
+    }
+  }
+  (void)rc;
+  return SGM_OK;
 }
-extern "C" int sgm_server_create(sgm_context *, const sgm_params *, int64_t, int, int, sgm_server **out) {
-  if (out) *out = nullptr;
-  return fail(SGM_ERR_INVALID_ARGUMENT, "serving mode not built yet");
+
+extern "C" int sgm_batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pairs,
+                                int count, int flags, sgm_batch **out) {
+  if (!ctx || !params || !pairs || count <= 0 || !out)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "bad batch arguments");
+  *out = nullptr;
+  std::lock_guard<std::mutex> g(ctx->lock);
+  (void)hipSetDevice(ctx->device);
+  return batch_create(ctx, params, pairs, count, flags, nullptr, 0, ctx->stream, out);
 }
-extern "C" int sgm_server_submit(sgm_server *, const sgm_pair *, uint64_t *) {
-  return fail(SGM_ERR_INVALID_ARGUMENT, "serving mode not built yet");
+
+namespace sgm {
+
+size_t batch_arena_bytes(const sgm_params *params, const sgm_pair *pairs, int count, int flags) {
+  // must mirror the carving in batch_create
+  const bool host_io = !(flags & SGM_BATCH_DEVICE);
+  const int words = words_of(params), sb = aggregate_bytes(params);
+  const bool zncc = params->cost == SGM_COST_ZNCC || params->cost == SGM_COST_ZNCC_FP32;
+  Carver cv;
+  const int nph = BPH_PATH0 + params->directions + 1, stride = 2 * count + 1;
+  cv.take(4);
+  cv.take(4);
+  cv.take((size_t)nph * stride * 4);
+  cv.take((size_t)count * 16 * 8);
+  for (int i = 0; i < count; i++) {
+    const sgm_pair &pr = pairs[i];
+    const size_t n = (size_t)pr.width * pr.height, Dp = padded_disparities(pr.disparities);
+    for (int d = 0; d < 2; d++) {
+      if (host_io) cv.take(n);
+      if (host_io && pr.valid_left) cv.take(n);
+      cv.take(n * 8 * words);
+      cv.take(n);
+      cv.take(n * 2);
+      if (zncc) {
+        cv.take(n * 4);
+        cv.take(n * 4);
+      }
+      cv.take(n * Dp * 2);
+      cv.take(n * Dp * sb);
+      const bool own_disp = host_io || (d == 0 ? pr.dl == nullptr : pr.dr == nullptr);
+      if (own_disp) cv.take(n * 4);
+    }
+    cv.take(n);
+    cv.take(n);
+    cv.take(n);
+    if (host_io) cv.take(n * 4);
+    if (host_io && pr.reasons) cv.take(n);
+    if (host_io && pr.boundary) cv.take(n);
+  }
+  return cv.off;
 }
-extern "C" int sgm_server_wait(sgm_server *, uint64_t, double, sgm_request_info *) {
-  return fail(SGM_ERR_INVALID_ARGUMENT, "serving mode not built yet");
+
+size_t batch_table_bytes(int count, const sgm_params *params) {
+  const int nph = BPH_PATH0 + params->directions + 1, stride = 2 * count + 1;
+  Carver tc;
+  tc.take(2 * count * sizeof(BatchImage));
+  tc.take(2 * count * sizeof(PathJob));
+  tc.take(2 * count * sizeof(CostJob));
+  tc.take(count * sizeof(BatchPairDev));
+  tc.take((nph + 1) * sizeof(int));
+  tc.take((size_t)nph * stride * sizeof(int));
+  return tc.off;
 }
-extern "C" double sgm_server_now(sgm_server *) { return 0; }
-extern "C" int sgm_server_destroy(sgm_server *) { return SGM_OK; }
+
+// Creates a batch.  With `region` non-null the arena and tables are carved from
+// that caller-owned device region (serving pool) instead of being allocated.
+int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pairs, int count,
+                 int flags, void *region, size_t region_bytes, hipStream_t stream,
+                 sgm_batch **out) {
+  for (int i = 0; i < count; i++) {
+    int rc = validate_pair(params, pairs[i], flags);
+    if (rc) return rc;
+  }
+  const bool host_io = !(flags & SGM_BATCH_DEVICE);
+  sgm_batch *b = new sgm_batch();
+  b->ctx = ctx;
+  b->stream = stream;
+  b->params = *params;
+  b->flags = flags;
+  b->pairs.assign(pairs, pairs + count);
+  b->words = words_of(params);
+  b->sum_bytes = aggregate_bytes(params);
+  const bool zncc = params->cost == SGM_COST_ZNCC || params->cost == SGM_COST_ZNCC_FP32;
+  // every pair of the batch has R <= 8, so the fused cost needs only the mode and width
+  b->fused = params->cost == SGM_COST_CENSUS && b->words == 1 && b->sum_bytes == 2;
+  b->waves_per_block = ctx->batch_waves_per_block;
+  b->lines_per_item = ctx->batch_lines_per_item;
+  b->blocks = ctx->info.compute_units * ctx->batch_blocks_per_cu;
+
+  // ---- arena layout
+  Carver cv;
+  struct Off {
+    size_t gray[2], valid[2], desc[2], mask[2], gm[2], wsum[2], wsum2[2], cost[2], sum[2], disp[2];
+    size_t code, rtmp, btmp, checked, reasons, boundary;
+  };
+  std::vector<Off> offs(count);
+  const int nph = BPH_PATH0 + params->directions + 1;
+  const int stride = 2 * count + 1;
+  const size_t done_words = (size_t)nph * stride;
+  const size_t ctrl_head = cv.take(4), ctrl_error = cv.take(4);
+  const size_t ctrl_done = cv.take(done_words * 4);
+  const size_t ctrl_stats = cv.take((size_t)count * 16 * 8);
+  const size_t ctrl_end = cv.off;
+  for (int i = 0; i < count; i++) {
+    const sgm_pair &pr = pairs[i];
+    const size_t n = (size_t)pr.width * pr.height, Dp = padded_disparities(pr.disparities);
+    Off &o = offs[i];
+    for (int d = 0; d < 2; d++) {
+      o.gray[d] = host_io ? cv.take(n) : 0;
+      o.valid[d] = (host_io && pr.valid_left) ? cv.take(n) : 0;
+      o.desc[d] = cv.take(n * 8 * b->words);
+      o.mask[d] = cv.take(n);
+      o.gm[d] = cv.take(n * 2);
+      o.wsum[d] = zncc ? cv.take(n * 4) : 0;
+      o.wsum2[d] = zncc ? cv.take(n * 4) : 0;
+      o.cost[d] = cv.take(n * Dp * 2);
+      o.sum[d] = cv.take(n * Dp * b->sum_bytes);
+      const bool own_disp = host_io || (d == 0 ? pr.dl == nullptr : pr.dr == nullptr);
+      o.disp[d] = own_disp ? cv.take(n * 4) : 0;
+    }
+    o.code = cv.take(n);
+    o.rtmp = cv.take(n);
+    o.btmp = cv.take(n);
+    o.checked = host_io ? cv.take(n * 4) : 0;
+    o.reasons = (host_io && pr.reasons) ? cv.take(n) : 0;
+    o.boundary = (host_io && pr.boundary) ? cv.take(n) : 0;
+  }
+  b->arena_bytes = cv.off;
+  int rc = SGM_OK;
+  const size_t table_bytes = batch_table_bytes(count, params);
+  if (region) {
+    if (b->arena_bytes + table_bytes > region_bytes) {
+      delete b;
+      return fail(SGM_ERR_NO_PLAN, "batch region too small");
+    }
+    b->arena = region;
+    b->tables = (uint8_t *)region + b->arena_bytes;
+    b->external = true;
+  } else {
+    if ((int64_t)(b->arena_bytes + table_bytes) > ctx->budget - ctx->allocated) {
+      const size_t need = b->arena_bytes;
+      delete b;
+      return fail(SGM_ERR_NO_PLAN, "batch needs %zu MiB of device memory; budget left %lld MiB",
+                  need >> 20, (long long)((ctx->budget - ctx->allocated) >> 20));
+    }
+    rc = context_alloc(ctx, &b->arena, b->arena_bytes, "batch arena");
+    if (rc) {
+      delete b;
+      return rc;
+    }
+  }
+  uint8_t *base = (uint8_t *)b->arena;
+  b->d_control = (unsigned *)(base + ctrl_head);
+  b->control_words = (ctrl_end - ctrl_head) / 4;
+  b->d_stats = (unsigned long long *)(base + ctrl_stats);
+
+  // ---- job tables
+  std::vector<BatchImage> images(2 * count);
+  std::vector<PathJob> jobs(2 * count);
+  std::vector<CostJob> cjobs(2 * count);
+  std::vector<BatchPairDev> pdev(count);
+  b->slots.resize(count);
+  const int census_rows = kBatchCensusRows, cost_rows = 4, lr_rows = 16;
+  std::vector<int> job_begin((size_t)nph * stride, 0), phase_begin(nph + 1, 0);
+  auto P8 = [&](size_t o) { return base + o; };
+  for (int i = 0; i < count; i++) {
+    const sgm_pair &pr = pairs[i];
+    const int w = pr.width, h = pr.height, D = pr.disparities, Dp = padded_disparities(D);
+    const Off &o = offs[i];
+    PairSlots &s = b->slots[i];
+    for (int d = 0; d < 2; d++) {
+      s.gray[d] = host_io ? P8(o.gray[d]) : (uint8_t *)(d == 0 ? pr.left : pr.right);
+      s.valid[d] = pr.valid_left ? (host_io ? P8(o.valid[d])
+                                            : (uint8_t *)(d == 0 ? pr.valid_left : pr.valid_right))
+                                 : nullptr;
+      s.disp[d] = o.disp[d] ? (float *)P8(o.disp[d]) : (d == 0 ? pr.dl : pr.dr);
+    }
+    s.checked = host_io ? (float *)P8(o.checked) : pr.checked;
+    s.reasons = host_io ? (o.reasons ? P8(o.reasons) : nullptr) : pr.reasons;
+    s.boundary = host_io ? (o.boundary ? P8(o.boundary) : nullptr) : pr.boundary;
+    for (int d = 0; d < 2; d++) {
+      BatchImage &I = images[2 * i + d];
+      I.gray = s.gray[d];
+      I.valid = s.valid[d];
+      I.desc = (uint64_t *)P8(o.desc[d]);
+      I.mask = P8(o.mask[d]);
+      I.gm = (uint16_t *)P8(o.gm[d]);
+      I.wsum = zncc ? (int32_t *)P8(o.wsum[d]) : nullptr;
+      I.wsum2 = zncc ? (int32_t *)P8(o.wsum2[d]) : nullptr;
+      I.reasons_init = d == 0 ? P8(o.rtmp) : nullptr;
+      I.boundary_init = d == 0 ? P8(o.btmp) : nullptr;
+      I.w = w;
+      I.h = h;
+    }
+    for (int d = 0; d < 2; d++) {
+      const int od = 1 - d;
+      PathJob &J = jobs[2 * i + d];
+      memset(&J, 0, sizeof J);
+      J.cost = (const uint16_t *)P8(o.cost[d]);
+      J.sum = P8(o.sum[d]);
+      J.mask = P8(o.mask[d]);
+      J.gray = s.gray[d];
+      J.gm = (const uint16_t *)P8(o.gm[d]);
+      J.desc_a = (const uint64_t *)P8(o.desc[d]);
+      J.desc_b = (const uint64_t *)P8(o.desc[od]);
+      J.mask_b = P8(o.mask[od]);
+      J.cost_out = (uint16_t *)P8(o.cost[d]);
+      J.disp = s.disp[d];
+      J.reasons = d == 0 ? P8(o.rtmp) : nullptr;
+      J.boundary = d == 0 ? P8(o.btmp) : nullptr;
+      J.code = d == 0 ? P8(o.code) : nullptr;
+      J.w = w;
+      J.h = h;
+      J.D = D;
+      J.Dp = Dp;
+      J.minD = pr.min_disparity;
+      J.rx = params->window_w / 2;
+      J.ry = params->window_h / 2;
+      J.reverse = d;
+      J.R = b->sum_bytes == 2 ? std::max(2, lanes_group(D)) : lanes_group(D);
+      CostJob &C = cjobs[2 * i + d];
+      memset(&C, 0, sizeof C);
+      C.desc_a = J.desc_a;
+      C.desc_b = J.desc_b;
+      C.mask_a = J.mask;
+      C.mask_b = J.mask_b;
+      C.gray_a = s.gray[d];
+      C.gray_b = s.gray[od];
+      C.sum_a = images[2 * i + d].wsum;
+      C.sum2_a = images[2 * i + d].wsum2;
+      C.sum_b = images[2 * i + od].wsum;
+      C.sum2_b = images[2 * i + od].wsum2;
+      C.cost = (uint16_t *)P8(o.cost[d]);
+      C.w = w;
+      C.h = h;
+      C.D = D;
+      C.Dp = Dp;
+      C.minD = pr.min_disparity;
+      C.reverse = d;
+    }
+    BatchPairDev &PD = pdev[i];
+    PD.dl = s.disp[0];
+    PD.dr = s.disp[1];
+    PD.code = P8(o.code);
+    PD.reasons_tmp = P8(o.rtmp);
+    PD.boundary_tmp = P8(o.btmp);
+    PD.checked = s.checked;
+    PD.reasons = s.reasons;
+    PD.boundary = s.boundary;
+    PD.stats = b->d_stats + (size_t)i * 16;
+    PD.w = w;
+    PD.h = h;
+  }
+  // items per (phase, job): census per image, cost / paths per direction, L-R per pair
+  int total = 0;
+  for (int ph = 0; ph < nph; ph++) {
+    phase_begin[ph] = total;
+    const int nj = ph == nph - 1 ? count : 2 * count;
+    int acc = 0;
+    for (int j = 0; j < nj; j++) {
+      job_begin[(size_t)ph * stride + j] = acc;
+      const sgm_pair &pr = pairs[ph == nph - 1 ? j : j / 2];
+      const int w = pr.width, h = pr.height;
+      int items;
+      if (ph == BPH_CENSUS) items = (h + census_rows - 1) / census_rows;
+      else if (ph == BPH_COST) items = b->fused ? 0 : (h + cost_rows - 1) / cost_rows;
+      else if (ph == nph - 1) items = (h + lr_rows - 1) / lr_rows;
+      else {
+        static const int dirs[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
+        const int dir = ph - BPH_PATH0;
+        const int cls = dirs[dir][1] == 0 ? 0 : dirs[dir][0] == 0 ? 1 : 2;
+        items = (lines_of(cls, w, h) + b->lines_per_item - 1) / b->lines_per_item;
+      }
+      acc += items;
+    }
+    job_begin[(size_t)ph * stride + nj] = acc;
+    total += acc;
+  }
+  phase_begin[nph] = total;
+  b->total_items = total;
+
+  // ---- upload tables (one small allocation)
+  Carver tc;
+  const size_t t_img = tc.take(images.size() * sizeof(BatchImage));
+  const size_t t_job = tc.take(jobs.size() * sizeof(PathJob));
+  const size_t t_cjob = tc.take(cjobs.size() * sizeof(CostJob));
+  const size_t t_pair = tc.take(pdev.size() * sizeof(BatchPairDev));
+  const size_t t_pb = tc.take(phase_begin.size() * sizeof(int));
+  const size_t t_jb = tc.take(job_begin.size() * sizeof(int));
+  std::vector<uint8_t> host(tc.off, 0);
+  memcpy(host.data() + t_img, images.data(), images.size() * sizeof(BatchImage));
+  memcpy(host.data() + t_job, jobs.data(), jobs.size() * sizeof(PathJob));
+  memcpy(host.data() + t_cjob, cjobs.data(), cjobs.size() * sizeof(CostJob));
+  memcpy(host.data() + t_pair, pdev.data(), pdev.size() * sizeof(BatchPairDev));
+  memcpy(host.data() + t_pb, phase_begin.data(), phase_begin.size() * sizeof(int));
+  memcpy(host.data() + t_jb, job_begin.data(), job_begin.size() * sizeof(int));
+  if (!b->external) rc = context_alloc(ctx, &b->tables, tc.off, "batch tables");
+  if (rc == SGM_OK) {
+    hipError_t e = hipMemcpyAsync(b->tables, host.data(), tc.off, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) rc = hip_fail(e, "batch table upload");
+  }
+  if (rc) {
+    sgm_batch_destroy(b);
+    return rc;
+  }
+  b->tables_bytes = tc.off;
+  uint8_t *tb = (uint8_t *)b->tables;
+  BatchArgs &A = b->args;
+  memset(&A, 0, sizeof A);
+  A.images = (const BatchImage *)(tb + t_img);
+  A.jobs = (const PathJob *)(tb + t_job);
+  A.cost_jobs = (const CostJob *)(tb + t_cjob);
+  A.pairs = (const BatchPairDev *)(tb + t_pair);
+  A.phase_begin = (const int *)(tb + t_pb);
+  A.job_begin = (const int *)(tb + t_jb);
+  A.head = (unsigned *)(base + ctrl_head);
+  A.error = (int *)(base + ctrl_error);
+  A.done = (unsigned *)(base + ctrl_done);
+  A.nph = nph;
+  A.stride = stride;
+  A.npairs = count;
+  A.directions = params->directions;
+  A.fused = b->fused ? 1 : 0;
+  A.lines_per_item = b->lines_per_item;
+  A.census_rows = census_rows;
+  A.cost_rows = cost_rows;
+  A.lr_rows = lr_rows;
+  A.pa = path_args(params, 0, 0, 0, ctx->keep_volumes ? 1 : 0);
+  A.ca = cost_args(params);
+  A.ww = params->window_w;
+  A.wh = params->window_h;
+  A.lr_threshold = params->lr_threshold;
+  {
+    // dependency waits give up after 20 s of wall clock (error flag, grid drains)
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess ||
+        khz <= 0)
+      khz = 100000;
+    A.spin_limit = (long long)khz * 1000LL * 20LL;
+  }
+
+  if (flags & SGM_BATCH_GRAPH) {
+    hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      sgm_batch_destroy(b);
+      return hip_fail(e, "hipStreamBeginCapture");
+    }
+    const bool prof = ctx->profiling;
+    ctx->profiling = false;  // no event records inside the graph
+    rc = batch_enqueue(b);
+    ctx->profiling = prof;
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(stream, &graph);
+    if (rc == SGM_OK && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
+    if (rc == SGM_OK) {
+      e = hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc) {
+      sgm_batch_destroy(b);
+      return rc;
+    }
+  }
+  *out = b;
+  return SGM_OK;
+}
+
+// Enqueue one run of the batch on its stream (graph replay or direct launches).
+int batch_launch(sgm_batch *b) {
+  if (b->graph_exec) {
+    ProfileScope ps(b->ctx, "batch_graph");
+    hipError_t e = hipGraphLaunch(b->graph_exec, b->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    return SGM_OK;
+  }
+  return batch_enqueue(b);
+}
+
+}  // namespace sgm
+
+extern "C" int sgm_batch_run(sgm_batch *b) {
+  if (!b) return fail(SGM_ERR_INVALID_ARGUMENT, "null batch");
+  sgm_context *ctx = b->ctx;
+  std::lock_guard<std::mutex> g(ctx->lock);
+  (void)hipSetDevice(ctx->device);
+  int rc = batch_launch(b);
+  if (rc) return rc;
+  if ((b->flags & SGM_BATCH_NO_SYNC) && (b->flags & SGM_BATCH_DEVICE)) return SGM_OK;
+  return batch_finish(b);
+}
+
+namespace sgm {
+
+// Waits for the batch, checks the queue's error flag, delivers the counters.
+int batch_finish(sgm_batch *b) {
+  sgm_context *ctx = b->ctx;
+  hipError_t e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) return hip_fail(e, "batch synchronize");
+  collect_profile(ctx);
+  int err = 0;
+  e = hipMemcpy(&err, b->args.error, sizeof err, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "batch status read");
+  if (err) return fail(SGM_ERR_TIMEOUT, "batch work queue wait expired (dependency never completed)");
+  const size_t count = b->pairs.size();
+  std::vector<unsigned long long> st(count * 16);
+  e = hipMemcpy(st.data(), b->d_stats, st.size() * 8, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "batch counters read");
+  for (size_t i = 0; i < count; i++) {
+    sgm_stats *out = b->pairs[i].stats;
+    if (!out) continue;
+    sgm_stats s;
+    memset(&s, 0, sizeof s);
+    const unsigned long long *c = &st[i * 16];
+    s.lower_boundary_rejected = (int64_t)c[0];
+    s.full_interval_pixels = (int64_t)c[1];
+    s.boundary_rejected = (int64_t)c[2];
+    s.upper_boundary_rejected = (int64_t)c[3];
+    s.ambiguity_rejected = (int64_t)c[4];
+    s.selected = (int64_t)c[5];
+    s.lr_rejected = (int64_t)c[6];
+    s.valid = (int64_t)c[10];
+    if (b->flags & SGM_BATCH_DEVICE) {
+      e = hipMemcpy(out, &s, sizeof s, hipMemcpyHostToDevice);
+      if (e != hipSuccess) return hip_fail(e, "batch counters write");
+    } else {
+      *out = s;
+    }
+  }
+  return SGM_OK;
+}
+
+}  // namespace sgm
+
+extern "C" int sgm_batch_destroy(sgm_batch *b) {
+  if (!b) return SGM_OK;
+  sgm_context *ctx = b->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(b->stream);
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->external) {
+    delete b;
+    return SGM_OK;
+  }
+  if (b->arena) {
+    (void)hipFree(b->arena);
+    ctx->allocated -= b->arena_bytes;
+  }
+  if (b->tables) {
+    (void)hipFree(b->tables);
+    ctx->allocated -= b->tables_bytes;
+  }
+  delete b;
+  return SGM_OK;
+}
+
+extern "C" int64_t sgm_batch_device_bytes(sgm_batch *b) {
+  return b ? (int64_t)(b->arena_bytes + b->tables_bytes) : 0;
+}

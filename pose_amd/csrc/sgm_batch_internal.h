@@ -1,0 +1,52 @@
+// Host-side state of the batched and served modes.
+#pragma once
+
+#include <vector>
+
+#include "sgm_batch_kernel.h"
+#include "sgm_internal.h"
+
+namespace sgm {
+
+// Device buffers of one pair inside a batch arena (or the caller's buffers).
+struct PairSlots {
+  uint8_t *gray[2] = {nullptr, nullptr};
+  uint8_t *valid[2] = {nullptr, nullptr};
+  float *disp[2] = {nullptr, nullptr};
+  float *checked = nullptr;
+  uint8_t *reasons = nullptr, *boundary = nullptr;
+};
+
+int64_t batch_pair_bytes(const sgm_params *p, int w, int h, int D, bool host_io);
+size_t batch_arena_bytes(const sgm_params *params, const sgm_pair *pairs, int count, int flags);
+size_t batch_table_bytes(int count, const sgm_params *params);
+int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pairs, int count,
+                 int flags, void *region, size_t region_bytes, hipStream_t stream,
+                 sgm_batch **out);
+int batch_launch(sgm_batch *b);
+int batch_finish(sgm_batch *b);
+
+}  // namespace sgm
+
+struct sgm_batch {
+  sgm_context *ctx = nullptr;
+  hipStream_t stream = nullptr;
+  bool external = false;  // arena and tables belong to a serving pool
+  sgm_params params{};
+  int flags = 0;
+  std::vector<sgm_pair> pairs;
+  std::vector<sgm::PairSlots> slots;
+  int words = 1, sum_bytes = 2;
+  bool fused = false;
+  int waves_per_block = 4, lines_per_item = 1, blocks = 0;
+  void *arena = nullptr;
+  size_t arena_bytes = 0;
+  void *tables = nullptr;
+  size_t tables_bytes = 0;
+  unsigned *d_control = nullptr;
+  size_t control_words = 0;
+  unsigned long long *d_stats = nullptr;
+  int total_items = 0;
+  sgm::BatchArgs args{};
+  hipGraphExec_t graph_exec = nullptr;
+};

@@ -82,6 +82,10 @@ struct sgm_context {
   std::vector<sgm::PendingEvent> pending_events;
   std::map<std::string, std::pair<double, long long>> kernel_stats;
   std::map<int, int> tuned_waves;   // lane group R -> waves per workgroup in path sweeps
+  // batched mode launch shape (autotuned)
+  int batch_waves_per_block = 4;
+  int batch_lines_per_item = 2;
+  int batch_blocks_per_cu = 2;
   int last_w = 0, last_h = 0;
   sgm_params last_params{};
 };

@@ -81,6 +81,10 @@ __device__ __forceinline__ void line_start(int line, int dx, int dy, int w, int 
   }
 }
 
+__device__ __forceinline__ int lines_of_dev(int cls, int w, int h) {
+  return cls == 0 ? h : cls == 1 ? w : w + h - 1;
+}
+
 // Element k (uniform) of a lane-distributed vector, or -1 where invalid.
 template <int R>
 __device__ __forceinline__ int lane_pick_valid(const int *S, const int *valid, int kbase, int k) {

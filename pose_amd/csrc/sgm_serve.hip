@@ -1,0 +1,512 @@
+// Serving mode (continuous batching) and the launch-shape autotuner.
+//
+// A server thread admits submitted pairs in arrival order as soon as their
+// device footprint fits the free part of a budgeted memory pool and a stream
+// slot is free, launches every admitted pair waiting at that moment as one
+// batch (persistent work-queue kernel) on that stream, and keeps up to
+// `streams` batches in flight, so requests that arrive while others run
+// overlap with them on the device.  Volumes stay device-resident for the life
+// of their batch.  Neither mode has a counterpart in the reference, which
+// matches pairs sequentially (DenseReconstruction.java:64-72).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "sgm_batch_internal.h"
+
+using namespace sgm;
+
+namespace {
+
+// First-fit allocator over one device region (offsets, 256-byte granules).
+class Pool {
+ public:
+  void reset(size_t bytes) {
+    size_ = bytes;
+    free_.clear();
+    free_[0] = bytes;
+  }
+  bool alloc(size_t bytes, size_t *off) {
+    bytes = (bytes + 255) / 256 * 256;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        *off = it->first;
+        const size_t rest = it->second - bytes, o = it->first + bytes;
+        free_.erase(it);
+        if (rest) free_[o] = rest;
+        used_ += bytes;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    bytes = (bytes + 255) / 256 * 256;
+    used_ -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto next = std::next(it);
+    if (next != free_.end() && it->first + it->second == next->first) {
+      it->second += next->second;
+      free_.erase(next);
+    }
+    if (it != free_.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  size_t used() const { return used_; }
+  size_t size() const { return size_; }
+
+ private:
+  size_t size_ = 0, used_ = 0;
+  std::map<size_t, size_t> free_;
+};
+
+struct Request {
+  uint64_t ticket;
+  sgm_pair pair;
+  sgm_stats stats;
+  size_t bytes = 0, off = 0;
+  double arrival = 0, admitted = 0, started = 0, finished = 0;
+  int status = 1;  // 1 pending, 0 done, < 0 error
+  int batch_id = -1;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  sgm_batch *batch = nullptr;
+  std::vector<Request *> reqs;
+  size_t off = 0, bytes = 0;
+  bool busy = false;
+};
+
+}  // namespace
+
+struct sgm_server {
+  sgm_context *ctx = nullptr;
+  sgm_params params{};
+  int max_batch = 16;
+  void *region = nullptr;
+  size_t region_bytes = 0;
+  Pool pool;
+  std::vector<Slot> slots;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request *> pending;
+  std::map<uint64_t, Request *> all;
+  uint64_t next_ticket = 1;
+  int next_batch = 0;
+  bool stop = false;
+  std::thread worker;
+  std::chrono::steady_clock::time_point t0;
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  void run();
+};
+
+static size_t request_bytes(const sgm_params *p, const sgm_pair &pr) {
+  return batch_arena_bytes(p, &pr, 1, 0) + batch_table_bytes(1, p);
+}
+
+void sgm_server::run() {
+  (void)hipSetDevice(ctx->device);
+  std::unique_lock<std::mutex> lk(mu);
+  while (true) {
+    // retire finished batches
+    for (Slot &s : slots) {
+      if (!s.busy) continue;
+      if (hipEventQuery(s.done) != hipSuccess) continue;
+      lk.unlock();
+      const int rc = batch_finish(s.batch);
+      sgm_batch_destroy(s.batch);
+      lk.lock();
+      const double t = now();
+      for (Request *r : s.reqs) {
+        r->finished = t;
+        r->status = rc == SGM_OK ? 0 : rc;
+        if (r->pair.stats) *r->pair.stats = r->stats;
+      }
+      pool.release(s.off, s.bytes);
+      s.busy = false;
+      s.batch = nullptr;
+      s.reqs.clear();
+      cv_done.notify_all();
+    }
+    if (stop && pending.empty()) {
+      bool any = false;
+      for (Slot &s : slots) any |= s.busy;
+      if (!any) break;
+    }
+    // admit and launch: one batch per free stream slot
+    for (Slot &s : slots) {
+      if (s.busy || pending.empty()) continue;
+      std::vector<Request *> take;
+      size_t bytes = 0;
+      for (Request *r : pending) {
+        if ((int)take.size() >= max_batch) break;
+        take.push_back(r);
+        bytes += r->bytes;
+      }
+      // shrink until the group fits the pool (FIFO: keep the oldest)
+      size_t off = 0;
+      while (!take.empty()) {
+        std::vector<sgm_pair> prs;
+        for (Request *r : take) prs.push_back(r->pair);
+        bytes = batch_arena_bytes(&params, prs.data(), (int)prs.size(), 0) +
+                batch_table_bytes((int)prs.size(), &params);
+        if (pool.alloc(bytes, &off)) break;
+        take.pop_back();
+      }
+      if (take.empty()) break;  // wait for memory to free up
+      const double t = now();
+      for (size_t i = 0; i < take.size(); i++) {
+        take[i]->admitted = t;
+        take[i]->batch_id = next_batch;
+        pending.pop_front();
+      }
+      std::vector<sgm_pair> prs;
+      for (Request *r : take) {
+        sgm_pair p = r->pair;
+        p.stats = &r->stats;
+        prs.push_back(p);
+      }
+      s.off = off;
+      s.bytes = bytes;
+      s.reqs = take;
+      s.busy = true;
+      const int id = next_batch++;
+      (void)id;
+      lk.unlock();
+      sgm_batch *b = nullptr;
+      int rc = batch_create(ctx, &params, prs.data(), (int)prs.size(), 0,
+                            (uint8_t *)region + off, bytes, s.stream, &b);
+      if (rc == SGM_OK) rc = batch_launch(b);
+      if (rc == SGM_OK) {
+        hipError_t e = hipEventRecord(s.done, s.stream);
+        if (e != hipSuccess) rc = hip_fail(e, "hipEventRecord");
+      }
+      lk.lock();
+      const double ts = now();
+      if (rc != SGM_OK) {
+        if (b) {
+          lk.unlock();
+          sgm_batch_destroy(b);
+          lk.lock();
+        }
+        for (Request *r : take) {
+          r->status = rc;
+          r->finished = ts;
+        }
+        pool.release(s.off, s.bytes);
+        s.busy = false;
+        s.reqs.clear();
+        cv_done.notify_all();
+        continue;
+      }
+      s.batch = b;
+      for (Request *r : take) r->started = ts;
+    }
+    bool any_busy = false;
+    for (Slot &s : slots) any_busy |= s.busy;
+    if (any_busy) {
+      cv_work.wait_for(lk, std::chrono::microseconds(200));
+    } else {
+      cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+    }
+  }
+}
+
+extern "C" int sgm_server_create(sgm_context *ctx, const sgm_params *params, int64_t budget_bytes,
+                                 int max_batch, int streams, sgm_server **out) {
+  if (!ctx || !params || !out || streams < 1 || streams > 8 || max_batch < 1)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "bad server arguments");
+  *out = nullptr;
+  int rc = validate_params(params, params->window_w + 2, params->window_h + 2);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(ctx->lock);
+  (void)hipSetDevice(ctx->device);
+  const int64_t avail = ctx->budget - ctx->allocated;
+  const int64_t bytes = budget_bytes > 0 ? std::min(budget_bytes, avail) : avail;
+  if (bytes <= 0) return fail(SGM_ERR_NO_PLAN, "no device memory left for the serving pool");
+  sgm_server *s = new sgm_server();
+  s->ctx = ctx;
+  s->params = *params;
+  s->max_batch = max_batch;
+  s->region_bytes = (size_t)bytes;
+  rc = context_alloc(ctx, &s->region, s->region_bytes, "serving pool");
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  s->pool.reset(s->region_bytes);
+  s->slots.resize(streams);
+  for (Slot &sl : s->slots) {
+    hipError_t e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "server stream");
+      break;
+    }
+  }
+  if (rc) {
+    for (Slot &sl : s->slots) {
+      if (sl.done) (void)hipEventDestroy(sl.done);
+      if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    }
+    (void)hipFree(s->region);
+    ctx->allocated -= s->region_bytes;
+    delete s;
+    return rc;
+  }
+  s->t0 = std::chrono::steady_clock::now();
+  s->worker = std::thread([s] { s->run(); });
+  *out = s;
+  return SGM_OK;
+}
+
+extern "C" int sgm_server_submit(sgm_server *srv, const sgm_pair *pair, uint64_t *ticket) {
+  if (!srv || !pair || !ticket) return fail(SGM_ERR_INVALID_ARGUMENT, "null argument");
+  sgm_params q = srv->params;
+  q.disparities = pair->disparities;
+  q.min_disparity = pair->min_disparity;
+  int rc = validate_params(&q, pair->width, pair->height);
+  if (rc) return rc;
+  if (!pair->left || !pair->right || !pair->checked)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "pair needs left, right and checked buffers");
+  if (lanes_group(pair->disparities) > 8)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "serving supports up to 512 disparities per pair");
+  const size_t bytes = request_bytes(&srv->params, *pair);
+  if (bytes > srv->region_bytes)
+    return fail(SGM_ERR_REJECTED, "pair needs %zu MiB, more than the serving budget (%zu MiB)",
+                bytes >> 20, srv->region_bytes >> 20);
+  Request *r = new Request();
+  r->pair = *pair;
+  memset(&r->stats, 0, sizeof r->stats);
+  r->bytes = bytes;
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (srv->stop) {
+      delete r;
+      return fail(SGM_ERR_REJECTED, "server is stopping");
+    }
+    r->ticket = srv->next_ticket++;
+    r->arrival = srv->now();
+    srv->pending.push_back(r);
+    srv->all[r->ticket] = r;
+    *ticket = r->ticket;
+  }
+  srv->cv_work.notify_all();
+  return SGM_OK;
+}
+
+extern "C" int sgm_server_wait(sgm_server *srv, uint64_t ticket, double timeout_s,
+                               sgm_request_info *info) {
+  if (!srv) return fail(SGM_ERR_INVALID_ARGUMENT, "null server");
+  std::unique_lock<std::mutex> lk(srv->mu);
+  auto it = srv->all.find(ticket);
+  if (it == srv->all.end()) return fail(SGM_ERR_NOT_FOUND, "unknown ticket %llu", (unsigned long long)ticket);
+  Request *r = it->second;
+  auto ready = [&] { return r->status != 1; };
+  if (timeout_s < 0) srv->cv_done.wait(lk, ready);
+  else if (!srv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready))
+    return fail(SGM_ERR_TIMEOUT, "request %llu not finished", (unsigned long long)ticket);
+  if (info) {
+    info->arrival_s = r->arrival;
+    info->admitted_s = r->admitted;
+    info->started_s = r->started;
+    info->finished_s = r->finished;
+    info->device_bytes = (int64_t)r->bytes;
+    info->status = r->status;
+    info->batch_id = r->batch_id;
+  }
+  const int status = r->status;
+  srv->all.erase(it);
+  delete r;
+  if (status != 0) return fail(status, "request failed (%d)", status);
+  return SGM_OK;
+}
+
+extern "C" double sgm_server_now(sgm_server *srv) { return srv ? srv->now() : 0.0; }
+
+extern "C" int sgm_server_destroy(sgm_server *srv) {
+  if (!srv) return SGM_OK;
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->stop = true;
+  }
+  srv->cv_work.notify_all();
+  if (srv->worker.joinable()) srv->worker.join();
+  (void)hipSetDevice(srv->ctx->device);
+  for (Slot &sl : srv->slots) {
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  }
+  for (auto &kv : srv->all) delete kv.second;
+  {
+    std::lock_guard<std::mutex> g(srv->ctx->lock);
+    (void)hipFree(srv->region);
+    srv->ctx->allocated -= srv->region_bytes;
+  }
+  delete srv;
+  return SGM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: times candidate launch shapes on this device and keeps the best.
+namespace {
+
+uint32_t lcg(uint32_t &s) {
+  s = s * 1664525u + 1013904223u;
+  return s >> 8;
+}
+
+// A textured synthetic pair (host buffers): smooth random texture shifted by a
+// constant disparity, enough to exercise realistic path-sweep traffic.
+void tune_pair(int w, int h, int shift, uint32_t seed, std::vector<uint8_t> &l,
+               std::vector<uint8_t> &r) {
+  l.resize((size_t)w * h);
+  r.resize((size_t)w * h);
+  for (size_t i = 0; i < l.size(); i++) l[i] = (uint8_t)(lcg(seed) & 255);
+  for (int y = 0; y < h; y++)
+    for (int x = 0; x < w; x++) r[(size_t)y * w + x] = l[(size_t)y * w + std::min(w - 1, x + shift)];
+}
+
+double time_plain(sgm_context *ctx, const sgm_params *p, const std::vector<uint8_t> &l,
+                  const std::vector<uint8_t> &r, int w, int h, std::vector<float> &out) {
+  sgm_stats st;
+  sgm_match_pair(ctx, p, l.data(), r.data(), nullptr, nullptr, w, h, out.data(), nullptr, nullptr,
+                 &st, nullptr, nullptr);  // warm-up
+  const auto t0 = std::chrono::steady_clock::now();
+  const int reps = 3;
+  for (int i = 0; i < reps; i++)
+    if (sgm_match_pair(ctx, p, l.data(), r.data(), nullptr, nullptr, w, h, out.data(), nullptr,
+                       nullptr, &st, nullptr, nullptr) != SGM_OK)
+      return 1e30;
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps;
+}
+
+}  // namespace
+
+extern "C" int sgm_autotune(sgm_context *ctx, const sgm_params *params, const char *path) {
+  if (!ctx || !params) return fail(SGM_ERR_INVALID_ARGUMENT, "null argument");
+  int rc = validate_params(params, 64, 64);
+  if (rc) return rc;
+  // 1. waves per workgroup of the plain path sweeps, for this D's lane group
+  const int R = path_lanes(params);
+  const int w = std::max(512, params->disparities + 256), h = 384;
+  std::vector<uint8_t> l, r;
+  tune_pair(w, h, std::min(params->disparities / 2, 40), 7, l, r);
+  std::vector<float> out((size_t)w * h);
+  double best = 1e30;
+  int best_waves = 4;
+  for (int waves : {1, 2, 4}) {
+    ctx->tuned_waves[R] = waves;
+    const double t = time_plain(ctx, params, l, r, w, h, out);
+    if (t < best) {
+      best = t;
+      best_waves = waves;
+    }
+  }
+  ctx->tuned_waves[R] = best_waves;
+  // 2. batched mode: waves per block x lines per work item x blocks per CU
+  const int npairs = 6;
+  std::vector<std::vector<uint8_t>> L(npairs), Rr(npairs);
+  std::vector<std::vector<float>> outs(npairs);
+  std::vector<sgm_pair> pairs(npairs);
+  uint32_t seed = 11;
+  for (int i = 0; i < npairs; i++) {
+    const int pw = 640 + (int)(lcg(seed) % 640), ph = pw * 3 / 4;
+    const int D = std::min(params->disparities, 64 + (int)(lcg(seed) % 192));
+    tune_pair(pw, ph, D / 3, 100 + i, L[i], Rr[i]);
+    outs[i].resize((size_t)pw * ph);
+    memset(&pairs[i], 0, sizeof(sgm_pair));
+    pairs[i].width = pw;
+    pairs[i].height = ph;
+    pairs[i].disparities = std::max(3, D);
+    pairs[i].min_disparity = 0;
+    pairs[i].left = L[i].data();
+    pairs[i].right = Rr[i].data();
+    pairs[i].checked = outs[i].data();
+  }
+  double bbest = 1e30;
+  int bw = ctx->batch_waves_per_block, bl = ctx->batch_lines_per_item, bb = ctx->batch_blocks_per_cu;
+  for (int waves : {2, 4, 8})
+    for (int lines : {1, 2, 4})
+      for (int bpc : {1, 2, 4}) {
+        if (waves * bpc > 16) continue;
+        ctx->batch_waves_per_block = waves;
+        ctx->batch_lines_per_item = lines;
+        ctx->batch_blocks_per_cu = bpc;
+        sgm_batch *b = nullptr;
+        if (sgm_batch_create(ctx, params, pairs.data(), npairs, 0, &b) != SGM_OK) continue;
+        double t = 1e30;
+        if (sgm_batch_run(b) == SGM_OK) {
+          const auto t0 = std::chrono::steady_clock::now();
+          if (sgm_batch_run(b) == SGM_OK && sgm_batch_run(b) == SGM_OK)
+            t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / 2;
+        }
+        sgm_batch_destroy(b);
+        if (t < bbest) {
+          bbest = t;
+          bw = waves;
+          bl = lines;
+          bb = bpc;
+        }
+      }
+  ctx->batch_waves_per_block = bw;
+  ctx->batch_lines_per_item = bl;
+  ctx->batch_blocks_per_cu = bb;
+  if (path) {
+    FILE *f = fopen(path, "w");
+    if (!f) return fail(SGM_ERR_INVALID_ARGUMENT, "cannot write %s", path);
+    fprintf(f,
+            "{\"device\": \"%s\", \"arch\": \"%s\", \"disparities\": %d, \"lane_group\": %d,\n"
+            " \"path_waves\": %d, \"path_pair_seconds\": %.6f,\n"
+            " \"batch_waves_per_block\": %d, \"batch_lines_per_item\": %d, "
+            "\"batch_blocks_per_cu\": %d, \"batch_seconds\": %.6f}\n",
+            ctx->info.name, ctx->info.arch, params->disparities, R, best_waves, best, bw, bl, bb,
+            bbest);
+    fclose(f);
+  }
+  return SGM_OK;
+}
+
+extern "C" int sgm_autotune_load(sgm_context *ctx, const char *path) {
+  if (!ctx || !path) return fail(SGM_ERR_INVALID_ARGUMENT, "null argument");
+  FILE *f = fopen(path, "r");
+  if (!f) return fail(SGM_ERR_NOT_FOUND, "no tuning file %s", path);
+  std::string s;
+  char buf[4096];
+  size_t k;
+  while ((k = fread(buf, 1, sizeof buf, f)) > 0) s.append(buf, k);
+  fclose(f);
+  auto field = [&](const char *name, int *v) {
+    const std::string key = std::string("\"") + name + "\":";
+    const size_t at = s.find(key);
+    if (at == std::string::npos) return false;
+    *v = atoi(s.c_str() + at + key.size());
+    return true;
+  };
+  int R = 0, waves = 0, bw = 0, bl = 0, bb = 0;
+  if (field("lane_group", &R) && field("path_waves", &waves) && waves >= 1 && waves <= 4)
+    ctx->tuned_waves[R] = waves;
+  if (field("batch_waves_per_block", &bw) && bw >= 1 && bw <= 8) ctx->batch_waves_per_block = bw;
+  if (field("batch_lines_per_item", &bl) && bl >= 1 && bl <= 64) ctx->batch_lines_per_item = bl;
+  if (field("batch_blocks_per_cu", &bb) && bb >= 1 && bb <= 16) ctx->batch_blocks_per_cu = bb;
+  return SGM_OK;
+}

@@ -595,3 +595,151 @@ class CensusSgm:
         st = Stats()
         st.set_all(r["stats"])
         return Result(width, height, r["checked"], st, r["reasons"], r["boundary"], r["dl"], r["dr"])
+
+
+# ---------------------------------------------------------------------------
+# Batched and served modes.
+
+BATCH_DEVICE, BATCH_GRAPH, BATCH_NO_SYNC = 1, 2, 4
+
+
+class Batch:
+    """Many uneven pairs in one persistent launch (host numpy buffers).
+
+    ``pairs``: list of dicts with left, right (uint8 (h, w)), disparities,
+    min_disparity and optional valid_left / valid_right.  Outputs (checked, dl, dr,
+    reasons, boundary, stats) are allocated here and refreshed by ``run()``."""
+
+    def __init__(self, ctx, params, pairs, graph=False):
+        self.ctx = ctx
+        self.params = params
+        self.keep = []
+        arr = (SgmPair * len(pairs))()
+        self.outputs = []
+        for i, pr in enumerate(pairs):
+            left = np.ascontiguousarray(pr["left"], np.uint8)
+            right = np.ascontiguousarray(pr["right"], np.uint8)
+            h, w = left.shape
+            n = w * h
+            o = dict(checked=np.empty(n, np.float32), dl=np.empty(n, np.float32),
+                     dr=np.empty(n, np.float32), reasons=np.empty(n, np.uint8),
+                     boundary=np.empty(n, np.uint8), stats=SgmStats())
+            vl = pr.get("valid_left")
+            vr = pr.get("valid_right")
+            vl = None if vl is None else np.ascontiguousarray(vl, np.uint8)
+            vr = None if vr is None else np.ascontiguousarray(vr, np.uint8)
+            self.keep += [left, right, vl, vr]
+            arr[i] = SgmPair(width=w, height=h, disparities=pr["disparities"],
+                             min_disparity=pr.get("min_disparity", 0), left=_p(left),
+                             right=_p(right), valid_left=_p(vl), valid_right=_p(vr),
+                             checked=_p(o["checked"]), dl=_p(o["dl"]), dr=_p(o["dr"]),
+                             reasons=_p(o["reasons"]), boundary=_p(o["boundary"]),
+                             stats=ctypes.addressof(o["stats"]))
+            self.outputs.append(o)
+        self.arr = arr
+        h = ctypes.c_void_p()
+        check(lib().sgm_batch_create(ctx.h, ctypes.byref(params), arr, len(pairs),
+                                     BATCH_GRAPH if graph else 0, ctypes.byref(h)))
+        self.h = h
+
+    def run(self):
+        check(lib().sgm_batch_run(self.h))
+        return [dict(checked=o["checked"], dl=o["dl"], dr=o["dr"], reasons=o["reasons"],
+                     boundary=o["boundary"], stats=o["stats"].as_dict()) for o in self.outputs]
+
+    def device_bytes(self):
+        return lib().sgm_batch_device_bytes(self.h)
+
+    def close(self):
+        if self.h:
+            lib().sgm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBatch:
+    """A batch over caller device buffers (inputs resident in HBM): build it from
+    sgm_pair records whose pointers are device pointers."""
+
+    def __init__(self, ctx, params, records, graph=True, no_sync=False):
+        self.ctx = ctx
+        arr = (SgmPair * len(records))(*records)
+        self.arr = arr
+        flags = BATCH_DEVICE | (BATCH_GRAPH if graph else 0) | (BATCH_NO_SYNC if no_sync else 0)
+        h = ctypes.c_void_p()
+        check(lib().sgm_batch_create(ctx.h, ctypes.byref(params), arr, len(records), flags,
+                                     ctypes.byref(h)))
+        self.h = h
+
+    def run(self):
+        check(lib().sgm_batch_run(self.h))
+
+    def device_bytes(self):
+        return lib().sgm_batch_device_bytes(self.h)
+
+    def close(self):
+        if self.h:
+            lib().sgm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching server over host buffers."""
+
+    def __init__(self, ctx, params, budget_bytes=0, max_batch=16, streams=2):
+        self.ctx = ctx
+        self.params = params
+        h = ctypes.c_void_p()
+        check(lib().sgm_server_create(ctx.h, ctypes.byref(params), int(budget_bytes), max_batch,
+                                      streams, ctypes.byref(h)))
+        self.h = h
+        self.live = {}
+
+    def submit(self, left, right, disparities, min_disparity=0):
+        left = np.ascontiguousarray(left, np.uint8)
+        right = np.ascontiguousarray(right, np.uint8)
+        h, w = left.shape
+        n = w * h
+        out = dict(checked=np.empty(n, np.float32), dl=np.empty(n, np.float32),
+                   dr=np.empty(n, np.float32), stats=SgmStats(), left=left, right=right)
+        rec = SgmPair(width=w, height=h, disparities=disparities, min_disparity=min_disparity,
+                      left=_p(left), right=_p(right), checked=_p(out["checked"]),
+                      dl=_p(out["dl"]), dr=_p(out["dr"]), stats=ctypes.addressof(out["stats"]))
+        t = ctypes.c_uint64()
+        check(lib().sgm_server_submit(self.h, ctypes.byref(rec), ctypes.byref(t)))
+        self.live[t.value] = out
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        info = SgmRequestInfo()
+        rc = lib().sgm_server_wait(self.h, ticket, float(timeout), ctypes.byref(info))
+        out = self.live.pop(ticket)
+        check(rc)
+        out["info"] = {f: getattr(info, f) for f, _ in SgmRequestInfo._fields_}
+        out["stats"] = out["stats"].as_dict()
+        return out
+
+    def now(self):
+        return lib().sgm_server_now(self.h)
+
+    def close(self):
+        if self.h:
+            lib().sgm_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

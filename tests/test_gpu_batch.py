@@ -1,0 +1,215 @@
+"""GPU tests of the batched (persistent work-queue kernel, hipGraph) and served
+(continuous batching) modes: every pair must be bit-exact to the oracle and to
+the plain one-pair path, alone or in the mix."""
+import json
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pose_amd import sgm, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def to_sgm(op):
+    return sgm.SgmParams(**{f: getattr(op, f) for f, _ in sgm.SgmParams._fields_})
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def same(got, want, what):
+    for k in ("checked", "dl", "dr"):
+        assert np.array_equal(bits(got[k]), bits(want[k])), (what, k)
+    for k in ("reasons", "boundary"):
+        if k in got and k in want:
+            assert np.array_equal(got[k], want[k]), (what, k)
+    for k in O.STAT_FIELDS:
+        assert got["stats"][k] == want["stats"][k], (what, k, got["stats"][k], want["stats"][k])
+
+
+def uneven_pairs(seed, count, wrange=(96, 420), drange=(3, 500)):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        w = int(rng.integers(*wrange))
+        h = int(rng.integers(40, 160))
+        D = int(rng.integers(drange[0], min(drange[1], w - 20)))
+        md = int(rng.integers(-8, 9))
+        left, right, _ = synth.stereo_pair(w, h, D, md, seed=seed * 100 + i)
+        pr = dict(left=left, right=right, disparities=D, min_disparity=md)
+        if i % 3 == 1:
+            v = (rng.random((h, w)) > 0.05).astype(np.uint8)
+            pr["valid_left"] = v
+            pr["valid_right"] = np.roll(v, 3, axis=1)
+        out.append(pr)
+    return out
+
+
+def oracle_of(base_kw, pr):
+    op = O.params(**dict(base_kw, disparities=pr["disparities"], min_disparity=pr["min_disparity"]))
+    h, w = pr["left"].shape
+    return O.compute(op, pr["left"], pr["right"], w, h, pr.get("valid_left"), pr.get("valid_right"))
+
+
+CENSUS97 = dict(window_w=9, window_h=7, p1=10, p2=120, lr_threshold=1.0)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_census_fused_bit_exact(gpu_context, graph):
+    pairs = uneven_pairs(1, 7)
+    print("batch pairs", [(p["left"].shape, p["disparities"], p["min_disparity"]) for p in pairs],
+          flush=True)
+    params = to_sgm(O.params(**dict(CENSUS97, disparities=64)))
+    b = sgm.Batch(gpu_context, params, pairs, graph=graph)
+    try:
+        for rep in range(2 if graph else 1):
+            res = b.run()
+            for i, pr in enumerate(pairs):
+                same(res[i], oracle_of(CENSUS97, pr), "pair %d rep %d" % (i, rep))
+    finally:
+        b.close()
+
+
+@pytest.mark.parametrize("kw", [
+    dict(window_w=5, window_h=5, cost=1, p1=4, p2=24),
+    dict(window_w=11, window_h=11, cost=0, p1=20, p2=120, directions=4),
+    dict(window_w=7, window_h=7, cost=3, p1=12, p2=100),
+    dict(window_w=5, window_h=5, cost=2, p1=40, p2=9000),          # 32-bit aggregates
+], ids=["ad-census", "census11x11-4paths", "zncc", "census-gradient-agg32"])
+def test_batch_other_costs_bit_exact(gpu_context, kw):
+    pairs = uneven_pairs(2, 4, wrange=(96, 260), drange=(3, 200))
+    params = to_sgm(O.params(**dict(kw, disparities=64)))
+    b = sgm.Batch(gpu_context, params, pairs)
+    try:
+        res = b.run()
+        for i, pr in enumerate(pairs):
+            same(res[i], oracle_of(kw, pr), "pair %d" % i)
+    finally:
+        b.close()
+
+
+def test_batch_graph_replays_match_eager(gpu_context):
+    """Graph replays with eager work between them all equal the eager result."""
+    pairs = uneven_pairs(3, 5)
+    params = to_sgm(O.params(**dict(CENSUS97, disparities=64)))
+    eager = sgm.Batch(gpu_context, params, pairs)
+    graph = sgm.Batch(gpu_context, params, pairs, graph=True)
+    try:
+        want = [dict((k, np.copy(v)) if k != "stats" else (k, dict(v)) for k, v in r.items())
+                for r in eager.run()]
+        for rep in range(3):
+            got = graph.run()
+            for i in range(len(pairs)):
+                same(got[i], want[i], "replay %d pair %d" % (rep, i))
+            eager.run()  # eager work between replays
+    finally:
+        eager.close()
+        graph.close()
+
+
+def test_batch_device_buffers(gpu_context):
+    ctx = gpu_context
+    pairs = uneven_pairs(4, 4)
+    params = to_sgm(O.params(**dict(CENSUS97, disparities=64)))
+    bufs, recs = [], []
+    for pr in pairs:
+        h, w = pr["left"].shape
+        n = w * h
+        b = dict(left=ctx.alloc(n).upload(pr["left"]), right=ctx.alloc(n).upload(pr["right"]),
+                 checked=ctx.alloc(4 * n), dl=ctx.alloc(4 * n), dr=ctx.alloc(4 * n),
+                 stats=ctx.alloc(128))
+        if "valid_left" in pr:
+            b["vl"] = ctx.alloc(n).upload(pr["valid_left"])
+            b["vr"] = ctx.alloc(n).upload(pr["valid_right"])
+        bufs.append(b)
+        recs.append(sgm.SgmPair(width=w, height=h, disparities=pr["disparities"],
+                                min_disparity=pr["min_disparity"], left=b["left"].ptr,
+                                right=b["right"].ptr,
+                                valid_left=b["vl"].ptr if "vl" in b else None,
+                                valid_right=b["vr"].ptr if "vr" in b else None,
+                                checked=b["checked"].ptr, dl=b["dl"].ptr,
+                                dr=b["dr"].ptr, stats=b["stats"].ptr))
+    db = sgm.DeviceBatch(ctx, params, recs, graph=True)
+    try:
+        for rep in range(2):
+            db.run()
+            for i, pr in enumerate(pairs):
+                h, w = pr["left"].shape
+                want = oracle_of(CENSUS97, pr)
+                n = w * h
+                for k in ("checked", "dl", "dr"):
+                    assert np.array_equal(bits(bufs[i][k].download(np.float32, n)),
+                                          bits(want[k])), (rep, i, k)
+                st = bufs[i]["stats"].download(np.int64, 16)
+                assert st[5] == want["stats"]["selected"] and st[10] == want["stats"]["valid"]
+    finally:
+        db.close()
+        for b in bufs:
+            for v in b.values():
+                v.free()
+
+
+def test_server_results_equal_alone(gpu_context):
+    """Requests admitted while others run: each result equals its solo plain run."""
+    ctx = gpu_context
+    pairs = uneven_pairs(5, 9, wrange=(120, 600), drange=(16, 300))
+    params = to_sgm(O.params(**dict(CENSUS97, disparities=64)))
+    alone = []
+    for pr in pairs:
+        p = to_sgm(O.params(**dict(CENSUS97, disparities=pr["disparities"],
+                                   min_disparity=pr["min_disparity"])))
+        alone.append(ctx.match_pair(p, pr["left"], pr["right"]))
+    srv = sgm.Server(ctx, params, budget_bytes=2 << 30, max_batch=4, streams=2)
+    try:
+        tickets = []
+        for i, pr in enumerate(pairs):
+            tickets.append(srv.submit(pr["left"], pr["right"], pr["disparities"], pr["min_disparity"]))
+        batches = set()
+        for i, t in enumerate(tickets):
+            got = srv.wait(t, timeout=60.0)
+            batches.add(got["info"]["batch_id"])
+            for k in ("checked", "dl", "dr"):
+                assert np.array_equal(bits(got[k]), bits(alone[i][k])), (i, k)
+            for k in ("selected", "boundary_rejected", "ambiguity_rejected", "lr_rejected", "valid"):
+                assert got["stats"][k] == alone[i]["stats"][k], (i, k)
+            info = got["info"]
+            assert info["arrival_s"] <= info["admitted_s"] <= info["started_s"] <= info["finished_s"]
+        assert len(batches) >= 2  # more than one batch was formed while others ran
+    finally:
+        srv.close()
+
+
+def test_server_rejects_what_never_fits(gpu_context):
+    params = to_sgm(O.params(**dict(CENSUS97, disparities=64)))
+    srv = sgm.Server(gpu_context, params, budget_bytes=64 << 20, max_batch=2, streams=1)
+    try:
+        left = np.zeros((600, 800), np.uint8)
+        with pytest.raises(sgm.SgmError) as e:
+            srv.submit(left, left, 256)
+        assert e.value.code == -6
+    finally:
+        srv.close()
+
+
+def test_autotune_writes_and_loads(gpu_context):
+    params = to_sgm(O.params(**dict(CENSUS97, disparities=128)))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "tune.json")
+        ctx = sgm.Context(0)
+        try:
+            ctx.autotune(params, path)
+            t = json.load(open(path))
+            assert t["lane_group"] == 2 and t["path_waves"] in (1, 2, 4)
+            assert t["batch_lines_per_item"] in (1, 2, 4)
+            ctx.autotune_load(path)
+            left, right, _ = synth.stereo_pair(200, 100, 128, 0, seed=3)
+            r = ctx.match_pair(params, left, right)
+            want = O.compute(O.params(**dict(CENSUS97, disparities=128)), left, right, 200, 100)
+            assert np.array_equal(bits(r["checked"]), bits(want["checked"]))
+        finally:
+            ctx.close()
