@@ -57,6 +57,201 @@ def cpu_baseline(w, D, rows=256, trials=1):
                        "C oracle single-threaded, %.2f s" % (w, rows, D, t))
 
 
+def batch_workload(count=48, seed=4):
+    """The batched workload: uneven landscape pairs, long side U[800, 3000], aspect
+    4:3 or 3:2, D U[64, 384], min disparity U[-32, 32], per-pair seeds."""
+    from pose_amd import synth
+    return synth.batch_pairs(count=count, seed=seed)
+
+
+def run_batched(sgm, params_for, steps=2):
+    """Same 48 pairs through (a) the plain path, one pair per launch sequence, and
+    (b) one batched persistent launch replayed from a hipGraph; inputs in HBM."""
+    import numpy as np
+    from pose_amd import synth
+    spec = batch_workload()
+    # the largest budget the library allows (95% of device memory): the batch
+    # keeps its cost volumes resident when they fit, else recomputes census costs
+    ctx = sgm.Context(0, budget_bytes=1 << 62)
+    bufs, recs = [], []
+    updates = 0
+    t_gen = time.perf_counter()
+    for (w, h, D, md, seed) in spec:
+        left, right, _ = synth.stereo_pair(w, h, D, md, seed=seed)
+        n = w * h
+        b = dict(left=ctx.alloc(n).upload(left), right=ctx.alloc(n).upload(right),
+                 checked=ctx.alloc(4 * n), plain=ctx.alloc(4 * n), stats=ctx.alloc(128))
+        bufs.append(b)
+        recs.append(sgm.SgmPair(width=w, height=h, disparities=D, min_disparity=md,
+                                left=b["left"].ptr, right=b["right"].ptr, checked=b["checked"].ptr,
+                                stats=b["stats"].ptr))
+        updates += w * h * D * 8 * 2
+    t_gen = time.perf_counter() - t_gen
+    base = params_for(64, 0)
+    # plain: one pair per launch sequence, all pairs back to back
+    def plain_all():
+        for (w, h, D, md, _), b in zip(spec, bufs):
+            ctx.match_pair_device(params_for(D, md), b["left"], b["right"], w, h, b["plain"])
+        ctx.synchronize()
+    plain_all()  # warm-up (allocates the plain path's buffers)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plain_all()
+    t_plain = (time.perf_counter() - t0) / steps
+    ctx.release_buffers()
+    batch = sgm.DeviceBatch(ctx, base, recs, graph=True)
+    batch.run()  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run()
+    t_batch = (time.perf_counter() - t0) / steps
+    same = all(np.array_equal(b["checked"].download(np.uint32, (w * h)),
+                              b["plain"].download(np.uint32, (w * h)))
+               for (w, h, _, _, _), b in zip(spec, bufs))
+    dev_bytes = batch.device_bytes()
+    shape = batch.describe()
+    batch.close()
+    for b in bufs:
+        for v in b.values():
+            v.free()
+    ctx.close()
+    return {
+        "workload": "48 uneven pairs, long side U[800,3000], aspect 4:3/3:2, D U[64,384], "
+                    "min disparity U[-32,32], census 9x7, P1=10 P2=120, 8 paths, L+R + L-R",
+        "pairs": len(spec), "cell_updates": updates,
+        "plain_ms": round(t_plain * 1e3, 3), "batched_ms": round(t_batch * 1e3, 3),
+        "plain_gupdates_per_s": round(updates / t_plain / 1e9, 3),
+        "batched_gupdates_per_s": round(updates / t_batch / 1e9, 3),
+        "plain_pairs_per_s": round(len(spec) / t_plain, 3),
+        "batched_pairs_per_s": round(len(spec) / t_batch, 3),
+        "speedup_vs_plain": round(t_plain / t_batch, 3),
+        "bit_exact_vs_plain": bool(same), "batch_device_bytes": dev_bytes,
+        "batch": shape,
+        "input_generation_s": round(t_gen, 2),
+    }
+
+
+def serve_trace(count, rate, seed=7):
+    """Served stream: Poisson arrivals at `rate` pairs/s (rate None: all at t=0);
+    sizes 70% from the batched distribution, 25% 1600x1067 D=256, 5% 6000x4000
+    D=512.  Requests draw from a pool of pre-generated pairs of each class."""
+    import numpy as np
+    from pose_amd import synth
+    rng = np.random.default_rng(seed)
+    mixed = synth.batch_pairs(count=12, seed=seed)
+    pool = {"mixed": mixed,
+            "1600x1067": [(1600, 1067, 256, 0, seed * 1000 + 100 + i) for i in range(3)],
+            "6000x4000": [(6000, 4000, 512, 0, seed * 1000 + 200)]}
+    t, trace = 0.0, []
+    for i in range(count):
+        if rate:
+            t += float(rng.exponential(1.0 / rate))
+        u = rng.random()
+        cls = "mixed" if u < 0.70 else "1600x1067" if u < 0.95 else "6000x4000"
+        spec = pool[cls][int(rng.integers(len(pool[cls])))]
+        trace.append((t if rate else 0.0, cls, spec))
+    return trace
+
+
+def run_served(sgm, params_for, rate=20.0, count=60, budget=250e9, max_batch=8, streams=2):
+    """The same request trace through (a) the plain path, one pair at a time in
+    arrival order, and (b) the continuous-batching server (admission by HBM
+    footprint against `budget`); host buffers in and out for both.  Latency =
+    finish - arrival; sustained pairs/s = requests / (last finish - first arrival)."""
+    import threading
+    import numpy as np
+    from pose_amd import synth
+
+    def pct(v, q):
+        v = sorted(v)
+        return v[min(len(v) - 1, max(0, int(math.ceil(q * len(v))) - 1))]
+
+    images = {}
+    t_gen = time.perf_counter()
+    traces = {"poisson": serve_trace(count, rate), "saturated": serve_trace(count, None, seed=8)}
+    for tr in traces.values():
+        for _, _, (w, h, D, md, s) in tr:
+            if s not in images:
+                images[s] = synth.stereo_pair(w, h, D, md, seed=s)[:2]
+    t_gen = time.perf_counter() - t_gen
+    out = {"workload": "served stream: 70%% batched-distribution pairs, 25%% 1600x1067 D=256, "
+                       "5%% 6000x4000 D=512; census 9x7, P1=10 P2=120, 8 paths; host buffers "
+                       "in/out; Poisson %.0f pairs/s (%d requests) and a saturated burst "
+                       "(%d requests at t=0); server budget %.0f GB, max batch %d, %d streams"
+                       % (rate, count, count, budget / 1e9, max_batch, streams),
+           "input_generation_s": round(t_gen, 2), "distinct_pairs": len(images)}
+    for name, trace in traces.items():
+        updates = sum(w * h * D * 8 * 2 for _, _, (w, h, D, md, s) in trace)
+        classes = {}
+        for _, c, _ in trace:
+            classes[c] = classes.get(c, 0) + 1
+        # (a) plain path, FIFO, one pair at a time
+        ctx = sgm.Context(0)
+        ctx.match_pair(params_for(64, 0), *images[trace[0][2][4]])  # warm-up
+        plain_lat, plain_out, service = [], [], {}
+        t0 = time.perf_counter()
+        for (ta, c, (w, h, D, md, s)) in trace:
+            now = time.perf_counter() - t0
+            if ta > now:
+                time.sleep(ta - now)
+            s0 = time.perf_counter()
+            r = ctx.match_pair(params_for(D, md), *images[s])
+            t_end = time.perf_counter()
+            service.setdefault(c, []).append(t_end - s0)
+            plain_lat.append(t_end - t0 - ta)
+            plain_out.append(r["checked"])
+        plain_span = time.perf_counter() - t0 - trace[0][0]
+        ctx.close()
+        # (b) server: a submitter thread replays the arrivals
+        ctx = sgm.Context(0, budget_bytes=int(budget) + (8 << 30))
+        srv = sgm.Server(ctx, params_for(64, 0), budget_bytes=int(budget), max_batch=max_batch,
+                         streams=streams)
+        tickets = [None] * len(trace)
+        def submit():
+            t1 = time.perf_counter()
+            for i, (ta, _, (w, h, D, md, s)) in enumerate(trace):
+                now = time.perf_counter() - t1
+                if ta > now:
+                    time.sleep(ta - now)
+                tickets[i] = srv.submit(*images[s], D, md)
+        th = threading.Thread(target=submit)
+        th.start()
+        lat, first, last, same, batches = [], None, 0.0, True, set()
+        for i in range(len(trace)):
+            while tickets[i] is None:
+                time.sleep(0.0005)
+            got = srv.wait(tickets[i], timeout=600.0)
+            info = got["info"]
+            lat.append(info["finished_s"] - info["arrival_s"])
+            first = info["arrival_s"] if first is None else min(first, info["arrival_s"])
+            last = max(last, info["finished_s"])
+            batches.add(info["batch_id"])
+            same &= bool(np.array_equal(got["checked"].view(np.uint32),
+                                        plain_out[i].view(np.uint32)))
+        th.join()
+        srv.close()
+        ctx.close()
+        span = last - first
+        out[name] = {
+            "requests": len(trace), "classes": classes, "cell_updates": updates,
+            "served": {"p50_ms": round(pct(lat, 0.5) * 1e3, 2),
+                       "p99_ms": round(pct(lat, 0.99) * 1e3, 2),
+                       "pairs_per_s": round(len(trace) / span, 3),
+                       "gupdates_per_s": round(updates / span / 1e9, 3),
+                       "batches": len(batches)},
+            "plain": {"p50_ms": round(pct(plain_lat, 0.5) * 1e3, 2),
+                      "p99_ms": round(pct(plain_lat, 0.99) * 1e3, 2),
+                      "pairs_per_s": round(len(trace) / plain_span, 3),
+                      "gupdates_per_s": round(updates / plain_span / 1e9, 3),
+                      "mean_service_ms": {c: round(sum(v) / len(v) * 1e3, 2)
+                                          for c, v in service.items()}},
+            "speedup_vs_plain": round(plain_span / span, 3),
+            "bit_exact_vs_plain": same,
+        }
+        print("served leg %s done" % name, file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,10 +259,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
+    ap.add_argument("--no-served", action="store_true", help="skip the served-vs-plain leg")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals, pairs/s")
+    ap.add_argument("--serve-requests", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every box here has one MI355X; multi-GPU "
-              "sharding of this path is design-only (see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU only: this path runs on a single MI355X; "
+              "sharding pairs across GPUs is not implemented.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -77,6 +276,7 @@ def main():
     params = workload_params(sgm)
     left, right, _ = synth.stereo_pair(W, H, D, 0, seed=2)
     n = W * H
+    dev = sgm.device_info(0)
     ctx = sgm.Context(0)
     d_left = ctx.alloc(n).upload(left)
     d_right = ctx.alloc(n).upload(right)
@@ -122,6 +322,19 @@ def main():
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("bytes_per_launch")
 
+    for b in (d_left, d_right, d_out, d_stats):
+        b.free()
+    ctx.close()
+
+    def params_for(Dv, md):
+        return sgm.SgmParams(window_w=9, window_h=7, disparities=Dv, min_disparity=md, cost=0,
+                             p1=10, p2=120, edge_scale=32, directions=8, aggregate_bytes=0,
+                             census_weight=500, ad_truncation=30, gradient_truncation=60,
+                             reserved=0, uniqueness=0.03, lr_threshold=1.0)
+    batched = None if args.no_batched else run_batched(sgm, params_for)
+    served = None if args.no_served else run_served(sgm, params_for, rate=args.serve_rate,
+                                                    count=args.serve_requests)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(W, D)
@@ -143,6 +356,7 @@ def main():
         "config": {"workload": "plain SGM pair 1600x1067, census 9x7, D=256, P1=8 P2=96, "
                                "8 paths, uint16 volumes, left+right + L-R check",
                    "width": W, "height": H, "disparities": D, "paths": P, "pairs_per_step": 1},
+        "device": {k: dev[k] for k in ("name", "arch", "memory", "compute_units")},
         "pairs_per_s": round(1e3 / ms_per_step, 3),
         "latency_ms": {"p50": round(lat_ms[len(lat_ms) // 2], 4),
                        "p99": round(lat_ms[min(len(lat_ms) - 1, int(math.ceil(0.99 * len(lat_ms))) - 1)], 4)},
@@ -154,10 +368,9 @@ def main():
                      "launches_per_step": path_launches,
                      "algorithmic_bytes_per_launch": bytes_step / max(1, path_launches)},
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
-    for b in (d_left, d_right, d_out, d_stats):
-        b.free()
-    ctx.close()
     print(json.dumps(line))
 
 

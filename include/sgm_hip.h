@@ -34,7 +34,8 @@ extern "C" {
 
 /* Matching costs: 0..3 are StereoCost.code (StereoCost.java:7-15); 3 is the
  * reference's exact integer/FP64 ZNCC.  4 is the FP32 ZNCC variant with cost
- * round((1 - ncc) * 1024), invalid cells marked separately. */
+ * round((1 - ncc) * 1024) from FP32 window sums; invalid cells are stored as
+ * 0x8800 and enter the recurrence as 2048 (the flag bit is masked off). */
 #define SGM_COST_CENSUS 0
 #define SGM_COST_AD_CENSUS 1
 #define SGM_COST_CENSUS_GRADIENT 2
@@ -162,7 +163,20 @@ int sgm_memcpy(sgm_context *ctx, void *dst, const void *src, size_t bytes, int k
 int sgm_set_keep_volumes(sgm_context *ctx, int keep);
 int sgm_read_volumes(sgm_context *ctx, int direction, uint16_t *cost, uint32_t *aggregate);
 
+/* Cost-volume policy for 1-word Census with uint16 aggregates (no counterpart
+ * in the reference, whose GpuStereo.plan drops to the streamed plan when the
+ * cost volume does not fit).  AUTO keeps the uint16 cost volume resident when
+ * the pair (or batch) fits the budget with it and otherwise recomputes the
+ * census cost inside every path sweep (no cost volume, one third less HBM);
+ * RESIDENT and RECOMPUTE force one of the two.  Results are identical. */
+#define SGM_VOLUME_AUTO 0
+#define SGM_VOLUME_RESIDENT 1
+#define SGM_VOLUME_RECOMPUTE 2
+int sgm_set_cost_volume(sgm_context *ctx, int policy);
+
 int sgm_synchronize(sgm_context *ctx);
+/* Free the per-pair buffers the plain entry points cache in the context. */
+int sgm_release_buffers(sgm_context *ctx);
 /* Kernel timing of the last calls (GpuSgm.Compute.profile, GpuSgm.java:592-604). */
 int sgm_set_profiling(sgm_context *ctx, int enable);
 int sgm_profile_json(sgm_context *ctx, char *buffer, size_t length);
@@ -205,6 +219,9 @@ int sgm_batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair 
 int sgm_batch_run(sgm_batch *batch);
 int sgm_batch_destroy(sgm_batch *batch);
 int64_t sgm_batch_device_bytes(sgm_batch *batch);
+/* JSON summary of a batch: pairs, work items, launch shape, cost-volume mode
+ * ("resident", "resident-fused" or "recomputed"), device bytes, graph. */
+int sgm_batch_describe(sgm_batch *batch, char *buffer, size_t length);
 
 /* Autotuner: time the candidate path-sweep configurations on this device for
  * the given params and store the winners in the context (and, if path is not

@@ -47,5 +47,18 @@ def build(verbose=False):
     return OUTPUT
 
 
+SYNTH_OUTPUT = os.path.join(HERE, "libsgm_synth.so")
+
+
+def build_synth():
+    """The synthetic-input generator (plain C, OpenMP)."""
+    cc = os.environ.get("CC", "gcc")
+    _run([cc, "-O2", "-fPIC", "-shared", "-fopenmp", "-std=c11", "-o", SYNTH_OUTPUT + ".tmp",
+          os.path.join(HERE, "csrc", "synth.c"), "-lm"])
+    os.replace(SYNTH_OUTPUT + ".tmp", SYNTH_OUTPUT)
+    return SYNTH_OUTPUT
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_synth())

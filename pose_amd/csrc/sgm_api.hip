@@ -106,6 +106,9 @@ int validate_params(const sgm_params *p, int w, int h) {
   // CensusSgm.java:56-60
   if (w < p->window_w + 2 || h < p->window_h + 2)
     return fail(SGM_ERR_INVALID_ARGUMENT, "Invalid stereo image dimensions %dx%d", w, h);
+  // descriptor planes are addressed with 32-bit byte offsets (8 B per pixel)
+  if ((int64_t)w * h > (int64_t)1 << 28)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "image %dx%d exceeds 2^28 pixels", w, h);
   return SGM_OK;
 }
 
@@ -136,18 +139,32 @@ int padded_disparities(int D) {
 
 int words_of(const sgm_params *p) { return (p->window_w * p->window_h - 1 + 63) / 64; }
 
-int64_t direction_bytes(const sgm_params *p, int w, int h) {
+int64_t direction_bytes(const sgm_params *p, int w, int h, bool cost_volume) {
   const int64_t n = (int64_t)w * h, Dp = padded_disparities(p->disparities);
   const int sb = std::max(aggregate_bytes(p), 2);
   const bool zncc = p->cost == SGM_COST_ZNCC || p->cost == SGM_COST_ZNCC_FP32;
-  // cost + aggregate volumes, descriptors, mask, gray copy, disparity, window sums
-  return n * Dp * (2 + sb) + n * (8 * words_of(p) + 1 + 1 + 4 + (zncc ? 8 : 0));
+  // (cost +) aggregate volumes, descriptors, mask, gray|mask, gray copy, disparity, window sums
+  return n * Dp * ((cost_volume ? 2 : 0) + sb) + n * (8 * words_of(p) + 1 + 2 + 1 + 4 + (zncc ? 8 : 0));
+}
+
+bool census_recompute_possible(const sgm_params *p) {
+  return p->cost == SGM_COST_CENSUS && words_of(p) == 1 && aggregate_bytes(p) == 2 &&
+         path_lanes(p) <= 8;
+}
+
+int64_t direction_bytes(const sgm_params *p, int w, int h) {
+  return direction_bytes(p, w, h, !census_recompute_possible(p));
+}
+
+int64_t pair_bytes(const sgm_params *p, int w, int h, bool cost_volume) {
+  const int64_t n = (int64_t)w * h;
+  // both directions + input copies, validity, reasons/boundary scratch, checked, stats
+  return 2 * direction_bytes(p, w, h, cost_volume) + n * (2 + 2 + 2 + 4) + 256;
 }
 
 int64_t pair_bytes(const sgm_params *p, int w, int h) {
-  const int64_t n = (int64_t)w * h;
-  // both directions + input copies, validity, reasons/boundary scratch, checked, stats
-  return 2 * direction_bytes(p, w, h) + n * (2 + 2 + 2 + 4) + 256;
+  // smallest footprint of the pair (the fit test of the plan)
+  return pair_bytes(p, w, h, !census_recompute_possible(p));
 }
 
 }  // namespace sgm
@@ -266,6 +283,15 @@ extern "C" int sgm_set_keep_volumes(sgm_context *ctx, int keep) {
   return SGM_OK;
 }
 
+extern "C" int sgm_set_cost_volume(sgm_context *ctx, int policy) {
+  if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
+  if (policy < SGM_VOLUME_AUTO || policy > SGM_VOLUME_RECOMPUTE)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "cost volume policy %d not in 0..2", policy);
+  std::lock_guard<std::mutex> g(ctx->lock);
+  ctx->volume_policy = policy;
+  return SGM_OK;
+}
+
 extern "C" int sgm_set_profiling(sgm_context *ctx, int enable) {
   if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
   ctx->profiling = enable != 0;
@@ -275,6 +301,17 @@ extern "C" int sgm_set_profiling(sgm_context *ctx, int enable) {
 extern "C" int sgm_debug_fail_allocations(sgm_context *ctx, int count) {
   if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
   ctx->fail_allocations = count;
+  return SGM_OK;
+}
+
+extern "C" int sgm_release_buffers(sgm_context *ctx) {
+  if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> g(ctx->lock);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &kv : ctx->buffers) kv.second.release(ctx);
+  ctx->buffers.clear();
+  ctx->last_w = ctx->last_h = 0;
   return SGM_OK;
 }
 
@@ -442,6 +479,7 @@ static int launch_path_any(sgm_context *ctx, int R, int sum_bytes, int mode, con
 #define CASE_PK(RR)                                                         \
   case RR:                                                                  \
     if (mode == 2) launch_path_t<RR, uint16_t, 2>(ctx, jobs, A, phase, waves); \
+    else if (mode == 3) launch_path_t<RR, uint16_t, 3>(ctx, jobs, A, phase, waves); \
     else launch_path_t<RR, uint16_t, 1>(ctx, jobs, A, phase, waves);        \
     break;
   if (sum_bytes == 4) {
@@ -484,6 +522,16 @@ int path_lanes(const sgm_params *p) {
   // 16-bit aggregates run packed pairs: at least two disparities per lane.
   const int R = lanes_group(p->disparities);
   return aggregate_bytes(p) == 2 ? std::max(R, 2) : R;
+}
+
+bool recompute_cost(sgm_context *ctx, const sgm_params *p, int w, int h) {
+  // census cost recomputed in every sweep (no cost volume): forced by the
+  // policy, or when the resident cost volume would not fit the budget.  Kept
+  // volumes for inspection always use the resident volume.
+  if (ctx->keep_volumes || !census_recompute_possible(p)) return false;
+  if (ctx->volume_policy == SGM_VOLUME_RECOMPUTE) return true;
+  if (ctx->volume_policy == SGM_VOLUME_RESIDENT) return false;
+  return pair_bytes(p, w, h, true) > ctx->budget;
 }
 
 bool fused_cost(const sgm_params *p) {
@@ -533,7 +581,7 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
       ENSURE(("wsum" + s).c_str(), n * 4);
       ENSURE(("wsum2" + s).c_str(), n * 4);
     }
-    ENSURE(("cost" + s).c_str(), n * Dp * 2);
+    if (!recompute_cost(ctx, p, w, h)) ENSURE(("cost" + s).c_str(), n * Dp * 2);
     ENSURE(("sum" + s).c_str(), n * Dp * sb);
     ENSURE(("disp" + s).c_str(), n * 4);
     B.gray[d] = (uint8_t *)ctx->buffers["gray" + s].ptr;
@@ -543,7 +591,7 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     B.desc[d] = (uint64_t *)ctx->buffers["desc" + s].ptr;
     B.wsum[d] = zncc ? (int32_t *)ctx->buffers["wsum" + s].ptr : nullptr;
     B.wsum2[d] = zncc ? (int32_t *)ctx->buffers["wsum2" + s].ptr : nullptr;
-    B.cost[d] = (uint16_t *)ctx->buffers["cost" + s].ptr;
+    B.cost[d] = recompute_cost(ctx, p, w, h) ? nullptr : (uint16_t *)ctx->buffers["cost" + s].ptr;
     B.sum[d] = ctx->buffers["sum" + s].ptr;
     B.disp[d] = (float *)ctx->buffers["disp" + s].ptr;
   }
@@ -574,9 +622,10 @@ int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuff
   int rc;
   const int D = p->disparities, Dp = padded_disparities(D), R = path_lanes(p);
   const int sb = aggregate_bytes(p);
-  const bool fused = fused_cost(p);
-  const int mode = sb == 2 ? 1 : 0;
-  if (!fused) {
+  const bool recompute = recompute_cost(ctx, p, w, h);
+  const bool fused = !recompute && fused_cost(p);
+  const int mode = recompute ? 3 : sb == 2 ? 1 : 0;
+  if (!fused && !recompute) {
     CostJobs cj;
     memset(&cj, 0, sizeof cj);
     for (int i = 0; i < nd; i++) {

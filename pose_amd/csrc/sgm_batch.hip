@@ -33,14 +33,15 @@ namespace sgm {
 
 int batch_lines_per_item(sgm_context *ctx) { return ctx->batch_lines_per_item; }
 
-int64_t batch_pair_bytes(const sgm_params *p, int w, int h, int D, bool host_io) {
-  sgm_params q = *p;
-  q.disparities = D;
-  const int64_t n = (int64_t)w * h;
-  // two directions (volumes, descriptors, masks, disparities) + scratch
-  int64_t b = 2 * direction_bytes(&q, w, h) + n * 4;
-  if (host_io) b += n * (2 + 2 + 4 + 2);
-  return b + 4096;
+// The persistent kernel instance of a batch.
+const void *batch_kernel_of(const sgm_batch *b) {
+  if (b->sum_bytes == 2) {
+    if (b->recompute) return (const void *)&batch_kernel<uint16_t, 1, true>;
+    if (b->words == 1) return (const void *)&batch_kernel<uint16_t, 1, false>;
+    return (const void *)&batch_kernel<uint16_t, 2, false>;
+  }
+  if (b->words == 1) return (const void *)&batch_kernel<int32_t, 1, false>;
+  return (const void *)&batch_kernel<int32_t, 2, false>;
 }
 
 }  // namespace sgm
@@ -91,14 +92,9 @@ static int batch_enqueue(sgm_batch *b) {
   const unsigned blocks = (unsigned)b->blocks;
   {
     ProfileScope ps(ctx, "batch");
-    if (b->sum_bytes == 2) {
-      if (b->words == 1) batch_kernel<uint16_t, 1><<<blocks, threads, 0, st>>>(b->args);
-      else batch_kernel<uint16_t, 2><<<blocks, threads, 0, st>>>(b->args);
-    } else {
-      if (b->words == 1) batch_kernel<int32_t, 1><<<blocks, threads, 0, st>>>(b->args);
-      else batch_kernel<int32_t, 2><<<blocks, threads, 0, st>>>(b->args);
-    }
-    hipError_t e = hipGetLastError();
+    void *kargs[] = {(void *)&b->args};
+    hipError_t e = hipLaunchKernel(batch_kernel_of(b), dim3(blocks), dim3(threads), kargs, 0, st);
+    if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
   }
   if (!(b->flags & SGM_BATCH_DEVICE)) {
@@ -127,12 +123,20 @@ extern "C" int sgm_batch_create(sgm_context *ctx, const sgm_params *params, cons
   *out = nullptr;
   std::lock_guard<std::mutex> g(ctx->lock);
   (void)hipSetDevice(ctx->device);
-  return batch_create(ctx, params, pairs, count, flags, nullptr, 0, ctx->stream, out);
+  return batch_create(ctx, params, pairs, count, flags, nullptr, 0, ctx->stream, out, -1);
 }
 
 namespace sgm {
 
-size_t batch_arena_bytes(const sgm_params *params, const sgm_pair *pairs, int count, int flags) {
+bool batch_recompute_possible(const sgm_params *params, bool keep_volumes) {
+  // every pair of a batch has R <= 8 (validate_pair): 1-word census with
+  // uint16 aggregates can recompute its cost in the sweeps
+  return !keep_volumes && params->cost == SGM_COST_CENSUS && words_of(params) == 1 &&
+         aggregate_bytes(params) == 2;
+}
+
+size_t batch_arena_bytes(const sgm_params *params, const sgm_pair *pairs, int count, int flags,
+                         bool recompute) {
   // must mirror the carving in batch_create
   const bool host_io = !(flags & SGM_BATCH_DEVICE);
   const int words = words_of(params), sb = aggregate_bytes(params);
@@ -156,7 +160,7 @@ size_t batch_arena_bytes(const sgm_params *params, const sgm_pair *pairs, int co
         cv.take(n * 4);
         cv.take(n * 4);
       }
-      cv.take(n * Dp * 2);
+      if (!recompute) cv.take(n * Dp * 2);
       cv.take(n * Dp * sb);
       const bool own_disp = host_io || (d == 0 ? pr.dl == nullptr : pr.dr == nullptr);
       if (own_disp) cv.take(n * 4);
@@ -187,7 +191,7 @@ size_t batch_table_bytes(int count, const sgm_params *params) {
 // that caller-owned device region (serving pool) instead of being allocated.
 int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pairs, int count,
                  int flags, void *region, size_t region_bytes, hipStream_t stream,
-                 sgm_batch **out) {
+                 sgm_batch **out, int recompute) {
   for (int i = 0; i < count; i++) {
     int rc = validate_pair(params, pairs[i], flags);
     if (rc) return rc;
@@ -202,11 +206,41 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
   b->words = words_of(params);
   b->sum_bytes = aggregate_bytes(params);
   const bool zncc = params->cost == SGM_COST_ZNCC || params->cost == SGM_COST_ZNCC_FP32;
-  // every pair of the batch has R <= 8, so the fused cost needs only the mode and width
-  b->fused = params->cost == SGM_COST_CENSUS && b->words == 1 && b->sum_bytes == 2;
+  // Census cost: recomputed in the sweeps (no cost volume) when forced by the
+  // policy or when the batch with resident cost volumes does not fit the
+  // memory at hand; otherwise fused into the first sweep and kept resident.
+  const bool possible = batch_recompute_possible(params, ctx->keep_volumes);
+  if (recompute < 0) {
+    if (!possible || ctx->volume_policy == SGM_VOLUME_RESIDENT) {
+      recompute = 0;
+    } else if (ctx->volume_policy == SGM_VOLUME_RECOMPUTE) {
+      recompute = 1;
+    } else {
+      const int64_t avail = region ? (int64_t)region_bytes : ctx->budget - ctx->allocated;
+      const int64_t need = (int64_t)(batch_arena_bytes(params, pairs, count, flags, false) +
+                                     batch_table_bytes(count, params));
+      recompute = need > avail ? 1 : 0;
+    }
+  }
+  b->recompute = possible && recompute != 0;
+  b->fused = !b->recompute && params->cost == SGM_COST_CENSUS && b->words == 1 &&
+             b->sum_bytes == 2;
   b->waves_per_block = ctx->batch_waves_per_block;
   b->lines_per_item = ctx->batch_lines_per_item;
-  b->blocks = ctx->info.compute_units * ctx->batch_blocks_per_cu;
+  {
+    // workgroups per CU: as many as stay resident (occupancy of the chosen
+    // instance) unless the tuned shape asks for fewer
+    int resident = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, batch_kernel_of(b),
+                                                     b->waves_per_block * 64, 0) != hipSuccess ||
+        resident < 1) {
+      (void)hipGetLastError();
+      resident = 1;
+    }
+    const int per_cu = ctx->batch_blocks_per_cu > 0 ? std::min(ctx->batch_blocks_per_cu, resident)
+                                                    : resident;
+    b->blocks = ctx->info.compute_units * per_cu;
+  }
 
   // ---- arena layout
   Carver cv;
@@ -234,7 +268,7 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
       o.gm[d] = cv.take(n * 2);
       o.wsum[d] = zncc ? cv.take(n * 4) : 0;
       o.wsum2[d] = zncc ? cv.take(n * 4) : 0;
-      o.cost[d] = cv.take(n * Dp * 2);
+      o.cost[d] = b->recompute ? 0 : cv.take(n * Dp * 2);
       o.sum[d] = cv.take(n * Dp * b->sum_bytes);
       const bool own_disp = host_io || (d == 0 ? pr.dl == nullptr : pr.dr == nullptr);
       o.disp[d] = own_disp ? cv.take(n * 4) : 0;
@@ -317,7 +351,7 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
       const int od = 1 - d;
       PathJob &J = jobs[2 * i + d];
       memset(&J, 0, sizeof J);
-      J.cost = (const uint16_t *)P8(o.cost[d]);
+      J.cost = b->recompute ? nullptr : (const uint16_t *)P8(o.cost[d]);
       J.sum = P8(o.sum[d]);
       J.mask = P8(o.mask[d]);
       J.gray = s.gray[d];
@@ -325,7 +359,7 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
       J.desc_a = (const uint64_t *)P8(o.desc[d]);
       J.desc_b = (const uint64_t *)P8(o.desc[od]);
       J.mask_b = P8(o.mask[od]);
-      J.cost_out = (uint16_t *)P8(o.cost[d]);
+      J.cost_out = b->recompute ? nullptr : (uint16_t *)P8(o.cost[d]);
       J.disp = s.disp[d];
       J.reasons = d == 0 ? P8(o.rtmp) : nullptr;
       J.boundary = d == 0 ? P8(o.btmp) : nullptr;
@@ -351,7 +385,7 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
       C.sum2_a = images[2 * i + d].wsum2;
       C.sum_b = images[2 * i + od].wsum;
       C.sum2_b = images[2 * i + od].wsum2;
-      C.cost = (uint16_t *)P8(o.cost[d]);
+      C.cost = b->recompute ? nullptr : (uint16_t *)P8(o.cost[d]);
       C.w = w;
       C.h = h;
       C.D = D;
@@ -384,7 +418,7 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
       const int w = pr.width, h = pr.height;
       int items;
       if (ph == BPH_CENSUS) items = (h + census_rows - 1) / census_rows;
-      else if (ph == BPH_COST) items = b->fused ? 0 : (h + cost_rows - 1) / cost_rows;
+      else if (ph == BPH_COST) items = (b->fused || b->recompute) ? 0 : (h + cost_rows - 1) / cost_rows;
       else if (ph == nph - 1) items = (h + lr_rows - 1) / lr_rows;
       else {
         static const int dirs[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
@@ -442,7 +476,8 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
   A.stride = stride;
   A.npairs = count;
   A.directions = params->directions;
-  A.fused = b->fused ? 1 : 0;
+  A.fused = (b->fused || b->recompute) ? 1 : 0;  // first sweep waits on census, not on a cost phase
+  A.cmode = b->recompute ? CM_RECOMPUTE : b->fused ? CM_FUSED : CM_LOAD;
   A.lines_per_item = b->lines_per_item;
   A.census_rows = census_rows;
   A.cost_rows = cost_rows;
@@ -578,4 +613,19 @@ extern "C" int sgm_batch_destroy(sgm_batch *b) {
 
 extern "C" int64_t sgm_batch_device_bytes(sgm_batch *b) {
   return b ? (int64_t)(b->arena_bytes + b->tables_bytes) : 0;
+}
+
+extern "C" int sgm_batch_describe(sgm_batch *b, char *buffer, size_t length) {
+  if (!b || !buffer || length == 0) return fail(SGM_ERR_INVALID_ARGUMENT, "null argument");
+  const int n = snprintf(buffer, length,
+                         "{\"pairs\":%zu,\"work_items\":%d,\"workgroups\":%d,"
+                         "\"waves_per_workgroup\":%d,\"lines_per_item\":%d,"
+                         "\"cost_volume\":\"%s\",\"device_bytes\":%lld,\"graph\":%s}",
+                         b->pairs.size(), b->total_items, b->blocks, b->waves_per_block,
+                         b->lines_per_item,
+                         b->recompute ? "recomputed" : b->fused ? "resident-fused" : "resident",
+                         (long long)(b->arena_bytes + b->tables_bytes),
+                         b->graph_exec ? "true" : "false");
+  if (n < 0 || (size_t)n >= length) return fail(SGM_ERR_INVALID_ARGUMENT, "buffer too small");
+  return SGM_OK;
 }

@@ -17,12 +17,16 @@ struct PairSlots {
   uint8_t *reasons = nullptr, *boundary = nullptr;
 };
 
-int64_t batch_pair_bytes(const sgm_params *p, int w, int h, int D, bool host_io);
-size_t batch_arena_bytes(const sgm_params *params, const sgm_pair *pairs, int count, int flags);
+bool batch_recompute_possible(const sgm_params *params, bool keep_volumes);
+size_t batch_arena_bytes(const sgm_params *params, const sgm_pair *pairs, int count, int flags,
+                         bool recompute);
 size_t batch_table_bytes(int count, const sgm_params *params);
+// recompute: 1 census cost recomputed in the sweeps (no cost volume), 0 resident
+// cost volume, -1 chosen by the context's policy and the memory at hand.
 int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pairs, int count,
                  int flags, void *region, size_t region_bytes, hipStream_t stream,
-                 sgm_batch **out);
+                 sgm_batch **out, int recompute);
+const void *batch_kernel_of(const sgm_batch *b);
 int batch_launch(sgm_batch *b);
 int batch_finish(sgm_batch *b);
 
@@ -38,6 +42,7 @@ struct sgm_batch {
   std::vector<sgm::PairSlots> slots;
   int words = 1, sum_bytes = 2;
   bool fused = false;
+  bool recompute = false;
   int waves_per_block = 4, lines_per_item = 1, blocks = 0;
   void *arena = nullptr;
   size_t arena_bytes = 0;

@@ -50,6 +50,7 @@ struct BatchArgs {
   unsigned *done;          // [nph][stride] completed items per job
   int *error;
   int nph, stride, npairs, directions, fused;
+  int cmode;               // 0 cost volume, 1 fused first sweep, 2 recomputed census cost
   int lines_per_item, census_rows, cost_rows, lr_rows;
   PathArgs pa;             // penalties, codes, uniqueness (direction fields set per item)
   CostArgs ca;
@@ -145,9 +146,9 @@ __device__ __forceinline__ void batch_census(const BatchArgs &B, const BatchImag
             bit++;
           }
       }
-      I.desc[p] = d0;
-      if (WORDS > 1) I.desc[n + p] = d1;
       const uint8_t m = interior && (hi - lo) >= 4 && (I.valid == nullptr || I.valid[p] != 0);
+      I.desc[p] = WORDS == 1 ? (d0 | ((uint64_t)m << 63)) : d0;  // bit 63: texture & valid
+      if (WORDS > 1) I.desc[n + p] = d1;
       I.mask[p] = m;
       I.gm[p] = (uint16_t)(c | (m << 8));
       if (I.wsum) {
@@ -250,15 +251,27 @@ __device__ __forceinline__ void batch_lr(const BatchArgs &B, const BatchPairDev 
   }
 }
 
-template <int R, typename SumT>
+// kind: 0 first / 1 first with fused census cost / 2 middle / 3 last (cost
+// volume), 4 first / 5 middle / 6 last with the census cost recomputed.  The
+// recomputing sweeps need far more registers (target descriptor prefetch), so
+// they live in their own kernel instance (RECOMP) and do not lower the
+// occupancy of the cost-volume instance.
+template <int R, typename SumT, bool RECOMP>
 __device__ __forceinline__ void batch_sweep_r(const PathJob &J, const PathArgs &A, int line,
                                               int kind, const int *p2_table) {
-  // kind: 0 first, 1 first with fused census cost, 2 middle, 3 last
-  if constexpr (sizeof(SumT) == 2) {
-    if (kind == 0) sweep_line_pk<R, PHASE_FIRST, false>(J, A, line, p2_table);
-    else if (kind == 1) sweep_line_pk<R, PHASE_FIRST, true>(J, A, line, p2_table);
-    else if (kind == 2) sweep_line_pk<R, PHASE_MID, false>(J, A, line, p2_table);
-    else sweep_line_pk<R, PHASE_LAST, false>(J, A, line, p2_table);
+  if constexpr (sizeof(SumT) == 2 && RECOMP) {
+    switch (kind) {
+      case 4: sweep_line_pk<R, PHASE_FIRST, CM_RECOMPUTE>(J, A, line, p2_table); break;
+      case 5: sweep_line_pk<R, PHASE_MID, CM_RECOMPUTE>(J, A, line, p2_table); break;
+      default: sweep_line_pk<R, PHASE_LAST, CM_RECOMPUTE>(J, A, line, p2_table); break;
+    }
+  } else if constexpr (sizeof(SumT) == 2) {
+    switch (kind) {
+      case 0: sweep_line_pk<R, PHASE_FIRST, CM_LOAD>(J, A, line, p2_table); break;
+      case 1: sweep_line_pk<R, PHASE_FIRST, CM_FUSED>(J, A, line, p2_table); break;
+      case 2: sweep_line_pk<R, PHASE_MID, CM_LOAD>(J, A, line, p2_table); break;
+      default: sweep_line_pk<R, PHASE_LAST, CM_LOAD>(J, A, line, p2_table); break;
+    }
   } else {
     if (kind <= 1) sweep_line<R, SumT, PHASE_FIRST>(J, A, line, p2_table);
     else if (kind == 2) sweep_line<R, SumT, PHASE_MID>(J, A, line, p2_table);
@@ -266,25 +279,34 @@ __device__ __forceinline__ void batch_sweep_r(const PathJob &J, const PathArgs &
   }
 }
 
-template <typename SumT>
+template <typename SumT, bool RECOMP>
 __device__ __forceinline__ void batch_sweep(const PathJob &J, const PathArgs &A, int line, int kind,
                                             const int *p2_table) {
   if constexpr (sizeof(SumT) == 2) {
-    if (J.R == 2) batch_sweep_r<2, SumT>(J, A, line, kind, p2_table);
-    else if (J.R == 4) batch_sweep_r<4, SumT>(J, A, line, kind, p2_table);
-    else batch_sweep_r<8, SumT>(J, A, line, kind, p2_table);
+    if (J.R == 2) batch_sweep_r<2, SumT, RECOMP>(J, A, line, kind, p2_table);
+    else if (J.R == 4) batch_sweep_r<4, SumT, RECOMP>(J, A, line, kind, p2_table);
+    else batch_sweep_r<8, SumT, RECOMP>(J, A, line, kind, p2_table);
   } else {
-    if (J.R == 1) batch_sweep_r<1, SumT>(J, A, line, kind, p2_table);
-    else if (J.R == 2) batch_sweep_r<2, SumT>(J, A, line, kind, p2_table);
-    else if (J.R == 4) batch_sweep_r<4, SumT>(J, A, line, kind, p2_table);
-    else batch_sweep_r<8, SumT>(J, A, line, kind, p2_table);
+    if (J.R == 1) batch_sweep_r<1, SumT, false>(J, A, line, kind, p2_table);
+    else if (J.R == 2) batch_sweep_r<2, SumT, false>(J, A, line, kind, p2_table);
+    else if (J.R == 4) batch_sweep_r<4, SumT, false>(J, A, line, kind, p2_table);
+    else batch_sweep_r<8, SumT, false>(J, A, line, kind, p2_table);
   }
 }
 
 constexpr int kBatchMaxWaves = 8;
 
-template <typename SumT, int WORDS>
-__global__ __launch_bounds__(512) void batch_kernel(BatchArgs B) {
+// Waves per SIMD the register allocation must allow: the cost-volume uint16
+// instance fits 128 VGPRs (4 waves per SIMD) without spilling in its sweeps;
+// the recomputing and 32-bit instances need more and run at 2.
+template <typename SumT, bool RECOMP>
+constexpr int batch_min_waves() {
+  return (sizeof(SumT) == 2 && !RECOMP) ? 4 : 2;
+}
+
+template <typename SumT, int WORDS, bool RECOMP>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(batch_min_waves<SumT, RECOMP>())))
+void batch_kernel(BatchArgs B) {
   __shared__ int p2_table[256];
   __shared__ uint8_t tiles[kBatchMaxWaves][kBatchTileH * kBatchTileW];
   build_p2_table(p2_table, B.pa);
@@ -350,10 +372,11 @@ __global__ __launch_bounds__(512) void batch_kernel(BatchArgs B) {
       A.dx = dirs[dir][0];
       A.dy = dirs[dir][1];
       A.cls = A.dy == 0 ? 0 : A.dx == 0 ? 1 : 2;
-      const int kind = dir == 0 ? (B.fused ? 1 : 0) : dir == B.directions - 1 ? 3 : 2;
+      int kind = dir == 0 ? (B.fused ? 1 : 0) : dir == B.directions - 1 ? 3 : 2;
+      if (B.cmode == CM_RECOMPUTE) kind = dir == 0 ? 4 : dir == B.directions - 1 ? 6 : 5;
       const int nlines = lines_of_dev(A.cls, J.w, J.h);
       const int l0 = tile_idx * B.lines_per_item, l1 = min(nlines, l0 + B.lines_per_item);
-      for (int l = l0; l < l1; l++) batch_sweep<SumT>(J, A, l, kind, p2_table);
+      for (int l = l0; l < l1; l++) batch_sweep<SumT, RECOMP>(J, A, l, kind, p2_table);
     } else {
       const BatchPairDev P = B.pairs[job];
       const int r0 = tile_idx * B.lr_rows, r1 = min(P.h, r0 + B.lr_rows);

@@ -39,7 +39,11 @@ int padded_disparities(int D);
 int words_of(const sgm_params *p);
 int invalid_code(const sgm_params *p);
 int64_t direction_bytes(const sgm_params *p, int w, int h);
+int64_t direction_bytes(const sgm_params *p, int w, int h, bool cost_volume);
+bool census_recompute_possible(const sgm_params *p);
+bool recompute_cost(sgm_context *ctx, const sgm_params *p, int w, int h);
 int64_t pair_bytes(const sgm_params *p, int w, int h);
+int64_t pair_bytes(const sgm_params *p, int w, int h, bool cost_volume);
 int lines_of(int cls, int w, int h);
 void collect_profile(sgm_context *ctx);
 
@@ -77,15 +81,16 @@ struct sgm_context {
   std::mutex lock;
   std::map<std::string, sgm::Buffer> buffers;
   bool keep_volumes = false;
+  int volume_policy = 0;  // SGM_VOLUME_*
   bool profiling = false;
   int fail_allocations = 0;
   std::vector<sgm::PendingEvent> pending_events;
   std::map<std::string, std::pair<double, long long>> kernel_stats;
   std::map<int, int> tuned_waves;   // lane group R -> waves per workgroup in path sweeps
   // batched mode launch shape (autotuned)
-  int batch_waves_per_block = 4;
+  int batch_waves_per_block = 8;
   int batch_lines_per_item = 2;
-  int batch_blocks_per_cu = 2;
+  int batch_blocks_per_cu = 0;  // 0: as many as stay resident
   int last_w = 0, last_h = 0;
   sgm_params last_params{};
 };

@@ -128,11 +128,13 @@ __global__ __launch_bounds__(256) void census_kernel(CensusJobs jobs, int ww, in
         bit++;
       }
   }
-  J.desc[p] = d0;
-  if (WORDS > 1) J.desc[n + p] = d1;
   uint8_t m;
   if (J.mask_in) m = J.mask_in[p] != 0;
   else m = interior && (hi - lo) >= 4 && (J.valid == nullptr || J.valid[p] != 0);
+  // One-word descriptors use at most 62 bits; bit 63 carries texture&valid so a
+  // sweep can test a target's usability from its descriptor alone.
+  J.desc[p] = WORDS == 1 ? (d0 | ((uint64_t)m << 63)) : d0;
+  if (WORDS > 1) J.desc[n + p] = d1;
   J.mask[p] = m;
   if (J.gm) J.gm[p] = (uint16_t)(tile[threadIdx.y + ry][threadIdx.x + rx] | (m << 8));
   if (J.wsum) {

@@ -550,40 +550,67 @@ __device__ __forceinline__ uint64_t fetch_target(const PathJob &J, __amdgpu_buff
   return (ok && m) ? (dd | 0x8000000000000000ull) : 0ull;
 }
 
+// Cost modes of the packed sweep.
+//   CM_LOAD:      read the cost volume.
+//   CM_FUSED:     compute the census cost (first, left-to-right sweep) and
+//                 write the cost volume for the later sweeps.
+//   CM_RECOMPUTE: compute the census cost in every sweep from descriptors
+//                 whose bit 63 carries the texture&valid flag; no cost volume.
+//                 Horizontal lines slide a DPP-shifted descriptor window,
+//                 other lines load each pixel's target row segment (L2-resident).
+enum { CM_LOAD = 0, CM_FUSED = 1, CM_RECOMPUTE = 2 };
+
 // Per-step metadata of a window of PF steps, one step per lane (lane i holds
 // step base + (i % PF)): the edge-adaptive P2 of the step, or -1 when the
 // predecessor is outside the image or untextured (no transition term).
 struct Meta {
   int p2;       // >= 0: P2 to use; -1: L = C
-  int ok;       // fused cost: reference pixel interior and textured
-  uint32_t da_lo, da_hi;  // fused cost: reference descriptor
-  uint32_t e_lo, e_hi;    // fused cost: target descriptor entering after this step
+  int ok;       // computed cost: reference pixel interior and textured
+  uint32_t da_lo, da_hi;  // computed cost: reference descriptor
+  uint32_t e_lo, e_hi;    // sliding window: target descriptor entering after this step
 };
 
-// Buffer resources of one line: gray|mask of the reference, and for the
-// fused sweep the reference descriptors and the target row.
+// Buffer resources of one line: gray|mask of the reference, the reference
+// descriptors and the target row (computed costs).
 struct LineRsrc {
   __amdgpu_buffer_rsrc_t gm, da, tdesc, tmask;
 };
 
-template <int PF, bool FUSED>
-__device__ __forceinline__ void meta_load(const PathJob &J, const LineRsrc &rs, int base, int len,
-                                          ptrdiff_t p0, ptrdiff_t step_off, int x0, int &g_raw,
-                                          int &gp_raw, uint64_t &da, uint64_t &e) {
+__device__ __forceinline__ uint64_t ld64(__amdgpu_buffer_rsrc_t r, int off) {
+  const v2u32 d = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+  return ((uint64_t)d.y << 32) | d.x;
+}
+
+// Target entering the sliding window after step s of a horizontal line.
+__device__ __forceinline__ int slide_target(const PathJob &J, int x0, int dx, int s, int R) {
+  const int xn = x0 + (s + 1) * dx;
+  const bool up = (dx > 0) != (J.reverse != 0);
+  if (up) return J.reverse ? xn + J.minD : xn - J.minD;
+  return J.reverse ? xn + J.minD + 64 * R - 1 : xn - J.minD - (64 * R - 1);
+}
+
+template <int PF, int CM>
+__device__ __forceinline__ void meta_load(const PathJob &J, const PathArgs &A, const LineRsrc &rs,
+                                          int base, int len, ptrdiff_t p0, ptrdiff_t step_off,
+                                          int x0, int &g_raw, int &gp_raw, uint64_t &da,
+                                          uint64_t &e) {
   const int lane = threadIdx.x & 63;
   const int s = min(base + (lane % PF), len - 1);
   const int sp = max(s - 1, 0);
-  g_raw = __builtin_amdgcn_raw_buffer_load_b16(rs.gm, (int)((p0 + (ptrdiff_t)s * step_off) * 2), 0, 0);
+  const ptrdiff_t ps = p0 + (ptrdiff_t)s * step_off;
+  g_raw = __builtin_amdgcn_raw_buffer_load_b16(rs.gm, (int)(ps * 2), 0, 0);
   gp_raw = __builtin_amdgcn_raw_buffer_load_b16(rs.gm, (int)((p0 + (ptrdiff_t)sp * step_off) * 2), 0, 0);
-  if constexpr (FUSED) {
-    const v2u32 d = __builtin_amdgcn_raw_buffer_load_b64(rs.da, (int)((p0 + s) * 8), 0, 0);
-    da = ((uint64_t)d.y << 32) | d.x;
+  if constexpr (CM == CM_FUSED) {
+    da = ld64(rs.da, (int)(ps * 8));
     const int t = J.reverse ? x0 + s + 1 + J.minD + 64 * J.R - 1 : x0 + s + 1 - J.minD;
     e = fetch_target(J, rs.tdesc, rs.tmask, t);
+  } else if constexpr (CM == CM_RECOMPUTE) {
+    da = ld64(rs.da, (int)(ps * 8));
+    if (A.dy == 0) e = ld64(rs.tdesc, slide_target(J, x0, A.dx, s, J.R) * 8);  // dropped if outside
   }
 }
 
-template <int PF, bool FUSED>
+template <int PF, int CM>
 __device__ __forceinline__ Meta meta_finish(const PathJob &J, const PathArgs &A, int base,
                                             int len, int x0, int y0, int g_raw, int gp_raw,
                                             uint64_t da, uint64_t e, const int *p2_table) {
@@ -593,9 +620,9 @@ __device__ __forceinline__ Meta meta_finish(const PathJob &J, const PathArgs &A,
   const int g = g_raw & 0xFF, gp = gp_raw & 0xFF;
   const bool has = s > 0 && (gp_raw >> 8) != 0;
   m.p2 = has ? p2_table[abs(g - gp)] : -1;
-  if constexpr (FUSED) {
-    const int x = x0 + s;  // the fused sweep runs along +x
-    m.ok = (y0 >= J.ry && y0 < J.h - J.ry && x >= J.rx && x < J.w - J.rx && (g_raw >> 8) != 0);
+  if constexpr (CM != CM_LOAD) {
+    const int x = x0 + s * A.dx, y = y0 + s * A.dy;
+    m.ok = (y >= J.ry && y < J.h - J.ry && x >= J.rx && x < J.w - J.rx && (g_raw >> 8) != 0);
     m.da_lo = (uint32_t)da;
     m.da_hi = (uint32_t)(da >> 32);
     m.e_lo = (uint32_t)e;
@@ -644,21 +671,42 @@ __device__ __forceinline__ Winner wta_reduce_pk(const int *S, const int *valid, 
   return r;
 }
 
-template <int R, int PHASE, bool FUSED>
+// Census Hamming costs of one lane's R disparities from flagged target
+// descriptors (StereoCost.value mode 0 with the validity test of the
+// reference `costs` kernel, GpuStereo.java:706-719).
+template <int R>
+__device__ __forceinline__ void census_costs(const uint64_t *tw, uint32_t alo, uint32_t ahi,
+                                             bool ok, int kbase, int D, int inv, uint32_t *c) {
+  int cv[R];
+#pragma unroll
+  for (int j = 0; j < R; j++) {
+    const uint32_t wl = (uint32_t)tw[j], wh = (uint32_t)(tw[j] >> 32);
+    const int ham = __builtin_popcount(wl ^ alo) + __builtin_popcount((wh ^ ahi) & 0x7FFFFFFFu);
+    cv[j] = (ok && (wh >> 31) && kbase + j < D) ? ham : inv;
+  }
+#pragma unroll
+  for (int i = 0; i < R / 2; i++) c[i] = (uint32_t)cv[2 * i] | ((uint32_t)cv[2 * i + 1] << 16);
+}
+
+template <int R, int PHASE, int CM>
 struct PkSweep {
   static constexpr int NP = R / 2;
-  static constexpr int PF = pk_prefetch_depth<R>();
+  static constexpr int PF = CM == CM_FUSED ? 1 : pk_prefetch_depth<R>();
+  static constexpr bool kComputed = CM != CM_LOAD;
+  static constexpr int kDirect = CM == CM_RECOMPUTE ? PF : 1;  // direct target-row prefetch slots
 
   const PathJob &J;
   const PathArgs &A;
   const int *p2_table;
   int lane, kbase, loff, x0, y0, len;
+  bool slide;  // computed cost on a horizontal line: sliding window
   ptrdiff_t p0, step_off;
   uint32_t row_bytes;
   uint32_t keep[NP], infm[NP];
   uint32_t vmask, p1pk;
-  uint32_t cb[PF][NP], sb[PF][NP];
-  uint64_t win[FUSED ? R : 1];
+  uint32_t cb[CM == CM_LOAD ? PF : 1][NP], sb[PF][NP];
+  uint64_t win[kComputed ? R : 1];
+  uint64_t dsc[kDirect][CM == CM_RECOMPUTE ? R : 1];
   uint32_t prev[NP];
   int minp;
   LineRsrc rs;
@@ -677,6 +725,7 @@ struct PkSweep {
     p0 = (ptrdiff_t)y0 * J.w + x0;
     row_bytes = (uint32_t)J.Dp * 2u;
     sum_store_bytes = (PHASE != PHASE_LAST || A.keep_sum) ? (int)row_bytes : 0;
+    slide = CM == CM_FUSED || (CM == CM_RECOMPUTE && A.dy == 0);
 #pragma unroll
     for (int i = 0; i < NP; i++) {
       const bool lo = kbase + 2 * i < J.D, hi = kbase + 2 * i + 1 < J.D;
@@ -684,13 +733,17 @@ struct PkSweep {
       infm[i] = (lo ? 0u : 0x7FFFu) | (hi ? 0u : 0x7FFF0000u);
       prev[i] = kInfPk;
     }
+#pragma unroll
+    for (int j = 0; j < PF; j++)
+#pragma unroll
+      for (int i = 0; i < NP; i++) sb[j][i] = 0;
     vmask = A.val_mask == 0xFFFF ? 0xFFFFFFFFu : splat16((uint32_t)A.val_mask);
     p1pk = splat16((uint32_t)A.p1);
     minp = 0;
     dfr = Deferred{};
     const uint32_t n = (uint32_t)J.w * (uint32_t)J.h;
     rs.gm = make_rsrc(J.gm, n * 2u);
-    if constexpr (FUSED) {
+    if constexpr (kComputed) {
       rs.da = make_rsrc(J.desc_a, n * 8u);
       rs.tdesc = make_rsrc(J.desc_b + (size_t)y0 * J.w, (uint32_t)J.w * 8u);
       rs.tmask = make_rsrc(J.mask_b + (size_t)y0 * J.w, (uint32_t)J.w);
@@ -701,23 +754,62 @@ struct PkSweep {
 
   __device__ __forceinline__ ptrdiff_t pix(int s) const { return p0 + (ptrdiff_t)s * step_off; }
 
-  __device__ __forceinline__ void load_step(int slot, int s) {
-    const ptrdiff_t pp = pix(min(s, len - 1));
-    if constexpr (!FUSED) {
-      bload_pairs<NP>(cb[slot], make_rsrc(J.cost + pp * J.Dp, row_bytes), loff);
-      if constexpr (PHASE != PHASE_FIRST)
-        bload_pairs<NP>(sb[slot], make_rsrc((const uint16_t *)J.sum + pp * J.Dp, row_bytes), loff);
+  // Target row segment of one pixel step (direct mode): R flagged descriptors.
+  __device__ __forceinline__ void load_targets(uint64_t *dst, int s) {
+    const int sc = min(s, len - 1);
+    const int px = x0 + sc * A.dx, py = y0 + sc * A.dy;
+    const __amdgpu_buffer_rsrc_t r = make_rsrc(J.desc_b + (size_t)py * J.w, (uint32_t)J.w * 8u);
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+      const int t = J.reverse ? px + J.minD + kbase + j : px - J.minD - kbase - j;
+      dst[j] = ld64(r, t * 8);  // outside the row: dropped (0 = unusable target)
     }
   }
 
+  __device__ __forceinline__ void load_step(int slot, int s) {
+    const ptrdiff_t pp = pix(min(s, len - 1));
+    if constexpr (CM == CM_LOAD)
+      bload_pairs<NP>(cb[slot], make_rsrc(J.cost + pp * J.Dp, row_bytes), loff);
+    if constexpr (CM == CM_RECOMPUTE)
+      if (!slide) load_targets(dsc[slot], s);
+    if constexpr (PHASE != PHASE_FIRST)
+      bload_pairs<NP>(sb[slot], make_rsrc((const uint16_t *)J.sum + pp * J.Dp, row_bytes), loff);
+  }
+
   __device__ __forceinline__ void init_window() {
-    if constexpr (FUSED) {
+    if constexpr (CM == CM_FUSED) {
 #pragma unroll
       for (int j = 0; j < R; j++) {
         const int k = kbase + j;
         const int t = J.reverse ? x0 + J.minD + k : x0 - J.minD - k;
         win[j] = fetch_target(J, rs.tdesc, rs.tmask, t);
       }
+    } else if constexpr (CM == CM_RECOMPUTE) {
+      if (slide) {
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+          const int k = kbase + j;
+          const int t = J.reverse ? x0 + J.minD + k : x0 - J.minD - k;
+          win[j] = ld64(rs.tdesc, t * 8);
+        }
+      }
+    }
+  }
+
+  __device__ __forceinline__ void slide_window(uint32_t elo, uint32_t ehi) {
+    const bool up = CM == CM_FUSED ? !J.reverse : ((A.dx > 0) != (J.reverse != 0));
+    if (!up) {
+      const uint32_t ilo = (uint32_t)dpp_shl1((int)(uint32_t)win[0], (int)elo);
+      const uint32_t ihi = (uint32_t)dpp_shl1((int)(uint32_t)(win[0] >> 32), (int)ehi);
+#pragma unroll
+      for (int j = 0; j < R - 1; j++) win[j] = win[j + 1];
+      win[R - 1] = ((uint64_t)ihi << 32) | ilo;
+    } else {
+      const uint32_t ilo = (uint32_t)dpp_shr1((int)(uint32_t)win[R - 1], (int)elo);
+      const uint32_t ihi = (uint32_t)dpp_shr1((int)(uint32_t)(win[R - 1] >> 32), (int)ehi);
+#pragma unroll
+      for (int j = R - 1; j > 0; j--) win[j] = win[j - 1];
+      win[0] = ((uint64_t)ihi << 32) | ilo;
     }
   }
 
@@ -727,35 +819,21 @@ struct PkSweep {
     const ptrdiff_t pp = pix(s);
     const int p2h = __builtin_amdgcn_readlane(meta.p2, ml);
     uint32_t c[NP];
-    if constexpr (FUSED) {
+    if constexpr (kComputed) {
       const uint32_t alo = (uint32_t)__builtin_amdgcn_readlane((int)meta.da_lo, ml);
       const uint32_t ahi = (uint32_t)__builtin_amdgcn_readlane((int)meta.da_hi, ml);
       const bool ok = __builtin_amdgcn_readlane(meta.ok, ml) != 0;
-      int cv[R];
-#pragma unroll
-      for (int j = 0; j < R; j++) {
-        const uint32_t wl = (uint32_t)win[j], wh = (uint32_t)(win[j] >> 32);
-        const int ham = __builtin_popcount(wl ^ alo) + __builtin_popcount((wh ^ ahi) & 0x7FFFFFFFu);
-        cv[j] = (ok && (wh >> 31) && kbase + j < J.D) ? ham : A.inv_code;
-      }
-#pragma unroll
-      for (int i = 0; i < NP; i++) c[i] = (uint32_t)cv[2 * i] | ((uint32_t)cv[2 * i + 1] << 16);
-      bstore_pairs<NP>(c, make_rsrc(J.cost_out + pp * J.Dp, row_bytes), loff);
-      const uint32_t elo = (uint32_t)__builtin_amdgcn_readlane((int)meta.e_lo, ml);
-      const uint32_t ehi = (uint32_t)__builtin_amdgcn_readlane((int)meta.e_hi, ml);
-      if (J.reverse) {
-        const uint32_t ilo = (uint32_t)dpp_shl1((int)(uint32_t)win[0], (int)elo);
-        const uint32_t ihi = (uint32_t)dpp_shl1((int)(uint32_t)(win[0] >> 32), (int)ehi);
-#pragma unroll
-        for (int j = 0; j < R - 1; j++) win[j] = win[j + 1];
-        win[R - 1] = ((uint64_t)ihi << 32) | ilo;
+      if (slide) {
+        census_costs<R>(win, alo, ahi, ok, kbase, J.D, A.inv_code, c);
+        const uint32_t elo = (uint32_t)__builtin_amdgcn_readlane((int)meta.e_lo, ml);
+        const uint32_t ehi = (uint32_t)__builtin_amdgcn_readlane((int)meta.e_hi, ml);
+        slide_window(elo, ehi);
       } else {
-        const uint32_t ilo = (uint32_t)dpp_shr1((int)(uint32_t)win[R - 1], (int)elo);
-        const uint32_t ihi = (uint32_t)dpp_shr1((int)(uint32_t)(win[R - 1] >> 32), (int)ehi);
-#pragma unroll
-        for (int j = R - 1; j > 0; j--) win[j] = win[j - 1];
-        win[0] = ((uint64_t)ihi << 32) | ilo;
+        census_costs<R>(dsc[CM == CM_RECOMPUTE ? slot : 0], alo, ahi, ok, kbase, J.D,
+                        A.inv_code, c);
       }
+      if constexpr (CM == CM_FUSED)
+        bstore_pairs<NP>(c, make_rsrc(J.cost_out + pp * J.Dp, row_bytes), loff);
     } else {
 #pragma unroll
       for (int i = 0; i < NP; i++) c[i] = cb[slot][i];
@@ -818,20 +896,20 @@ struct PkSweep {
     // prologue: metadata of window 0, loads of steps 0 .. PF-1
     int g_raw, gp_raw;
     uint64_t da = 0, e = 0;
-    meta_load<PF, FUSED>(J, rs, 0, len, p0, step_off, x0, g_raw, gp_raw, da, e);
-    meta = meta_finish<PF, FUSED>(J, A, 0, len, x0, y0, g_raw, gp_raw, da, e, p2_table);
+    meta_load<PF, CM>(J, A, rs, 0, len, p0, step_off, x0, g_raw, gp_raw, da, e);
+    meta = meta_finish<PF, CM>(J, A, 0, len, x0, y0, g_raw, gp_raw, da, e, p2_table);
 #pragma unroll
     for (int j = 0; j < PF; j++) load_step(j, j);
     int s0 = 0;
     for (; s0 + PF <= len; s0 += PF) {
       // metadata of the next window, consumed next iteration
-      meta_load<PF, FUSED>(J, rs, s0 + PF, len, p0, step_off, x0, g_raw, gp_raw, da, e);
+      meta_load<PF, CM>(J, A, rs, s0 + PF, len, p0, step_off, x0, g_raw, gp_raw, da, e);
 #pragma unroll
       for (int jj = 0; jj < PF; jj++) {
         step(s0 + jj, jj, jj);
         load_step(jj, s0 + jj + PF);
       }
-      meta = meta_finish<PF, FUSED>(J, A, s0 + PF, len, x0, y0, g_raw, gp_raw, da, e, p2_table);
+      meta = meta_finish<PF, CM>(J, A, s0 + PF, len, x0, y0, g_raw, gp_raw, da, e, p2_table);
       maybe_flush(s0 + PF);
     }
     // tail: fewer than PF steps; their loads are already in the slots
@@ -845,11 +923,11 @@ struct PkSweep {
   }
 };
 
-template <int R, int PHASE, bool FUSED>
+template <int R, int PHASE, int CM>
 __device__ __forceinline__ void sweep_line_pk(const PathJob &J, const PathArgs &A, int line,
                                               const int *p2_table) {
   static_assert(R % 2 == 0, "packed sweep needs an even lane group");
-  PkSweep<R, PHASE, FUSED> sw(J, A, line, p2_table);
+  PkSweep<R, PHASE, CM> sw(J, A, line, p2_table);
   sw.run();
 }
 
@@ -883,12 +961,16 @@ struct PathJobs {
   PathJob local[2];
 };
 
-// MODE: 0 generic int sweep, 1 packed u16 sweep, 2 packed u16 sweep with fused census cost
+// MODE: 0 generic int sweep; 1 packed u16 reading the cost volume; 2 packed u16
+// first sweep with the census cost fused in (writes the cost volume); 3 packed
+// u16 with the census cost recomputed in the sweep (no cost volume).
 template <int R, typename SumT, int PHASE, int MODE>
 __device__ __forceinline__ void sweep_dispatch(const PathJob &J, const PathArgs &A, int line,
                                                const int *p2_table) {
   if constexpr (MODE == 0) sweep_line<R, SumT, PHASE>(J, A, line, p2_table);
-  else sweep_line_pk<R, PHASE, MODE == 2>(J, A, line, p2_table);
+  else if constexpr (MODE == 1) sweep_line_pk<R, PHASE, CM_LOAD>(J, A, line, p2_table);
+  else if constexpr (MODE == 2) sweep_line_pk<R, PHASE, CM_FUSED>(J, A, line, p2_table);
+  else sweep_line_pk<R, PHASE, CM_RECOMPUTE>(J, A, line, p2_table);
 }
 
 template <int R, typename SumT, int PHASE, int MODE>

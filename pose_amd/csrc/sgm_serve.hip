@@ -119,8 +119,11 @@ struct sgm_server {
   void run();
 };
 
-static size_t request_bytes(const sgm_params *p, const sgm_pair &pr) {
-  return batch_arena_bytes(p, &pr, 1, 0) + batch_table_bytes(1, p);
+// Smallest device footprint of a request served alone (admission test).
+static size_t request_bytes(const sgm_context *ctx, const sgm_params *p, const sgm_pair &pr) {
+  const bool recompute = batch_recompute_possible(p, ctx->keep_volumes) &&
+                         ctx->volume_policy != SGM_VOLUME_RESIDENT;
+  return batch_arena_bytes(p, &pr, 1, 0, recompute) + batch_table_bytes(1, p);
 }
 
 void sgm_server::run() {
@@ -162,14 +165,26 @@ void sgm_server::run() {
         take.push_back(r);
         bytes += r->bytes;
       }
-      // shrink until the group fits the pool (FIFO: keep the oldest)
+      // Shrink until the group fits the pool (FIFO: keep the oldest).  Each
+      // group first tries resident cost volumes, then recomputed census costs.
       size_t off = 0;
+      int recompute = 0;
+      const bool can_recompute = batch_recompute_possible(&params, ctx->keep_volumes);
       while (!take.empty()) {
         std::vector<sgm_pair> prs;
         for (Request *r : take) prs.push_back(r->pair);
-        bytes = batch_arena_bytes(&params, prs.data(), (int)prs.size(), 0) +
-                batch_table_bytes((int)prs.size(), &params);
-        if (pool.alloc(bytes, &off)) break;
+        const int n = (int)prs.size();
+        const size_t tables = batch_table_bytes(n, &params);
+        if (ctx->volume_policy != SGM_VOLUME_RECOMPUTE || !can_recompute) {
+          recompute = 0;
+          bytes = batch_arena_bytes(&params, prs.data(), n, 0, false) + tables;
+          if (pool.alloc(bytes, &off)) break;
+        }
+        if (can_recompute && ctx->volume_policy != SGM_VOLUME_RESIDENT) {
+          recompute = 1;
+          bytes = batch_arena_bytes(&params, prs.data(), n, 0, true) + tables;
+          if (pool.alloc(bytes, &off)) break;
+        }
         take.pop_back();
       }
       if (take.empty()) break;  // wait for memory to free up
@@ -194,7 +209,7 @@ void sgm_server::run() {
       lk.unlock();
       sgm_batch *b = nullptr;
       int rc = batch_create(ctx, &params, prs.data(), (int)prs.size(), 0,
-                            (uint8_t *)region + off, bytes, s.stream, &b);
+                            (uint8_t *)region + off, bytes, s.stream, &b, recompute);
       if (rc == SGM_OK) rc = batch_launch(b);
       if (rc == SGM_OK) {
         hipError_t e = hipEventRecord(s.done, s.stream);
@@ -290,7 +305,7 @@ extern "C" int sgm_server_submit(sgm_server *srv, const sgm_pair *pair, uint64_t
     return fail(SGM_ERR_INVALID_ARGUMENT, "pair needs left, right and checked buffers");
   if (lanes_group(pair->disparities) > 8)
     return fail(SGM_ERR_INVALID_ARGUMENT, "serving supports up to 512 disparities per pair");
-  const size_t bytes = request_bytes(&srv->params, *pair);
+  const size_t bytes = request_bytes(srv->ctx, &srv->params, *pair);
   if (bytes > srv->region_bytes)
     return fail(SGM_ERR_REJECTED, "pair needs %zu MiB, more than the serving budget (%zu MiB)",
                 bytes >> 20, srv->region_bytes >> 20);
@@ -445,10 +460,9 @@ extern "C" int sgm_autotune(sgm_context *ctx, const sgm_params *params, const ch
   }
   double bbest = 1e30;
   int bw = ctx->batch_waves_per_block, bl = ctx->batch_lines_per_item, bb = ctx->batch_blocks_per_cu;
-  for (int waves : {2, 4, 8})
+  for (int waves : {4, 8})
     for (int lines : {1, 2, 4})
-      for (int bpc : {1, 2, 4}) {
-        if (waves * bpc > 16) continue;
+      for (int bpc : {0, 1, 2}) {  // 0: all resident workgroups
         ctx->batch_waves_per_block = waves;
         ctx->batch_lines_per_item = lines;
         ctx->batch_blocks_per_cu = bpc;
@@ -507,6 +521,6 @@ extern "C" int sgm_autotune_load(sgm_context *ctx, const char *path) {
     ctx->tuned_waves[R] = waves;
   if (field("batch_waves_per_block", &bw) && bw >= 1 && bw <= 8) ctx->batch_waves_per_block = bw;
   if (field("batch_lines_per_item", &bl) && bl >= 1 && bl <= 64) ctx->batch_lines_per_item = bl;
-  if (field("batch_blocks_per_cu", &bb) && bb >= 1 && bb <= 16) ctx->batch_blocks_per_cu = bb;
+  if (field("batch_blocks_per_cu", &bb) && bb >= 0 && bb <= 16) ctx->batch_blocks_per_cu = bb;
   return SGM_OK;
 }

@@ -10,6 +10,7 @@ there is no CPU fallback in this package: when the library is missing or fails,
 calls raise.
 """
 import ctypes
+import json
 import math
 import os
 import threading
@@ -25,6 +26,7 @@ ERRORS = {
     -5: "HIP error", -6: "rejected", -7: "timeout", -8: "not found",
 }
 
+VOLUME_AUTO, VOLUME_RESIDENT, VOLUME_RECOMPUTE = 0, 1, 2
 COSTS = {"census": 0, "ad-census": 1, "census-gradient": 2, "zncc": 3, "zncc-fp32": 4}
 
 
@@ -90,10 +92,12 @@ class SgmRequestInfo(ctypes.Structure):
 EXPORTS = [
     "sgm_device_count", "sgm_device_info_get", "sgm_open", "sgm_close", "sgm_plan_get",
     "sgm_disparity_limit", "sgm_match", "sgm_match_pair", "sgm_match_pair_device",
-    "sgm_set_keep_volumes", "sgm_read_volumes", "sgm_synchronize", "sgm_set_profiling",
+    "sgm_set_keep_volumes", "sgm_read_volumes", "sgm_set_cost_volume", "sgm_synchronize",
+    "sgm_release_buffers", "sgm_set_profiling",
     "sgm_profile_json", "sgm_stream", "sgm_debug_fail_allocations", "sgm_last_error",
     "sgm_device_alloc", "sgm_device_free", "sgm_memcpy",
     "sgm_batch_create", "sgm_batch_run", "sgm_batch_destroy", "sgm_batch_device_bytes",
+    "sgm_batch_describe",
     "sgm_autotune", "sgm_autotune_load",
     "sgm_server_create", "sgm_server_submit", "sgm_server_wait", "sgm_server_now",
     "sgm_server_destroy",
@@ -131,8 +135,10 @@ def lib():
             L.sgm_match_pair.argtypes = [P, pp, P, P, P, P, I, I, P, P, P, ps, P, P]
             L.sgm_match_pair_device.argtypes = [P, pp, P, P, P, P, I, I, P, P, P, P, P, P]
             L.sgm_set_keep_volumes.argtypes = [P, I]
+            L.sgm_set_cost_volume.argtypes = [P, I]
             L.sgm_read_volumes.argtypes = [P, I, P, P]
             L.sgm_synchronize.argtypes = [P]
+            L.sgm_release_buffers.argtypes = [P]
             L.sgm_set_profiling.argtypes = [P, I]
             L.sgm_profile_json.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
             L.sgm_stream.argtypes = [P]
@@ -146,6 +152,7 @@ def lib():
             L.sgm_batch_destroy.argtypes = [P]
             L.sgm_batch_device_bytes.argtypes = [P]
             L.sgm_batch_device_bytes.restype = I64
+            L.sgm_batch_describe.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
             L.sgm_autotune.argtypes = [P, pp, ctypes.c_char_p]
             L.sgm_autotune_load.argtypes = [P, ctypes.c_char_p]
             L.sgm_server_create.argtypes = [P, pp, I64, I, I, ctypes.POINTER(P)]
@@ -389,6 +396,10 @@ class Context:
     def keep_volumes(self, keep=True):
         check(lib().sgm_set_keep_volumes(self.h, int(keep)))
 
+    def cost_volume(self, policy):
+        """Cost-volume policy: VOLUME_AUTO, VOLUME_RESIDENT or VOLUME_RECOMPUTE."""
+        check(lib().sgm_set_cost_volume(self.h, int(policy)))
+
     def read_volumes(self, direction, n, D):
         cost = np.empty(n * D, np.uint16)
         agg = np.empty(n * D, np.uint32)
@@ -397,6 +408,9 @@ class Context:
 
     def synchronize(self):
         check(lib().sgm_synchronize(self.h))
+
+    def release_buffers(self):
+        check(lib().sgm_release_buffers(self.h))
 
     def set_profiling(self, on=True):
         check(lib().sgm_set_profiling(self.h, int(on)))
@@ -650,6 +664,11 @@ class Batch:
     def device_bytes(self):
         return lib().sgm_batch_device_bytes(self.h)
 
+    def describe(self):
+        buf = ctypes.create_string_buffer(1024)
+        check(lib().sgm_batch_describe(self.h, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
     def close(self):
         if self.h:
             lib().sgm_batch_destroy(self.h)
@@ -681,6 +700,11 @@ class DeviceBatch:
 
     def device_bytes(self):
         return lib().sgm_batch_device_bytes(self.h)
+
+    def describe(self):
+        buf = ctypes.create_string_buffer(1024)
+        check(lib().sgm_batch_describe(self.h, buf, len(buf)))
+        return json.loads(buf.value.decode())
 
     def close(self):
         if self.h:

@@ -59,13 +59,19 @@ def oracle_of(base_kw, pr):
 CENSUS97 = dict(window_w=9, window_h=7, p1=10, p2=120, lr_threshold=1.0)
 
 
+@pytest.mark.parametrize("policy", [sgm.VOLUME_RESIDENT, sgm.VOLUME_RECOMPUTE],
+                         ids=["resident", "recompute"])
 @pytest.mark.parametrize("graph", [False, True])
-def test_batch_census_fused_bit_exact(gpu_context, graph):
+def test_batch_census_bit_exact(gpu_context, graph, policy):
     pairs = uneven_pairs(1, 7)
     print("batch pairs", [(p["left"].shape, p["disparities"], p["min_disparity"]) for p in pairs],
           flush=True)
     params = to_sgm(O.params(**dict(CENSUS97, disparities=64)))
-    b = sgm.Batch(gpu_context, params, pairs, graph=graph)
+    gpu_context.cost_volume(policy)
+    try:
+        b = sgm.Batch(gpu_context, params, pairs, graph=graph)
+    finally:
+        gpu_context.cost_volume(sgm.VOLUME_AUTO)
     try:
         for rep in range(2 if graph else 1):
             res = b.run()
@@ -154,7 +160,9 @@ def test_batch_device_buffers(gpu_context):
                 v.free()
 
 
-def test_server_results_equal_alone(gpu_context):
+@pytest.mark.parametrize("policy", [sgm.VOLUME_AUTO, sgm.VOLUME_RECOMPUTE],
+                         ids=["auto", "recompute"])
+def test_server_results_equal_alone(gpu_context, policy):
     """Requests admitted while others run: each result equals its solo plain run."""
     ctx = gpu_context
     pairs = uneven_pairs(5, 9, wrange=(120, 600), drange=(16, 300))
@@ -164,6 +172,7 @@ def test_server_results_equal_alone(gpu_context):
         p = to_sgm(O.params(**dict(CENSUS97, disparities=pr["disparities"],
                                    min_disparity=pr["min_disparity"])))
         alone.append(ctx.match_pair(p, pr["left"], pr["right"]))
+    ctx.cost_volume(policy)
     srv = sgm.Server(ctx, params, budget_bytes=2 << 30, max_batch=4, streams=2)
     try:
         tickets = []
@@ -182,6 +191,7 @@ def test_server_results_equal_alone(gpu_context):
         assert len(batches) >= 2  # more than one batch was formed while others ran
     finally:
         srv.close()
+        ctx.cost_volume(sgm.VOLUME_AUTO)
 
 
 def test_server_rejects_what_never_fits(gpu_context):

@@ -122,6 +122,35 @@ def test_parameter_sweep(gpu_context, cfg):
     run_both(gpu_context, op, left, right, what="sweep%r" % (cfg,))
 
 
+RECOMPUTE_CASES = [
+    # (w, h, D, minD, window, dirs, p1, p2, edge, masks)
+    (64, 40, 3, 0, (5, 5), 8, 4, 24, 32, False),
+    (200, 90, 64, -7, (9, 7), 8, 10, 120, 32, True),
+    (256, 70, 100, 5, (7, 5), 4, 6, 80, 0, False),
+    (300, 60, 250, -20, (9, 7), 8, 8, 96, 16, True),
+    (640, 40, 500, 3, (9, 7), 8, 8, 96, 32, False),
+]
+
+
+@pytest.mark.parametrize("cfg", RECOMPUTE_CASES, ids=[str(i) for i in range(len(RECOMPUTE_CASES))])
+def test_recomputed_census_cost_bit_exact(gpu_context, cfg):
+    """Census cost recomputed inside every sweep (no cost volume) against the oracle."""
+    w, h, D, md, (ww, wh), dirs, p1, p2, edge, masks = cfg
+    left, right, _ = synth.stereo_pair(w, h, D, md, seed=w + D)
+    vl = vr = None
+    if masks:
+        rng = np.random.default_rng(D)
+        vl = (rng.random((h, w)) > 0.04).astype(np.uint8)
+        vr = np.roll(vl, 2, axis=1)
+    op = O.params(window_w=ww, window_h=wh, disparities=D, min_disparity=md, directions=dirs,
+                  p1=p1, p2=p2, edge_scale=edge)
+    gpu_context.cost_volume(sgm.VOLUME_RECOMPUTE)
+    try:
+        run_both(gpu_context, op, left, right, vl, vr, what="recompute%r" % (cfg,))
+    finally:
+        gpu_context.cost_volume(sgm.VOLUME_AUTO)
+
+
 @pytest.mark.parametrize("cost", [0, 1, 2, 3])
 @pytest.mark.parametrize("directions", [4, 8])
 def test_reference_upgrade_fixture(gpu_context, cost, directions):

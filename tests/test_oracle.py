@@ -247,3 +247,51 @@ def test_golden_fixtures_reproduce():
         assert np.array_equal(r["dr"].view(np.uint32), z["dr"].view(np.uint32)), f
         assert (r["reasons"] == z["reasons"]).all() and (r["boundary"] == z["boundary"]).all(), f
         assert [r["stats"][k] for k in O.STAT_FIELDS] == list(z["stats"]), f
+
+
+def test_zncc1024_cost_matches_library_correlation():
+    """The FP64 ZNCC-1024 oracle cost against numpy's Pearson correlation
+    (np.corrcoef) of the two windows: round((1 - ncc) * 1024), 0x8800 where a
+    window is constant; gain and offset between the windows leave it unchanged."""
+    rng = np.random.default_rng(11)
+    w, h, ww, wh, D = 40, 22, 7, 5, 6
+    ga = rng.integers(0, 256, (h, w)).astype(np.uint8)
+    ga[8:14, 20:30] = 77  # constant patch: zero variance
+    gb = np.clip(np.roll(ga, 2, axis=1).astype(int) + rng.integers(-20, 21, ga.shape), 0, 255)
+    gb = gb.astype(np.uint8)
+    ones = np.ones(w * h, np.uint8)
+    p = O.params(window_w=ww, window_h=wh, disparities=D, min_disparity=0, cost=O.ZNCC1024,
+                 p1=12, p2=150)
+    da, db = O.census(ga, w, h, ww, wh), O.census(gb, w, h, ww, wh)
+    C = O.match(p, da, db, ga.reshape(-1), gb.reshape(-1), ones, ones, w, h, 0,
+                volumes=True)[4].reshape(h, w, D)
+    rx, ry = ww // 2, wh // 2
+    checked = 0
+    for y in range(ry, h - ry):
+        for x in range(rx, w - rx):
+            for k in range(D):
+                t = x - k
+                if t < rx:  # target outside the image interior: invalid cell
+                    assert C[y, x, k] == 0x8800
+                    continue
+                a = ga[y - ry:y + ry + 1, x - rx:x + rx + 1].astype(np.float64).reshape(-1)
+                b = gb[y - ry:y + ry + 1, t - rx:t + rx + 1].astype(np.float64).reshape(-1)
+                if a.std() == 0 or b.std() == 0:
+                    assert C[y, x, k] == 0x8800
+                    continue
+                v = (1 - min(1.0, max(-1.0, np.corrcoef(a, b)[0, 1]))) * 1024
+                if abs(v - math.floor(v) - 0.5) < 1e-6:
+                    continue  # a rounding tie decided by the last ulp
+                assert C[y, x, k] == math.floor(v + 0.5), (y, x, k)
+                checked += 1
+    assert checked > 2000
+    # invariance: gain 2 and offset -30 on the target (kept inside 0..255)
+    base = rng.integers(40, 110, (h, w)).astype(np.uint8)
+    lin = (base.astype(int) * 2 - 30).astype(np.uint8)
+    d0 = O.census(base, w, h, ww, wh)
+    C0 = O.match(p, d0, d0, base.reshape(-1), base.reshape(-1), ones, ones, w, h, 0,
+                 volumes=True)[4].reshape(h, w, D)
+    C1 = O.match(p, d0, O.census(lin, w, h, ww, wh), base.reshape(-1), lin.reshape(-1), ones,
+                 ones, w, h, 0, volumes=True)[4].reshape(h, w, D)
+    assert (C0[ry:h - ry, rx + D:w - rx, 0] == 0).all()
+    assert (C1[ry:h - ry, rx + D:w - rx] == C0[ry:h - ry, rx + D:w - rx]).all()
