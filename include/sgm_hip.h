@@ -155,6 +155,11 @@ int sgm_match_pair_device(sgm_context *ctx, const sgm_params *params, const void
 int sgm_device_alloc(sgm_context *ctx, size_t bytes, void **out);
 int sgm_device_free(sgm_context *ctx, void *ptr);
 int sgm_memcpy(sgm_context *ctx, void *dst, const void *src, size_t bytes, int kind);
+/* Device-to-device copy by a plain streaming kernel on the context's stream
+ * (asynchronous; 16-byte aligned pointers and size).  No counterpart in the
+ * reference: it measures the achievable HBM bandwidth and calibrates HBM
+ * traffic counters against a kernel of known size. */
+int sgm_device_copy(sgm_context *ctx, void *dst, const void *src, size_t bytes);
 
 /* Copy the last call's cost (uint16, n*D, 512 = invalid) and aggregate
  * (widened to uint32, n*D) volumes of one matching direction (0 = left
@@ -220,7 +225,8 @@ int sgm_batch_run(sgm_batch *batch);
 int sgm_batch_destroy(sgm_batch *batch);
 int64_t sgm_batch_device_bytes(sgm_batch *batch);
 /* JSON summary of a batch: pairs, work items, launch shape, cost-volume mode
- * ("resident", "resident-fused" or "recomputed"), device bytes, graph. */
+ * ("resident-u16", "resident-u8", "resident-u8-fused" or "recomputed"),
+ * device bytes, graph. */
 int sgm_batch_describe(sgm_batch *batch, char *buffer, size_t length);
 
 /* Autotuner: time the candidate path-sweep configurations on this device for

@@ -143,8 +143,9 @@ int64_t direction_bytes(const sgm_params *p, int w, int h, bool cost_volume) {
   const int64_t n = (int64_t)w * h, Dp = padded_disparities(p->disparities);
   const int sb = std::max(aggregate_bytes(p), 2);
   const bool zncc = p->cost == SGM_COST_ZNCC || p->cost == SGM_COST_ZNCC_FP32;
+  const int cb = cost_volume ? (cost_u8(p) ? 1 : 2) : 0;
   // (cost +) aggregate volumes, descriptors, mask, gray|mask, gray copy, disparity, window sums
-  return n * Dp * ((cost_volume ? 2 : 0) + sb) + n * (8 * words_of(p) + 1 + 2 + 1 + 4 + (zncc ? 8 : 0));
+  return n * Dp * (cb + sb) + n * (8 * words_of(p) + 1 + 2 + 1 + 4 + (zncc ? 8 : 0));
 }
 
 bool census_recompute_possible(const sgm_params *p) {
@@ -423,6 +424,7 @@ CostArgs cost_args(const sgm_params *p) {
   A.adcap = p->ad_truncation;
   A.gradcap = p->gradient_truncation;
   A.inv_code = invalid_code(p);
+  A.u8 = cost_u8(p) ? 1 : 0;
   return A;
 }
 
@@ -480,6 +482,7 @@ static int launch_path_any(sgm_context *ctx, int R, int sum_bytes, int mode, con
   case RR:                                                                  \
     if (mode == 2) launch_path_t<RR, uint16_t, 2>(ctx, jobs, A, phase, waves); \
     else if (mode == 3) launch_path_t<RR, uint16_t, 3>(ctx, jobs, A, phase, waves); \
+    else if (mode == 4) launch_path_t<RR, uint16_t, 4>(ctx, jobs, A, phase, waves); \
     else launch_path_t<RR, uint16_t, 1>(ctx, jobs, A, phase, waves);        \
     break;
   if (sum_bytes == 4) {
@@ -490,9 +493,13 @@ static int launch_path_any(sgm_context *ctx, int R, int sum_bytes, int mode, con
   } else {
     switch (R) {
       CASE_PK(2) CASE_PK(4) CASE_PK(8)
-      case 16: launch_path_t<16, uint16_t, 1>(ctx, jobs, A, phase, waves); break;
-      case 32: launch_path_t<32, uint16_t, 1>(ctx, jobs, A, phase, waves); break;
-      case 64: launch_path_t<64, uint16_t, 1>(ctx, jobs, A, phase, waves); break;
+#define CASE_PK_LOAD(RR)                                                    \
+  case RR:                                                                  \
+    if (mode == 4) launch_path_t<RR, uint16_t, 4>(ctx, jobs, A, phase, waves); \
+    else launch_path_t<RR, uint16_t, 1>(ctx, jobs, A, phase, waves);        \
+    break;
+      CASE_PK_LOAD(16) CASE_PK_LOAD(32) CASE_PK_LOAD(64)
+#undef CASE_PK_LOAD
       default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
     }
   }
@@ -517,6 +524,13 @@ int launch_lr(sgm_context *ctx, const LrArgs &A) {
 }
 
 int invalid_code(const sgm_params *p) { return p->cost == SGM_COST_ZNCC_FP32 ? 0x8800 : 512; }
+
+bool cost_u8(const sgm_params *p) {
+  // the packed 16-bit sweeps read census-family costs (<= 120, invalid 512 stored
+  // as 255) from a uint8 volume; FP32 ZNCC costs (0..2048) and the 32-bit
+  // aggregate sweeps keep uint16
+  return aggregate_bytes(p) == 2 && p->cost != SGM_COST_ZNCC_FP32;
+}
 
 int path_lanes(const sgm_params *p) {
   // 16-bit aggregates run packed pairs: at least two disparities per lane.
@@ -581,7 +595,7 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
       ENSURE(("wsum" + s).c_str(), n * 4);
       ENSURE(("wsum2" + s).c_str(), n * 4);
     }
-    if (!recompute_cost(ctx, p, w, h)) ENSURE(("cost" + s).c_str(), n * Dp * 2);
+    if (!recompute_cost(ctx, p, w, h)) ENSURE(("cost" + s).c_str(), n * Dp * (cost_u8(p) ? 1 : 2));
     ENSURE(("sum" + s).c_str(), n * Dp * sb);
     ENSURE(("disp" + s).c_str(), n * 4);
     B.gray[d] = (uint8_t *)ctx->buffers["gray" + s].ptr;
@@ -624,7 +638,7 @@ int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuff
   const int sb = aggregate_bytes(p);
   const bool recompute = recompute_cost(ctx, p, w, h);
   const bool fused = !recompute && fused_cost(p);
-  const int mode = recompute ? 3 : sb == 2 ? 1 : 0;
+  const int mode = recompute ? 3 : sb == 2 ? (cost_u8(p) ? 4 : 1) : 0;
   if (!fused && !recompute) {
     CostJobs cj;
     memset(&cj, 0, sizeof cj);
@@ -943,15 +957,20 @@ extern "C" int sgm_read_volumes(sgm_context *ctx, int direction, uint16_t *cost,
   std::string s = std::to_string(direction);
   const void *dc = ctx->buffers["cost" + s].ptr, *ds = ctx->buffers["sum" + s].ptr;
   if (!dc || !ds) return fail(SGM_ERR_NOT_FOUND, "no volumes");
-  std::vector<uint16_t> hc(n * Dp);
+  // uint8 volumes hold 255 for an invalid cell (512)
+  const bool u8 = cost_u8(p);
+  std::vector<uint8_t> hc(n * Dp * (u8 ? 1 : 2));
   std::vector<uint8_t> hs(n * Dp * sb);
   int rc;
-  if ((rc = copy_d2h(ctx, hc.data(), dc, hc.size() * 2))) return rc;
+  if ((rc = copy_d2h(ctx, hc.data(), dc, hc.size()))) return rc;
   if ((rc = copy_d2h(ctx, hs.data(), ds, hs.size()))) return rc;
   if ((rc = finish(ctx))) return rc;
   for (size_t i = 0; i < n; i++)
     for (size_t k = 0; k < D; k++) {
-      if (cost) cost[i * D + k] = hc[i * Dp + k];
+      if (cost) {
+        if (u8) cost[i * D + k] = hc[i * Dp + k] == 255 ? 512 : hc[i * Dp + k];
+        else cost[i * D + k] = ((const uint16_t *)hc.data())[i * Dp + k];
+      }
       if (aggregate) {
         if (sb == 2) aggregate[i * D + k] = ((const uint16_t *)hs.data())[i * Dp + k];
         else aggregate[i * D + k] = (uint32_t)((const int32_t *)hs.data())[i * Dp + k];
@@ -975,6 +994,21 @@ extern "C" int sgm_device_free(sgm_context *ctx, void *ptr) {
   (void)hipStreamSynchronize(ctx->stream);
   hipError_t e = hipFree(ptr);
   return e == hipSuccess ? SGM_OK : hip_fail(e, "hipFree");
+}
+
+extern "C" int sgm_device_copy(sgm_context *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx || !dst || !src) return fail(SGM_ERR_INVALID_ARGUMENT, "null argument");
+  if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+    return fail(SGM_ERR_INVALID_ARGUMENT, "device copy needs 16-byte aligned pointers and size");
+  if (bytes == 0) return SGM_OK;
+  std::lock_guard<std::mutex> g(ctx->lock);
+  (void)hipSetDevice(ctx->device);
+  ProfileScope ps(ctx, "copy");
+  const size_t n = bytes / 16;
+  const size_t want = (n + 255) / 256;
+  const unsigned blocks = (unsigned)std::min<size_t>(want, (size_t)ctx->info.compute_units * 16);
+  copy_kernel<<<blocks, 256, 0, ctx->stream>>>((const uint4 *)src, (uint4 *)dst, n);
+  return launch_check("copy kernel");
 }
 
 extern "C" int sgm_memcpy(sgm_context *ctx, void *dst, const void *src, size_t bytes, int kind) {

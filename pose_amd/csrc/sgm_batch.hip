@@ -36,12 +36,16 @@ int batch_lines_per_item(sgm_context *ctx) { return ctx->batch_lines_per_item; }
 // The persistent kernel instance of a batch.
 const void *batch_kernel_of(const sgm_batch *b) {
   if (b->sum_bytes == 2) {
-    if (b->recompute) return (const void *)&batch_kernel<uint16_t, 1, true>;
-    if (b->words == 1) return (const void *)&batch_kernel<uint16_t, 1, false>;
-    return (const void *)&batch_kernel<uint16_t, 2, false>;
+    if (b->recompute) return (const void *)&batch_kernel<uint16_t, 1, VOL_NONE>;
+    if (b->cost8) {
+      if (b->words == 1) return (const void *)&batch_kernel<uint16_t, 1, VOL_U8>;
+      return (const void *)&batch_kernel<uint16_t, 2, VOL_U8>;
+    }
+    if (b->words == 1) return (const void *)&batch_kernel<uint16_t, 1, VOL_U16>;
+    return (const void *)&batch_kernel<uint16_t, 2, VOL_U16>;
   }
-  if (b->words == 1) return (const void *)&batch_kernel<int32_t, 1, false>;
-  return (const void *)&batch_kernel<int32_t, 2, false>;
+  if (b->words == 1) return (const void *)&batch_kernel<int32_t, 1, VOL_U16>;
+  return (const void *)&batch_kernel<int32_t, 2, VOL_U16>;
 }
 
 }  // namespace sgm
@@ -160,7 +164,7 @@ size_t batch_arena_bytes(const sgm_params *params, const sgm_pair *pairs, int co
         cv.take(n * 4);
         cv.take(n * 4);
       }
-      if (!recompute) cv.take(n * Dp * 2);
+      if (!recompute) cv.take(n * Dp * (cost_u8(params) ? 1 : 2));
       cv.take(n * Dp * sb);
       const bool own_disp = host_io || (d == 0 ? pr.dl == nullptr : pr.dr == nullptr);
       if (own_disp) cv.take(n * 4);
@@ -223,6 +227,7 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
     }
   }
   b->recompute = possible && recompute != 0;
+  b->cost8 = !b->recompute && cost_u8(params);
   b->fused = !b->recompute && params->cost == SGM_COST_CENSUS && b->words == 1 &&
              b->sum_bytes == 2;
   b->waves_per_block = ctx->batch_waves_per_block;
@@ -268,7 +273,7 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
       o.gm[d] = cv.take(n * 2);
       o.wsum[d] = zncc ? cv.take(n * 4) : 0;
       o.wsum2[d] = zncc ? cv.take(n * 4) : 0;
-      o.cost[d] = b->recompute ? 0 : cv.take(n * Dp * 2);
+      o.cost[d] = b->recompute ? 0 : cv.take(n * Dp * (b->cost8 ? 1 : 2));
       o.sum[d] = cv.take(n * Dp * b->sum_bytes);
       const bool own_disp = host_io || (d == 0 ? pr.dl == nullptr : pr.dr == nullptr);
       o.disp[d] = own_disp ? cv.take(n * 4) : 0;
@@ -477,7 +482,7 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
   A.npairs = count;
   A.directions = params->directions;
   A.fused = (b->fused || b->recompute) ? 1 : 0;  // first sweep waits on census, not on a cost phase
-  A.cmode = b->recompute ? CM_RECOMPUTE : b->fused ? CM_FUSED : CM_LOAD;
+  A.cmode = b->recompute ? CM_RECOMPUTE : b->fused ? CM_FUSED : b->cost8 ? CM_LOAD8 : CM_LOAD;
   A.lines_per_item = b->lines_per_item;
   A.census_rows = census_rows;
   A.cost_rows = cost_rows;
@@ -623,7 +628,10 @@ extern "C" int sgm_batch_describe(sgm_batch *b, char *buffer, size_t length) {
                          "\"cost_volume\":\"%s\",\"device_bytes\":%lld,\"graph\":%s}",
                          b->pairs.size(), b->total_items, b->blocks, b->waves_per_block,
                          b->lines_per_item,
-                         b->recompute ? "recomputed" : b->fused ? "resident-fused" : "resident",
+                         b->recompute ? "recomputed"
+                         : b->fused   ? "resident-u8-fused"
+                         : b->cost8   ? "resident-u8"
+                                      : "resident-u16",
                          (long long)(b->arena_bytes + b->tables_bytes),
                          b->graph_exec ? "true" : "false");
   if (n < 0 || (size_t)n >= length) return fail(SGM_ERR_INVALID_ARGUMENT, "buffer too small");

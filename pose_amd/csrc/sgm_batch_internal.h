@@ -43,6 +43,7 @@ struct sgm_batch {
   int words = 1, sum_bytes = 2;
   bool fused = false;
   bool recompute = false;
+  bool cost8 = false;  // uint8 cost volume
   int waves_per_block = 4, lines_per_item = 1, blocks = 0;
   void *arena = nullptr;
   size_t arena_bytes = 0;

@@ -50,7 +50,7 @@ struct BatchArgs {
   unsigned *done;          // [nph][stride] completed items per job
   int *error;
   int nph, stride, npairs, directions, fused;
-  int cmode;               // 0 cost volume, 1 fused first sweep, 2 recomputed census cost
+  int cmode;               // CM_*: the first sweep's cost mode
   int lines_per_item, census_rows, cost_rows, lr_rows;
   PathArgs pa;             // penalties, codes, uniqueness (direction fields set per item)
   CostArgs ca;
@@ -191,10 +191,7 @@ __device__ __forceinline__ void batch_cost(const BatchArgs &B, const CostJob &J,
         }
         v[j] = (unsigned)value & 0xFFFFu;
       }
-      uint2 packed;
-      packed.x = v[0] | (v[1] << 16);
-      packed.y = v[2] | (v[3] << 16);
-      *reinterpret_cast<uint2 *>(J.cost + p * J.Dp + k0) = packed;
+      store_cost4(J.cost, p, J.Dp, k0, v, B.ca.u8 != 0);
     }
   }
 }
@@ -251,46 +248,55 @@ __device__ __forceinline__ void batch_lr(const BatchArgs &B, const BatchPairDev 
   }
 }
 
-// kind: 0 first / 1 first with fused census cost / 2 middle / 3 last (cost
-// volume), 4 first / 5 middle / 6 last with the census cost recomputed.  The
-// recomputing sweeps need far more registers (target descriptor prefetch), so
-// they live in their own kernel instance (RECOMP) and do not lower the
-// occupancy of the cost-volume instance.
-template <int R, typename SumT, bool RECOMP>
+// Cost-volume instance of the batch kernel (each its own kernel, so the
+// register-heavy recomputing sweeps do not lower the occupancy of the others):
+//   VOL_U16:  uint16 cost volume (FP32 ZNCC costs, 32-bit aggregates)
+//   VOL_U8:   uint8 cost volume, census cost fused into the first sweep when
+//             the batch is 1-word census, else written by the cost phase
+//   VOL_NONE: census cost recomputed in every sweep
+enum { VOL_U16 = 0, VOL_U8 = 1, VOL_NONE = 2 };
+
+template <int R, typename SumT, int VOL>
 __device__ __forceinline__ void batch_sweep_r(const PathJob &J, const PathArgs &A, int line,
-                                              int kind, const int *p2_table) {
-  if constexpr (sizeof(SumT) == 2 && RECOMP) {
-    switch (kind) {
-      case 4: sweep_line_pk<R, PHASE_FIRST, CM_RECOMPUTE>(J, A, line, p2_table); break;
-      case 5: sweep_line_pk<R, PHASE_MID, CM_RECOMPUTE>(J, A, line, p2_table); break;
-      default: sweep_line_pk<R, PHASE_LAST, CM_RECOMPUTE>(J, A, line, p2_table); break;
+                                              int phase, bool fused, const int *p2_table) {
+  if constexpr (sizeof(SumT) == 2 && VOL == VOL_NONE) {
+    if (phase == PHASE_FIRST) sweep_line_pk<R, PHASE_FIRST, CM_RECOMPUTE>(J, A, line, p2_table);
+    else if (phase == PHASE_MID) sweep_line_pk<R, PHASE_MID, CM_RECOMPUTE>(J, A, line, p2_table);
+    else sweep_line_pk<R, PHASE_LAST, CM_RECOMPUTE>(J, A, line, p2_table);
+  } else if constexpr (sizeof(SumT) == 2 && VOL == VOL_U8) {
+    if (phase == PHASE_FIRST) {
+      // the fused R = 8 sweep runs without metadata prefetch so it fits the
+      // instance's 128-register budget
+      if (fused) sweep_line_pk<R, PHASE_FIRST, CM_FUSED, (R >= 8 ? 1 : 64)>(J, A, line, p2_table);
+      else sweep_line_pk<R, PHASE_FIRST, CM_LOAD8>(J, A, line, p2_table);
+    } else if (phase == PHASE_MID) {
+      sweep_line_pk<R, PHASE_MID, CM_LOAD8>(J, A, line, p2_table);
+    } else {
+      sweep_line_pk<R, PHASE_LAST, CM_LOAD8>(J, A, line, p2_table);
     }
   } else if constexpr (sizeof(SumT) == 2) {
-    switch (kind) {
-      case 0: sweep_line_pk<R, PHASE_FIRST, CM_LOAD>(J, A, line, p2_table); break;
-      case 1: sweep_line_pk<R, PHASE_FIRST, CM_FUSED>(J, A, line, p2_table); break;
-      case 2: sweep_line_pk<R, PHASE_MID, CM_LOAD>(J, A, line, p2_table); break;
-      default: sweep_line_pk<R, PHASE_LAST, CM_LOAD>(J, A, line, p2_table); break;
-    }
+    if (phase == PHASE_FIRST) sweep_line_pk<R, PHASE_FIRST, CM_LOAD>(J, A, line, p2_table);
+    else if (phase == PHASE_MID) sweep_line_pk<R, PHASE_MID, CM_LOAD>(J, A, line, p2_table);
+    else sweep_line_pk<R, PHASE_LAST, CM_LOAD>(J, A, line, p2_table);
   } else {
-    if (kind <= 1) sweep_line<R, SumT, PHASE_FIRST>(J, A, line, p2_table);
-    else if (kind == 2) sweep_line<R, SumT, PHASE_MID>(J, A, line, p2_table);
+    if (phase == PHASE_FIRST) sweep_line<R, SumT, PHASE_FIRST>(J, A, line, p2_table);
+    else if (phase == PHASE_MID) sweep_line<R, SumT, PHASE_MID>(J, A, line, p2_table);
     else sweep_line<R, SumT, PHASE_LAST>(J, A, line, p2_table);
   }
 }
 
-template <typename SumT, bool RECOMP>
-__device__ __forceinline__ void batch_sweep(const PathJob &J, const PathArgs &A, int line, int kind,
-                                            const int *p2_table) {
+template <typename SumT, int VOL>
+__device__ __forceinline__ void batch_sweep(const PathJob &J, const PathArgs &A, int line, int phase,
+                                            bool fused, const int *p2_table) {
   if constexpr (sizeof(SumT) == 2) {
-    if (J.R == 2) batch_sweep_r<2, SumT, RECOMP>(J, A, line, kind, p2_table);
-    else if (J.R == 4) batch_sweep_r<4, SumT, RECOMP>(J, A, line, kind, p2_table);
-    else batch_sweep_r<8, SumT, RECOMP>(J, A, line, kind, p2_table);
+    if (J.R == 2) batch_sweep_r<2, SumT, VOL>(J, A, line, phase, fused, p2_table);
+    else if (J.R == 4) batch_sweep_r<4, SumT, VOL>(J, A, line, phase, fused, p2_table);
+    else batch_sweep_r<8, SumT, VOL>(J, A, line, phase, fused, p2_table);
   } else {
-    if (J.R == 1) batch_sweep_r<1, SumT, false>(J, A, line, kind, p2_table);
-    else if (J.R == 2) batch_sweep_r<2, SumT, false>(J, A, line, kind, p2_table);
-    else if (J.R == 4) batch_sweep_r<4, SumT, false>(J, A, line, kind, p2_table);
-    else batch_sweep_r<8, SumT, false>(J, A, line, kind, p2_table);
+    if (J.R == 1) batch_sweep_r<1, SumT, VOL_U16>(J, A, line, phase, fused, p2_table);
+    else if (J.R == 2) batch_sweep_r<2, SumT, VOL_U16>(J, A, line, phase, fused, p2_table);
+    else if (J.R == 4) batch_sweep_r<4, SumT, VOL_U16>(J, A, line, phase, fused, p2_table);
+    else batch_sweep_r<8, SumT, VOL_U16>(J, A, line, phase, fused, p2_table);
   }
 }
 
@@ -299,13 +305,13 @@ constexpr int kBatchMaxWaves = 8;
 // Waves per SIMD the register allocation must allow: the cost-volume uint16
 // instance fits 128 VGPRs (4 waves per SIMD) without spilling in its sweeps;
 // the recomputing and 32-bit instances need more and run at 2.
-template <typename SumT, bool RECOMP>
+template <typename SumT, int VOL>
 constexpr int batch_min_waves() {
-  return (sizeof(SumT) == 2 && !RECOMP) ? 4 : 2;
+  return (sizeof(SumT) == 2 && VOL != VOL_NONE) ? 4 : 2;
 }
 
-template <typename SumT, int WORDS, bool RECOMP>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(batch_min_waves<SumT, RECOMP>())))
+template <typename SumT, int WORDS, int VOL>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(batch_min_waves<SumT, VOL>())))
 void batch_kernel(BatchArgs B) {
   __shared__ int p2_table[256];
   __shared__ uint8_t tiles[kBatchMaxWaves][kBatchTileH * kBatchTileW];
@@ -372,11 +378,11 @@ void batch_kernel(BatchArgs B) {
       A.dx = dirs[dir][0];
       A.dy = dirs[dir][1];
       A.cls = A.dy == 0 ? 0 : A.dx == 0 ? 1 : 2;
-      int kind = dir == 0 ? (B.fused ? 1 : 0) : dir == B.directions - 1 ? 3 : 2;
-      if (B.cmode == CM_RECOMPUTE) kind = dir == 0 ? 4 : dir == B.directions - 1 ? 6 : 5;
+      const int phase = dir == 0 ? PHASE_FIRST : dir == B.directions - 1 ? PHASE_LAST : PHASE_MID;
+      const bool fused = B.cmode == CM_FUSED;
       const int nlines = lines_of_dev(A.cls, J.w, J.h);
       const int l0 = tile_idx * B.lines_per_item, l1 = min(nlines, l0 + B.lines_per_item);
-      for (int l = l0; l < l1; l++) batch_sweep<SumT, RECOMP>(J, A, l, kind, p2_table);
+      for (int l = l0; l < l1; l++) batch_sweep<SumT, VOL>(J, A, l, phase, fused, p2_table);
     } else {
       const BatchPairDev P = B.pairs[job];
       const int r0 = tile_idx * B.lr_rows, r1 = min(P.h, r0 + B.lr_rows);

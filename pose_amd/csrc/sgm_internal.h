@@ -38,6 +38,7 @@ int lanes_group(int D);
 int padded_disparities(int D);
 int words_of(const sgm_params *p);
 int invalid_code(const sgm_params *p);
+bool cost_u8(const sgm_params *p);
 int64_t direction_bytes(const sgm_params *p, int w, int h);
 int64_t direction_bytes(const sgm_params *p, int w, int h, bool cost_volume);
 bool census_recompute_possible(const sgm_params *p);

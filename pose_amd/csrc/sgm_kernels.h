@@ -158,7 +158,23 @@ struct CostJob {
 
 struct CostArgs {
   int mode, ww, wh, words, weight, adcap, gradcap, inv_code;
+  int u8;  // uint8 cost volume (census-family costs <= 120, 255 = invalid 512)
 };
+
+// Stores the costs of disparities k0 .. k0+3 of pixel p.
+__device__ __forceinline__ void store_cost4(uint16_t *cost, size_t p, int Dp, int k0,
+                                            const unsigned *v, bool u8) {
+  if (u8) {
+    const unsigned b = min(v[0], 255u) | (min(v[1], 255u) << 8) | (min(v[2], 255u) << 16) |
+                       (min(v[3], 255u) << 24);
+    *reinterpret_cast<uint32_t *>((uint8_t *)cost + p * Dp + k0) = b;
+  } else {
+    uint2 packed;
+    packed.x = v[0] | (v[1] << 16);
+    packed.y = v[2] | (v[3] << 16);
+    *reinterpret_cast<uint2 *>(cost + p * Dp + k0) = packed;
+  }
+}
 
 __device__ __forceinline__ int cost_value(const CostJob &J, const CostArgs &A, size_t p, size_t q,
                                           size_t n) {
@@ -245,10 +261,7 @@ static __global__ __launch_bounds__(256) void cost_kernel(CostJobs jobs, CostArg
     }
     v[j] = (unsigned)value & 0xFFFFu;
   }
-  uint2 packed;
-  packed.x = v[0] | (v[1] << 16);
-  packed.y = v[2] | (v[3] << 16);
-  *reinterpret_cast<uint2 *>(J.cost + p * J.Dp + k0) = packed;
+  store_cost4(J.cost, p, J.Dp, k0, v, A.u8 != 0);
 }
 
 }  // namespace sgm
@@ -336,6 +349,23 @@ static __global__ void fill_u8_kernel(uint8_t *p, uint8_t v, size_t n) {
 static __global__ void fill_u64_kernel(unsigned long long *p, unsigned long long v, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// Plain 16-byte-vector device copy (n vectors), grid-stride: the HBM
+// calibration kernel (reads n*16 B, writes n*16 B).
+static __global__ __launch_bounds__(256) void copy_kernel(const uint4 *__restrict__ src,
+                                                          uint4 *__restrict__ dst, size_t n) {
+  // four independent 16-byte loads in flight per thread and iteration
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+    dst[i] = a;
+    dst[i + stride] = b;
+    dst[i + 2 * stride] = c;
+    dst[i + 3 * stride] = d;
+  }
+  for (; i < n; i += stride) dst[i] = src[i];
 }
 
 }  // namespace sgm

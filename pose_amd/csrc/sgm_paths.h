@@ -558,7 +558,59 @@ __device__ __forceinline__ uint64_t fetch_target(const PathJob &J, __amdgpu_buff
 //                 whose bit 63 carries the texture&valid flag; no cost volume.
 //                 Horizontal lines slide a DPP-shifted descriptor window,
 //                 other lines load each pixel's target row segment (L2-resident).
-enum { CM_LOAD = 0, CM_FUSED = 1, CM_RECOMPUTE = 2 };
+//   CM_LOAD8:     read a uint8 cost volume (census-family costs are <= 120;
+//                 255 marks an invalid cell and is widened back to 512).
+// The fused sweep writes the uint8 volume that CM_LOAD8 reads.
+enum { CM_LOAD = 0, CM_FUSED = 1, CM_RECOMPUTE = 2, CM_LOAD8 = 3 };
+
+// R bytes per lane through a buffer resource (R = 2 .. 64, byte offset `off`).
+template <int R>
+__device__ __forceinline__ void bload_bytes(uint32_t *dst, __amdgpu_buffer_rsrc_t r, int off) {
+  if constexpr (R == 2) {
+    dst[0] = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+  } else if constexpr (R == 4) {
+    dst[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+  } else {
+    bload_pairs<R / 4>(dst, r, off);
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void bstore_bytes(const uint32_t *src, __amdgpu_buffer_rsrc_t r, int off) {
+  if constexpr (R == 2) {
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)src[0], r, off, 0, 0);
+  } else if constexpr (R == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(src[0], r, off, 0, 0);
+  } else {
+    bstore_pairs<R / 4>(src, r, off);
+  }
+}
+
+// uint8 costs -> packed uint16 pairs: pair i of the lane from raw bytes 2i, 2i+1;
+// 255 (invalid) becomes 512.  Each half is <= 255, so +1 never carries across
+// halves; t holds 0x100 in a half that was 255, t + (t >> 8) adds 257 there.
+template <int NP>
+__device__ __forceinline__ void widen_costs(const uint32_t *raw, uint32_t *c) {
+#pragma unroll
+  for (int i = 0; i < NP; i++) {
+    const uint32_t v = __builtin_amdgcn_perm(0u, raw[i >> 1], (i & 1) ? 0x0C030C02u : 0x0C010C00u);
+    const uint32_t t = (v + 0x00010001u) & 0x01000100u;
+    c[i] = v + t + (t >> 8);
+  }
+}
+
+// packed uint16 pairs (values <= 120 or 512) -> uint8 bytes, 512 -> 255.
+template <int NP>
+__device__ __forceinline__ void narrow_costs(const uint32_t *c, uint32_t *raw) {
+  if constexpr (NP == 1) {
+    raw[0] = __builtin_amdgcn_perm(0u, pk_min(c[0], 0x00FF00FFu), 0x0C0C0200u);
+  } else {
+#pragma unroll
+    for (int i = 0; i < NP / 2; i++)
+      raw[i] = __builtin_amdgcn_perm(pk_min(c[2 * i + 1], 0x00FF00FFu), pk_min(c[2 * i], 0x00FF00FFu),
+                                     0x06040200u);
+  }
+}
 
 // Per-step metadata of a window of PF steps, one step per lane (lane i holds
 // step base + (i % PF)): the edge-adaptive P2 of the step, or -1 when the
@@ -620,7 +672,7 @@ __device__ __forceinline__ Meta meta_finish(const PathJob &J, const PathArgs &A,
   const int g = g_raw & 0xFF, gp = gp_raw & 0xFF;
   const bool has = s > 0 && (gp_raw >> 8) != 0;
   m.p2 = has ? p2_table[abs(g - gp)] : -1;
-  if constexpr (CM != CM_LOAD) {
+  if constexpr (CM == CM_FUSED || CM == CM_RECOMPUTE) {
     const int x = x0 + s * A.dx, y = y0 + s * A.dy;
     m.ok = (y >= J.ry && y < J.h - J.ry && x >= J.rx && x < J.w - J.rx && (g_raw >> 8) != 0);
     m.da_lo = (uint32_t)da;
@@ -688,23 +740,25 @@ __device__ __forceinline__ void census_costs(const uint64_t *tw, uint32_t alo, u
   for (int i = 0; i < R / 2; i++) c[i] = (uint32_t)cv[2 * i] | ((uint32_t)cv[2 * i + 1] << 16);
 }
 
-template <int R, int PHASE, int CM>
+template <int R, int PHASE, int CM, int PFCAP = 64>
 struct PkSweep {
   static constexpr int NP = R / 2;
-  static constexpr int PF = CM == CM_FUSED ? 1 : pk_prefetch_depth<R>();
-  static constexpr bool kComputed = CM != CM_LOAD;
+  static constexpr int PF = pk_prefetch_depth<R>() < PFCAP ? pk_prefetch_depth<R>() : PFCAP;
+  static constexpr bool kComputed = CM == CM_FUSED || CM == CM_RECOMPUTE;
+  static constexpr bool kLoaded = CM == CM_LOAD || CM == CM_LOAD8;
+  static constexpr int NC = CM == CM_LOAD8 ? (R + 3) / 4 : NP;  // dwords of loaded cost per slot
   static constexpr int kDirect = CM == CM_RECOMPUTE ? PF : 1;  // direct target-row prefetch slots
 
   const PathJob &J;
   const PathArgs &A;
   const int *p2_table;
-  int lane, kbase, loff, x0, y0, len;
+  int lane, kbase, loff, boff, x0, y0, len;
   bool slide;  // computed cost on a horizontal line: sliding window
   ptrdiff_t p0, step_off;
-  uint32_t row_bytes;
+  uint32_t row_bytes, cost_row_bytes;
   uint32_t keep[NP], infm[NP];
   uint32_t vmask, p1pk;
-  uint32_t cb[CM == CM_LOAD ? PF : 1][NP], sb[PF][NP];
+  uint32_t cb[kLoaded ? PF : 1][NC], sb[PF][NP];
   uint64_t win[kComputed ? R : 1];
   uint64_t dsc[kDirect][CM == CM_RECOMPUTE ? R : 1];
   uint32_t prev[NP];
@@ -720,10 +774,12 @@ struct PkSweep {
     lane = threadIdx.x & 63;
     kbase = lane * R;
     loff = kbase < J.D ? kbase * 2 : kOutOfRange;
+    boff = kbase < J.D ? kbase : kOutOfRange;  // uint8 cost volume
     line_start(line, A.dx, A.dy, J.w, J.h, x0, y0, len);
     step_off = (ptrdiff_t)A.dy * J.w + A.dx;
     p0 = (ptrdiff_t)y0 * J.w + x0;
     row_bytes = (uint32_t)J.Dp * 2u;
+    cost_row_bytes = (CM == CM_LOAD8 || CM == CM_FUSED) ? (uint32_t)J.Dp : row_bytes;
     sum_store_bytes = (PHASE != PHASE_LAST || A.keep_sum) ? (int)row_bytes : 0;
     slide = CM == CM_FUSED || (CM == CM_RECOMPUTE && A.dy == 0);
 #pragma unroll
@@ -770,6 +826,8 @@ struct PkSweep {
     const ptrdiff_t pp = pix(min(s, len - 1));
     if constexpr (CM == CM_LOAD)
       bload_pairs<NP>(cb[slot], make_rsrc(J.cost + pp * J.Dp, row_bytes), loff);
+    if constexpr (CM == CM_LOAD8)
+      bload_bytes<R>(cb[slot], make_rsrc((const uint8_t *)J.cost + pp * J.Dp, cost_row_bytes), boff);
     if constexpr (CM == CM_RECOMPUTE)
       if (!slide) load_targets(dsc[slot], s);
     if constexpr (PHASE != PHASE_FIRST)
@@ -832,8 +890,13 @@ struct PkSweep {
         census_costs<R>(dsc[CM == CM_RECOMPUTE ? slot : 0], alo, ahi, ok, kbase, J.D,
                         A.inv_code, c);
       }
-      if constexpr (CM == CM_FUSED)
-        bstore_pairs<NP>(c, make_rsrc(J.cost_out + pp * J.Dp, row_bytes), loff);
+      if constexpr (CM == CM_FUSED) {
+        uint32_t raw[(R + 3) / 4];
+        narrow_costs<NP>(c, raw);
+        bstore_bytes<R>(raw, make_rsrc((uint8_t *)J.cost_out + pp * J.Dp, cost_row_bytes), boff);
+      }
+    } else if constexpr (CM == CM_LOAD8) {
+      widen_costs<NP>(cb[slot], c);
     } else {
 #pragma unroll
       for (int i = 0; i < NP; i++) c[i] = cb[slot][i];
@@ -923,11 +986,11 @@ struct PkSweep {
   }
 };
 
-template <int R, int PHASE, int CM>
+template <int R, int PHASE, int CM, int PFCAP = 64>
 __device__ __forceinline__ void sweep_line_pk(const PathJob &J, const PathArgs &A, int line,
                                               const int *p2_table) {
   static_assert(R % 2 == 0, "packed sweep needs an even lane group");
-  PkSweep<R, PHASE, CM> sw(J, A, line, p2_table);
+  PkSweep<R, PHASE, CM, PFCAP> sw(J, A, line, p2_table);
   sw.run();
 }
 
@@ -961,16 +1024,18 @@ struct PathJobs {
   PathJob local[2];
 };
 
-// MODE: 0 generic int sweep; 1 packed u16 reading the cost volume; 2 packed u16
-// first sweep with the census cost fused in (writes the cost volume); 3 packed
-// u16 with the census cost recomputed in the sweep (no cost volume).
+// MODE: 0 generic int sweep; 1 packed u16 reading the uint16 cost volume; 2
+// packed u16 first sweep with the census cost fused in (writes the uint8 cost
+// volume); 3 packed u16 with the census cost recomputed in the sweep (no cost
+// volume); 4 packed u16 reading the uint8 cost volume.
 template <int R, typename SumT, int PHASE, int MODE>
 __device__ __forceinline__ void sweep_dispatch(const PathJob &J, const PathArgs &A, int line,
                                                const int *p2_table) {
   if constexpr (MODE == 0) sweep_line<R, SumT, PHASE>(J, A, line, p2_table);
   else if constexpr (MODE == 1) sweep_line_pk<R, PHASE, CM_LOAD>(J, A, line, p2_table);
   else if constexpr (MODE == 2) sweep_line_pk<R, PHASE, CM_FUSED>(J, A, line, p2_table);
-  else sweep_line_pk<R, PHASE, CM_RECOMPUTE>(J, A, line, p2_table);
+  else if constexpr (MODE == 3) sweep_line_pk<R, PHASE, CM_RECOMPUTE>(J, A, line, p2_table);
+  else sweep_line_pk<R, PHASE, CM_LOAD8>(J, A, line, p2_table);
 }
 
 template <int R, typename SumT, int PHASE, int MODE>

@@ -95,7 +95,7 @@ EXPORTS = [
     "sgm_set_keep_volumes", "sgm_read_volumes", "sgm_set_cost_volume", "sgm_synchronize",
     "sgm_release_buffers", "sgm_set_profiling",
     "sgm_profile_json", "sgm_stream", "sgm_debug_fail_allocations", "sgm_last_error",
-    "sgm_device_alloc", "sgm_device_free", "sgm_memcpy",
+    "sgm_device_alloc", "sgm_device_free", "sgm_memcpy", "sgm_device_copy",
     "sgm_batch_create", "sgm_batch_run", "sgm_batch_destroy", "sgm_batch_device_bytes",
     "sgm_batch_describe",
     "sgm_autotune", "sgm_autotune_load",
@@ -147,6 +147,7 @@ def lib():
             L.sgm_device_alloc.argtypes = [P, ctypes.c_size_t, ctypes.POINTER(P)]
             L.sgm_device_free.argtypes = [P, P]
             L.sgm_memcpy.argtypes = [P, P, P, ctypes.c_size_t, I]
+            L.sgm_device_copy.argtypes = [P, P, P, ctypes.c_size_t]
             L.sgm_batch_create.argtypes = [P, pp, ctypes.POINTER(SgmPair), I, I, ctypes.POINTER(P)]
             L.sgm_batch_run.argtypes = [P]
             L.sgm_batch_destroy.argtypes = [P]
@@ -425,6 +426,10 @@ class Context:
         return lib().sgm_stream(self.h)
 
     # -- device buffers owned by the caller (plain device pointers) --
+    def device_copy(self, dst, src, nbytes):
+        """Asynchronous device-to-device copy by the library's streaming kernel."""
+        check(lib().sgm_device_copy(self.h, dst.ptr, src.ptr, int(nbytes)))
+
     def alloc(self, nbytes):
         ptr = ctypes.c_void_p()
         check(lib().sgm_device_alloc(self.h, int(nbytes), ctypes.byref(ptr)))

@@ -99,6 +99,26 @@ def test_batch_other_costs_bit_exact(gpu_context, kw):
         b.close()
 
 
+def test_batch_zncc_fp32_equals_plain(gpu_context):
+    """FP32 ZNCC (uint16 cost volume, 16-bit aggregates) in a batch: each pair equals
+    the plain GPU path bit for bit (same FP32 arithmetic)."""
+    kw = dict(window_w=7, window_h=7, cost=4, p1=12, p2=150, lr_threshold=1.0)
+    pairs = uneven_pairs(6, 4, wrange=(96, 260), drange=(3, 200))
+    params = to_sgm(O.params(**dict(kw, disparities=64)))
+    b = sgm.Batch(gpu_context, params, pairs)
+    try:
+        assert b.describe()["cost_volume"] == "resident-u16"
+        res = b.run()
+        for i, pr in enumerate(pairs):
+            p = to_sgm(O.params(**dict(kw, disparities=pr["disparities"],
+                                       min_disparity=pr["min_disparity"])))
+            want = gpu_context.match_pair(p, pr["left"], pr["right"], pr.get("valid_left"),
+                                          pr.get("valid_right"))
+            same(res[i], want, "pair %d" % i)
+    finally:
+        b.close()
+
+
 def test_batch_graph_replays_match_eager(gpu_context):
     """Graph replays with eager work between them all equal the eager result."""
     pairs = uneven_pairs(3, 5)

@@ -289,3 +289,23 @@ def test_zncc_fp32_against_fp64_oracle(gpu_context):
         one = np.isfinite(a) ^ np.isfinite(b)
         bad = (np.abs(a[both] - b[both]) > 1).sum() + one.sum()
         assert bad <= 0.001 * a.size, (k, int(bad))
+
+
+def test_zncc_fp32_workload_strip_1600_d192(gpu_context):
+    """The ZNCC workload's shape (1600 wide, D=192, 7x7, P1=12, P2=150, gain field
+    0.8-1.2) on a 1600x160 strip the FP64 oracle finishes in seconds: at most 0.1%
+    of pixels with |dd| > 1 (or valid on one side only)."""
+    w, h = 1600, 160
+    left, right, _ = synth.stereo_pair(w, h, 192, 0, seed=33, gain=True)
+    op = O.params(window_w=7, window_h=7, disparities=192, cost=4, p1=12, p2=150,
+                  lr_threshold=1.0)
+    print("zncc strip", w, h, flush=True)
+    want = O.compute(op, left, right, w, h)
+    got = gpu_context.match_pair(to_sgm(op), left, right)
+    for k in ("dl", "dr", "checked"):
+        a, b = got[k], want[k]
+        both = np.isfinite(a) & np.isfinite(b)
+        one = np.isfinite(a) ^ np.isfinite(b)
+        bad = int((np.abs(a[both] - b[both]) > 1).sum() + one.sum())
+        print(k, "differ", bad, "of", a.size, flush=True)
+        assert bad <= 0.001 * a.size, (k, bad)
