@@ -132,6 +132,12 @@ extern "C" int sgm_batch_create(sgm_context *ctx, const sgm_params *params, cons
 
 namespace sgm {
 
+bool batch_cost8(const sgm_params *params) {
+  // uint8 cost volume in a batch: the census-family costs without ZNCC (the
+  // ZNCC window arithmetic stays in the uint16-volume kernel instance)
+  return cost_u8(params) && params->cost != SGM_COST_ZNCC;
+}
+
 bool batch_recompute_possible(const sgm_params *params, bool keep_volumes) {
   // every pair of a batch has R <= 8 (validate_pair): 1-word census with
   // uint16 aggregates can recompute its cost in the sweeps
@@ -164,7 +170,7 @@ size_t batch_arena_bytes(const sgm_params *params, const sgm_pair *pairs, int co
         cv.take(n * 4);
         cv.take(n * 4);
       }
-      if (!recompute) cv.take(n * Dp * (cost_u8(params) ? 1 : 2));
+      if (!recompute) cv.take(n * Dp * (batch_cost8(params) ? 1 : 2));
       cv.take(n * Dp * sb);
       const bool own_disp = host_io || (d == 0 ? pr.dl == nullptr : pr.dr == nullptr);
       if (own_disp) cv.take(n * 4);
@@ -227,7 +233,7 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
     }
   }
   b->recompute = possible && recompute != 0;
-  b->cost8 = !b->recompute && cost_u8(params);
+  b->cost8 = !b->recompute && batch_cost8(params);
   b->fused = !b->recompute && params->cost == SGM_COST_CENSUS && b->words == 1 &&
              b->sum_bytes == 2;
   b->waves_per_block = ctx->batch_waves_per_block;
@@ -489,6 +495,7 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
   A.lr_rows = lr_rows;
   A.pa = path_args(params, 0, 0, 0, ctx->keep_volumes ? 1 : 0);
   A.ca = cost_args(params);
+  A.ca.u8 = b->cost8 ? 1 : 0;
   A.ww = params->window_w;
   A.wh = params->window_h;
   A.lr_threshold = params->lr_threshold;

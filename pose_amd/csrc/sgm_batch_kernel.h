@@ -109,7 +109,7 @@ constexpr int kBatchTileW = 64 + 10, kBatchTileH = kBatchCensusRows + 10;
 template <int WORDS>
 __device__ __forceinline__ void batch_census(const BatchArgs &B, const BatchImage &I, int r0,
                                              int r1, uint8_t *tile) {
-  const int lane = threadIdx.x & 63;
+  const int lane = lane_id();
   const int rx = B.ww >> 1, ry = B.wh >> 1;
   const size_t n = (size_t)I.w * I.h;
   for (int x0 = 0; x0 < I.w; x0 += 64) {
@@ -166,8 +166,9 @@ __device__ __forceinline__ void batch_census(const BatchArgs &B, const BatchImag
 }
 
 // Cost volume rows [r0, r1) of one matching direction (non-fused costs).
+template <bool ZNCC>
 __device__ __forceinline__ void batch_cost(const BatchArgs &B, const CostJob &J, int r0, int r1) {
-  const int lane = threadIdx.x & 63;
+  const int lane = lane_id();
   const int rx = B.ww >> 1, ry = B.wh >> 1;
   const size_t n = (size_t)J.w * J.h;
   const int quads = J.Dp >> 2;
@@ -186,7 +187,7 @@ __device__ __forceinline__ void batch_cost(const BatchArgs &B, const CostJob &J,
           const long long t = (long long)x + (J.reverse ? (long long)disp : -(long long)disp);
           if (t >= rx && t < J.w - rx) {
             const size_t q = (size_t)y * J.w + (size_t)t;
-            if (J.mask_b[q]) value = cost_value(J, B.ca, p, q, n);
+            if (J.mask_b[q]) value = cost_value<ZNCC>(J, B.ca, p, q, n);
           }
         }
         v[j] = (unsigned)value & 0xFFFFu;
@@ -198,7 +199,7 @@ __device__ __forceinline__ void batch_cost(const BatchArgs &B, const CostJob &J,
 
 // Left-right check + counters of rows [r0, r1) of one pair (as lr_kernel).
 __device__ __forceinline__ void batch_lr(const BatchArgs &B, const BatchPairDev &P, int r0, int r1) {
-  const int lane = threadIdx.x & 63;
+  const int lane = lane_id();
   const int rx = B.ww >> 1, ry = B.wh >> 1;
   unsigned c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int y = r0; y < r1; y++)
@@ -302,12 +303,12 @@ __device__ __forceinline__ void batch_sweep(const PathJob &J, const PathArgs &A,
 
 constexpr int kBatchMaxWaves = 8;
 
-// Waves per SIMD the register allocation must allow: the cost-volume uint16
-// instance fits 128 VGPRs (4 waves per SIMD) without spilling in its sweeps;
-// the recomputing and 32-bit instances need more and run at 2.
+// Waves per SIMD the register allocation must allow, the most each instance
+// reaches without spilling: cost-volume instances 128 VGPRs (4 waves),
+// recomputed census costs 256 (2).
 template <typename SumT, int VOL>
 constexpr int batch_min_waves() {
-  return (sizeof(SumT) == 2 && VOL != VOL_NONE) ? 4 : 2;
+  return VOL == VOL_NONE ? 2 : 4;
 }
 
 template <typename SumT, int WORDS, int VOL>
@@ -317,7 +318,7 @@ void batch_kernel(BatchArgs B) {
   __shared__ uint8_t tiles[kBatchMaxWaves][kBatchTileH * kBatchTileW];
   build_p2_table(p2_table, B.pa);
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = lane_id(), wave = threadIdx.x >> 6;
   uint8_t *tile = tiles[wave];
   const int total = B.phase_begin[B.nph];
   static const int dirs[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
@@ -370,7 +371,7 @@ void batch_kernel(BatchArgs B) {
     } else if (ph == BPH_COST) {
       const CostJob J = B.cost_jobs[job];
       const int r0 = tile_idx * B.cost_rows, r1 = min(J.h, r0 + B.cost_rows);
-      batch_cost(B, J, r0, r1);
+      batch_cost<VOL == VOL_U16>(B, J, r0, r1);  // ZNCC costs only use uint16 volumes
     } else if (ph <= last_path) {
       const PathJob J = B.jobs[job];
       const int dir = ph - BPH_PATH0;

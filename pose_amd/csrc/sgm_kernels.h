@@ -176,10 +176,13 @@ __device__ __forceinline__ void store_cost4(uint16_t *cost, size_t p, int Dp, in
   }
 }
 
+// ZNCC = false compiles only the census-family modes 0..2 (register budget of
+// the batched kernel instance that never sees ZNCC).
+template <bool ZNCC = true>
 __device__ __forceinline__ int cost_value(const CostJob &J, const CostArgs &A, size_t p, size_t q,
                                           size_t n) {
   const int bits = A.ww * A.wh - 1;
-  if (A.mode <= 2) {
+  if (!ZNCC || A.mode <= 2) {
     int ham = __popcll(J.desc_a[p] ^ J.desc_b[q]);
     if (A.words > 1) ham += __popcll(J.desc_a[n + p] ^ J.desc_b[n + q]);
     if (A.mode == 0) return ham;
@@ -199,6 +202,7 @@ __device__ __forceinline__ int cost_value(const CostJob &J, const CostArgs &A, s
     const int den = 1000 * cap;
     return (A.weight * ham * cap + (1000 - A.weight) * min(difference, cap) * bits + den / 2) / den;
   }
+  if constexpr (!ZNCC) return A.inv_code;
   // ZNCC: exact integer window sums.
   const int rx = A.ww >> 1, ry = A.wh >> 1, w = J.w;
   int sab = 0;

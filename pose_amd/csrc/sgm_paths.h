@@ -18,6 +18,17 @@
 
 namespace sgm {
 
+// Lane index within the wave, produced by inline asm so the compiler treats it
+// as opaque: values derived from it are recomputed where used instead of being
+// hoisted out of the batched kernel's work loop, where they would stay live in
+// VGPRs across every inlined sweep variant (2 instructions per use).
+__device__ __forceinline__ int lane_id() {
+  int l;
+  asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+  return l;
+}
+
+
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 struct PathJob {
@@ -156,7 +167,7 @@ __device__ __forceinline__ Winner wta_reduce(const int *S, const int *valid, int
 struct Deferred {
   int best, value, second, before, after, flags;
   __device__ __forceinline__ void put(const Winner &w, int slot) {
-    const bool mine = (int)(threadIdx.x & 63) == slot;
+    const bool mine = (int)(lane_id()) == slot;
     best = mine ? w.best : best;
     value = mine ? w.value : value;
     second = mine ? w.second : second;
@@ -166,15 +177,100 @@ struct Deferred {
   }
 };
 
-__device__ __forceinline__ void select_flush(const PathJob &J, const PathArgs &A,
-                                             const Deferred &d, int count, int base, int x0,
-                                             int y0) {
-  const int lane = threadIdx.x & 63;
+// Per-lane winner summary of one pixel, decoded (the fields of Winner).
+struct WinnerView {
+  int best, value, second, before, after, flags;
+};
+
+// Winner summaries of up to 64 pixel steps of the packed sweep, lane i holding
+// step i, in three registers:
+//   f0 = (best + 1) | flags << 13      (best + 1 = 0: no valid disparity)
+//   f1 = value | second << 16          (second 0xFFFF: none)
+//   f2 = before | after << 16          (0xFFFF: neighbour invalid or absent)
+// Aggregates are at most 65532 (dirs * (512 + P2) <= 65535 with dirs 4 or 8),
+// so 0xFFFF never collides with a value.
+struct DeferredPk {
+  uint32_t f0, f1, f2;
+  __device__ __forceinline__ void put(bool mine, uint32_t a, uint32_t b, uint32_t c) {
+    f0 = mine ? a : f0;
+    f1 = mine ? b : f1;
+    f2 = mine ? c : f2;
+  }
+  __device__ __forceinline__ WinnerView view() const {
+    WinnerView w;
+    w.best = (int)(f0 & 0x1FFFu) - 1;
+    w.flags = (int)(f0 >> 13);
+    w.value = (int)(f1 & 0xFFFFu);
+    w.second = (f1 >> 16) == 0xFFFFu ? 0x7FFFFFFF : (int)(f1 >> 16);
+    w.before = (f2 & 0xFFFFu) == 0xFFFFu ? -1 : (int)(f2 & 0xFFFFu);
+    w.after = (f2 >> 16) == 0xFFFFu ? -1 : (int)(f2 >> 16);
+    return w;
+  }
+};
+
+// The uniform inputs of select_flush, one per lane of a single VGPR.  The
+// flush runs once per 64 steps; holding its ~20 scalars in SGPRs across the
+// sweep loop would exhaust the SGPR file (the compiler then spills SGPRs to
+// VGPR lanes inside the hot loop), so they are parked here and read back with
+// readlane at flush time.
+struct FlushArgs {
+  int v;
+  __device__ __forceinline__ int get(int i) const { return __builtin_amdgcn_readlane(v, i); }
+  template <typename T>
+  __device__ __forceinline__ T *ptr(int i) const {
+    return (T *)(((uint64_t)(uint32_t)get(i + 1) << 32) | (uint32_t)get(i));
+  }
+};
+
+enum {
+  FA_DISP = 0, FA_CODE = 2, FA_REASONS = 4, FA_BOUNDARY = 6, FA_W = 8, FA_H, FA_RX, FA_RY,
+  FA_D, FA_MIND, FA_DX, FA_DY, FA_UNIQ = 16, FA_COUNT = 18
+};
+
+__device__ __forceinline__ FlushArgs make_flush_args(const PathJob &J, const PathArgs &A) {
+  const uint64_t ptrs[4] = {(uint64_t)J.disp, (uint64_t)J.code, (uint64_t)J.reasons,
+                            (uint64_t)J.boundary};
+  const uint64_t uq = __builtin_bit_cast(uint64_t, A.uniq);
+  int vals[FA_COUNT];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    vals[2 * i] = (int)(uint32_t)ptrs[i];
+    vals[2 * i + 1] = (int)(uint32_t)(ptrs[i] >> 32);
+  }
+  vals[FA_W] = J.w;
+  vals[FA_H] = J.h;
+  vals[FA_RX] = J.rx;
+  vals[FA_RY] = J.ry;
+  vals[FA_D] = J.D;
+  vals[FA_MIND] = J.minD;
+  vals[FA_DX] = A.dx;
+  vals[FA_DY] = A.dy;
+  vals[FA_UNIQ] = (int)(uint32_t)uq;
+  vals[FA_UNIQ + 1] = (int)(uint32_t)(uq >> 32);
+  const int lane = lane_id();
+  int v = 0;
+#pragma unroll
+  for (int i = 0; i < FA_COUNT; i++) v = lane == i ? vals[i] : v;
+  asm volatile("" : "+v"(v));
+  return FlushArgs{v};
+}
+
+__device__ __forceinline__ void select_flush(const FlushArgs &F, const WinnerView &d, int count,
+                                             int base, int x0, int y0) {
+  const int lane = lane_id();
   if (lane >= count) return;
+  const int Jw = F.get(FA_W), Jh = F.get(FA_H), Jrx = F.get(FA_RX), Jry = F.get(FA_RY);
+  const int JD = F.get(FA_D), JminD = F.get(FA_MIND), Adx = F.get(FA_DX), Ady = F.get(FA_DY);
+  const double uniq = __builtin_bit_cast(
+      double, ((uint64_t)(uint32_t)F.get(FA_UNIQ + 1) << 32) | (uint32_t)F.get(FA_UNIQ));
+  float *disp = F.ptr<float>(FA_DISP);
+  uint8_t *codes = F.ptr<uint8_t>(FA_CODE);
+  uint8_t *reasons = F.ptr<uint8_t>(FA_REASONS);
+  uint8_t *boundary = F.ptr<uint8_t>(FA_BOUNDARY);
   const int s = base + lane;
-  const int px = x0 + s * A.dx, py = y0 + s * A.dy;
-  const size_t pp = (size_t)py * J.w + px;
-  const bool interior = px >= J.rx && px < J.w - J.rx && py >= J.ry && py < J.h - J.ry;
+  const int px = x0 + s * Adx, py = y0 + s * Ady;
+  const size_t pp = (size_t)py * Jw + px;
+  const bool interior = px >= Jrx && px < Jw - Jrx && py >= Jry && py < Jh - Jry;
   float out = __builtin_nanf("");
   int reason = -1, bflag = -1, code = 0;
   const int best = d.best, value = d.value;
@@ -184,19 +280,19 @@ __device__ __forceinline__ void select_flush(const PathJob &J, const PathArgs &A
       bflag = 4;
       code |= CODE_FULL;
     }
-    if (best == 0 || best == J.D - 1) {
+    if (best == 0 || best == JD - 1) {
       reason = 2;
       code |= CODE_BOUNDARY;
       if (full && best == 0) {
         bflag |= 1;
         code |= CODE_LOWER;
       }
-      if (full && best == J.D - 1) {
+      if (full && best == JD - 1) {
         bflag |= 2;
         code |= CODE_UPPER;
       }
     } else {
-      double allowed = (double)value * A.uniq;
+      double allowed = (double)value * uniq;
       if (!(allowed >= 1.0)) allowed = 1.0;
       if (d.second != 0x7FFFFFFF && (double)(d.second - value) <= allowed) {
         reason = 3;
@@ -211,16 +307,16 @@ __device__ __forceinline__ void select_flush(const PathJob &J, const PathArgs &A
             offset = t > -.5 ? t : -.5;
           }
         }
-        out = (float)((double)(J.minD + best) + offset);
+        out = (float)((double)(JminD + best) + offset);
         reason = 0;
         code |= CODE_SELECTED;
       }
     }
   }
-  J.disp[pp] = out;
-  if (J.code) J.code[pp] = (uint8_t)code;
-  if (J.reasons && reason >= 0) J.reasons[pp] = (uint8_t)reason;
-  if (J.boundary && bflag >= 0) J.boundary[pp] = (uint8_t)bflag;
+  disp[pp] = out;
+  if (codes) codes[pp] = (uint8_t)code;
+  if (reasons && reason >= 0) reasons[pp] = (uint8_t)reason;
+  if (boundary && bflag >= 0) boundary[pp] = (uint8_t)bflag;
 }
 
 // Per-64-step chunk of a per-pixel value along the line, loaded one chunk ahead.
@@ -341,7 +437,7 @@ template <int R, typename SumT, int PHASE>
 __device__ __forceinline__ void sweep_line(const PathJob &J, const PathArgs &A, int line,
                                            const int *p2_table) {
   constexpr int PF = prefetch_depth<R>();
-  const int lane = threadIdx.x & 63;
+  const int lane = lane_id();
   const int kbase = lane * R;
   const bool lane_mem = kbase < J.D;
   int x, y, len;
@@ -451,7 +547,10 @@ __device__ __forceinline__ void sweep_line(const PathJob &J, const PathArgs &A, 
         for (int j = 0; j < R; j++) valid[j] = (kbase + j < J.D) && (c[j] != A.inv_code);
         const Winner win = wta_reduce<R, sizeof(SumT) == 4>(S, valid, kbase, J.D);
         dfr.put(win, s & 63);
-        if ((s & 63) == 63 || s == len - 1) select_flush(J, A, dfr, (s & 63) + 1, s & ~63, x, y);
+        if ((s & 63) == 63 || s == len - 1)
+          select_flush(make_flush_args(J, A),
+                       WinnerView{dfr.best, dfr.value, dfr.second, dfr.before, dfr.after, dfr.flags},
+                       (s & 63) + 1, s & ~63, x, y);
       }
     }
   }
@@ -646,7 +745,7 @@ __device__ __forceinline__ void meta_load(const PathJob &J, const PathArgs &A, c
                                           int base, int len, ptrdiff_t p0, ptrdiff_t step_off,
                                           int x0, int &g_raw, int &gp_raw, uint64_t &da,
                                           uint64_t &e) {
-  const int lane = threadIdx.x & 63;
+  const int lane = lane_id();
   const int s = min(base + (lane % PF), len - 1);
   const int sp = max(s - 1, 0);
   const ptrdiff_t ps = p0 + (ptrdiff_t)s * step_off;
@@ -666,7 +765,7 @@ template <int PF, int CM>
 __device__ __forceinline__ Meta meta_finish(const PathJob &J, const PathArgs &A, int base,
                                             int len, int x0, int y0, int g_raw, int gp_raw,
                                             uint64_t da, uint64_t e, const int *p2_table) {
-  const int lane = threadIdx.x & 63;
+  const int lane = lane_id();
   const int s = min(base + (lane % PF), len - 1);
   Meta m;
   const int g = g_raw & 0xFF, gp = gp_raw & 0xFF;
@@ -684,43 +783,6 @@ __device__ __forceinline__ Meta meta_finish(const PathJob &J, const PathArgs &A,
     m.da_lo = m.da_hi = m.e_lo = m.e_hi = 0;
   }
   return m;
-}
-
-// Branch-free winner summary (the branchy wta_reduce, with the dependent
-// reductions evaluated unconditionally; their results are ignored when no
-// disparity is valid).
-template <int R>
-__device__ __forceinline__ Winner wta_reduce_pk(const int *S, const int *valid, int kbase, int D) {
-  Winner r;
-  unsigned key = 0xFFFFFFFFu;
-#pragma unroll
-  for (int j = 0; j < R; j++)
-    key = valid[j] ? min(key, ((unsigned)S[j] << 12) | (unsigned)(kbase + j)) : key;
-  key = wave_min_u(key);
-  const bool any = key != 0xFFFFFFFFu;
-  const int best = any ? (int)(key & 0xFFFu) : -1;
-  r.best = best;
-  r.value = (int)(key >> 12);
-  int sec = 0x7FFFFFFF;
-#pragma unroll
-  for (int j = 0; j < R; j++)
-    sec = (valid[j] && abs(kbase + j - best) > 1) ? min(sec, S[j]) : sec;
-  r.second = wave_min(sec);
-  const int kb = max(best - 1, 0), ka = min(best + 1, D - 1);
-  int vb = -1, va = -1, vlast = -1;
-#pragma unroll
-  for (int j = 0; j < R; j++) {
-    const int k = kbase + j;
-    vb = (k == kb) ? (valid[j] ? S[j] : -1) : vb;
-    va = (k == ka) ? (valid[j] ? S[j] : -1) : va;
-    vlast = (k == D - 1) ? valid[j] : vlast;
-  }
-  r.before = best > 0 ? __builtin_amdgcn_readlane(vb, kb / R) : -1;
-  r.after = best + 1 < D ? __builtin_amdgcn_readlane(va, ka / R) : -1;
-  const int v0 = __builtin_amdgcn_readlane(valid[0], 0);
-  const int vl = __builtin_amdgcn_readlane(vlast, (D - 1) / R);
-  r.flags = (v0 ? 1 : 0) | (vl > 0 ? 2 : 0);
-  return r;
 }
 
 // Census Hamming costs of one lane's R disparities from flagged target
@@ -756,6 +818,9 @@ struct PkSweep {
   bool slide;  // computed cost on a horizontal line: sliding window
   ptrdiff_t p0, step_off;
   uint32_t row_bytes, cost_row_bytes;
+  const uint8_t *ld_c, *ld_s;  // rows of the next prefetched step
+  uint8_t *st_c, *st_s;        // rows of the next stored step
+  ptrdiff_t cstep, sstep;      // bytes per pixel step in the cost / aggregate volume
   uint32_t keep[NP], infm[NP];
   uint32_t vmask, p1pk;
   uint32_t cb[kLoaded ? PF : 1][NC], sb[PF][NP];
@@ -765,13 +830,17 @@ struct PkSweep {
   int minp;
   LineRsrc rs;
   Meta meta;
-  Deferred dfr;
+  DeferredPk dfr;
+  FlushArgs flush_args;
+  uint32_t kpk[PHASE == PHASE_LAST ? NP : 1], keep01[PHASE == PHASE_LAST ? NP : 1];
+  uint32_t invpk;
+  int last_lane, last_pair, last_half;
   int sum_store_bytes;
 
   __device__ __forceinline__ PkSweep(const PathJob &J_, const PathArgs &A_, int line,
                                      const int *tab)
       : J(J_), A(A_), p2_table(tab) {
-    lane = threadIdx.x & 63;
+    lane = lane_id();
     kbase = lane * R;
     loff = kbase < J.D ? kbase * 2 : kOutOfRange;
     boff = kbase < J.D ? kbase : kOutOfRange;  // uint8 cost volume
@@ -780,6 +849,15 @@ struct PkSweep {
     p0 = (ptrdiff_t)y0 * J.w + x0;
     row_bytes = (uint32_t)J.Dp * 2u;
     cost_row_bytes = (CM == CM_LOAD8 || CM == CM_FUSED) ? (uint32_t)J.Dp : row_bytes;
+    {
+      const ptrdiff_t cb_ = (CM == CM_LOAD8 || CM == CM_FUSED) ? 1 : 2;
+      cstep = step_off * J.Dp * cb_;
+      sstep = step_off * J.Dp * 2;
+      ld_c = (const uint8_t *)J.cost + p0 * J.Dp * cb_;
+      st_c = (uint8_t *)J.cost_out + p0 * J.Dp * cb_;
+      ld_s = (const uint8_t *)J.sum + p0 * J.Dp * 2;
+      st_s = (uint8_t *)J.sum + p0 * J.Dp * 2;
+    }
     sum_store_bytes = (PHASE != PHASE_LAST || A.keep_sum) ? (int)row_bytes : 0;
     slide = CM == CM_FUSED || (CM == CM_RECOMPUTE && A.dy == 0);
 #pragma unroll
@@ -796,7 +874,21 @@ struct PkSweep {
     vmask = A.val_mask == 0xFFFF ? 0xFFFFFFFFu : splat16((uint32_t)A.val_mask);
     p1pk = splat16((uint32_t)A.p1);
     minp = 0;
-    dfr = Deferred{};
+    dfr = DeferredPk{0u, 0u, 0u};
+    flush_args = FlushArgs{0};
+    invpk = splat16((uint32_t)A.inv_code);
+    last_lane = last_pair = last_half = 0;
+    if constexpr (PHASE == PHASE_LAST) {
+      flush_args = make_flush_args(J, A);
+#pragma unroll
+      for (int i = 0; i < NP; i++) {
+        kpk[i] = (uint32_t)(kbase + 2 * i) | ((uint32_t)(kbase + 2 * i + 1) << 16);
+        keep01[i] = keep[i] & 0x00010001u;
+      }
+      last_lane = (J.D - 1) / R;
+      last_pair = ((J.D - 1) % R) >> 1;
+      last_half = (J.D - 1) & 1;
+    }
     const uint32_t n = (uint32_t)J.w * (uint32_t)J.h;
     rs.gm = make_rsrc(J.gm, n * 2u);
     if constexpr (kComputed) {
@@ -822,16 +914,21 @@ struct PkSweep {
     }
   }
 
+  // Loads of pixel step s (called for s = 0, 1, 2, ... in order): the row
+  // pointers advance by one step each call instead of being recomputed with
+  // 64-bit multiplies; steps past the line end get an empty range (no access).
   __device__ __forceinline__ void load_step(int slot, int s) {
-    const ptrdiff_t pp = pix(min(s, len - 1));
+    const bool in = s < len;
     if constexpr (CM == CM_LOAD)
-      bload_pairs<NP>(cb[slot], make_rsrc(J.cost + pp * J.Dp, row_bytes), loff);
+      bload_pairs<NP>(cb[slot], make_rsrc(ld_c, in ? row_bytes : 0u), loff);
     if constexpr (CM == CM_LOAD8)
-      bload_bytes<R>(cb[slot], make_rsrc((const uint8_t *)J.cost + pp * J.Dp, cost_row_bytes), boff);
+      bload_bytes<R>(cb[slot], make_rsrc(ld_c, in ? cost_row_bytes : 0u), boff);
     if constexpr (CM == CM_RECOMPUTE)
       if (!slide) load_targets(dsc[slot], s);
     if constexpr (PHASE != PHASE_FIRST)
-      bload_pairs<NP>(sb[slot], make_rsrc((const uint16_t *)J.sum + pp * J.Dp, row_bytes), loff);
+      bload_pairs<NP>(sb[slot], make_rsrc(ld_s, in ? row_bytes : 0u), loff);
+    ld_c += cstep;
+    ld_s += sstep;
   }
 
   __device__ __forceinline__ void init_window() {
@@ -874,7 +971,6 @@ struct PkSweep {
   // One pixel step with metadata lane `ml` (uniform, compile-time in the
   // unrolled loop) and prefetch slot `slot`.
   __device__ __forceinline__ void step(int s, int slot, int ml) {
-    const ptrdiff_t pp = pix(s);
     const int p2h = __builtin_amdgcn_readlane(meta.p2, ml);
     uint32_t c[NP];
     if constexpr (kComputed) {
@@ -893,7 +989,8 @@ struct PkSweep {
       if constexpr (CM == CM_FUSED) {
         uint32_t raw[(R + 3) / 4];
         narrow_costs<NP>(c, raw);
-        bstore_bytes<R>(raw, make_rsrc((uint8_t *)J.cost_out + pp * J.Dp, cost_row_bytes), boff);
+        bstore_bytes<R>(raw, make_rsrc(st_c, cost_row_bytes), boff);
+        st_c += cstep;
       }
     } else if constexpr (CM == CM_LOAD8) {
       widen_costs<NP>(cb[slot], c);
@@ -927,21 +1024,72 @@ struct PkSweep {
     uint32_t S[NP];
 #pragma unroll
     for (int i = 0; i < NP; i++) S[i] = PHASE == PHASE_FIRST ? L[i] : pk_add(sb[slot][i], L[i]);
-    bstore_pairs<NP>(S, make_rsrc((uint16_t *)J.sum + pp * J.Dp, (uint32_t)sum_store_bytes), loff);
+    bstore_pairs<NP>(S, make_rsrc(st_s, (uint32_t)sum_store_bytes), loff);
+    st_s += sstep;
     minp = wave_min(lmin);
 #pragma unroll
     for (int i = 0; i < NP; i++) prev[i] = L[i];
-    if constexpr (PHASE == PHASE_LAST) {
-      int Si[R], valid[R];
+    if constexpr (PHASE == PHASE_LAST) select(S, c, s);
+  }
+
+  // Pair i (uniform) of the lane group, selected without dynamic register indexing.
+  __device__ __forceinline__ static uint32_t pick_pair(const uint32_t *v, int i) {
+    uint32_t r = v[0];
 #pragma unroll
-      for (int i = 0; i < NP; i++) {
-        Si[2 * i] = (int)(S[i] & 0xFFFFu);
-        Si[2 * i + 1] = (int)(S[i] >> 16);
-        valid[2 * i] = (kbase + 2 * i < J.D) && ((c[i] & 0xFFFFu) != (uint32_t)A.inv_code);
-        valid[2 * i + 1] = (kbase + 2 * i + 1 < J.D) && ((c[i] >> 16) != (uint32_t)A.inv_code);
-      }
-      dfr.put(wta_reduce_pk<R>(Si, valid, kbase, J.D), s & 63);
+    for (int j = 1; j < NP; j++) r = i == j ? v[j] : r;
+    return r;
+  }
+
+  // Masked aggregate of disparity k (uniform, 0 <= k < D): 0xFFFF when invalid.
+  __device__ __forceinline__ static uint32_t cell(const uint32_t *Sm, int k) {
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)pick_pair(Sm, (k % R) >> 1), k / R);
+    return (k & 1) ? (w >> 16) : (w & 0xFFFFu);
+  }
+
+  // Winner-take-all summary of this step (the reference's first argmin over
+  // valid disparities, the second best at |k - best| > 1, the aggregates beside
+  // the winner and the validity of both range ends; CensusSgm.java:460-515),
+  // on packed pairs: invalid cells (cost == invalid code, or k >= D) are masked
+  // to 0xFFFF, above every aggregate.
+  __device__ __forceinline__ void select(const uint32_t *S, const uint32_t *c, int s) {
+    uint32_t Sm[NP];
+    unsigned key = 0xFFFFFFFFu;
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+      const uint32_t ne = pk_min(c[i] ^ invpk, 0x00010001u) & keep01[i];  // 1: valid
+      Sm[i] = S[i] | pk_sub(ne, 0x00010001u);
+      const unsigned k0 = ((Sm[i] & 0xFFFFu) << 12) | (kpk[i] & 0xFFFFu);
+      const unsigned k1 = ((Sm[i] >> 16) << 12) | (kpk[i] >> 16);
+      key = min(key, min(k0, k1));
     }
+    key = wave_min_u(key);
+    const bool any = key < (0xFFFFu << 12);
+    const int best = (int)(key & 0xFFFu);
+    // second best: cells with |k - best| <= 1 masked to 0xFFFF
+    const u16x2 bm1 = as_pk(splat16((uint32_t)(best - 1) & 0xFFFFu));
+    const u16x2 two = as_pk(0x00020002u);
+    uint32_t sec = 0xFFFFFFFFu;
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+      const u16x2 e = __builtin_elementwise_sub_sat(as_pk(kpk[i]) - bm1, two);  // 0: excluded
+      const uint32_t far = pk_min(as_u32(e), 0x00010001u);
+      sec = pk_min(sec, Sm[i] | pk_sub(far, 0x00010001u));
+    }
+    const unsigned second = wave_min_u(min(sec & 0xFFFFu, sec >> 16));
+    uint32_t f0 = 0u, f1 = 0u, f2 = 0xFFFFFFFFu;
+    if (any) {
+      const uint32_t before = best > 0 ? cell(Sm, best - 1) : 0xFFFFu;
+      const uint32_t after = best + 1 < J.D ? cell(Sm, best + 1) : 0xFFFFu;
+      const uint32_t v0 = (uint32_t)__builtin_amdgcn_readlane((int)Sm[0], 0) & 0xFFFFu;
+      const uint32_t wl =
+          (uint32_t)__builtin_amdgcn_readlane((int)pick_pair(Sm, last_pair), last_lane);
+      const uint32_t vl = last_half ? (wl >> 16) : (wl & 0xFFFFu);
+      const uint32_t flags = (v0 != 0xFFFFu ? 1u : 0u) | (vl != 0xFFFFu ? 2u : 0u);
+      f0 = (uint32_t)(best + 1) | (flags << 13);
+      f1 = ((key >> 12) & 0xFFFFu) | (second << 16);
+      f2 = before | (after << 16);
+    }
+    dfr.put(lane == (s & 63), f0, f1, f2);
   }
 
   __device__ __forceinline__ void maybe_flush(int s_done) {
@@ -949,7 +1097,7 @@ struct PkSweep {
     if constexpr (PHASE == PHASE_LAST) {
       if ((s_done & 63) == 0 || s_done == len) {
         const int base = (s_done - 1) & ~63;
-        select_flush(J, A, dfr, s_done - base, base, x0, y0);
+        select_flush(flush_args, dfr.view(), s_done - base, base, x0, y0);
       }
     }
   }
@@ -964,16 +1112,29 @@ struct PkSweep {
 #pragma unroll
     for (int j = 0; j < PF; j++) load_step(j, j);
     int s0 = 0;
-    for (; s0 + PF <= len; s0 += PF) {
+    auto window = [&](int w0) {
       // metadata of the next window, consumed next iteration
-      meta_load<PF, CM>(J, A, rs, s0 + PF, len, p0, step_off, x0, g_raw, gp_raw, da, e);
+      meta_load<PF, CM>(J, A, rs, w0 + PF, len, p0, step_off, x0, g_raw, gp_raw, da, e);
 #pragma unroll
       for (int jj = 0; jj < PF; jj++) {
-        step(s0 + jj, jj, jj);
-        load_step(jj, s0 + jj + PF);
+        step(w0 + jj, jj, jj);
+        load_step(jj, w0 + jj + PF);
       }
-      meta = meta_finish<PF, CM>(J, A, s0 + PF, len, x0, y0, g_raw, gp_raw, da, e, p2_table);
-      maybe_flush(s0 + PF);
+      meta = meta_finish<PF, CM>(J, A, w0 + PF, len, x0, y0, g_raw, gp_raw, da, e, p2_table);
+    };
+    if constexpr (PHASE == PHASE_LAST) {
+      // 64-step chunks with the selection flush outside the inner loop, so the
+      // flush's scalar pressure does not force spills into the step code
+      while (s0 + PF <= len) {
+        const int lim = min(len, (s0 | 63) + 1);
+        do {
+          window(s0);
+          s0 += PF;
+        } while (s0 + PF <= lim);
+        if (s0 == lim) maybe_flush(s0);
+      }
+    } else {
+      for (; s0 + PF <= len; s0 += PF) window(s0);
     }
     // tail: fewer than PF steps; their loads are already in the slots
 #pragma unroll
