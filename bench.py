@@ -136,6 +136,44 @@ def run_batched(sgm, params_for, steps=2):
     }
 
 
+def run_zncc(sgm, steps=5):
+    """The FP32 ZNCC workload: 1600x1067, 7x7 windows, D=192, P1=12, P2=150, gain
+    field 0.8-1.2 on the right image; inputs in HBM; uint16 cost volume."""
+    import numpy as np
+    from pose_amd import synth
+    W, H, D = 1600, 1067, 192
+    p = sgm.SgmParams(window_w=7, window_h=7, disparities=D, min_disparity=0, cost=4, p1=12,
+                      p2=150, edge_scale=32, directions=8, aggregate_bytes=0, census_weight=500,
+                      ad_truncation=30, gradient_truncation=60, reserved=0, uniqueness=0.03,
+                      lr_threshold=1.0)
+    left, right, _ = synth.stereo_pair(W, H, D, 0, seed=3, gain=True)
+    ctx = sgm.Context(0)
+    n = W * H
+    dl, dr, out = ctx.alloc(n).upload(left), ctx.alloc(n).upload(right), ctx.alloc(4 * n)
+    ctx.match_pair_device(p, dl, dr, W, H, out)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.match_pair_device(p, dl, dr, W, H, out)
+    ctx.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    ctx.set_profiling(True)
+    ctx.profile()
+    ctx.match_pair_device(p, dl, dr, W, H, out)
+    ctx.synchronize()
+    prof = {k: round(v["total_ms"], 3) for k, v in ctx.profile()["device_kernel_milliseconds"].items()}
+    ctx.set_profiling(False)
+    valid = int(np.isfinite(out.download(np.float32, n)).sum())
+    for b in (dl, dr, out):
+        b.free()
+    ctx.close()
+    upd = W * H * D * 8 * 2
+    return {"workload": "ZNCC FP32 pair 1600x1067, 7x7, D=192, P1=12 P2=150, gain field, 8 paths, "
+                        "uint16 cost and aggregate volumes, L+R + L-R",
+            "ms_per_pair": round(t * 1e3, 3), "gupdates_per_s": round(upd / t / 1e9, 3),
+            "kernel_ms": prof, "valid_pixels": valid}
+
+
 def serve_trace(count, rate, seed=7):
     """Served stream: Poisson arrivals at `rate` pairs/s (rate None: all at t=0);
     sizes 70% from the batched distribution, 25% 1600x1067 D=256, 5% 6000x4000
@@ -158,7 +196,7 @@ def serve_trace(count, rate, seed=7):
     return trace
 
 
-def run_served(sgm, params_for, rate=20.0, count=60, budget=250e9, max_batch=8, streams=2):
+def run_served(sgm, params_for, rate=20.0, count=60, budget=250e9, max_batch=8, streams=3):
     """The same request trace through (a) the plain path, one pair at a time in
     arrival order, and (b) the continuous-batching server (admission by HBM
     footprint against `budget`); host buffers in and out for both.  Latency =
@@ -270,6 +308,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--no-served", action="store_true", help="skip the served-vs-plain leg")
+    ap.add_argument("--no-zncc", action="store_true", help="skip the FP32 ZNCC leg")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals, pairs/s")
     ap.add_argument("--serve-requests", type=int, default=60)
     args = ap.parse_args()
@@ -359,6 +398,7 @@ def main():
                              p1=10, p2=120, edge_scale=32, directions=8, aggregate_bytes=0,
                              census_weight=500, ad_truncation=30, gradient_truncation=60,
                              reserved=0, uniqueness=0.03, lr_threshold=1.0)
+    zncc = None if args.no_zncc else run_zncc(sgm)
     batched = None if args.no_batched else run_batched(sgm, params_for)
     served = None if args.no_served else run_served(sgm, params_for, rate=args.serve_rate,
                                                     count=args.serve_requests)
@@ -397,6 +437,7 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_step / max(1, path_launches),
                      "copy_kernel": hbm_copy},
         "cpu_baseline": cpu,
+        "zncc": zncc,
         "batched": batched,
         "served": served,
     }

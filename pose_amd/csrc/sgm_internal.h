@@ -90,7 +90,7 @@ struct sgm_context {
   std::map<int, int> tuned_waves;   // lane group R -> waves per workgroup in path sweeps
   // batched mode launch shape (autotuned)
   int batch_waves_per_block = 8;
-  int batch_lines_per_item = 2;
+  int batch_lines_per_item = 1;
   int batch_blocks_per_cu = 0;  // 0: as many as stay resident
   int last_w = 0, last_h = 0;
   sgm_params last_params{};

@@ -158,12 +158,21 @@ void sgm_server::run() {
     // admit and launch: one batch per free stream slot
     for (Slot &s : slots) {
       if (s.busy || pending.empty()) continue;
+      // A request whose footprint exceeds an eighth of the pool is batched
+      // alone when it reaches the head of the queue; the batches formed
+      // before that take the small requests behind it in arrival order, so
+      // they do not wait for its completion.
+      const size_t large = region_bytes / 8;
       std::vector<Request *> take;
       size_t bytes = 0;
-      for (Request *r : pending) {
-        if ((int)take.size() >= max_batch) break;
-        take.push_back(r);
-        bytes += r->bytes;
+      if (pending.front()->bytes > large) {
+        take.push_back(pending.front());
+      } else {
+        for (Request *r : pending) {
+          if ((int)take.size() >= max_batch) break;
+          if (r->bytes > large) continue;
+          take.push_back(r);
+        }
       }
       // Shrink until the group fits the pool (FIFO: keep the oldest).  Each
       // group first tries resident cost volumes, then recomputed census costs.
@@ -192,7 +201,7 @@ void sgm_server::run() {
       for (size_t i = 0; i < take.size(); i++) {
         take[i]->admitted = t;
         take[i]->batch_id = next_batch;
-        pending.pop_front();
+        pending.erase(std::find(pending.begin(), pending.end(), take[i]));
       }
       std::vector<sgm_pair> prs;
       for (Request *r : take) {

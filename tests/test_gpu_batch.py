@@ -214,6 +214,32 @@ def test_server_results_equal_alone(gpu_context, policy):
         ctx.cost_volume(sgm.VOLUME_AUTO)
 
 
+def test_server_large_request_batched_alone(gpu_context):
+    """A request above an eighth of the serving pool runs in a batch of its own;
+    the small requests around it still share batches; all equal their solo runs."""
+    ctx = gpu_context
+    small = uneven_pairs(7, 6, wrange=(96, 200), drange=(16, 64))
+    big_l, big_r, _ = synth.stereo_pair(900, 700, 256, 0, seed=77)
+    pairs = small[:3] + [dict(left=big_l, right=big_r, disparities=256, min_disparity=0)] + small[3:]
+    params = to_sgm(O.params(**dict(CENSUS97, disparities=64)))
+    srv = sgm.Server(ctx, params, budget_bytes=2 << 30, max_batch=4, streams=2)
+    try:
+        tickets = [srv.submit(p["left"], p["right"], p["disparities"], p["min_disparity"])
+                   for p in pairs]
+        got = [srv.wait(t, timeout=60.0) for t in tickets]
+    finally:
+        srv.close()
+    ids = [g["info"]["batch_id"] for g in got]
+    assert ids.count(ids[3]) == 1, ids
+    assert len(set(ids[:3] + ids[4:])) < 6, ids  # small requests were batched together
+    for p, g in zip(pairs, got):
+        op = to_sgm(O.params(**dict(CENSUS97, disparities=p["disparities"],
+                                    min_disparity=p["min_disparity"])))
+        want = ctx.match_pair(op, p["left"], p["right"])
+        for k in ("checked", "dl", "dr"):
+            assert np.array_equal(bits(g[k]), bits(want[k])), k
+
+
 def test_server_rejects_what_never_fits(gpu_context):
     params = to_sgm(O.params(**dict(CENSUS97, disparities=64)))
     srv = sgm.Server(gpu_context, params, budget_bytes=64 << 20, max_batch=2, streams=1)
