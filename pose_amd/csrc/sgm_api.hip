@@ -431,6 +431,12 @@ CostArgs cost_args(const sgm_params *p) {
 int launch_cost(sgm_context *ctx, const CostJobs &jobs, int njobs, int w, int h, int Dp,
                 const sgm_params *p) {
   ProfileScope ps(ctx, "cost");
+  if (p->cost == SGM_COST_ZNCC || p->cost == SGM_COST_ZNCC_FP32) {
+    const int ntx = (w + kZnccTX - 1) / kZnccTX, ntd = (Dp + kZnccTD - 1) / kZnccTD;
+    dim3 grid(ntx * ntd, h, njobs);
+    zncc_cost_kernel<<<grid, 256, 0, ctx->stream>>>(jobs, cost_args(p));
+    return launch_check("zncc cost kernel");
+  }
   const int quads = w * (Dp / 4);
   dim3 grid((quads + 255) / 256, h, njobs);
   cost_kernel<<<grid, 256, 0, ctx->stream>>>(jobs, cost_args(p));

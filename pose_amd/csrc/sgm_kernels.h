@@ -176,6 +176,9 @@ __device__ __forceinline__ void store_cost4(uint16_t *cost, size_t p, int Dp, in
   }
 }
 
+__device__ __forceinline__ int zncc_cost(const CostArgs &A, int sab, long long sa, long long saa,
+                                         long long sb, long long sbb);
+
 // ZNCC = false compiles only the census-family modes 0..2 (register budget of
 // the batched kernel instance that never sees ZNCC).
 template <bool ZNCC = true>
@@ -211,8 +214,16 @@ __device__ __forceinline__ int cost_value(const CostJob &J, const CostArgs &A, s
     const uint8_t *rb = J.gray_b + (ptrdiff_t)q + (ptrdiff_t)dy * w;
     for (int dx = -rx; dx <= rx; dx++) sab += (int)ra[dx] * (int)rb[dx];
   }
+  return zncc_cost(A, sab, J.sum_a[p], J.sum2_a[p], J.sum_b[q], J.sum2_b[q]);
+}
+
+// ZNCC cost from the exact integer window sums of the reference window (sa,
+// saa), the target window (sb, sbb) and their cross products (sab):
+// StereoCost.value mode 3 (StereoCost.java:75-89) or the FP32 variant.
+__device__ __forceinline__ int zncc_cost(const CostArgs &A, int sab, long long sa, long long saa,
+                                         long long sb, long long sbb) {
+  const int bits = A.ww * A.wh - 1;
   const long long cnt = (long long)A.ww * A.wh;
-  const long long sa = J.sum_a[p], sb = J.sum_b[q], saa = J.sum2_a[p], sbb = J.sum2_b[q];
   const long long va = cnt * saa - sa * sa, vb = cnt * sbb - sb * sb;
   if (A.mode == 3) {
     if (va <= 0 || vb <= 0) return 512;
@@ -238,6 +249,74 @@ __device__ __forceinline__ int cost_value(const CostJob &J, const CostArgs &A, s
 struct CostJobs {
   CostJob job[2];
 };
+
+// ZNCC cost volume rows (modes 3 and 4): one image row, 64 columns x 64
+// disparities per workgroup.  The gray rows under the window are staged in
+// LDS; the cross sums are separable: for every window column x' and disparity
+// d the 7-row (wh-row) column sum of a(x') b(x' -/+ d) is formed once in LDS,
+// then each cell adds ww of them.  All sums are exact integers, so the result
+// equals the per-cell window loop.
+constexpr int kZnccTX = 64, kZnccTD = 64, kZnccMaxR = 5;
+
+static __global__ __launch_bounds__(256) void zncc_cost_kernel(CostJobs jobs, CostArgs A) {
+  __shared__ uint8_t arow[2 * kZnccMaxR + 1][kZnccTX + 2 * kZnccMaxR];
+  __shared__ uint8_t brow[2 * kZnccMaxR + 1][kZnccTX + 2 * kZnccMaxR + kZnccTD];
+  __shared__ int col[kZnccTX + 2 * kZnccMaxR][kZnccTD + 1];
+  const CostJob J = jobs.job[blockIdx.z];
+  const int y = blockIdx.y;
+  const int ntx = (J.w + kZnccTX - 1) / kZnccTX;
+  const int x0 = (int)(blockIdx.x % ntx) * kZnccTX, d0 = (int)(blockIdx.x / ntx) * kZnccTD;
+  if (y >= J.h || d0 >= J.Dp) return;  // whole workgroup
+  const int rx = A.ww >> 1, ry = A.wh >> 1;
+  const int XW = kZnccTX + 2 * rx, BW = XW + kZnccTD - 1;
+  // b column j holds target t = tb0 + j
+  const int tb0 = J.reverse ? x0 - rx + J.minD + d0 : x0 - rx - J.minD - d0 - (kZnccTD - 1);
+  const bool rows_ok = y >= ry && y < J.h - ry;
+  for (int i = threadIdx.x; i < A.wh * XW; i += 256) {
+    const int r = i / XW, c = i - r * XW, xx = x0 - rx + c;
+    arow[r][c] = (rows_ok && xx >= 0 && xx < J.w) ? J.gray_a[(size_t)(y - ry + r) * J.w + xx] : 0;
+  }
+  for (int i = threadIdx.x; i < A.wh * BW; i += 256) {
+    const int r = i / BW, c = i - r * BW, t = tb0 + c;
+    brow[r][c] = (rows_ok && t >= 0 && t < J.w) ? J.gray_b[(size_t)(y - ry + r) * J.w + t] : 0;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < XW * kZnccTD; i += 256) {
+    const int c = i / kZnccTD, dd = i - c * kZnccTD;
+    const int j = J.reverse ? c + dd : c - dd + kZnccTD - 1;
+    int s = 0;
+    for (int r = 0; r < A.wh; r++) s += (int)arow[r][c] * (int)brow[r][j];
+    col[c][dd] = s;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kZnccTX * (kZnccTD / 4); i += 256) {
+    const int xo = i / (kZnccTD / 4), q4 = i - xo * (kZnccTD / 4);
+    const int x = x0 + xo, k0 = d0 + 4 * q4;
+    if (x >= J.w || k0 >= J.Dp) continue;
+    const size_t p = (size_t)y * J.w + x;
+    const bool pix_ok = rows_ok && x >= rx && x < J.w - rx && J.mask_a[p];
+    unsigned v[4];
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+      const int k = k0 + jj, dd = 4 * q4 + jj;
+      int value = A.inv_code;
+      if (pix_ok && k < J.D) {
+        const int disp = J.minD + k;
+        const long long t = (long long)x + (J.reverse ? (long long)disp : -(long long)disp);
+        if (t >= rx && t < J.w - rx) {
+          const size_t q = (size_t)y * J.w + (size_t)t;
+          if (J.mask_b[q]) {
+            int sab = 0;
+            for (int c = xo; c <= xo + 2 * rx; c++) sab += col[c][dd];
+            value = zncc_cost(A, sab, J.sum_a[p], J.sum2_a[p], J.sum_b[q], J.sum2_b[q]);
+          }
+        }
+      }
+      v[jj] = (unsigned)value & 0xFFFFu;
+    }
+    store_cost4(J.cost, p, J.Dp, k0, v, A.u8 != 0);
+  }
+}
 
 static __global__ __launch_bounds__(256) void cost_kernel(CostJobs jobs, CostArgs A) {
   const CostJob J = jobs.job[blockIdx.z];
