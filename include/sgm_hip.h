@@ -139,6 +139,37 @@ int sgm_match_pair(sgm_context *ctx, const sgm_params *params, const uint8_t *le
                    int width, int height, float *checked, float *dl, float *dr, sgm_stats *stats,
                    uint8_t *reasons, uint8_t *boundary);
 
+/* Post-L-R filters of CensusSgm.computeRecorded (CensusSgm.java:203-271):
+ * median (Options.median, medianMode, medianMaxChange, medianEdgeThreshold),
+ * post-median reciprocity and photometric ZNCC (Options.correlationWindow,
+ * correlation), speckle removal (speckleSize, speckleRange,
+ * speckleConnectivity).  Reference defaults: 3, edge-preserving (0), 24, 7, 50,
+ * 8, 0.5, 0.55, 1.0.  median <= 1 or speckle_size <= 1 disables that step;
+ * median windows up to 7. */
+typedef struct sgm_filter_params {
+  int32_t median;
+  int32_t median_classic;       /* 1: medianValid, 0: medianEdgePreserving */
+  int32_t median_edge_threshold;
+  int32_t correlation_window;
+  int32_t speckle_size;
+  int32_t speckle_connectivity; /* 4 or 8 */
+  int32_t reserved0, reserved1;
+  double median_max_change;
+  double correlation;
+  double speckle_range;
+  double reserved2;
+} sgm_filter_params;
+
+/* CensusSgm.computeRecorded through 05_final_stereo: sgm_match_pair, then the
+ * filters above on the device.  disparity receives Result.disparity; stats
+ * include post_median_lr_rejected, photometric_rejected, speckle_rejected and
+ * the final valid count; reasons (may be NULL) the codes 0..7. */
+int sgm_match_pair_filtered(sgm_context *ctx, const sgm_params *params,
+                            const sgm_filter_params *filters, const uint8_t *left,
+                            const uint8_t *right, const uint8_t *valid_left,
+                            const uint8_t *valid_right, int width, int height, float *disparity,
+                            sgm_stats *stats, uint8_t *reasons);
+
 /* Same as sgm_match_pair with every buffer in device memory (no transfers, no
  * host synchronisation).  d_stats (device, sgm_stats) may be NULL.  Work is
  * enqueued on the context's stream; call sgm_synchronize before reading. */

@@ -28,6 +28,28 @@ class OracleParams(ctypes.Structure):
     ]
 
 
+class OracleFilter(ctypes.Structure):
+    _fields_ = [
+        ("median", ctypes.c_int32), ("median_classic", ctypes.c_int32),
+        ("median_edge_threshold", ctypes.c_int32), ("correlation_window", ctypes.c_int32),
+        ("speckle_size", ctypes.c_int32), ("speckle_connectivity", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32), ("reserved1", ctypes.c_int32),
+        ("median_max_change", ctypes.c_double), ("correlation", ctypes.c_double),
+        ("speckle_range", ctypes.c_double), ("reserved2", ctypes.c_double),
+    ]
+
+
+def filters(**kw):
+    """Post-L-R filter options; defaults are the reference's Options defaults
+    (Options.java:19-23,67,76): edge-preserving 3x3 median, max change 0.5, edge
+    threshold 24, 7x7 photometric ZNCC >= 0.55, speckles < 50 px, range 1, 8-connected."""
+    d = dict(median=3, median_classic=0, median_edge_threshold=24, correlation_window=7,
+             speckle_size=50, speckle_connectivity=8, reserved0=0, reserved1=0,
+             median_max_change=0.5, correlation=0.55, speckle_range=1.0, reserved2=0.0)
+    d.update(kw)
+    return OracleFilter(**d)
+
+
 STAT_FIELDS = [
     "lower_boundary_rejected", "full_interval_pixels", "boundary_rejected",
     "upper_boundary_rejected", "ambiguity_rejected", "selected", "lr_rejected",
@@ -63,6 +85,12 @@ def lib():
                                    P, P, P, P]
         L.oracle_compute.argtypes = [ctypes.POINTER(OracleParams), P, P, P, P, ctypes.c_int,
                                      ctypes.c_int, P, P, P, ctypes.POINTER(OracleStats), P, P]
+        L.oracle_postfilter.argtypes = [ctypes.POINTER(OracleParams), ctypes.POINTER(OracleFilter),
+                                        P, P, P, ctypes.c_int, ctypes.c_int, P,
+                                        ctypes.POINTER(OracleStats), P]
+        L.oracle_photo_zncc.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_double, ctypes.c_int]
+        L.oracle_photo_zncc.restype = ctypes.c_double
         L.oracle_round_float.argtypes = [ctypes.c_float]
         L.oracle_round_float.restype = ctypes.c_int32
         L.oracle_round_double.argtypes = [ctypes.c_double]
@@ -142,6 +170,39 @@ def compute(p, left, right, w, h, valid_left=None, valid_right=None):
         raise ValueError("oracle_compute failed: %d" % rc)
     return dict(checked=checked, dl=dl, dr=dr, stats=st.as_dict(), reasons=reasons,
                 boundary=boundary)
+
+
+def postfilter(p, f, left, right, dr, checked, w, h, stats=None, reasons=None):
+    """Median, reciprocity + photometric ZNCC and speckles on a copy of `checked`.
+    Returns (filtered, stats dict, reasons)."""
+    out = np.array(checked, np.float32, copy=True).reshape(-1)
+    st = OracleStats()
+    if stats:
+        for k, v in stats.items():
+            setattr(st, k, v)
+    rs = None if reasons is None else np.array(reasons, np.uint8, copy=True).reshape(-1)
+    left = np.ascontiguousarray(left, np.uint8).reshape(-1)
+    right = np.ascontiguousarray(right, np.uint8).reshape(-1)
+    dr = np.ascontiguousarray(dr, np.float32).reshape(-1)
+    rc = lib().oracle_postfilter(ctypes.byref(p), ctypes.byref(f), _ptr(left), _ptr(right),
+                                 _ptr(dr), w, h, _ptr(out), ctypes.byref(st), _ptr(rs))
+    if rc != 0:
+        raise ValueError("oracle_postfilter failed: %d" % rc)
+    return out, st.as_dict(), rs
+
+
+def photo_zncc(a, b, w, h, x, y, disparity, r):
+    a = np.ascontiguousarray(a, np.uint8).reshape(-1)
+    b = np.ascontiguousarray(b, np.uint8).reshape(-1)
+    return lib().oracle_photo_zncc(_ptr(a), _ptr(b), w, h, x, y, float(disparity), r)
+
+
+def compute_filtered(p, f, left, right, w, h, valid_left=None, valid_right=None):
+    """CensusSgm.computeRecorded through 05_final_stereo."""
+    r = compute(p, left, right, w, h, valid_left, valid_right)
+    out, st, rs = postfilter(p, f, left, right, r["dr"], r["checked"], w, h, r["stats"],
+                             r["reasons"])
+    return dict(r, disparity=out, stats=st, reasons=rs)
 
 
 def round_float(v):

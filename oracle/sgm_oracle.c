@@ -371,3 +371,188 @@ int oracle_compute(const oracle_params *o, const uint8_t *left, const uint8_t *r
   if (!dr_out) free(dr);
   return rc;
 }
+
+
+/* ------------------------------------------------------------------------ */
+/* Post-L-R filters (CensusSgm.java:203-271, 557-681).                       */
+
+/* Float.compare order for finite values (-0.0 < 0.0). */
+static int float_cmp(const void *a, const void *b) {
+  float x = *(const float *)a, y = *(const float *)b;
+  if (x < y) return -1;
+  if (x > y) return 1;
+  int sx = signbit(x) != 0, sy = signbit(y) != 0;
+  return sy - sx;
+}
+
+/* CensusSgm.medianValid (CensusSgm.java:557-573). */
+static void median_valid(const float *a, int w, int h, int window, float *out) {
+  int r = window / 2;
+  float *values = (float *)malloc(sizeof(float) * window * window);
+  memcpy(out, a, sizeof(float) * (size_t)w * h);
+  for (int y = 0; y < h; y++)
+    for (int x = 0; x < w; x++) {
+      if (!isfinite_f(a[(size_t)y * w + x])) continue;
+      int n = 0;
+      for (int yy = y - r < 0 ? 0 : y - r; yy <= (y + r < h - 1 ? y + r : h - 1); yy++)
+        for (int xx = x - r < 0 ? 0 : x - r; xx <= (x + r < w - 1 ? x + r : w - 1); xx++) {
+          float v = a[(size_t)yy * w + xx];
+          if (isfinite_f(v)) values[n++] = v;
+        }
+      qsort(values, n, sizeof(float), float_cmp);
+      out[(size_t)y * w + x] = values[n / 2];
+    }
+  free(values);
+}
+
+/* CensusSgm.medianEdgePreserving (CensusSgm.java:576-600). */
+static void median_edge(const float *a, int w, int h, int window, double max_change,
+                        const uint8_t *gray, int edge, float *out) {
+  int r = window / 2;
+  float *values = (float *)malloc(sizeof(float) * window * window);
+  median_valid(a, w, h, window, out);
+  for (int y = 0; y < h; y++)
+    for (int x = 0; x < w; x++) {
+      float center = a[(size_t)y * w + x];
+      if (!isfinite_f(center)) continue;
+      if ((double)fabsf(out[(size_t)y * w + x] - center) < max_change) continue;
+      int n = 0;
+      for (int yy = y - r < 0 ? 0 : y - r; yy <= (y + r < h - 1 ? y + r : h - 1); yy++)
+        for (int xx = x - r < 0 ? 0 : x - r; xx <= (x + r < w - 1 ? x + r : w - 1); xx++) {
+          float value = a[(size_t)yy * w + xx];
+          if (isfinite_f(value) && (double)fabsf(value - center) <= max_change &&
+              (edge == 0 || abs((int)gray[(size_t)yy * w + xx] - (int)gray[(size_t)y * w + x]) <= edge))
+            values[n++] = value;
+        }
+      if (n > 0) {
+        qsort(values, n, sizeof(float), float_cmp);
+        out[(size_t)y * w + x] = values[n / 2];
+      }
+    }
+  free(values);
+}
+
+/* CensusSgm.reciprocal (CensusSgm.java:602-612), border per axis. */
+static int reciprocal(const float *right, int w, int h, int x, int y, float disparity,
+                      double tolerance, int bx, int by) {
+  if (!isfinite_f(disparity) || y < by || y >= h - by) return 0;
+  int xr = oracle_round_float((float)x - disparity);
+  return xr >= bx && xr < w - bx && isfinite_f(right[(size_t)y * w + xr]) &&
+         (double)fabsf(disparity - right[(size_t)y * w + xr]) <= tolerance;
+}
+
+/* CensusSgm.zncc (CensusSgm.java:614-644), one channel. */
+static double photo_zncc(const uint8_t *a, const uint8_t *b, int w, int h, int x, int y,
+                         double disparity, int r) {
+  double target = x - disparity;
+  int bx = (int)floor(target);
+  double fraction = target - bx;
+  if (x - r < 0 || x + r >= w || y - r < 0 || y + r >= h || bx - r < 0 || bx + r + 1 >= w)
+    return -1;
+  int n = (2 * r + 1) * (2 * r + 1);
+  double sa = 0, sb = 0, saa = 0, sbb = 0, sab = 0;
+  for (int dy = -r; dy <= r; dy++)
+    for (int dx = -r; dx <= r; dx++) {
+      size_t ai = (size_t)(y + dy) * w + x + dx, bi = (size_t)(y + dy) * w + bx + dx;
+      double av = a[ai];
+      double bv = (1 - fraction) * b[bi] + fraction * b[bi + 1];
+      sa += av;
+      sb += bv;
+      saa += av * av;
+      sbb += bv * bv;
+      sab += av * bv;
+    }
+  double covariance = sab - sa * sb / n, var_a = saa - sa * sa / n, var_b = sbb - sb * sb / n;
+  if (var_a < n * 4 || var_b < n * 4) return -1;
+  double prod = var_a * var_b;
+  return covariance / sqrt(prod > 1e-12 ? prod : 1e-12);
+}
+
+/* CensusSgm.removeSpeckles (CensusSgm.java:651-681). */
+static int64_t remove_speckles(float *a, int w, int h, int minimum, double range, int connectivity) {
+  if (minimum <= 1) return 0;
+  size_t n = (size_t)w * h;
+  uint8_t *seen = (uint8_t *)calloc(n, 1);
+  int *queue = (int *)malloc(sizeof(int) * n);
+  int64_t rejected = 0;
+  for (size_t start = 0; start < n; start++) {
+    if (seen[start] || !isfinite_f(a[start])) continue;
+    size_t head = 0, tail = 0;
+    queue[tail++] = (int)start;
+    seen[start] = 1;
+    while (head < tail) {
+      int p = queue[head++], x = p % w, y = p / w;
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          if ((dx == 0 && dy == 0) || (connectivity == 4 && dx != 0 && dy != 0)) continue;
+          int xx = x + dx, yy = y + dy;
+          if (xx < 0 || xx >= w || yy < 0 || yy >= h) continue;
+          int q = yy * w + xx;
+          if (seen[q] || !isfinite_f(a[q]) || (double)fabsf(a[q] - a[p]) > range) continue;
+          seen[q] = 1;
+          queue[tail++] = q;
+        }
+    }
+    if ((int64_t)tail < minimum)
+      for (size_t i = 0; i < tail; i++) {
+        a[queue[i]] = NAN;
+        rejected++;
+      }
+  }
+  free(seen);
+  free(queue);
+  return rejected;
+}
+
+double oracle_photo_zncc(const uint8_t *a, const uint8_t *b, int w, int h, int x, int y,
+                         double disparity, int r) {
+  return photo_zncc(a, b, w, h, x, y, disparity, r);
+}
+
+int oracle_postfilter(const oracle_params *o, const oracle_filter *f, const uint8_t *left,
+                      const uint8_t *right, const float *dr, int w, int h, float *checked,
+                      oracle_stats *stats, uint8_t *reasons) {
+  size_t n = (size_t)w * h;
+  int bx = o->window_w / 2, by = o->window_h / 2;
+  if (f->median > 1) {
+    float *out = (float *)malloc(sizeof(float) * n);
+    if (f->median_classic)
+      median_valid(checked, w, h, f->median, out);
+    else
+      median_edge(checked, w, h, f->median, f->median_max_change, left, f->median_edge_threshold,
+                  out);
+    memcpy(checked, out, sizeof(float) * n);
+    free(out);
+  }
+  for (int y = 0; y < h; y++)
+    for (int x = 0; x < w; x++) {
+      size_t p = (size_t)y * w + x;
+      float disparity = checked[p];
+      if (!isfinite_f(disparity)) continue;
+      if (!reciprocal(dr, w, h, x, y, disparity, o->lr_threshold, bx, by)) {
+        checked[p] = NAN;
+        stats->post_median_lr_rejected++;
+        if (reasons) reasons[p] = 5;
+        continue;
+      }
+      double score = photo_zncc(left, right, w, h, x, y, disparity, f->correlation_window / 2);
+      if (score < f->correlation) {
+        checked[p] = NAN;
+        stats->photometric_rejected++;
+        if (reasons) reasons[p] = 6;
+      }
+    }
+  float *before = reasons ? (float *)malloc(sizeof(float) * n) : NULL;
+  if (before) memcpy(before, checked, sizeof(float) * n);
+  stats->speckle_rejected +=
+      remove_speckles(checked, w, h, f->speckle_size, f->speckle_range, f->speckle_connectivity);
+  stats->valid = 0;
+  for (size_t i = 0; i < n; i++)
+    if (isfinite_f(checked[i])) stats->valid++;
+  if (before) {
+    for (size_t i = 0; i < n; i++)
+      if (isfinite_f(before[i]) && !isfinite_f(checked[i])) reasons[i] = 7;
+    free(before);
+  }
+  return 0;
+}

@@ -88,6 +88,36 @@ int oracle_compute(const oracle_params *o, const uint8_t *left, const uint8_t *r
                    float *checked, float *dl, float *dr, oracle_stats *stats, uint8_t *reasons,
                    uint8_t *boundary);
 
+/* Post-L-R filters of CensusSgm.computeRecorded (CensusSgm.java:203-271);
+ * same layout as sgm_filter_params (include/sgm_hip.h). */
+typedef struct {
+  int32_t median;               /* window; <= 1 disables (Options.median) */
+  int32_t median_classic;       /* 1: medianValid, 0: medianEdgePreserving */
+  int32_t median_edge_threshold;
+  int32_t correlation_window;   /* photometric ZNCC window (odd) */
+  int32_t speckle_size;         /* minimum component size; <= 1 disables */
+  int32_t speckle_connectivity; /* 4 or 8 */
+  int32_t reserved0, reserved1;
+  double median_max_change;
+  double correlation;           /* photometric threshold: score < correlation rejects */
+  double speckle_range;
+  double reserved2;
+} oracle_filter;
+
+/* Median (03_median), post-median reciprocity and photometric ZNCC
+ * (04_photometric), speckle removal (05_final_stereo) on `checked` in place
+ * (CensusSgm.java:203-271, 557-681).  dr: right-reference disparities; the
+ * reciprocity border is (window_w/2, window_h/2) of o.  Adds
+ * post_median_lr_rejected, photometric_rejected, speckle_rejected and sets
+ * valid; reasons (may be NULL) get 5, 6, 7. */
+int oracle_postfilter(const oracle_params *o, const oracle_filter *f, const uint8_t *left,
+                      const uint8_t *right, const float *dr, int w, int h, float *checked,
+                      oracle_stats *stats, uint8_t *reasons);
+
+/* CensusSgm.zncc (CensusSgm.java:614-644), one channel: the photometric score. */
+double oracle_photo_zncc(const uint8_t *a, const uint8_t *b, int w, int h, int x, int y,
+                         double disparity, int r);
+
 /* Java Math.round(float) / Math.round(double) (closest, ties toward +inf). */
 int32_t oracle_round_float(float v);
 int64_t oracle_round_double(double v);

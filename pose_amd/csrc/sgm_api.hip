@@ -19,6 +19,7 @@
 
 #include "../../include/sgm_hip.h"
 #include "sgm_internal.h"
+#include "sgm_filters.h"
 
 using namespace sgm;
 
@@ -831,6 +832,131 @@ extern "C" int sgm_match_pair(sgm_context *ctx, const sgm_params *params, const 
   ctx->last_w = w;
   ctx->last_h = h;
   ctx->last_params = *params;
+  return SGM_OK;
+}
+
+namespace {
+
+int validate_filters(const sgm_filter_params *f) {
+  if (!f) return fail(SGM_ERR_INVALID_ARGUMENT, "null filter params");
+  if (f->median > kMedianMaxWindow)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "median window %d above %d", f->median, kMedianMaxWindow);
+  if (f->correlation_window < 1 || f->correlation_window > 31)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "correlation window must be 1..31");
+  if (f->speckle_connectivity != 4 && f->speckle_connectivity != 8)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "speckle connectivity must be 4 or 8");
+  if (!(f->median_max_change >= 0) || !(f->speckle_range >= 0))
+    return fail(SGM_ERR_INVALID_ARGUMENT, "median max change and speckle range must be >= 0");
+  return SGM_OK;
+}
+
+// 03_median .. 05_final_stereo on the device (CensusSgm.java:203-271).  `work`
+// holds the L-R-validated map on entry and the final map on return.
+int run_filters(sgm_context *ctx, const sgm_params *p, const sgm_filter_params *f, int w, int h,
+                PairBuffers &B, float *&work) {
+  int rc;
+  const size_t n = (size_t)w * h;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  FilterArgs F;
+  F.w = w;
+  F.h = h;
+  F.median = f->median;
+  F.median_classic = f->median_classic;
+  F.median_edge = f->median_edge_threshold;
+  F.median_max_change = f->median_max_change;
+  F.bx = p->window_w / 2;
+  F.by = p->window_h / 2;
+  F.lr_threshold = p->lr_threshold;
+  F.corr_r = f->correlation_window / 2;
+  F.correlation = f->correlation;
+  F.speckle_size = f->speckle_size;
+  F.speckle_conn = f->speckle_connectivity;
+  F.speckle_range = f->speckle_range;
+#define ENSURE(name, bytes)                                                   \
+  if ((rc = ctx->buffers[name].ensure(ctx, (bytes), name)) != SGM_OK) return rc;
+  ENSURE("fmed", n * 4);
+  ENSURE("cc_parent", n * 4);
+  ENSURE("cc_size", n * 4);
+  ENSURE("cc_error", 64);
+#undef ENSURE
+  float *fmed = (float *)ctx->buffers["fmed"].ptr;
+  int *parent = (int *)ctx->buffers["cc_parent"].ptr, *size = (int *)ctx->buffers["cc_size"].ptr;
+  int *error = (int *)ctx->buffers["cc_error"].ptr;
+  ProfileScope ps(ctx, "filters");
+  // the final valid count replaces the L-R checkpoint's
+  if ((rc = launch_fill_u64(ctx, B.stats + 10, 0ULL, 1))) return rc;
+  if ((rc = launch_fill_u8(ctx, (uint8_t *)error, 0, 64))) return rc;
+  work = B.checked;
+  if (f->median > 1) {
+    median_kernel<<<blocks, 256, 0, ctx->stream>>>(B.checked, fmed, B.gray[0], F);
+    if ((rc = launch_check("median kernel"))) return rc;
+    work = fmed;
+  }
+  photo_kernel<<<blocks, 256, 0, ctx->stream>>>(work, B.disp[1], B.gray[0], B.gray[1],
+                                                B.reasons_out, B.stats, F);
+  if ((rc = launch_check("photometric kernel"))) return rc;
+  if (f->speckle_size > 1) {
+    cc_init_kernel<<<blocks, 256, 0, ctx->stream>>>(work, parent, size, n);
+    cc_union_kernel<<<blocks, 256, 0, ctx->stream>>>(work, parent, error, F);
+    cc_size_kernel<<<blocks, 256, 0, ctx->stream>>>(parent, size, error, n);
+    cc_remove_kernel<<<blocks, 256, 0, ctx->stream>>>(work, parent, size, B.reasons_out, B.stats,
+                                                      error, F);
+    if ((rc = launch_check("speckle kernels"))) return rc;
+  } else {
+    count_valid_kernel<<<blocks, 256, 0, ctx->stream>>>(work, B.stats, n);
+    if ((rc = launch_check("count kernel"))) return rc;
+  }
+  return SGM_OK;
+}
+
+}  // namespace
+
+extern "C" int sgm_match_pair_filtered(sgm_context *ctx, const sgm_params *params,
+                                       const sgm_filter_params *filters, const uint8_t *left,
+                                       const uint8_t *right, const uint8_t *valid_left,
+                                       const uint8_t *valid_right, int w, int h, float *disparity,
+                                       sgm_stats *stats, uint8_t *reasons) {
+  if (!ctx || !left || !right || !disparity) return fail(SGM_ERR_INVALID_ARGUMENT, "null argument");
+  if ((valid_left == nullptr) != (valid_right == nullptr))
+    return fail(SGM_ERR_INVALID_ARGUMENT, "Invalid stereo mask dimensions");
+  std::lock_guard<std::mutex> g(ctx->lock);
+  int rc = validate_params(params, w, h);
+  if (rc) return rc;
+  if ((rc = validate_filters(filters))) return rc;
+  if (pair_bytes(params, w, h) > ctx->budget)
+    return fail(SGM_ERR_NO_PLAN, "No GPU SGM memory plan fits the requested range");
+  (void)hipSetDevice(ctx->device);
+  PairBuffers B;
+  if ((rc = ensure_pair_buffers(ctx, params, w, h, B))) return rc;
+  const size_t n = (size_t)w * h;
+  if ((rc = copy_h2d(ctx, B.gray[0], left, n))) return rc;
+  if ((rc = copy_h2d(ctx, B.gray[1], right, n))) return rc;
+  const uint8_t *vl = nullptr, *vr = nullptr;
+  if (valid_left) {
+    if ((rc = copy_h2d(ctx, B.valid[0], valid_left, n))) return rc;
+    if ((rc = copy_h2d(ctx, B.valid[1], valid_right, n))) return rc;
+    vl = B.valid[0];
+    vr = B.valid[1];
+  }
+  if ((rc = run_pair(ctx, params, w, h, B, B.gray[0], B.gray[1], vl, vr, B.checked, B.reasons_out,
+                     B.boundary_out, B.stats)))
+    return rc;
+  float *work = nullptr;
+  if ((rc = run_filters(ctx, params, filters, w, h, B, work))) return rc;
+  unsigned long long counters[16];
+  int error = 0;
+  if ((rc = copy_d2h(ctx, disparity, work, n * 4))) return rc;
+  if (reasons && (rc = copy_d2h(ctx, reasons, B.reasons_out, n))) return rc;
+  if ((rc = copy_d2h(ctx, counters, B.stats, sizeof counters))) return rc;
+  if ((rc = copy_d2h(ctx, &error, ctx->buffers["cc_error"].ptr, sizeof error))) return rc;
+  if ((rc = finish(ctx))) return rc;
+  if (error) return fail(SGM_ERR_HIP, "speckle component search did not converge");
+  if (stats) {
+    stats_from_device(counters, stats);
+    stats->post_median_lr_rejected = (int64_t)counters[7];
+    stats->photometric_rejected = (int64_t)counters[8];
+    stats->speckle_rejected = (int64_t)counters[9];
+  }
   return SGM_OK;
 }
 

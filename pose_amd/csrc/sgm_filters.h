@@ -1,0 +1,279 @@
+// Post-left-right filters of the reference's CensusSgm.computeRecorded
+// (src/main/java/sgm/CensusSgm.java:203-271, 557-681), on the device:
+//   03_median        median_kernel (classic or edge-preserving)
+//   04_photometric   photo_kernel  (post-median reciprocity + subpixel ZNCC)
+//   05_final_stereo  cc_* kernels  (speckle removal by connected components)
+// Every kernel is per pixel and reproduces the reference's float/double
+// arithmetic in the same order (the library is built with -ffp-contract=off).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sgm {
+
+constexpr int kMedianMaxWindow = 7;  // up to 7x7 = 49 values per pixel
+
+struct FilterArgs {
+  int w, h;
+  int median, median_classic, median_edge;
+  double median_max_change;
+  int bx, by;            // reciprocity border (census radius per axis)
+  double lr_threshold;
+  int corr_r;            // photometric window radius
+  double correlation;
+  int speckle_size, speckle_conn;
+  double speckle_range;
+};
+
+// Float.compare order for finite values (-0.0 before 0.0) as a signed key.
+__device__ __forceinline__ int float_order_key(float v) {
+  const int b = __float_as_int(v);
+  return b < 0 ? b ^ 0x7FFFFFFF : b;
+}
+
+// Element n / 2 of the sorted values (insertion sort; n <= 49).
+__device__ __forceinline__ float sorted_middle(float *v, int n) {
+  for (int i = 1; i < n; i++) {
+    const float x = v[i];
+    const int kx = float_order_key(x);
+    int j = i - 1;
+    while (j >= 0 && float_order_key(v[j]) > kx) {
+      v[j + 1] = v[j];
+      j--;
+    }
+    v[j + 1] = x;
+  }
+  return v[n / 2];
+}
+
+// CensusSgm.medianValid / medianEdgePreserving (CensusSgm.java:557-600).
+static __global__ __launch_bounds__(256) void median_kernel(const float *__restrict__ in,
+                                                            float *__restrict__ out,
+                                                            const uint8_t *__restrict__ gray,
+                                                            FilterArgs F) {
+  const size_t n = (size_t)F.w * F.h;
+  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  const int x = (int)(p % F.w), y = (int)(p / F.w), r = F.median / 2;
+  const float center = in[p];
+  if (!isfinite(center)) {
+    out[p] = center;
+    return;
+  }
+  float v[kMedianMaxWindow * kMedianMaxWindow];
+  const int y0 = max(0, y - r), y1 = min(F.h - 1, y + r), x0 = max(0, x - r), x1 = min(F.w - 1, x + r);
+  int cnt = 0;
+  for (int yy = y0; yy <= y1; yy++)
+    for (int xx = x0; xx <= x1; xx++) {
+      const float a = in[(size_t)yy * F.w + xx];
+      if (isfinite(a)) v[cnt++] = a;
+    }
+  float m = sorted_middle(v, cnt);
+  if (!F.median_classic && !((double)fabsf(m - center) < F.median_max_change)) {
+    const int g = gray[p];
+    cnt = 0;
+    for (int yy = y0; yy <= y1; yy++)
+      for (int xx = x0; xx <= x1; xx++) {
+        const size_t q = (size_t)yy * F.w + xx;
+        const float a = in[q];
+        if (isfinite(a) && (double)fabsf(a - center) <= F.median_max_change &&
+            (F.median_edge == 0 || abs((int)gray[q] - g) <= F.median_edge))
+          v[cnt++] = a;
+      }
+    if (cnt > 0) m = sorted_middle(v, cnt);
+  }
+  out[p] = m;
+}
+
+// Block-wide sums of two counters, one atomic per block and counter.
+__device__ __forceinline__ void add_counts(unsigned long long *stats, int f0, unsigned c0, int f1,
+                                           unsigned c1) {
+  __shared__ unsigned cnt[2];
+  if (threadIdx.x == 0) cnt[0] = cnt[1] = 0;
+  __syncthreads();
+  if (c0) atomicAdd(&cnt[0], c0);
+  if (c1) atomicAdd(&cnt[1], c1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (cnt[0]) atomicAdd(stats + f0, (unsigned long long)cnt[0]);
+    if (cnt[1]) atomicAdd(stats + f1, (unsigned long long)cnt[1]);
+  }
+}
+
+// CensusSgm.reciprocal + CensusSgm.zncc with the photometric threshold
+// (CensusSgm.java:224-257, 602-644); one channel.
+static __global__ __launch_bounds__(256) void photo_kernel(float *__restrict__ disp,
+                                                           const float *__restrict__ dr,
+                                                           const uint8_t *__restrict__ left,
+                                                           const uint8_t *__restrict__ right,
+                                                           uint8_t *__restrict__ reasons,
+                                                           unsigned long long *stats, FilterArgs F) {
+  const size_t n = (size_t)F.w * F.h;
+  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+  unsigned lr_rej = 0, photo_rej = 0;
+  if (p < n) {
+    const int x = (int)(p % F.w), y = (int)(p / F.w);
+    const float d = disp[p];
+    if (isfinite(d)) {
+      bool ok = y >= F.by && y < F.h - F.by;
+      if (ok) {
+        const int xr = round_half_up_f((float)x - d);
+        ok = xr >= F.bx && xr < F.w - F.bx;
+        if (ok) {
+          const float o = dr[(size_t)y * F.w + xr];
+          ok = isfinite(o) && (double)fabsf(d - o) <= F.lr_threshold;
+        }
+      }
+      if (!ok) {
+        disp[p] = __builtin_nanf("");
+        lr_rej = 1;
+        if (reasons) reasons[p] = 5;
+      } else {
+        const int r = F.corr_r;
+        const double target = (double)x - (double)d;
+        const int bxi = (int)floor(target);
+        const double fraction = target - bxi;
+        double score = -1.0;
+        if (!(x - r < 0 || x + r >= F.w || y - r < 0 || y + r >= F.h || bxi - r < 0 ||
+              bxi + r + 1 >= F.w)) {
+          const int cnt = (2 * r + 1) * (2 * r + 1);
+          double sa = 0, sb = 0, saa = 0, sbb = 0, sab = 0;
+          for (int dy = -r; dy <= r; dy++)
+            for (int dx = -r; dx <= r; dx++) {
+              const size_t ai = (size_t)(y + dy) * F.w + x + dx;
+              const size_t bi = (size_t)(y + dy) * F.w + bxi + dx;
+              const double av = left[ai];
+              const double bv = (1 - fraction) * right[bi] + fraction * right[bi + 1];
+              sa += av;
+              sb += bv;
+              saa += av * av;
+              sbb += bv * bv;
+              sab += av * bv;
+            }
+          const double cov = sab - sa * sb / cnt, va = saa - sa * sa / cnt,
+                       vb = sbb - sb * sb / cnt;
+          if (!(va < cnt * 4 || vb < cnt * 4)) {
+            const double prod = va * vb;
+            score = cov / sqrt(prod > 1e-12 ? prod : 1e-12);
+          }
+        }
+        if (score < F.correlation) {
+          disp[p] = __builtin_nanf("");
+          photo_rej = 1;
+          if (reasons) reasons[p] = 6;
+        }
+      }
+    }
+  }
+  add_counts(stats, 7, lr_rej, 8, photo_rej);
+}
+
+// ---- speckles: lock-free union-find over 4/8-connected pixels whose values
+// differ by at most the range (CensusSgm.removeSpeckles, CensusSgm.java:651-681).
+// Roots only ever point to smaller indices, so there are no cycles; every
+// loop is also capped at n steps and raises the error flag if it runs out.
+
+__device__ __forceinline__ int cc_parent(int *parent, int x) {
+  return __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int cc_find(int *parent, int x, int limit, int *error) {
+  for (int it = 0; it < limit; it++) {
+    const int px = cc_parent(parent, x);
+    if (px == x) return x;
+    const int gp = cc_parent(parent, px);
+    if (gp != px) atomicCAS(parent + x, px, gp);  // path halving
+    x = px;
+  }
+  atomicExch(error, 1);
+  return x;
+}
+
+__device__ __forceinline__ void cc_unite(int *parent, int a, int b, int limit, int *error) {
+  for (int it = 0; it < limit; it++) {
+    a = cc_find(parent, a, limit, error);
+    b = cc_find(parent, b, limit, error);
+    if (a == b) return;
+    if (a > b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    if (atomicCAS(parent + b, b, a) == b) return;
+  }
+  atomicExch(error, 1);
+}
+
+static __global__ __launch_bounds__(256) void cc_init_kernel(const float *__restrict__ a, int *parent,
+                                                             int *size, size_t n) {
+  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  parent[p] = isfinite(a[p]) ? (int)p : -1;
+  size[p] = 0;
+}
+
+__device__ __forceinline__ bool cc_linked(const float *a, size_t p, size_t q, double range) {
+  const float aq = a[q];
+  return isfinite(aq) && !((double)fabsf(aq - a[p]) > range);
+}
+
+static __global__ __launch_bounds__(256) void cc_union_kernel(const float *__restrict__ a, int *parent,
+                                                              int *error, FilterArgs F) {
+  const size_t n = (size_t)F.w * F.h;
+  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= n || !isfinite(a[p])) return;
+  const int x = (int)(p % F.w), y = (int)(p / F.w), limit = (int)n + 1;
+  // each undirected edge once: right, down, and for 8-connectivity the two lower diagonals
+  if (x + 1 < F.w && cc_linked(a, p, p + 1, F.speckle_range))
+    cc_unite(parent, (int)p, (int)p + 1, limit, error);
+  if (y + 1 < F.h) {
+    const size_t q = p + F.w;
+    if (cc_linked(a, p, q, F.speckle_range)) cc_unite(parent, (int)p, (int)q, limit, error);
+    if (F.speckle_conn == 8) {
+      if (x > 0 && cc_linked(a, p, q - 1, F.speckle_range))
+        cc_unite(parent, (int)p, (int)q - 1, limit, error);
+      if (x + 1 < F.w && cc_linked(a, p, q + 1, F.speckle_range))
+        cc_unite(parent, (int)p, (int)q + 1, limit, error);
+    }
+  }
+}
+
+static __global__ __launch_bounds__(256) void cc_size_kernel(int *parent, int *size, int *error,
+                                                             size_t n) {
+  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= n || parent[p] < 0) return;
+  const int root = cc_find(parent, (int)p, (int)n + 1, error);
+  atomicAdd(size + root, 1);
+}
+
+static __global__ __launch_bounds__(256) void cc_remove_kernel(float *a, int *parent, const int *size,
+                                                               uint8_t *reasons,
+                                                               unsigned long long *stats, int *error,
+                                                               FilterArgs F) {
+  const size_t n = (size_t)F.w * F.h;
+  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+  unsigned removed = 0, valid = 0;
+  if (p < n && parent[p] >= 0) {
+    const int root = cc_find(parent, (int)p, (int)n + 1, error);
+    if (size[root] < F.speckle_size) {
+      a[p] = __builtin_nanf("");
+      removed = 1;
+      if (reasons) reasons[p] = 7;
+    } else {
+      valid = 1;
+    }
+  }
+  add_counts(stats, 9, removed, 10, valid);
+}
+
+// Valid count when speckle removal is off.
+static __global__ __launch_bounds__(256) void count_valid_kernel(const float *a,
+                                                                 unsigned long long *stats,
+                                                                 size_t n) {
+  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const unsigned v = (p < n && isfinite(a[p])) ? 1u : 0u;
+  add_counts(stats, 10, v, 9, 0u);
+}
+
+}  // namespace sgm

@@ -50,6 +50,17 @@ STAT_FIELDS = [
 ]
 
 
+class SgmFilterParams(ctypes.Structure):
+    _fields_ = [
+        ("median", ctypes.c_int32), ("median_classic", ctypes.c_int32),
+        ("median_edge_threshold", ctypes.c_int32), ("correlation_window", ctypes.c_int32),
+        ("speckle_size", ctypes.c_int32), ("speckle_connectivity", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32), ("reserved1", ctypes.c_int32),
+        ("median_max_change", ctypes.c_double), ("correlation", ctypes.c_double),
+        ("speckle_range", ctypes.c_double), ("reserved2", ctypes.c_double),
+    ]
+
+
 class SgmStats(ctypes.Structure):
     _fields_ = [(f, ctypes.c_int64) for f in STAT_FIELDS] + [("reserved", ctypes.c_int64 * 5)]
 
@@ -92,6 +103,7 @@ class SgmRequestInfo(ctypes.Structure):
 EXPORTS = [
     "sgm_device_count", "sgm_device_info_get", "sgm_open", "sgm_close", "sgm_plan_get",
     "sgm_disparity_limit", "sgm_match", "sgm_match_pair", "sgm_match_pair_device",
+    "sgm_match_pair_filtered",
     "sgm_set_keep_volumes", "sgm_read_volumes", "sgm_set_cost_volume", "sgm_synchronize",
     "sgm_release_buffers", "sgm_set_profiling",
     "sgm_profile_json", "sgm_stream", "sgm_debug_fail_allocations", "sgm_last_error",
@@ -134,6 +146,8 @@ def lib():
             L.sgm_match.argtypes = [P, pp, P, P, P, P, I, I, I, P, ps, P, P]
             L.sgm_match_pair.argtypes = [P, pp, P, P, P, P, I, I, P, P, P, ps, P, P]
             L.sgm_match_pair_device.argtypes = [P, pp, P, P, P, P, I, I, P, P, P, P, P, P]
+            L.sgm_match_pair_filtered.argtypes = [P, pp, ctypes.POINTER(SgmFilterParams), P, P, P,
+                                                  P, I, I, P, ps, P]
             L.sgm_set_keep_volumes.argtypes = [P, I]
             L.sgm_set_cost_volume.argtypes = [P, I]
             L.sgm_read_volumes.argtypes = [P, I, P, P]
@@ -221,6 +235,16 @@ class Options:
         self.gpuDevice = 0
         self.gpuMemoryMb = 0
         self.rawPenalties = False  # p1/p2 already in raw scaled units
+        # post-L-R filters (Options.java:19-23,67,76)
+        self.median = 3
+        self.medianMode = "edge-preserving"
+        self.medianMaxChange = 0.5
+        self.medianEdgeThreshold = 24
+        self.correlationWindow = 7
+        self.correlation = 0.55
+        self.speckleSize = 50
+        self.speckleRange = 1.0
+        self.speckleConnectivity = 8
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise AttributeError("unknown option " + k)
@@ -272,6 +296,18 @@ class StereoCost:
         return 2 if o.sgmAggregate != "32" and fits else 4
 
 
+def filters_from_options(o):
+    """sgm_filter_params from the reference's option names."""
+    if o.medianMode not in ("edge-preserving", "classic"):
+        raise ValueError("--median-mode must be edge-preserving or classic")
+    return SgmFilterParams(
+        median=o.median, median_classic=1 if o.medianMode == "classic" else 0,
+        median_edge_threshold=o.medianEdgeThreshold, correlation_window=o.correlationWindow,
+        speckle_size=o.speckleSize, speckle_connectivity=4 if o.speckleConnectivity == 4 else 8,
+        reserved0=0, reserved1=0, median_max_change=o.medianMaxChange,
+        correlation=o.correlation, speckle_range=o.speckleRange, reserved2=0.0)
+
+
 def params_from_options(o):
     ww, wh = o.window()
     return SgmParams(
@@ -312,7 +348,7 @@ class Stats:
 
 
 class Result:
-    """CensusSgm.Result (CensusSgm.java:739-767) at the left-right-validated checkpoint."""
+    """CensusSgm.Result (CensusSgm.java:739-767): final disparity, stats, reason codes."""
 
     def __init__(self, width, height, disparity, stats, reasons=None, boundary=None,
                  left=None, right=None):
@@ -376,6 +412,23 @@ class Context:
                                    _p(reasons), _p(boundary)))
         return dict(checked=checked, dl=dl, dr=dr, stats=st.as_dict(), reasons=reasons,
                     boundary=boundary)
+
+    def match_pair_filtered(self, params, filters, left, right, valid_left=None,
+                            valid_right=None):
+        """CensusSgm.computeRecorded through 05_final_stereo on host arrays (h, w)."""
+        left = np.ascontiguousarray(left, np.uint8)
+        right = np.ascontiguousarray(right, np.uint8)
+        h, w = left.shape
+        n = w * h
+        vl = None if valid_left is None else np.ascontiguousarray(valid_left, np.uint8)
+        vr = None if valid_right is None else np.ascontiguousarray(valid_right, np.uint8)
+        disp = np.empty(n, np.float32)
+        reasons = np.empty(n, np.uint8)
+        st = SgmStats()
+        check(lib().sgm_match_pair_filtered(self.h, ctypes.byref(params), ctypes.byref(filters),
+                                            _p(left), _p(right), _p(vl), _p(vr), w, h, _p(disp),
+                                            ctypes.byref(st), _p(reasons)))
+        return dict(disparity=disp, stats=st.as_dict(), reasons=reasons)
 
     def match(self, params, gray_a, gray_b, mask_a, mask_b, w, h, reverse, stats=None,
               reasons=None, boundary=None):
@@ -586,9 +639,9 @@ class GpuSgm:
 
 
 class CensusSgm:
-    """CensusSgm.compute (CensusSgm.java:18-280) through the device path, up to the
-    02_left_right_validated checkpoint.  Post-L-R CPU filters (median, photometric,
-    speckles) are outside this engine; options that enable them are rejected."""
+    """CensusSgm.compute (CensusSgm.java:18-280) through the device path: matching,
+    the left-right check and the post-L-R filters (median, reciprocity and
+    photometric ZNCC, speckles), all on the GPU."""
 
     def __init__(self, options):
         self.options = options
@@ -606,14 +659,15 @@ class CensusSgm:
             raise RuntimeError("this engine has no CPU SGM path (compute=cpu)")
         try:
             ctx = GpuSgm.session(o)
-            r = ctx.match_pair(params_from_options(o), left, right, validLeft, validRight)
+            r = ctx.match_pair_filtered(params_from_options(o), filters_from_options(o), left,
+                                        right, validLeft, validRight)
         except (RuntimeError, OSError) as e:
             GpuSgm.closeSession()
             GpuSgm.failure(o, e)
             raise RuntimeError("GPU SGM failed and this engine has no CPU path: %s" % e) from e
         st = Stats()
         st.set_all(r["stats"])
-        return Result(width, height, r["checked"], st, r["reasons"], r["boundary"], r["dl"], r["dr"])
+        return Result(width, height, r["disparity"], st, r["reasons"])
 
 
 # ---------------------------------------------------------------------------

@@ -35,9 +35,10 @@ def test_struct_layouts_match_header():
 #include <stddef.h>
 #include "sgm_hip.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(sgm_params), sizeof(sgm_stats),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(sgm_params), sizeof(sgm_stats),
          sizeof(sgm_device_info), sizeof(sgm_plan), sizeof(sgm_pair), sizeof(sgm_request_info),
-         offsetof(sgm_params, uniqueness), offsetof(sgm_stats, valid));
+         offsetof(sgm_params, uniqueness), offsetof(sgm_stats, valid),
+         sizeof(sgm_filter_params), offsetof(sgm_filter_params, median_max_change));
   return 0;
 }
 '''
@@ -49,7 +50,8 @@ int main(void) {
         got = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
     want = [ctypes.sizeof(sgm.SgmParams), ctypes.sizeof(sgm.SgmStats), ctypes.sizeof(sgm.SgmDeviceInfo),
             ctypes.sizeof(sgm.SgmPlan), ctypes.sizeof(sgm.SgmPair), ctypes.sizeof(sgm.SgmRequestInfo),
-            sgm.SgmParams.uniqueness.offset, sgm.SgmStats.valid.offset]
+            sgm.SgmParams.uniqueness.offset, sgm.SgmStats.valid.offset,
+            ctypes.sizeof(sgm.SgmFilterParams), sgm.SgmFilterParams.median_max_change.offset]
     assert got == want
 
 
@@ -58,6 +60,10 @@ def test_oracle_params_share_the_layout():
     assert [f for f, _ in O.OracleParams._fields_] == [f for f, _ in sgm.SgmParams._fields_]
     assert ctypes.sizeof(O.OracleParams) == ctypes.sizeof(sgm.SgmParams)
     assert ctypes.sizeof(O.OracleStats) == ctypes.sizeof(sgm.SgmStats)
+    assert [f for f, _ in O.OracleFilter._fields_][0] == "median"
+    assert ctypes.sizeof(O.OracleFilter) == ctypes.sizeof(sgm.SgmFilterParams)
+    assert all(getattr(O.OracleFilter, f).offset == getattr(sgm.SgmFilterParams, f).offset
+               for f, _ in O.OracleFilter._fields_)
 
 
 def test_error_paths_without_device():

@@ -1,0 +1,226 @@
+"""CPU tests that pin the oracle's post-L-R filters (oracle_postfilter:
+CensusSgm.java:203-271, 557-681) with checks independent of its C code:
+library routines (numpy sort / corrcoef), brute-force component search, and
+closed forms."""
+import math
+from collections import deque
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def np_median_valid(a, window):
+    """medianValid (CensusSgm.java:557-573): finite values of the clipped window,
+    sorted, element n // 2."""
+    h, w = a.shape
+    r = window // 2
+    out = a.copy()
+    for y in range(h):
+        for x in range(w):
+            if not np.isfinite(a[y, x]):
+                continue
+            v = a[max(0, y - r):y + r + 1, max(0, x - r):x + r + 1].ravel()
+            v = np.sort(v[np.isfinite(v)])
+            out[y, x] = v[len(v) // 2]
+    return out
+
+
+def np_median_edge(a, window, max_change, gray, edge):
+    h, w = a.shape
+    r = window // 2
+    out = np_median_valid(a, window)
+    for y in range(h):
+        for x in range(w):
+            c = a[y, x]
+            if not np.isfinite(c) or abs(np.float32(out[y, x] - c)) < max_change:
+                continue
+            vals = []
+            for yy in range(max(0, y - r), min(h, y + r + 1)):
+                for xx in range(max(0, x - r), min(w, x + r + 1)):
+                    v = a[yy, xx]
+                    if (np.isfinite(v) and abs(np.float32(v - c)) <= max_change and
+                            (edge == 0 or abs(int(gray[yy, xx]) - int(gray[y, x])) <= edge)):
+                        vals.append(v)
+            if vals:
+                out[y, x] = np.sort(np.array(vals, np.float32))[len(vals) // 2]
+    return out
+
+
+def bfs_speckles(a, minimum, rng, conn):
+    h, w = a.shape
+    a = a.copy()
+    seen = np.zeros(a.shape, bool)
+    removed = 0
+    nb = [(dy, dx) for dy in (-1, 0, 1) for dx in (-1, 0, 1)
+          if (dy or dx) and (conn == 8 or dy == 0 or dx == 0)]
+    for y0 in range(h):
+        for x0 in range(w):
+            if seen[y0, x0] or not np.isfinite(a[y0, x0]):
+                continue
+            comp, q = [], deque([(y0, x0)])
+            seen[y0, x0] = True
+            while q:
+                y, x = q.popleft()
+                comp.append((y, x))
+                for dy, dx in nb:
+                    yy, xx = y + dy, x + dx
+                    if 0 <= yy < h and 0 <= xx < w and not seen[yy, xx] and np.isfinite(a[yy, xx]) \
+                            and abs(np.float32(a[yy, xx] - a[y, x])) <= rng:
+                        seen[yy, xx] = True
+                        q.append((yy, xx))
+            if len(comp) < minimum:
+                for y, x in comp:
+                    a[y, x] = np.nan
+                removed += len(comp)
+    return a, removed
+
+
+def random_map(seed, h=23, w=31):
+    rng = np.random.default_rng(seed)
+    a = (rng.integers(0, 8, (h, w)) + rng.random((h, w)) * 0.6).astype(np.float32)
+    a[rng.random((h, w)) < 0.2] = np.nan
+    return a
+
+
+def run_filter(a, gray=None, right_gray=None, dr=None, **kw):
+    h, w = a.shape
+    gray = np.zeros((h, w), np.uint8) if gray is None else gray
+    right_gray = gray if right_gray is None else right_gray
+    f = O.filters(**kw)
+    p = O.params(window_w=3, window_h=3, lr_threshold=1e9)
+    if dr is None:  # reciprocity always passes: dr equals the disparity at the matched pixel
+        dr = np.full((h, w), 0.0, np.float32)
+    out, st, rs = O.postfilter(p, f, gray, right_gray, dr, a, w, h, reasons=np.ones(w * h, np.uint8))
+    return out.reshape(h, w), st, rs.reshape(h, w)
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("window", [3, 5])
+def test_median_classic_matches_numpy(seed, window):
+    a = random_map(seed)
+    h, w = a.shape
+    want = np_median_valid(a, window)
+    f = O.filters(median=window, median_classic=1, correlation=-2.0, speckle_size=0)
+    p = O.params(window_w=3, window_h=3, lr_threshold=1e9)
+    dr = np.zeros((h, w), np.float32)
+    # reciprocity with a huge tolerance rejects only pixels whose match leaves the image
+    out, st, _ = O.postfilter(p, f, np.zeros((h, w), np.uint8), np.zeros((h, w), np.uint8), dr,
+                              a, w, h)
+    out = out.reshape(h, w)
+    keep = np.isfinite(out)
+    assert np.array_equal(out[keep].view(np.uint32), want[keep].view(np.uint32))
+    # everything that vanished did so at the image border (reciprocity) only
+    lost = np.isfinite(want) & ~keep
+    assert st["post_median_lr_rejected"] == int(lost.sum())
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_median_edge_preserving_matches_numpy(seed):
+    a = random_map(10 + seed)
+    h, w = a.shape
+    gray = np.random.default_rng(seed).integers(0, 256, (h, w)).astype(np.uint8)
+    want = np_median_edge(a, 3, 0.5, gray, 24)
+    f = O.filters(median=3, median_classic=0, correlation=-2.0, speckle_size=0)
+    p = O.params(window_w=3, window_h=3, lr_threshold=1e9)
+    out, _, _ = O.postfilter(p, f, gray, gray, np.zeros((h, w), np.float32), a, w, h)
+    out = out.reshape(h, w)
+    keep = np.isfinite(out)
+    assert np.array_equal(out[keep].view(np.uint32), want[keep].view(np.uint32))
+
+
+def test_median_closed_forms():
+    a = np.full((9, 9), 4.25, np.float32)
+    a[4, 4] = 9.0  # isolated outlier: the 3x3 median replaces it
+    out = np_median_valid(a, 3)
+    assert out[4, 4] == np.float32(4.25)
+    b = np.full((9, 9), np.nan, np.float32)
+    b[4, 4] = 2.0  # holes are not filled and a lone value stays itself
+    assert np.isnan(np_median_valid(b, 3)[3, 3]) and np_median_valid(b, 3)[4, 4] == 2.0
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("conn", [4, 8])
+def test_speckles_match_bruteforce(seed, conn):
+    a = random_map(20 + seed, 19, 27)
+    a = np.round(a * 2) / 2  # coarse values: linked neighbours within range 1
+    h, w = a.shape
+    want, removed = bfs_speckles(a, 6, 1.0, conn)
+    f = O.filters(median=1, correlation=-2.0, speckle_size=6, speckle_range=1.0,
+                  speckle_connectivity=conn)
+    p = O.params(window_w=3, window_h=3, lr_threshold=1e9)
+    # reciprocity: dr = 0 and a huge tolerance -> only pixels matched outside the
+    # border are dropped before the speckle pass
+    dr = np.zeros((h, w), np.float32)
+    z = np.zeros((h, w), np.uint8)
+    base, st0, _ = O.postfilter(p, O.filters(median=1, correlation=-2.0, speckle_size=0), z, z,
+                                dr, a, w, h)
+    want, removed = bfs_speckles(base.reshape(h, w), 6, 1.0, conn)
+    out, st, rs = O.postfilter(p, f, z, z, dr, a, w, h, reasons=np.ones(w * h, np.uint8))
+    out = out.reshape(h, w)
+    assert np.array_equal(np.isnan(out), np.isnan(want))
+    assert st["speckle_rejected"] == removed
+    assert int((rs == 7).sum()) == removed
+
+
+def test_speckle_connectivity_closed_form():
+    a = np.full((7, 7), np.nan, np.float32)
+    for i in range(5):
+        a[1 + i, 1 + i] = 3.0  # diagonal chain of 5
+    _, r8 = bfs_speckles(a, 5, 1.0, 8)
+    _, r4 = bfs_speckles(a, 5, 1.0, 4)
+    assert r8 == 0 and r4 == 5
+    f8 = O.filters(median=1, correlation=-2.0, speckle_size=5, speckle_connectivity=8)
+    f4 = O.filters(median=1, correlation=-2.0, speckle_size=5, speckle_connectivity=4)
+    p = O.params(window_w=1 * 3, window_h=3, lr_threshold=1e9)
+    z = np.zeros((7, 7), np.uint8)
+    dr = np.full((7, 7), 3.0, np.float32)
+    # disparity 3 matches x - 3; keep the chain inside the reciprocity border
+    o8, s8, _ = O.postfilter(p, f8, z, z, dr, a, 7, 7)
+    o4, s4, _ = O.postfilter(p, f4, z, z, dr, a, 7, 7)
+    assert s4["speckle_rejected"] >= s8["speckle_rejected"]
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_photometric_zncc_matches_corrcoef(seed):
+    rng = np.random.default_rng(seed)
+    w, h, r = 40, 20, 3
+    a = rng.integers(0, 256, (h, w)).astype(np.uint8)
+    b = np.clip(np.roll(a, -5, axis=1).astype(int) + rng.integers(-30, 31, (h, w)), 0, 255)
+    b = b.astype(np.uint8)
+    for trial in range(20):
+        x, y = int(rng.integers(r, w - r)), int(rng.integers(r, h - r))
+        d = float(np.float32(rng.random() * 12 - 2))
+        got = O.photo_zncc(a, b, w, h, x, y, d, r)
+        target = x - d
+        bx = math.floor(target)
+        fr = target - bx
+        if bx - r < 0 or bx + r + 1 >= w:
+            assert got == -1
+            continue
+        wa = a[y - r:y + r + 1, x - r:x + r + 1].astype(np.float64).ravel()
+        wb = ((1 - fr) * b[y - r:y + r + 1, bx - r:bx + r + 1] +
+              fr * b[y - r:y + r + 1, bx - r + 1:bx + r + 2]).astype(np.float64).ravel()
+        n = wa.size
+        if wa.var() * n < n * 4 or wb.var() * n < n * 4:
+            assert got == -1
+            continue
+        assert abs(got - np.corrcoef(wa, wb)[0, 1]) < 1e-9
+
+
+def test_reciprocity_rejects_with_reason_5():
+    h, w = 12, 20
+    a = np.full((h, w), np.nan, np.float32)
+    a[5, 10] = 3.0
+    a[6, 10] = 3.0
+    dr = np.zeros((h, w), np.float32)
+    dr[5, 7] = 3.0   # matches: |3 - 3| <= tolerance
+    dr[6, 7] = 5.5   # does not
+    p = O.params(window_w=3, window_h=3, lr_threshold=1.0)
+    f = O.filters(median=1, correlation=-2.0, speckle_size=0)
+    z = np.zeros((h, w), np.uint8)
+    out, st, rs = O.postfilter(p, f, z, z, dr, a, w, h, reasons=np.ones(w * h, np.uint8))
+    out = out.reshape(h, w)
+    assert out[5, 10] == 3.0 and np.isnan(out[6, 10])
+    assert st["post_median_lr_rejected"] == 1 and rs.reshape(h, w)[6, 10] == 5
