@@ -136,6 +136,31 @@ def run_batched(sgm, params_for, steps=2):
     }
 
 
+def run_filtered(sgm, left, right, W, H, steps=3):
+    """The benchmark pair through the reference's post-L-R filters (default
+    options: edge-preserving 3x3 median, 7x7 photometric ZNCC >= 0.55, speckles
+    < 50 px): device kernel times of the filter stage."""
+    p = workload_params(sgm)
+    f = sgm.SgmFilterParams(median=3, median_classic=0, median_edge_threshold=24,
+                            correlation_window=7, speckle_size=50, speckle_connectivity=8,
+                            reserved0=0, reserved1=0, median_max_change=0.5, correlation=0.55,
+                            speckle_range=1.0, reserved2=0.0)
+    ctx = sgm.Context(0)
+    r = ctx.match_pair_filtered(p, f, left, right)
+    ctx.set_profiling(True)
+    ctx.profile()
+    for _ in range(steps):
+        r = ctx.match_pair_filtered(p, f, left, right)
+    prof = ctx.profile()["device_kernel_milliseconds"]
+    ctx.set_profiling(False)
+    ctx.close()
+    st = r["stats"]
+    return {"filters_ms": round(prof["filters"]["total_ms"] / steps, 4),
+            "valid_pixels": st["valid"], "post_median_lr_rejected": st["post_median_lr_rejected"],
+            "photometric_rejected": st["photometric_rejected"],
+            "speckle_rejected": st["speckle_rejected"]}
+
+
 def run_zncc(sgm, steps=5):
     """The FP32 ZNCC workload: 1600x1067, 7x7 windows, D=192, P1=12, P2=150, gain
     field 0.8-1.2 on the right image; inputs in HBM; uint16 cost volume."""
@@ -399,6 +424,7 @@ def main():
                              census_weight=500, ad_truncation=30, gradient_truncation=60,
                              reserved=0, uniqueness=0.03, lr_threshold=1.0)
     zncc = None if args.no_zncc else run_zncc(sgm)
+    filtered = run_filtered(sgm, left, right, W, H)
     batched = None if args.no_batched else run_batched(sgm, params_for)
     served = None if args.no_served else run_served(sgm, params_for, rate=args.serve_rate,
                                                     count=args.serve_requests)
@@ -437,6 +463,7 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_step / max(1, path_launches),
                      "copy_kernel": hbm_copy},
         "cpu_baseline": cpu,
+        "filtered": filtered,
         "zncc": zncc,
         "batched": batched,
         "served": served,

@@ -99,6 +99,11 @@ typedef struct sgm_plan {
   int64_t pair_bytes;           /* footprint of a left+right pair (both volumes resident) */
   int64_t budget_bytes;
   int64_t aggregate_bound;      /* StereoCost.aggregateBound */
+  /* Streamed branch of GpuStereo.plan (GpuStereo.java:56-83), "ram" aggregate storage. */
+  int32_t streamed;             /* 1 if a pair runs strip-streamed under the plan mode */
+  int32_t strip_rows;           /* rows per strip of the streamed plan (0 if none fits) */
+  int64_t streamed_bytes;       /* device footprint of a pair under the streamed plan */
+  int64_t host_bytes;           /* pinned host memory of the streamed aggregate */
 } sgm_plan;
 
 typedef struct sgm_context sgm_context;
@@ -112,11 +117,11 @@ int sgm_device_info_get(int index, sgm_device_info *out);
 int sgm_open(int device, int64_t budget_bytes, sgm_context **out);
 int sgm_close(sgm_context *ctx);
 
-/* GpuStereo.plan resident branch (GpuStereo.java:33-54). */
+/* GpuStereo.plan (GpuStereo.java:33-84): resident and streamed footprints. */
 int sgm_plan_get(sgm_context *ctx, const sgm_params *params, int width, int height,
                  sgm_plan *out);
 /* GpuSgm.disparityLimit / GpuStereo.limit (GpuSgm.java:237-245, GpuStereo.java:86-94):
- * the largest D whose resident plan fits, 0 if none. */
+ * the largest D for which a plan of the context's plan mode fits, 0 if none. */
 int sgm_disparity_limit(sgm_context *ctx, const sgm_params *params, int width, int height,
                         int *limit);
 
@@ -198,6 +203,21 @@ int sgm_device_copy(sgm_context *ctx, void *dst, const void *src, size_t bytes);
  * when keep_volumes was set. */
 int sgm_set_keep_volumes(sgm_context *ctx, int keep);
 int sgm_read_volumes(sgm_context *ctx, int direction, uint16_t *cost, uint32_t *aggregate);
+
+/* Aggregation plan (the reference's --gpu-sgm auto|resident|streamed and
+ * --gpu-strip-rows, Options.java:32,39; GpuStereo.plan, GpuStereo.java:33-84).
+ * AUTO runs resident when the pair fits the budget and strip-streamed
+ * otherwise; RESIDENT fails with SGM_ERR_NO_PLAN when it does not fit;
+ * STREAMED always streams.  Streamed runs keep the cost and aggregate volumes
+ * of one strip of `strip_rows` rows on the device (halving the rows until the
+ * strip fits the budget, as the reference does), carry vertical and diagonal
+ * paths across strip seams, and park the aggregate of finished strips in
+ * pinned host memory between the top-down and the bottom-up pass.  Results
+ * are identical to the resident plan for every strip height. */
+#define SGM_PLAN_AUTO 0
+#define SGM_PLAN_RESIDENT 1
+#define SGM_PLAN_STREAMED 2
+int sgm_set_plan_mode(sgm_context *ctx, int mode, int strip_rows);
 
 /* Cost-volume policy for 1-word Census with uint16 aggregates (no counterpart
  * in the reference, whose GpuStereo.plan drops to the streamed plan when the

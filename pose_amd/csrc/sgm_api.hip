@@ -20,6 +20,7 @@
 #include "../../include/sgm_hip.h"
 #include "sgm_internal.h"
 #include "sgm_filters.h"
+#include "sgm_strips.h"
 
 using namespace sgm;
 
@@ -73,6 +74,26 @@ void Buffer::release(sgm_context *ctx) {
     (void)hipFree(ptr);
     ctx->allocated -= cap;
   }
+  ptr = nullptr;
+  cap = 0;
+}
+
+int HostBuffer::ensure(size_t bytes, const char *what) {
+  if (bytes <= cap && ptr) return SGM_OK;
+  release();
+  hipError_t e = hipHostMalloc(&ptr, bytes ? bytes : 8, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    ptr = nullptr;
+    return fail(SGM_ERR_OUT_OF_MEMORY, "pinned host allocation (%zu MiB) for %s failed (HIP %d)",
+                bytes >> 20, what, (int)e);
+  }
+  cap = bytes;
+  return SGM_OK;
+}
+
+void HostBuffer::release() {
+  if (ptr) (void)hipHostFree(ptr);
   ptr = nullptr;
   cap = 0;
 }
@@ -169,6 +190,31 @@ int64_t pair_bytes(const sgm_params *p, int w, int h) {
   return pair_bytes(p, w, h, !census_recompute_possible(p));
 }
 
+// Streamed plan (GpuStereo.java:56-83): whole-image inputs and outputs of both
+// images, the cost (uint16) and aggregate volumes of one strip, and the
+// ping-pong carry rows of up to three vertical/diagonal directions per pass.
+int64_t streamed_pair_bytes(const sgm_params *p, int w, int h, int rows) {
+  const int64_t n = (int64_t)w * h, Dp = padded_disparities(p->disparities);
+  const int sb = std::max(aggregate_bytes(p), 2);
+  const int64_t inputs = direction_bytes(p, w, h, false) - n * Dp * sb;
+  const int64_t strip = (int64_t)w * rows * Dp * (2 + sb);
+  const int64_t carries = 2 * 3 * (int64_t)w * Dp * 4;
+  return 2 * inputs + strip + carries + n * (2 + 2 + 2 + 4) + 256;
+}
+
+// The aggregate of one matching direction, parked between the two passes.
+int64_t streamed_host_bytes(const sgm_params *p, int w, int h) {
+  return (int64_t)w * h * padded_disparities(p->disparities) * std::max(aggregate_bytes(p), 2);
+}
+
+// Largest strip height (gpuStripRows, halved until it fits) of the streamed
+// plan, 0 if none fits.
+int streamed_rows(sgm_context *ctx, const sgm_params *p, int w, int h) {
+  for (int rows = std::min(h, ctx->strip_rows); rows >= 1; rows /= 2)
+    if (streamed_pair_bytes(p, w, h, rows) <= ctx->budget) return rows;
+  return 0;
+}
+
 }  // namespace sgm
 
 // ---------------------------------------------------------------------------
@@ -234,6 +280,7 @@ extern "C" int sgm_close(sgm_context *ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   for (auto &kv : ctx->buffers) kv.second.release(ctx);
+  for (auto &kv : ctx->host_buffers) kv.second.release();
   for (auto &pe : ctx->pending_events) {
     (void)hipEventDestroy(pe.start);
     (void)hipEventDestroy(pe.stop);
@@ -258,6 +305,23 @@ extern "C" int sgm_plan_get(sgm_context *ctx, const sgm_params *params, int w, i
   out->budget_bytes = ctx->budget;
   out->aggregate_bound = aggregate_bound(params);
   out->resident = out->pair_bytes <= ctx->budget;
+  out->strip_rows = streamed_rows(ctx, params, w, h);
+  out->streamed_bytes = out->strip_rows ? streamed_pair_bytes(params, w, h, out->strip_rows) : 0;
+  out->host_bytes = out->strip_rows ? streamed_host_bytes(params, w, h) : 0;
+  out->streamed = ctx->plan_mode == SGM_PLAN_STREAMED ? out->strip_rows > 0
+                  : ctx->plan_mode == SGM_PLAN_AUTO   ? !out->resident && out->strip_rows > 0
+                                                      : 0;
+  return SGM_OK;
+}
+
+extern "C" int sgm_set_plan_mode(sgm_context *ctx, int mode, int strip_rows) {
+  if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
+  if (mode < SGM_PLAN_AUTO || mode > SGM_PLAN_STREAMED)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "plan mode %d not in 0..2", mode);
+  if (strip_rows < 1) return fail(SGM_ERR_INVALID_ARGUMENT, "--gpu-strip-rows must be positive");
+  std::lock_guard<std::mutex> g(ctx->lock);
+  ctx->plan_mode = mode;
+  ctx->strip_rows = strip_rows;
   return SGM_OK;
 }
 
@@ -271,7 +335,8 @@ extern "C" int sgm_disparity_limit(sgm_context *ctx, const sgm_params *params, i
     int mid = (low + high + 1) / 2;
     q.disparities = mid;
     sgm_plan plan;
-    bool ok = mid >= 3 && sgm_plan_get(ctx, &q, w, h, &plan) == SGM_OK && plan.resident;
+    bool ok = mid >= 3 && sgm_plan_get(ctx, &q, w, h, &plan) == SGM_OK &&
+              ((plan.resident && ctx->plan_mode != SGM_PLAN_STREAMED) || plan.streamed);
     if (ok) low = mid;
     else high = mid - 1;
   }
@@ -313,6 +378,8 @@ extern "C" int sgm_release_buffers(sgm_context *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (auto &kv : ctx->buffers) kv.second.release(ctx);
   ctx->buffers.clear();
+  for (auto &kv : ctx->host_buffers) kv.second.release();
+  ctx->host_buffers.clear();
   ctx->last_w = ctx->last_h = 0;
   return SGM_OK;
 }
@@ -429,18 +496,22 @@ CostArgs cost_args(const sgm_params *p) {
   return A;
 }
 
+// Cost rows [J.y0, J.y0 + rows) of every job (rows < 0: the whole image).
 int launch_cost(sgm_context *ctx, const CostJobs &jobs, int njobs, int w, int h, int Dp,
-                const sgm_params *p) {
+                const sgm_params *p, int rows, bool force_u16) {
   ProfileScope ps(ctx, "cost");
+  CostArgs A = cost_args(p);
+  if (force_u16) A.u8 = 0;
+  const int gy = rows < 0 ? h : rows;
   if (p->cost == SGM_COST_ZNCC || p->cost == SGM_COST_ZNCC_FP32) {
     const int ntx = (w + kZnccTX - 1) / kZnccTX, ntd = (Dp + kZnccTD - 1) / kZnccTD;
-    dim3 grid(ntx * ntd, h, njobs);
-    zncc_cost_kernel<<<grid, 256, 0, ctx->stream>>>(jobs, cost_args(p));
+    dim3 grid(ntx * ntd, gy, njobs);
+    zncc_cost_kernel<<<grid, 256, 0, ctx->stream>>>(jobs, A);
     return launch_check("zncc cost kernel");
   }
   const int quads = w * (Dp / 4);
-  dim3 grid((quads + 255) / 256, h, njobs);
-  cost_kernel<<<grid, 256, 0, ctx->stream>>>(jobs, cost_args(p));
+  dim3 grid((quads + 255) / 256, gy, njobs);
+  cost_kernel<<<grid, 256, 0, ctx->stream>>>(jobs, A);
   return launch_check("cost kernel");
 }
 
@@ -582,9 +653,17 @@ struct PairBuffers {
   float *disp[2], *checked;
   uint8_t *reasons, *boundary, *reasons_out, *boundary_out, *code;
   unsigned long long *stats;
+  // streamed plan (strip_rows > 0): one strip's volumes, carry rows, host aggregate
+  int strip_rows;
+  uint16_t *strip_cost;
+  void *strip_sum;
+  int *carry;
+  uint8_t *host_sum;
 };
 
-int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuffers &B) {
+// Resident (rows = 0) or strip-streamed (rows > 0) buffers of a pair.
+int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuffers &B,
+                        int rows = 0) {
   const size_t n = (size_t)w * h, Dp = padded_disparities(p->disparities);
   const int sb = aggregate_bytes(p), words = words_of(p);
   const bool zncc = p->cost == SGM_COST_ZNCC || p->cost == SGM_COST_ZNCC_FP32;
@@ -602,8 +681,14 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
       ENSURE(("wsum" + s).c_str(), n * 4);
       ENSURE(("wsum2" + s).c_str(), n * 4);
     }
-    if (!recompute_cost(ctx, p, w, h)) ENSURE(("cost" + s).c_str(), n * Dp * (cost_u8(p) ? 1 : 2));
-    ENSURE(("sum" + s).c_str(), n * Dp * sb);
+    if (rows) {
+      // the streamed plan's footprint holds no whole-image volume
+      ctx->buffers["cost" + s].release(ctx);
+      ctx->buffers["sum" + s].release(ctx);
+    } else {
+      if (!recompute_cost(ctx, p, w, h)) ENSURE(("cost" + s).c_str(), n * Dp * (cost_u8(p) ? 1 : 2));
+      ENSURE(("sum" + s).c_str(), n * Dp * sb);
+    }
     ENSURE(("disp" + s).c_str(), n * 4);
     B.gray[d] = (uint8_t *)ctx->buffers["gray" + s].ptr;
     B.valid[d] = (uint8_t *)ctx->buffers["valid" + s].ptr;
@@ -612,8 +697,9 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     B.desc[d] = (uint64_t *)ctx->buffers["desc" + s].ptr;
     B.wsum[d] = zncc ? (int32_t *)ctx->buffers["wsum" + s].ptr : nullptr;
     B.wsum2[d] = zncc ? (int32_t *)ctx->buffers["wsum2" + s].ptr : nullptr;
-    B.cost[d] = recompute_cost(ctx, p, w, h) ? nullptr : (uint16_t *)ctx->buffers["cost" + s].ptr;
-    B.sum[d] = ctx->buffers["sum" + s].ptr;
+    B.cost[d] = rows || recompute_cost(ctx, p, w, h) ? nullptr
+                                                     : (uint16_t *)ctx->buffers["cost" + s].ptr;
+    B.sum[d] = rows ? nullptr : ctx->buffers["sum" + s].ptr;
     B.disp[d] = (float *)ctx->buffers["disp" + s].ptr;
   }
   ENSURE("checked", n * 4);
@@ -623,6 +709,22 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
   ENSURE("boundary_out", n);
   ENSURE("stats", 16 * 8);
   ENSURE("code", n);
+  B.strip_rows = rows;
+  B.strip_cost = nullptr;
+  B.strip_sum = nullptr;
+  B.carry = nullptr;
+  B.host_sum = nullptr;
+  if (rows) {
+    const size_t cells = (size_t)w * rows * Dp;
+    ENSURE("strip_cost", cells * 2);
+    ENSURE("strip_sum", cells * sb);
+    ENSURE("carry", 2 * 3 * (size_t)w * Dp * 4);
+    B.strip_cost = (uint16_t *)ctx->buffers["strip_cost"].ptr;
+    B.strip_sum = ctx->buffers["strip_sum"].ptr;
+    B.carry = (int *)ctx->buffers["carry"].ptr;
+    if ((rc = ctx->host_buffers["sum"].ensure(n * Dp * sb, "streamed aggregate"))) return rc;
+    B.host_sum = (uint8_t *)ctx->host_buffers["sum"].ptr;
+  }
 #undef ENSURE
   B.checked = (float *)ctx->buffers["checked"].ptr;
   B.reasons = (uint8_t *)ctx->buffers["reasons"].ptr;
@@ -634,12 +736,158 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
   return SGM_OK;
 }
 
+template <int R, typename SumT>
+void launch_strip_t(sgm_context *ctx, const StripJob &J, const PathArgs &A, int phase) {
+  constexpr int waves = 4;
+  const unsigned blocks = (unsigned)((A.total_lines + waves - 1) / waves);
+  if (phase == PHASE_FIRST)
+    strip_kernel<R, SumT, PHASE_FIRST><<<blocks, waves * 64, 0, ctx->stream>>>(J, A);
+  else if (phase == PHASE_MID)
+    strip_kernel<R, SumT, PHASE_MID><<<blocks, waves * 64, 0, ctx->stream>>>(J, A);
+  else
+    strip_kernel<R, SumT, PHASE_LAST><<<blocks, waves * 64, 0, ctx->stream>>>(J, A);
+}
+
+int launch_strip(sgm_context *ctx, int R, int sum_bytes, const StripJob &J, const PathArgs &A,
+                 int phase) {
+  ProfileScope ps(ctx, "strip");
+#define CASE_STRIP(RR, T)                                                   \
+  case RR:                                                                  \
+    launch_strip_t<RR, T>(ctx, J, A, phase);                                \
+    break;
+  if (sum_bytes == 4) {
+    switch (R) {
+      CASE_STRIP(1, int32_t) CASE_STRIP(2, int32_t) CASE_STRIP(4, int32_t) CASE_STRIP(8, int32_t)
+      CASE_STRIP(16, int32_t) CASE_STRIP(32, int32_t) CASE_STRIP(64, int32_t)
+      default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
+    }
+  } else {
+    switch (R) {
+      CASE_STRIP(2, uint16_t) CASE_STRIP(4, uint16_t) CASE_STRIP(8, uint16_t)
+      CASE_STRIP(16, uint16_t) CASE_STRIP(32, uint16_t) CASE_STRIP(64, uint16_t)
+      default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
+    }
+  }
+#undef CASE_STRIP
+  return launch_check("strip kernel");
+}
+
+// Strip-streamed counterpart of run_directions (the streamed loop of
+// GpuStereo.executeTargets, GpuStereo.java:336-451).  The reference runs every
+// direction over every strip and moves the strip's aggregate through host
+// memory between directions.  Here the directions are split by vertical sense:
+// a top-down pass over the strips runs all directions with dy >= 0 (horizontal
+// ones included) while the strip is on the device, parks the strip's partial
+// aggregate in pinned host memory, and a bottom-up pass brings it back, adds
+// the dy < 0 directions and selects.  Aggregates are integer sums, so the
+// order of the directions does not change them; each direction's path is the
+// same recurrence over the same values as in one whole-image sweep.  One host
+// round trip of the aggregate per pair instead of one per direction.
+int run_directions_streamed(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuffers &B,
+                            const int *slots, int nd, int out_slot, uint8_t *reasons,
+                            uint8_t *boundary) {
+  static const int dirs[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
+  int rc;
+  const int D = p->disparities, Dp = padded_disparities(D), R = path_lanes(p);
+  const int sb = aggregate_bytes(p), rows = B.strip_rows;
+  const int strips = (h + rows - 1) / rows;
+  int down[8], up[8], ndown = 0, nup = 0;
+  for (int dir = 0; dir < p->directions; dir++) {
+    if (dirs[dir][1] >= 0) down[ndown++] = dir;
+    else up[nup++] = dir;
+  }
+  const size_t carry_row = (size_t)w * Dp;  // ints
+  for (int i = 0; i < nd; i++) {
+    const int d = slots[i], o = 1 - d;
+    CostJobs cj;
+    memset(&cj, 0, sizeof cj);
+    CostJob &C = cj.job[0];
+    C.desc_a = B.desc[d];
+    C.desc_b = B.desc[o];
+    C.mask_a = B.mask[d];
+    C.mask_b = B.mask[o];
+    C.gray_a = B.gray[d];
+    C.gray_b = B.gray[o];
+    C.sum_a = B.wsum[d];
+    C.sum2_a = B.wsum2[d];
+    C.sum_b = B.wsum[o];
+    C.sum2_b = B.wsum2[o];
+    C.cost = B.strip_cost;
+    C.w = w;
+    C.h = h;
+    C.D = D;
+    C.Dp = Dp;
+    C.minD = p->min_disparity;
+    C.reverse = d;
+    StripJob J;
+    memset(&J, 0, sizeof J);
+    J.cost = B.strip_cost;
+    J.sum = B.strip_sum;
+    J.mask = B.mask[d];
+    J.gray = B.gray[d];
+    J.disp = B.disp[d];
+    J.reasons = d == out_slot ? reasons : nullptr;
+    J.boundary = d == out_slot ? boundary : nullptr;
+    J.code = d == out_slot ? B.code : nullptr;
+    J.w = w;
+    J.h = h;
+    J.D = D;
+    J.Dp = Dp;
+    J.minD = p->min_disparity;
+    J.rx = p->window_w / 2;
+    J.ry = p->window_h / 2;
+    const int passes = nup ? 2 : 1;
+    for (int pass = 0; pass < passes; pass++) {
+      const int *list = pass == 0 ? down : up;
+      const int count = pass == 0 ? ndown : nup;
+      for (int ord = 0; ord < strips; ord++) {
+        const int s = pass == 0 ? ord : strips - 1 - ord;
+        const int y0 = s * rows, act = std::min(rows, h - y0);
+        const size_t bytes = (size_t)w * act * Dp * sb;
+        uint8_t *host = B.host_sum + (size_t)y0 * w * Dp * sb;
+        C.y0 = y0;
+        if ((rc = launch_cost(ctx, cj, 1, w, h, Dp, p, act, true))) return rc;
+        if (pass == 1) {
+          hipError_t e = hipMemcpyAsync(B.strip_sum, host, bytes, hipMemcpyHostToDevice, ctx->stream);
+          if (e != hipSuccess) return hip_fail(e, "strip aggregate upload");
+        }
+        J.y0 = y0;
+        J.rows = act;
+        int carried = 0;
+        for (int t = 0; t < count; t++) {
+          const int dir = list[t];
+          const bool first = pass == 0 && t == 0, last = pass == passes - 1 && t == count - 1;
+          const int phase = first ? PHASE_FIRST : last ? PHASE_LAST : PHASE_MID;
+          PathArgs A = path_args(p, dir, 1, 0, 0);
+          A.total_lines = segments_of(A.cls, w, act);
+          J.carry_in = J.carry_out = nullptr;
+          if (A.dy != 0) {
+            int *in = B.carry + ((size_t)(ord & 1) * 3 + carried) * carry_row;
+            int *out = B.carry + ((size_t)((ord + 1) & 1) * 3 + carried) * carry_row;
+            J.carry_in = ord > 0 ? in : nullptr;
+            J.carry_out = ord < strips - 1 ? out : nullptr;
+            carried++;
+          }
+          if ((rc = launch_strip(ctx, R, sb, J, A, phase))) return rc;
+        }
+        if (pass == 0 && passes == 2) {
+          hipError_t e = hipMemcpyAsync(host, B.strip_sum, bytes, hipMemcpyDeviceToHost, ctx->stream);
+          if (e != hipSuccess) return hip_fail(e, "strip aggregate download");
+        }
+      }
+    }
+  }
+  return SGM_OK;
+}
+
 // Cost + path sweeps + fused selection for `nd` matching directions
 // (d = 0: left reference, target x - d; d = 1: right reference, target x + d).
 // The selection codes of slot `out_slot` go to B.code (counted later), its
 // reason / boundary updates to `reasons` / `boundary`.
 int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuffers &B,
                    const int *slots, int nd, int out_slot, uint8_t *reasons, uint8_t *boundary) {
+  if (B.strip_rows)
+    return run_directions_streamed(ctx, p, w, h, B, slots, nd, out_slot, reasons, boundary);
   int rc;
   const int D = p->disparities, Dp = padded_disparities(D), R = path_lanes(p);
   const int sb = aggregate_bytes(p);
@@ -789,6 +1037,18 @@ int finish(sgm_context *ctx) {
   return SGM_OK;
 }
 
+// GpuStereo.plan (GpuStereo.java:33-84) under the context's plan mode: *rows =
+// 0 for the resident plan, else the strip height of the streamed plan.
+int pick_plan(sgm_context *ctx, const sgm_params *p, int w, int h, int *rows) {
+  *rows = 0;
+  const bool fits = pair_bytes(p, w, h) <= ctx->budget;
+  if (ctx->plan_mode != SGM_PLAN_STREAMED && fits) return SGM_OK;
+  if (ctx->plan_mode != SGM_PLAN_RESIDENT) *rows = streamed_rows(ctx, p, w, h);
+  if (*rows) return SGM_OK;
+  return fail(SGM_ERR_NO_PLAN, "No GPU SGM memory plan fits the requested range (%lld > %lld bytes)",
+              (long long)pair_bytes(p, w, h), (long long)ctx->budget);
+}
+
 }  // namespace
 
 extern "C" int sgm_match_pair(sgm_context *ctx, const sgm_params *params, const uint8_t *left,
@@ -801,12 +1061,12 @@ extern "C" int sgm_match_pair(sgm_context *ctx, const sgm_params *params, const 
   std::lock_guard<std::mutex> g(ctx->lock);
   int rc = validate_params(params, w, h);
   if (rc) return rc;
-  if (pair_bytes(params, w, h) > ctx->budget)
-    return fail(SGM_ERR_NO_PLAN, "No GPU SGM memory plan fits the requested range (%lld > %lld bytes)",
-                (long long)pair_bytes(params, w, h), (long long)ctx->budget);
+  int rows;
+  if ((rc = pick_plan(ctx, params, w, h, &rows))) return rc;
   (void)hipSetDevice(ctx->device);
   PairBuffers B;
-  if ((rc = ensure_pair_buffers(ctx, params, w, h, B))) return rc;
+  if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows))) return rc;
+  ctx->last_streamed = rows > 0;
   const size_t n = (size_t)w * h;
   if ((rc = copy_h2d(ctx, B.gray[0], left, n))) return rc;
   if ((rc = copy_h2d(ctx, B.gray[1], right, n))) return rc;
@@ -923,11 +1183,12 @@ extern "C" int sgm_match_pair_filtered(sgm_context *ctx, const sgm_params *param
   int rc = validate_params(params, w, h);
   if (rc) return rc;
   if ((rc = validate_filters(filters))) return rc;
-  if (pair_bytes(params, w, h) > ctx->budget)
-    return fail(SGM_ERR_NO_PLAN, "No GPU SGM memory plan fits the requested range");
+  int rows;
+  if ((rc = pick_plan(ctx, params, w, h, &rows))) return rc;
   (void)hipSetDevice(ctx->device);
   PairBuffers B;
-  if ((rc = ensure_pair_buffers(ctx, params, w, h, B))) return rc;
+  if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows))) return rc;
+  ctx->last_streamed = rows > 0;
   const size_t n = (size_t)w * h;
   if ((rc = copy_h2d(ctx, B.gray[0], left, n))) return rc;
   if ((rc = copy_h2d(ctx, B.gray[1], right, n))) return rc;
@@ -971,11 +1232,12 @@ extern "C" int sgm_match_pair_device(sgm_context *ctx, const sgm_params *params,
   std::lock_guard<std::mutex> g(ctx->lock);
   int rc = validate_params(params, w, h);
   if (rc) return rc;
-  if (pair_bytes(params, w, h) > ctx->budget)
-    return fail(SGM_ERR_NO_PLAN, "No GPU SGM memory plan fits the requested range");
+  int rows;
+  if ((rc = pick_plan(ctx, params, w, h, &rows))) return rc;
   (void)hipSetDevice(ctx->device);
   PairBuffers B;
-  if ((rc = ensure_pair_buffers(ctx, params, w, h, B))) return rc;
+  if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows))) return rc;
+  ctx->last_streamed = rows > 0;
   const size_t n = (size_t)w * h;
   unsigned long long *stats = d_stats ? (unsigned long long *)d_stats : B.stats;
   if ((rc = run_pair(ctx, params, w, h, B, (const uint8_t *)d_left, (const uint8_t *)d_right,
@@ -1009,11 +1271,12 @@ extern "C" int sgm_match(sgm_context *ctx, const sgm_params *params, const uint8
   std::lock_guard<std::mutex> g(ctx->lock);
   int rc = validate_params(params, w, h);
   if (rc) return rc;
-  if (pair_bytes(params, w, h) > ctx->budget)
-    return fail(SGM_ERR_NO_PLAN, "No GPU SGM memory plan fits the requested range");
+  int rows;
+  if ((rc = pick_plan(ctx, params, w, h, &rows))) return rc;
   (void)hipSetDevice(ctx->device);
   PairBuffers B;
-  if ((rc = ensure_pair_buffers(ctx, params, w, h, B))) return rc;
+  if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows))) return rc;
+  ctx->last_streamed = rows > 0;
   const size_t n = (size_t)w * h;
   // slot d = reverse: the reference image sits in the slot whose cost job has J.reverse = reverse.
   const int ref = reverse ? 1 : 0, tgt = 1 - ref;
@@ -1083,6 +1346,8 @@ extern "C" int sgm_read_volumes(sgm_context *ctx, int direction, uint16_t *cost,
   std::lock_guard<std::mutex> g(ctx->lock);
   if (!ctx->keep_volumes || ctx->last_w == 0)
     return fail(SGM_ERR_NOT_FOUND, "no retained volumes (enable sgm_set_keep_volumes first)");
+  if (ctx->last_streamed)
+    return fail(SGM_ERR_NOT_FOUND, "no retained volumes: the last pair ran strip-streamed");
   const sgm_params *p = &ctx->last_params;
   const size_t n = (size_t)ctx->last_w * ctx->last_h, D = p->disparities, Dp = padded_disparities(p->disparities);
   const int sb = aggregate_bytes(p);

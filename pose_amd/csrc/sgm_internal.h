@@ -22,6 +22,14 @@ struct Buffer {
   void release(sgm_context *ctx);
 };
 
+// Pinned host memory (hipHostMalloc), grown on demand.
+struct HostBuffer {
+  void *ptr = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes, const char *what);
+  void release();
+};
+
 struct PendingEvent {
   hipEvent_t start, stop;
   const char *name;
@@ -61,7 +69,10 @@ int launch_fill_u64(sgm_context *ctx, unsigned long long *p, unsigned long long 
 int launch_census(sgm_context *ctx, const CensusJobs &jobs, int njobs, int w, int h,
                   const sgm_params *p);
 int launch_cost(sgm_context *ctx, const CostJobs &jobs, int njobs, int w, int h, int Dp,
-                const sgm_params *p);
+                const sgm_params *p, int rows = -1, bool force_u16 = false);
+int64_t streamed_pair_bytes(const sgm_params *p, int w, int h, int rows);
+int64_t streamed_host_bytes(const sgm_params *p, int w, int h);
+int streamed_rows(sgm_context *ctx, const sgm_params *p, int w, int h);
 CostArgs cost_args(const sgm_params *p);
 PathArgs path_args(const sgm_params *p, int dir, int njobs, int total_lines, int keep_sum);
 int launch_path(sgm_context *ctx, int R, int sum_bytes, int mode, const PathJobs &jobs,
@@ -81,6 +92,10 @@ struct sgm_context {
   hipStream_t stream = nullptr;
   std::mutex lock;
   std::map<std::string, sgm::Buffer> buffers;
+  std::map<std::string, sgm::HostBuffer> host_buffers;
+  int plan_mode = 0;       // SGM_PLAN_*
+  int strip_rows = 64;     // Options.gpuStripRows default
+  bool last_streamed = false;
   bool keep_volumes = false;
   int volume_policy = 0;  // SGM_VOLUME_*
   bool profiling = false;

@@ -154,6 +154,7 @@ struct CostJob {
   const int32_t *sum_a, *sum2_a, *sum_b, *sum2_b;   // ZNCC window sums
   uint16_t *cost;
   int w, h, D, Dp, minD, reverse;
+  int y0;  // first image row held by `cost` (strips): cost index ((y - y0) * w + x) * Dp + k
 };
 
 struct CostArgs {
@@ -263,7 +264,7 @@ static __global__ __launch_bounds__(256) void zncc_cost_kernel(CostJobs jobs, Co
   __shared__ uint8_t brow[2 * kZnccMaxR + 1][kZnccTX + 2 * kZnccMaxR + kZnccTD];
   __shared__ int col[kZnccTX + 2 * kZnccMaxR][kZnccTD + 1];
   const CostJob J = jobs.job[blockIdx.z];
-  const int y = blockIdx.y;
+  const int y = blockIdx.y + J.y0;
   const int ntx = (J.w + kZnccTX - 1) / kZnccTX;
   const int x0 = (int)(blockIdx.x % ntx) * kZnccTX, d0 = (int)(blockIdx.x / ntx) * kZnccTD;
   if (y >= J.h || d0 >= J.Dp) return;  // whole workgroup
@@ -314,13 +315,13 @@ static __global__ __launch_bounds__(256) void zncc_cost_kernel(CostJobs jobs, Co
       }
       v[jj] = (unsigned)value & 0xFFFFu;
     }
-    store_cost4(J.cost, p, J.Dp, k0, v, A.u8 != 0);
+    store_cost4(J.cost, p - (size_t)J.y0 * J.w, J.Dp, k0, v, A.u8 != 0);
   }
 }
 
 static __global__ __launch_bounds__(256) void cost_kernel(CostJobs jobs, CostArgs A) {
   const CostJob J = jobs.job[blockIdx.z];
-  const int y = blockIdx.y;
+  const int y = blockIdx.y + J.y0;
   if (y >= J.h) return;
   const int quads = J.Dp >> 2;
   const int qi = blockIdx.x * 256 + threadIdx.x;
@@ -344,7 +345,7 @@ static __global__ __launch_bounds__(256) void cost_kernel(CostJobs jobs, CostArg
     }
     v[j] = (unsigned)value & 0xFFFFu;
   }
-  store_cost4(J.cost, p, J.Dp, k0, v, A.u8 != 0);
+  store_cost4(J.cost, p - (size_t)J.y0 * J.w, J.Dp, k0, v, A.u8 != 0);
 }
 
 }  // namespace sgm

@@ -27,6 +27,9 @@ ERRORS = {
 }
 
 VOLUME_AUTO, VOLUME_RESIDENT, VOLUME_RECOMPUTE = 0, 1, 2
+# aggregation plan: --gpu-sgm auto|resident|streamed (Options.java:32)
+PLAN_AUTO, PLAN_RESIDENT, PLAN_STREAMED = 0, 1, 2
+PLAN_MODES = {"auto": PLAN_AUTO, "resident": PLAN_RESIDENT, "streamed": PLAN_STREAMED}
 COSTS = {"census": 0, "ad-census": 1, "census-gradient": 2, "zncc": 3, "zncc-fp32": 4}
 
 
@@ -80,7 +83,9 @@ class SgmPlan(ctypes.Structure):
                 ("padded_disparities", ctypes.c_int32),
                 ("lanes_per_disparity_group", ctypes.c_int32),
                 ("device_bytes", ctypes.c_int64), ("pair_bytes", ctypes.c_int64),
-                ("budget_bytes", ctypes.c_int64), ("aggregate_bound", ctypes.c_int64)]
+                ("budget_bytes", ctypes.c_int64), ("aggregate_bound", ctypes.c_int64),
+                ("streamed", ctypes.c_int32), ("strip_rows", ctypes.c_int32),
+                ("streamed_bytes", ctypes.c_int64), ("host_bytes", ctypes.c_int64)]
 
 
 class SgmPair(ctypes.Structure):
@@ -104,7 +109,8 @@ EXPORTS = [
     "sgm_device_count", "sgm_device_info_get", "sgm_open", "sgm_close", "sgm_plan_get",
     "sgm_disparity_limit", "sgm_match", "sgm_match_pair", "sgm_match_pair_device",
     "sgm_match_pair_filtered",
-    "sgm_set_keep_volumes", "sgm_read_volumes", "sgm_set_cost_volume", "sgm_synchronize",
+    "sgm_set_keep_volumes", "sgm_read_volumes", "sgm_set_cost_volume", "sgm_set_plan_mode",
+    "sgm_synchronize",
     "sgm_release_buffers", "sgm_set_profiling",
     "sgm_profile_json", "sgm_stream", "sgm_debug_fail_allocations", "sgm_last_error",
     "sgm_device_alloc", "sgm_device_free", "sgm_memcpy", "sgm_device_copy",
@@ -150,6 +156,7 @@ def lib():
                                                   P, I, I, P, ps, P]
             L.sgm_set_keep_volumes.argtypes = [P, I]
             L.sgm_set_cost_volume.argtypes = [P, I]
+            L.sgm_set_plan_mode.argtypes = [P, I, I]
             L.sgm_read_volumes.argtypes = [P, I, P, P]
             L.sgm_synchronize.argtypes = [P]
             L.sgm_release_buffers.argtypes = [P]
@@ -234,6 +241,8 @@ class Options:
         self.compute = "auto"
         self.gpuDevice = 0
         self.gpuMemoryMb = 0
+        self.gpuSgm = "auto"        # auto | resident | streamed (Options.java:32)
+        self.gpuStripRows = 64      # Options.java:39
         self.rawPenalties = False  # p1/p2 already in raw scaled units
         # post-L-R filters (Options.java:19-23,67,76)
         self.median = 3
@@ -454,6 +463,12 @@ class Context:
         """Cost-volume policy: VOLUME_AUTO, VOLUME_RESIDENT or VOLUME_RECOMPUTE."""
         check(lib().sgm_set_cost_volume(self.h, int(policy)))
 
+    def plan_mode(self, mode, strip_rows=64):
+        """Aggregation plan: PLAN_AUTO, PLAN_RESIDENT or PLAN_STREAMED (or the
+        reference's names "auto", "resident", "streamed") and gpuStripRows."""
+        mode = PLAN_MODES[mode] if isinstance(mode, str) else int(mode)
+        check(lib().sgm_set_plan_mode(self.h, mode, int(strip_rows)))
+
     def read_volumes(self, direction, n, D):
         cost = np.empty(n * D, np.uint16)
         agg = np.empty(n * D, np.uint32)
@@ -567,6 +582,15 @@ class GpuSgm:
         return s
 
     @staticmethod
+    def configured(o):
+        """The session with the options' plan mode (--gpu-sgm, --gpu-strip-rows)."""
+        if o.gpuSgm not in PLAN_MODES:
+            raise ValueError("--gpu-sgm must be auto, resident or streamed")
+        s = GpuSgm.session(o)
+        s.plan_mode(o.gpuSgm, o.gpuStripRows)
+        return s
+
+    @staticmethod
     def closeSession():
         s = getattr(_session, "ctx", None)
         _session.ctx = None
@@ -591,7 +615,7 @@ class GpuSgm:
         if o.compute == "cpu":
             return 0
         try:
-            return GpuSgm.session(o).disparity_limit(params_from_options(o), w, h)
+            return GpuSgm.configured(o).disparity_limit(params_from_options(o), w, h)
         except (RuntimeError, OSError) as e:
             GpuSgm.failure(o, e)
             return 0
@@ -618,7 +642,7 @@ class GpuSgm:
         if not GpuSgm.use(w, h, o):
             return None
         try:
-            ctx = GpuSgm.session(o)
+            ctx = GpuSgm.configured(o)
             st = SgmStats()
             codes = None if reasons is None else np.array(reasons, np.uint8, copy=True)
             bnd = None if searchBoundary is None else np.array(searchBoundary, np.uint8, copy=True)
@@ -658,7 +682,7 @@ class CensusSgm:
         if o.compute == "cpu":
             raise RuntimeError("this engine has no CPU SGM path (compute=cpu)")
         try:
-            ctx = GpuSgm.session(o)
+            ctx = GpuSgm.configured(o)
             r = ctx.match_pair_filtered(params_from_options(o), filters_from_options(o), left,
                                         right, validLeft, validRight)
         except (RuntimeError, OSError) as e:
