@@ -161,6 +161,39 @@ def run_filtered(sgm, left, right, W, H, steps=3):
             "speckle_rejected": st["speckle_rejected"]}
 
 
+def run_streamed(sgm, W, H, D, P, d_left, d_right, rows=64, steps=2):
+    """The benchmark pair under the strip-streamed plan (strips of `rows` rows;
+    strip cost + aggregate on the device, the aggregate parked in pinned host
+    memory between the top-down and bottom-up passes): inputs in HBM, the
+    aggregate's PCIe round trip inside the timed region."""
+    params = workload_params(sgm)
+    ctx = sgm.Context(0)
+    ctx.plan_mode(sgm.PLAN_STREAMED, rows)
+    n = W * H
+    out = ctx.alloc(4 * n)
+    ctx.match_pair_device(params, d_left, d_right, W, H, out)
+    ctx.synchronize()
+    ctx.set_profiling(True)
+    ctx.profile()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.match_pair_device(params, d_left, d_right, W, H, out)
+    ctx.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    prof = {k: round(v["total_ms"] / steps, 3)
+            for k, v in ctx.profile()["device_kernel_milliseconds"].items()}
+    ctx.set_profiling(False)
+    plan = ctx.plan(params, W, H)
+    out.free()
+    ctx.close()
+    upd = W * H * D * P * 2
+    return {"workload": "plain pair 1600x1067 D=256 under the streamed plan, strips of %d rows" % rows,
+            "ms_per_pair": round(t * 1e3, 3), "gupdates_per_s": round(upd / t / 1e9, 3),
+            "kernel_ms": prof, "device_bytes": plan["streamed_bytes"],
+            "host_aggregate_bytes_per_direction": plan["host_bytes"],
+            "pcie_bytes_per_pair": 2 * 2 * plan["host_bytes"]}
+
+
 def run_zncc(sgm, steps=5):
     """The FP32 ZNCC workload: 1600x1067, 7x7 windows, D=192, P1=12, P2=150, gain
     field 0.8-1.2 on the right image; inputs in HBM; uint16 cost volume."""
@@ -334,6 +367,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--no-served", action="store_true", help="skip the served-vs-plain leg")
     ap.add_argument("--no-zncc", action="store_true", help="skip the FP32 ZNCC leg")
+    ap.add_argument("--no-streamed", action="store_true", help="skip the strip-streamed leg")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals, pairs/s")
     ap.add_argument("--serve-requests", type=int, default=60)
     args = ap.parse_args()
@@ -414,6 +448,7 @@ def main():
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("bytes_per_launch")
 
+    streamed = None if args.no_streamed else run_streamed(sgm, W, H, D, P, d_left, d_right)
     for b in (d_left, d_right, d_out, d_stats):
         b.free()
     ctx.close()
@@ -464,6 +499,7 @@ def main():
                      "copy_kernel": hbm_copy},
         "cpu_baseline": cpu,
         "filtered": filtered,
+        "streamed": streamed,
         "zncc": zncc,
         "batched": batched,
         "served": served,

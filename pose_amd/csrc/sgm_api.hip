@@ -1156,11 +1156,12 @@ int run_filters(sgm_context *ctx, const sgm_params *p, const sgm_filter_params *
                                                 B.reasons_out, B.stats, F);
   if ((rc = launch_check("photometric kernel"))) return rc;
   if (f->speckle_size > 1) {
-    cc_init_kernel<<<blocks, 256, 0, ctx->stream>>>(work, parent, size, n);
-    cc_union_kernel<<<blocks, 256, 0, ctx->stream>>>(work, parent, error, F);
-    cc_size_kernel<<<blocks, 256, 0, ctx->stream>>>(parent, size, error, n);
-    cc_remove_kernel<<<blocks, 256, 0, ctx->stream>>>(work, parent, size, B.reasons_out, B.stats,
-                                                      error, F);
+    const dim3 tiles((w + kCcTx - 1) / kCcTx, (h + kCcTy - 1) / kCcTy), tile(kCcTx, kCcTy);
+    cc_local_kernel<<<tiles, tile, 0, ctx->stream>>>(work, parent, size, error, F);
+    cc_border_kernel<<<tiles, tile, 0, ctx->stream>>>(work, parent, error, F);
+    cc_flatten_kernel<<<blocks, 256, 0, ctx->stream>>>(parent, error, n);
+    cc_size_kernel<<<blocks, 256, 0, ctx->stream>>>(parent, size, n);
+    cc_remove_kernel<<<blocks, 256, 0, ctx->stream>>>(work, parent, size, B.reasons_out, B.stats, F);
     if ((rc = launch_check("speckle kernels"))) return rc;
   } else {
     count_valid_kernel<<<blocks, 256, 0, ctx->stream>>>(work, B.stats, n);

@@ -169,10 +169,19 @@ static __global__ __launch_bounds__(256) void photo_kernel(float *__restrict__ d
   add_counts(stats, 7, lr_rej, 8, photo_rej);
 }
 
-// ---- speckles: lock-free union-find over 4/8-connected pixels whose values
-// differ by at most the range (CensusSgm.removeSpeckles, CensusSgm.java:651-681).
-// Roots only ever point to smaller indices, so there are no cycles; every
-// loop is also capped at n steps and raises the error flag if it runs out.
+// ---- speckles: connected components of 4/8-connected pixels whose values
+// differ by at most the range (CensusSgm.removeSpeckles, CensusSgm.java:651-681),
+// labelled by union-find in three steps:
+//   cc_local_kernel   union-find inside a 32x8 tile in LDS, one thread per pixel;
+//                     the global parent of a pixel becomes its tile-local root
+//   cc_border_kernel  lock-free global unions only for links that cross tiles
+//   cc_flatten_kernel every pixel points straight at its component's root
+// Roots only ever point to smaller indices (a tile-local root is its
+// component's smallest pixel; global links go from the larger root to the
+// smaller), so there are no cycles; every loop is also capped at n steps and
+// raises the error flag if it runs out.
+
+constexpr int kCcTx = 32, kCcTy = 8;
 
 __device__ __forceinline__ int cc_parent(int *parent, int x) {
   return __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -205,12 +214,30 @@ __device__ __forceinline__ void cc_unite(int *parent, int a, int b, int limit, i
   atomicExch(error, 1);
 }
 
-static __global__ __launch_bounds__(256) void cc_init_kernel(const float *__restrict__ a, int *parent,
-                                                             int *size, size_t n) {
-  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (p >= n) return;
-  parent[p] = isfinite(a[p]) ? (int)p : -1;
-  size[p] = 0;
+// LDS counterparts over the kCcTx * kCcTy tile-local indices.
+__device__ __forceinline__ int cc_find_lds(int *lp, int x, int *bad) {
+  for (int it = 0; it < kCcTx * kCcTy; it++) {
+    const int px = __hip_atomic_load(lp + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (px == x) return x;
+    x = px;
+  }
+  *bad = 1;
+  return x;
+}
+
+__device__ __forceinline__ void cc_unite_lds(int *lp, int a, int b, int *bad) {
+  for (int it = 0; it < kCcTx * kCcTy; it++) {
+    a = cc_find_lds(lp, a, bad);
+    b = cc_find_lds(lp, b, bad);
+    if (a == b) return;
+    if (a > b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    if (atomicCAS(lp + b, b, a) == b) return;
+  }
+  *bad = 1;
 }
 
 __device__ __forceinline__ bool cc_linked(const float *a, size_t p, size_t q, double range) {
@@ -218,45 +245,102 @@ __device__ __forceinline__ bool cc_linked(const float *a, size_t p, size_t q, do
   return isfinite(aq) && !((double)fabsf(aq - a[p]) > range);
 }
 
-static __global__ __launch_bounds__(256) void cc_union_kernel(const float *__restrict__ a, int *parent,
-                                                              int *error, FilterArgs F) {
-  const size_t n = (size_t)F.w * F.h;
-  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (p >= n || !isfinite(a[p])) return;
-  const int x = (int)(p % F.w), y = (int)(p / F.w), limit = (int)n + 1;
-  // each undirected edge once: right, down, and for 8-connectivity the two lower diagonals
-  if (x + 1 < F.w && cc_linked(a, p, p + 1, F.speckle_range))
-    cc_unite(parent, (int)p, (int)p + 1, limit, error);
-  if (y + 1 < F.h) {
-    const size_t q = p + F.w;
-    if (cc_linked(a, p, q, F.speckle_range)) cc_unite(parent, (int)p, (int)q, limit, error);
-    if (F.speckle_conn == 8) {
-      if (x > 0 && cc_linked(a, p, q - 1, F.speckle_range))
-        cc_unite(parent, (int)p, (int)q - 1, limit, error);
-      if (x + 1 < F.w && cc_linked(a, p, q + 1, F.speckle_range))
-        cc_unite(parent, (int)p, (int)q + 1, limit, error);
+// Links of pixel (x, y) to its forward neighbours: right, down and, for
+// 8-connectivity, down-left and down-right (each undirected link once).
+template <typename F>
+__device__ __forceinline__ void cc_forward_links(int x, int y, const FilterArgs &A, F &&link) {
+  if (x + 1 < A.w) link(x + 1, y);
+  if (y + 1 < A.h) {
+    link(x, y + 1);
+    if (A.speckle_conn == 8) {
+      if (x > 0) link(x - 1, y + 1);
+      if (x + 1 < A.w) link(x + 1, y + 1);
     }
   }
 }
 
-static __global__ __launch_bounds__(256) void cc_size_kernel(int *parent, int *size, int *error,
-                                                             size_t n) {
+static __global__ __launch_bounds__(256) void cc_local_kernel(const float *__restrict__ a,
+                                                              int *parent, int *size, int *error,
+                                                              FilterArgs F) {
+  __shared__ int lp[kCcTx * kCcTy];
+  __shared__ int bad;
+  const int tx = threadIdx.x, ty = threadIdx.y, l = ty * kCcTx + tx;
+  const int x0 = blockIdx.x * kCcTx, y0 = blockIdx.y * kCcTy;
+  const int x = x0 + tx, y = y0 + ty;
+  const bool inside = x < F.w && y < F.h;
+  const size_t p = (size_t)y * F.w + x;
+  const bool valid = inside && isfinite(a[p]);
+  if (l == 0) bad = 0;
+  lp[l] = valid ? l : -1;
+  __syncthreads();
+  if (valid)
+    cc_forward_links(x, y, F, [&](int qx, int qy) {
+      if (qx < x0 || qx >= x0 + kCcTx || qy >= y0 + kCcTy) return;  // crosses the tile
+      if (cc_linked(a, p, (size_t)qy * F.w + qx, F.speckle_range))
+        cc_unite_lds(lp, l, (qy - y0) * kCcTx + (qx - x0), &bad);
+    });
+  __syncthreads();
+  if (inside) {
+    int root = -1;
+    if (valid) {
+      const int r = cc_find_lds(lp, l, &bad);
+      root = (int)((size_t)(y0 + r / kCcTx) * F.w + x0 + r % kCcTx);
+    }
+    parent[p] = root;
+    size[p] = 0;
+  }
+  __syncthreads();
+  if (l == 0 && bad) atomicExch(error, 1);
+}
+
+static __global__ __launch_bounds__(256) void cc_border_kernel(const float *__restrict__ a,
+                                                               int *parent, int *error,
+                                                               FilterArgs F) {
+  const int x = blockIdx.x * kCcTx + threadIdx.x, y = blockIdx.y * kCcTy + threadIdx.y;
+  if (x >= F.w || y >= F.h) return;
+  const size_t p = (size_t)y * F.w + x;
+  if (!isfinite(a[p])) return;
+  const int tx0 = x - (int)threadIdx.x, ty0 = y - (int)threadIdx.y, limit = F.w * F.h + 1;
+  cc_forward_links(x, y, F, [&](int qx, int qy) {
+    if (qx >= tx0 && qx < tx0 + kCcTx && qy < ty0 + kCcTy) return;  // inside the tile: done
+    const size_t q = (size_t)qy * F.w + qx;
+    if (cc_linked(a, p, q, F.speckle_range)) cc_unite(parent, (int)p, (int)q, limit, error);
+  });
+}
+
+static __global__ __launch_bounds__(256) void cc_flatten_kernel(int *parent, int *error, size_t n) {
   const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= n || parent[p] < 0) return;
   const int root = cc_find(parent, (int)p, (int)n + 1, error);
-  atomicAdd(size + root, 1);
+  __hip_atomic_store(parent + p, root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-static __global__ __launch_bounds__(256) void cc_remove_kernel(float *a, int *parent, const int *size,
-                                                               uint8_t *reasons,
-                                                               unsigned long long *stats, int *error,
+// Component sizes: lanes of a wave with the same root add their count with
+// one atomic (a large component otherwise serialises on its root's counter).
+static __global__ __launch_bounds__(256) void cc_size_kernel(const int *parent, int *size,
+                                                             size_t n) {
+  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const int root = p < n ? parent[p] : -1;
+  const int lane = (int)(threadIdx.x & 63);
+  unsigned long long active = __ballot(root >= 0);
+  while (active) {
+    const int leader = __ffsll((long long)active) - 1;
+    const int r = __shfl(root, leader);
+    const unsigned long long same = __ballot(root == r);
+    if (lane == leader) atomicAdd(size + r, (int)__popcll(same));
+    active &= ~same;
+  }
+}
+
+static __global__ __launch_bounds__(256) void cc_remove_kernel(float *a, const int *parent,
+                                                               const int *size, uint8_t *reasons,
+                                                               unsigned long long *stats,
                                                                FilterArgs F) {
   const size_t n = (size_t)F.w * F.h;
   const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
   unsigned removed = 0, valid = 0;
   if (p < n && parent[p] >= 0) {
-    const int root = cc_find(parent, (int)p, (int)n + 1, error);
-    if (size[root] < F.speckle_size) {
+    if (size[parent[p]] < F.speckle_size) {
       a[p] = __builtin_nanf("");
       removed = 1;
       if (reasons) reasons[p] = 7;
