@@ -585,18 +585,23 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, ui
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
 }
 
+// Cache policy of the cost / aggregate volume accesses: each cell is read and
+// written once per sweep and the volumes are far larger than the caches, so
+// they stream (nt: the SLC bit position of the buffer instructions' aux field).
+constexpr int kVolumePolicy = 2;
+
 template <int NP>
 __device__ __forceinline__ void bload_pairs(uint32_t *dst, __amdgpu_buffer_rsrc_t r, int off) {
   if constexpr (NP == 1) {
-    dst[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+    dst[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, kVolumePolicy);
   } else if constexpr (NP == 2) {
-    const v2u32 t = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    const v2u32 t = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, kVolumePolicy);
     dst[0] = t.x;
     dst[1] = t.y;
   } else {
 #pragma unroll
     for (int i = 0; i < NP / 4; i++) {
-      const v4u32 t = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16 * i, 0, 0);
+      const v4u32 t = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16 * i, 0, kVolumePolicy);
       dst[4 * i] = t.x;
       dst[4 * i + 1] = t.y;
       dst[4 * i + 2] = t.z;
@@ -608,12 +613,12 @@ __device__ __forceinline__ void bload_pairs(uint32_t *dst, __amdgpu_buffer_rsrc_
 template <int NP>
 __device__ __forceinline__ void bstore_pairs(const uint32_t *src, __amdgpu_buffer_rsrc_t r, int off) {
   if constexpr (NP == 1) {
-    __builtin_amdgcn_raw_buffer_store_b32(src[0], r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(src[0], r, off, 0, kVolumePolicy);
   } else if constexpr (NP == 2) {
     v2u32 t;
     t.x = src[0];
     t.y = src[1];
-    __builtin_amdgcn_raw_buffer_store_b64(t, r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(t, r, off, 0, kVolumePolicy);
   } else {
 #pragma unroll
     for (int i = 0; i < NP / 4; i++) {
@@ -622,7 +627,7 @@ __device__ __forceinline__ void bstore_pairs(const uint32_t *src, __amdgpu_buffe
       t.y = src[4 * i + 1];
       t.z = src[4 * i + 2];
       t.w = src[4 * i + 3];
-      __builtin_amdgcn_raw_buffer_store_b128(t, r, off + 16 * i, 0, 0);
+      __builtin_amdgcn_raw_buffer_store_b128(t, r, off + 16 * i, 0, kVolumePolicy);
     }
   }
 }
@@ -666,9 +671,9 @@ enum { CM_LOAD = 0, CM_FUSED = 1, CM_RECOMPUTE = 2, CM_LOAD8 = 3 };
 template <int R>
 __device__ __forceinline__ void bload_bytes(uint32_t *dst, __amdgpu_buffer_rsrc_t r, int off) {
   if constexpr (R == 2) {
-    dst[0] = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+    dst[0] = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, kVolumePolicy);
   } else if constexpr (R == 4) {
-    dst[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+    dst[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, kVolumePolicy);
   } else {
     bload_pairs<R / 4>(dst, r, off);
   }
@@ -677,9 +682,9 @@ __device__ __forceinline__ void bload_bytes(uint32_t *dst, __amdgpu_buffer_rsrc_
 template <int R>
 __device__ __forceinline__ void bstore_bytes(const uint32_t *src, __amdgpu_buffer_rsrc_t r, int off) {
   if constexpr (R == 2) {
-    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)src[0], r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)src[0], r, off, 0, kVolumePolicy);
   } else if constexpr (R == 4) {
-    __builtin_amdgcn_raw_buffer_store_b32(src[0], r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(src[0], r, off, 0, kVolumePolicy);
   } else {
     bstore_pairs<R / 4>(src, r, off);
   }
