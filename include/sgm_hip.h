@@ -175,6 +175,21 @@ int sgm_match_pair_filtered(sgm_context *ctx, const sgm_params *params,
                             const uint8_t *valid_right, int width, int height, float *disparity,
                             sgm_stats *stats, uint8_t *reasons);
 
+/* --census-mode color-distance (CensusSgm.java:109-121): as sgm_match_pair /
+ * sgm_match_pair_filtered, with the census descriptors and texture masks of
+ * CensusSgm.colorCensus (CensusSgm.java:302-332) computed on the device from
+ * the interleaved BGR rasters (3*width*height bytes each); costs and
+ * edge-adaptive penalties still read the gray rasters, and the photometric
+ * filter scores the BGR rasters over three channels (CensusSgm.java:223-225).
+ * filters NULL: disparity receives the L-R-validated map (02 checkpoint);
+ * otherwise the final map.  dl, dr, stats, reasons, boundary may be NULL. */
+int sgm_match_pair_color(sgm_context *ctx, const sgm_params *params,
+                         const sgm_filter_params *filters, const uint8_t *left,
+                         const uint8_t *right, const uint8_t *left_bgr, const uint8_t *right_bgr,
+                         const uint8_t *valid_left, const uint8_t *valid_right, int width,
+                         int height, float *disparity, float *dl, float *dr, sgm_stats *stats,
+                         uint8_t *reasons, uint8_t *boundary);
+
 /* Same as sgm_match_pair with every buffer in device memory (no transfers, no
  * host synchronisation).  d_stats (device, sgm_stats) may be NULL.  Work is
  * enqueued on the context's stream; call sgm_synchronize before reading. */

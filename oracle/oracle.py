@@ -91,6 +91,18 @@ def lib():
         L.oracle_photo_zncc.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_double, ctypes.c_int]
         L.oracle_photo_zncc.restype = ctypes.c_double
+        L.oracle_photo_zncc_ch.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.oracle_photo_zncc_ch.restype = ctypes.c_double
+        L.oracle_color_census.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, P, P]
+        L.oracle_compute_color.argtypes = [ctypes.POINTER(OracleParams), P, P, P, P, P, P,
+                                           ctypes.c_int, ctypes.c_int, P, P, P,
+                                           ctypes.POINTER(OracleStats), P, P]
+        L.oracle_postfilter_ch.argtypes = [ctypes.POINTER(OracleParams),
+                                           ctypes.POINTER(OracleFilter), P, P, P, ctypes.c_int,
+                                           P, ctypes.c_int, ctypes.c_int, P,
+                                           ctypes.POINTER(OracleStats), P]
         L.oracle_round_float.argtypes = [ctypes.c_float]
         L.oracle_round_float.restype = ctypes.c_int32
         L.oracle_round_double.argtypes = [ctypes.c_double]
@@ -130,6 +142,17 @@ def texture(gray, w, h, ww, wh, valid=None):
     v = None if valid is None else np.ascontiguousarray(valid, np.uint8)
     lib().oracle_texture(_ptr(np.ascontiguousarray(gray, np.uint8)), _ptr(v), w, h, ww, wh, _ptr(out))
     return out
+
+
+def color_census(bgr, w, h, ww, wh, valid=None):
+    """CensusSgm.colorCensus on interleaved BGR (h, w, 3): (descriptor words, textured)."""
+    words = (ww * wh - 1 + 63) // 64
+    out = np.zeros(words * w * h, dtype=np.uint64)
+    tex = np.zeros(w * h, dtype=np.uint8)
+    v = None if valid is None else np.ascontiguousarray(valid, np.uint8)
+    lib().oracle_color_census(_ptr(np.ascontiguousarray(bgr, np.uint8)), _ptr(v), w, h, ww, wh,
+                              _ptr(out), _ptr(tex))
+    return out.reshape(words, h * w), tex
 
 
 def match(p, da, db, ga, gb, ta, tb, w, h, reverse, volumes=False):
@@ -172,9 +195,29 @@ def compute(p, left, right, w, h, valid_left=None, valid_right=None):
                 boundary=boundary)
 
 
-def postfilter(p, f, left, right, dr, checked, w, h, stats=None, reasons=None):
+def compute_color(p, left, right, left_bgr, right_bgr, w, h, valid_left=None, valid_right=None):
+    """compute() with colour-distance census descriptors and texture from BGR."""
+    n = w * h
+    arrs = [np.ascontiguousarray(a, np.uint8).reshape(-1) for a in (left, right, left_bgr, right_bgr)]
+    vl = None if valid_left is None else np.ascontiguousarray(valid_left, np.uint8).reshape(-1)
+    vr = None if valid_right is None else np.ascontiguousarray(valid_right, np.uint8).reshape(-1)
+    checked, dl, dr = (np.empty(n, np.float32) for _ in range(3))
+    reasons = np.empty(n, np.uint8)
+    boundary = np.empty(n, np.uint8)
+    st = OracleStats()
+    rc = lib().oracle_compute_color(ctypes.byref(p), *(_ptr(a) for a in arrs), _ptr(vl), _ptr(vr),
+                                    w, h, _ptr(checked), _ptr(dl), _ptr(dr), ctypes.byref(st),
+                                    _ptr(reasons), _ptr(boundary))
+    if rc != 0:
+        raise ValueError("oracle_compute_color failed: %d" % rc)
+    return dict(checked=checked, dl=dl, dr=dr, stats=st.as_dict(), reasons=reasons,
+                boundary=boundary)
+
+
+def postfilter(p, f, left, right, dr, checked, w, h, stats=None, reasons=None, photo=None):
     """Median, reciprocity + photometric ZNCC and speckles on a copy of `checked`.
-    Returns (filtered, stats dict, reasons)."""
+    ``photo`` = (left_bgr, right_bgr) scores the photometric test over three
+    channels (colour-distance mode).  Returns (filtered, stats dict, reasons)."""
     out = np.array(checked, np.float32, copy=True).reshape(-1)
     st = OracleStats()
     if stats:
@@ -184,8 +227,14 @@ def postfilter(p, f, left, right, dr, checked, w, h, stats=None, reasons=None):
     left = np.ascontiguousarray(left, np.uint8).reshape(-1)
     right = np.ascontiguousarray(right, np.uint8).reshape(-1)
     dr = np.ascontiguousarray(dr, np.float32).reshape(-1)
-    rc = lib().oracle_postfilter(ctypes.byref(p), ctypes.byref(f), _ptr(left), _ptr(right),
-                                 _ptr(dr), w, h, _ptr(out), ctypes.byref(st), _ptr(rs))
+    if photo is None:
+        pl, pr, ch = left, right, 1
+    else:
+        pl, pr = (np.ascontiguousarray(a, np.uint8).reshape(-1) for a in photo)
+        ch = 3
+    rc = lib().oracle_postfilter_ch(ctypes.byref(p), ctypes.byref(f), _ptr(left), _ptr(pl),
+                                    _ptr(pr), ch, _ptr(dr), w, h, _ptr(out), ctypes.byref(st),
+                                    _ptr(rs))
     if rc != 0:
         raise ValueError("oracle_postfilter failed: %d" % rc)
     return out, st.as_dict(), rs
@@ -197,11 +246,24 @@ def photo_zncc(a, b, w, h, x, y, disparity, r):
     return lib().oracle_photo_zncc(_ptr(a), _ptr(b), w, h, x, y, float(disparity), r)
 
 
-def compute_filtered(p, f, left, right, w, h, valid_left=None, valid_right=None):
-    """CensusSgm.computeRecorded through 05_final_stereo."""
-    r = compute(p, left, right, w, h, valid_left, valid_right)
+def photo_zncc_ch(a, b, w, h, x, y, disparity, r, channels):
+    a = np.ascontiguousarray(a, np.uint8).reshape(-1)
+    b = np.ascontiguousarray(b, np.uint8).reshape(-1)
+    return lib().oracle_photo_zncc_ch(_ptr(a), _ptr(b), w, h, x, y, float(disparity), r, channels)
+
+
+def compute_filtered(p, f, left, right, w, h, valid_left=None, valid_right=None, left_bgr=None,
+                     right_bgr=None):
+    """CensusSgm.computeRecorded through 05_final_stereo (colour-distance census
+    and three-channel photometric test when the BGR rasters are given)."""
+    if left_bgr is None:
+        r = compute(p, left, right, w, h, valid_left, valid_right)
+        photo = None
+    else:
+        r = compute_color(p, left, right, left_bgr, right_bgr, w, h, valid_left, valid_right)
+        photo = (left_bgr, right_bgr)
     out, st, rs = postfilter(p, f, left, right, r["dr"], r["checked"], w, h, r["stats"],
-                             r["reasons"])
+                             r["reasons"], photo=photo)
     return dict(r, disparity=out, stats=st, reasons=rs)
 
 

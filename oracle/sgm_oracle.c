@@ -97,6 +97,42 @@ void oracle_texture(const uint8_t *in, const uint8_t *valid, int w, int h, int w
     }
 }
 
+/* CensusSgm.colorCensus (CensusSgm.java:302-332): distances and their sum in
+ * double, in window order; mean over all ww*wh cells (centre included). */
+void oracle_color_census(const uint8_t *bgr, const uint8_t *valid, int w, int h, int ww, int wh,
+                         uint64_t *desc, uint8_t *textured) {
+  int rx = ww / 2, ry = wh / 2, words = oracle_words(ww, wh), cells = ww * wh - 1;
+  size_t n = (size_t)w * h;
+  double *distances = (double *)malloc(sizeof(double) * cells);
+  memset(desc, 0, sizeof(uint64_t) * n * words);
+  memset(textured, 0, n);
+  for (int y = ry; y < h - ry; y++)
+    for (int x = rx; x < w - rx; x++) {
+      size_t p = (size_t)y * w + x, center = 3 * p;
+      int bit = 0;
+      double total = 0, maximum = 0;
+      for (int dy = -ry; dy <= ry; dy++)
+        for (int dx = -rx; dx <= rx; dx++) {
+          if (dx == 0 && dy == 0) continue;
+          size_t q = 3 * ((size_t)(y + dy) * w + x + dx);
+          double squared = 0;
+          for (int c = 0; c < 3; c++) {
+            int difference = (int)bgr[q + c] - (int)bgr[center + c];
+            squared += difference * difference;
+          }
+          double distance = sqrt(squared);
+          distances[bit++] = distance;
+          total += distance;
+          if (distance > maximum) maximum = distance;
+        }
+      double mean = total / (ww * wh);
+      textured[p] = maximum >= 4 && (valid == NULL || valid[p] != 0);
+      for (int k = 0; k < cells; k++)
+        if (distances[k] >= mean) desc[(size_t)(k / 64) * n + p] |= 1ULL << (k % 64);
+    }
+  free(distances);
+}
+
 static int popcount64(uint64_t v) { return __builtin_popcountll(v); }
 
 /* StereoCost.value (StereoCost.java:50-90) plus the FP64 ZNCC-1024 oracle. */
@@ -312,10 +348,11 @@ int oracle_match(const oracle_params *o, const uint64_t *a, const uint64_t *b,
 
 static int isfinite_f(float v) { return isfinite(v); }
 
-int oracle_compute(const oracle_params *o, const uint8_t *left, const uint8_t *right,
-                   const uint8_t *valid_left, const uint8_t *valid_right, int width, int height,
-                   float *checked, float *dl_out, float *dr_out, oracle_stats *stats,
-                   uint8_t *reasons, uint8_t *searchBoundary) {
+static int compute_pair(const oracle_params *o, const uint8_t *left, const uint8_t *right,
+                        const uint8_t *left_bgr, const uint8_t *right_bgr,
+                        const uint8_t *valid_left, const uint8_t *valid_right, int width,
+                        int height, float *checked, float *dl_out, float *dr_out,
+                        oracle_stats *stats, uint8_t *reasons, uint8_t *searchBoundary) {
   /* CensusSgm.computeRecorded (CensusSgm.java:56-60 dimension check). */
   if (width < o->window_w + 2 || height < o->window_h + 2) return -1;
   size_t n = (size_t)width * height;
@@ -328,10 +365,16 @@ int oracle_compute(const oracle_params *o, const uint8_t *left, const uint8_t *r
   if (!a || !b || !ta || !tb || !dl || !dr) return -2;
   if (reasons) memset(reasons, 1, n);
   if (searchBoundary) memset(searchBoundary, 0, n);
-  oracle_census(left, width, height, o->window_w, o->window_h, a);
-  oracle_census(right, width, height, o->window_w, o->window_h, b);
-  oracle_texture(left, valid_left, width, height, o->window_w, o->window_h, ta);
-  oracle_texture(right, valid_right, width, height, o->window_w, o->window_h, tb);
+  if (left_bgr) {
+    /* CensusSgm.java:109-121 and the validity AND of :127-134 */
+    oracle_color_census(left_bgr, valid_left, width, height, o->window_w, o->window_h, a, ta);
+    oracle_color_census(right_bgr, valid_right, width, height, o->window_w, o->window_h, b, tb);
+  } else {
+    oracle_census(left, width, height, o->window_w, o->window_h, a);
+    oracle_census(right, width, height, o->window_w, o->window_h, b);
+    oracle_texture(left, valid_left, width, height, o->window_w, o->window_h, ta);
+    oracle_texture(right, valid_right, width, height, o->window_w, o->window_h, tb);
+  }
   oracle_stats discarded;
   memset(&discarded, 0, sizeof discarded);
   int rc = oracle_match(o, a, b, left, right, ta, tb, width, height, 0, dl, stats, reasons,
@@ -370,6 +413,24 @@ int oracle_compute(const oracle_params *o, const uint8_t *left, const uint8_t *r
   if (!dl_out) free(dl);
   if (!dr_out) free(dr);
   return rc;
+}
+
+int oracle_compute(const oracle_params *o, const uint8_t *left, const uint8_t *right,
+                   const uint8_t *valid_left, const uint8_t *valid_right, int width, int height,
+                   float *checked, float *dl, float *dr, oracle_stats *stats, uint8_t *reasons,
+                   uint8_t *boundary) {
+  return compute_pair(o, left, right, NULL, NULL, valid_left, valid_right, width, height,
+                      checked, dl, dr, stats, reasons, boundary);
+}
+
+int oracle_compute_color(const oracle_params *o, const uint8_t *left, const uint8_t *right,
+                         const uint8_t *left_bgr, const uint8_t *right_bgr,
+                         const uint8_t *valid_left, const uint8_t *valid_right, int width,
+                         int height, float *checked, float *dl, float *dr, oracle_stats *stats,
+                         uint8_t *reasons, uint8_t *boundary) {
+  if (!left_bgr || !right_bgr) return -1;
+  return compute_pair(o, left, right, left_bgr, right_bgr, valid_left, valid_right, width,
+                      height, checked, dl, dr, stats, reasons, boundary);
 }
 
 
@@ -441,29 +502,35 @@ static int reciprocal(const float *right, int w, int h, int x, int y, float disp
          (double)fabsf(disparity - right[(size_t)y * w + xr]) <= tolerance;
 }
 
-/* CensusSgm.zncc (CensusSgm.java:614-644), one channel. */
+/* CensusSgm.zncc (CensusSgm.java:614-644) over `channels` interleaved channels. */
 static double photo_zncc(const uint8_t *a, const uint8_t *b, int w, int h, int x, int y,
-                         double disparity, int r) {
+                         double disparity, int r, int channels) {
   double target = x - disparity;
   int bx = (int)floor(target);
   double fraction = target - bx;
   if (x - r < 0 || x + r >= w || y - r < 0 || y + r >= h || bx - r < 0 || bx + r + 1 >= w)
     return -1;
   int n = (2 * r + 1) * (2 * r + 1);
-  double sa = 0, sb = 0, saa = 0, sbb = 0, sab = 0;
-  for (int dy = -r; dy <= r; dy++)
-    for (int dx = -r; dx <= r; dx++) {
-      size_t ai = (size_t)(y + dy) * w + x + dx, bi = (size_t)(y + dy) * w + bx + dx;
-      double av = a[ai];
-      double bv = (1 - fraction) * b[bi] + fraction * b[bi + 1];
-      sa += av;
-      sb += bv;
-      saa += av * av;
-      sbb += bv * bv;
-      sab += av * bv;
-    }
-  double covariance = sab - sa * sb / n, var_a = saa - sa * sa / n, var_b = sbb - sb * sb / n;
-  if (var_a < n * 4 || var_b < n * 4) return -1;
+  double covariance = 0, var_a = 0, var_b = 0;
+  for (int c = 0; c < channels; c++) {
+    double sa = 0, sb = 0, saa = 0, sbb = 0, sab = 0;
+    for (int dy = -r; dy <= r; dy++)
+      for (int dx = -r; dx <= r; dx++) {
+        size_t ai = ((size_t)(y + dy) * w + x + dx) * channels + c;
+        size_t bi = ((size_t)(y + dy) * w + bx + dx) * channels + c;
+        double av = a[ai];
+        double bv = (1 - fraction) * b[bi] + fraction * b[bi + channels];
+        sa += av;
+        sb += bv;
+        saa += av * av;
+        sbb += bv * bv;
+        sab += av * bv;
+      }
+    covariance += sab - sa * sb / n;
+    var_a += saa - sa * sa / n;
+    var_b += sbb - sb * sb / n;
+  }
+  if (var_a < channels * n * 4 || var_b < channels * n * 4) return -1;
   double prod = var_a * var_b;
   return covariance / sqrt(prod > 1e-12 ? prod : 1e-12);
 }
@@ -506,12 +573,24 @@ static int64_t remove_speckles(float *a, int w, int h, int minimum, double range
 
 double oracle_photo_zncc(const uint8_t *a, const uint8_t *b, int w, int h, int x, int y,
                          double disparity, int r) {
-  return photo_zncc(a, b, w, h, x, y, disparity, r);
+  return photo_zncc(a, b, w, h, x, y, disparity, r, 1);
+}
+
+double oracle_photo_zncc_ch(const uint8_t *a, const uint8_t *b, int w, int h, int x, int y,
+                            double disparity, int r, int channels) {
+  return photo_zncc(a, b, w, h, x, y, disparity, r, channels);
 }
 
 int oracle_postfilter(const oracle_params *o, const oracle_filter *f, const uint8_t *left,
                       const uint8_t *right, const float *dr, int w, int h, float *checked,
                       oracle_stats *stats, uint8_t *reasons) {
+  return oracle_postfilter_ch(o, f, left, left, right, 1, dr, w, h, checked, stats, reasons);
+}
+
+int oracle_postfilter_ch(const oracle_params *o, const oracle_filter *f, const uint8_t *left,
+                         const uint8_t *photo_left, const uint8_t *photo_right, int channels,
+                         const float *dr, int w, int h, float *checked, oracle_stats *stats,
+                         uint8_t *reasons) {
   size_t n = (size_t)w * h;
   int bx = o->window_w / 2, by = o->window_h / 2;
   if (f->median > 1) {
@@ -535,7 +614,8 @@ int oracle_postfilter(const oracle_params *o, const oracle_filter *f, const uint
         if (reasons) reasons[p] = 5;
         continue;
       }
-      double score = photo_zncc(left, right, w, h, x, y, disparity, f->correlation_window / 2);
+      double score = photo_zncc(photo_left, photo_right, w, h, x, y, disparity,
+                                f->correlation_window / 2, channels);
       if (score < f->correlation) {
         checked[p] = NAN;
         stats->photometric_rejected++;

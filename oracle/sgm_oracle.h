@@ -72,6 +72,14 @@ void oracle_census(const uint8_t *gray, int w, int h, int ww, int wh, uint64_t *
 void oracle_texture(const uint8_t *gray, const uint8_t *valid, int w, int h, int ww, int wh,
                     uint8_t *mask);
 
+/* CensusSgm.colorCensus (CensusSgm.java:302-332), rectangular window:
+ * interleaved BGR in; bit k (row-major, centre skipped) set where the colour
+ * distance to the centre >= the window mean (the centre's zero distance
+ * counts in the mean); textured = largest distance >= 4, ANDed with valid
+ * (may be NULL).  Border pixels keep 0 descriptors and are untextured. */
+void oracle_color_census(const uint8_t *bgr, const uint8_t *valid, int w, int h, int ww, int wh,
+                         uint64_t *desc, uint8_t *textured);
+
 /* One matching direction, CPU branch of CensusSgm.match (CensusSgm.java:344-517).
  * cost_out (n*D uint16) and agg_out (n*D uint32, the stored aggregate value)
  * are optional.  reasons/boundary may be NULL. Returns 0 or <0 on bad input. */
@@ -87,6 +95,15 @@ int oracle_compute(const oracle_params *o, const uint8_t *left, const uint8_t *r
                    const uint8_t *valid_left, const uint8_t *valid_right, int w, int h,
                    float *checked, float *dl, float *dr, oracle_stats *stats, uint8_t *reasons,
                    uint8_t *boundary);
+
+/* oracle_compute with colour-distance census descriptors and texture masks
+ * from interleaved BGR (--census-mode color-distance, CensusSgm.java:109-121);
+ * the costs and edge-adaptive penalties still read the gray rasters. */
+int oracle_compute_color(const oracle_params *o, const uint8_t *left, const uint8_t *right,
+                         const uint8_t *left_bgr, const uint8_t *right_bgr,
+                         const uint8_t *valid_left, const uint8_t *valid_right, int w, int h,
+                         float *checked, float *dl, float *dr, oracle_stats *stats,
+                         uint8_t *reasons, uint8_t *boundary);
 
 /* Post-L-R filters of CensusSgm.computeRecorded (CensusSgm.java:203-271);
  * same layout as sgm_filter_params (include/sgm_hip.h). */
@@ -114,9 +131,20 @@ int oracle_postfilter(const oracle_params *o, const oracle_filter *f, const uint
                       const uint8_t *right, const float *dr, int w, int h, float *checked,
                       oracle_stats *stats, uint8_t *reasons);
 
-/* CensusSgm.zncc (CensusSgm.java:614-644), one channel: the photometric score. */
+/* oracle_postfilter with the photometric score over `channels` interleaved
+ * channels of photo_left / photo_right (the BGR rasters in colour-distance
+ * mode, CensusSgm.java:223-225); the median still reads the gray left. */
+int oracle_postfilter_ch(const oracle_params *o, const oracle_filter *f, const uint8_t *left,
+                         const uint8_t *photo_left, const uint8_t *photo_right, int channels,
+                         const float *dr, int w, int h, float *checked, oracle_stats *stats,
+                         uint8_t *reasons);
+
+/* CensusSgm.zncc (CensusSgm.java:614-644) over `channels` interleaved channels:
+ * the photometric score. */
 double oracle_photo_zncc(const uint8_t *a, const uint8_t *b, int w, int h, int x, int y,
                          double disparity, int r);
+double oracle_photo_zncc_ch(const uint8_t *a, const uint8_t *b, int w, int h, int x, int y,
+                            double disparity, int r, int channels);
 
 /* Java Math.round(float) / Math.round(double) (closest, ties toward +inf). */
 int32_t oracle_round_float(float v);

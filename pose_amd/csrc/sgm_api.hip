@@ -653,6 +653,8 @@ struct PairBuffers {
   float *disp[2], *checked;
   uint8_t *reasons, *boundary, *reasons_out, *boundary_out, *code;
   unsigned long long *stats;
+  // colour-distance census: interleaved BGR of both images on the device, or null
+  const uint8_t *bgr[2];
   // streamed plan (strip_rows > 0): one strip's volumes, carry rows, host aggregate
   int strip_rows;
   uint16_t *strip_cost;
@@ -709,6 +711,7 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
   ENSURE("boundary_out", n);
   ENSURE("stats", 16 * 8);
   ENSURE("code", n);
+  B.bgr[0] = B.bgr[1] = nullptr;
   B.strip_rows = rows;
   B.strip_cost = nullptr;
   B.strip_sum = nullptr;
@@ -988,6 +991,29 @@ int run_pair(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuffers &B
     J.h = h;
   }
   if ((rc = launch_census(ctx, cj, 2, w, h, p))) return rc;
+  if (B.bgr[0]) {
+    ColorJobs xj;
+    memset(&xj, 0, sizeof xj);
+    for (int d = 0; d < 2; d++) {
+      ColorJob &J = xj.job[d];
+      J.bgr = B.bgr[d];
+      J.gray = B.gray[d];
+      J.valid = val[d];
+      J.desc = B.desc[d];
+      J.mask = B.mask[d];
+      J.gm = B.gm[d];
+      J.w = w;
+      J.h = h;
+    }
+    dim3 grid((w + kCensusTx - 1) / kCensusTx, (h + kCensusTy - 1) / kCensusTy, 2);
+    dim3 block(kCensusTx, kCensusTy);
+    ProfileScope ps(ctx, "census");
+    if (words_of(p) == 1)
+      color_census_kernel<1><<<grid, block, 0, ctx->stream>>>(xj, p->window_w, p->window_h);
+    else
+      color_census_kernel<2><<<grid, block, 0, ctx->stream>>>(xj, p->window_w, p->window_h);
+    if ((rc = launch_check("colour census kernel"))) return rc;
+  }
   static const int both[2] = {0, 1};
   if ((rc = run_directions(ctx, p, w, h, B, both, 2, 0, B.reasons, B.boundary))) return rc;
   LrArgs L;
@@ -1128,6 +1154,7 @@ int run_filters(sgm_context *ctx, const sgm_params *p, const sgm_filter_params *
   F.by = p->window_h / 2;
   F.lr_threshold = p->lr_threshold;
   F.corr_r = f->correlation_window / 2;
+  F.channels = B.bgr[0] ? 3 : 1;  // CensusSgm.java:223-225
   F.correlation = f->correlation;
   F.speckle_size = f->speckle_size;
   F.speckle_conn = f->speckle_connectivity;
@@ -1152,7 +1179,8 @@ int run_filters(sgm_context *ctx, const sgm_params *p, const sgm_filter_params *
     if ((rc = launch_check("median kernel"))) return rc;
     work = fmed;
   }
-  photo_kernel<<<blocks, 256, 0, ctx->stream>>>(work, B.disp[1], B.gray[0], B.gray[1],
+  photo_kernel<<<blocks, 256, 0, ctx->stream>>>(work, B.disp[1], B.bgr[0] ? B.bgr[0] : B.gray[0],
+                                                B.bgr[1] ? B.bgr[1] : B.gray[1],
                                                 B.reasons_out, B.stats, F);
   if ((rc = launch_check("photometric kernel"))) return rc;
   if (f->speckle_size > 1) {
@@ -1219,6 +1247,74 @@ extern "C" int sgm_match_pair_filtered(sgm_context *ctx, const sgm_params *param
     stats->photometric_rejected = (int64_t)counters[8];
     stats->speckle_rejected = (int64_t)counters[9];
   }
+  return SGM_OK;
+}
+
+extern "C" int sgm_match_pair_color(sgm_context *ctx, const sgm_params *params,
+                                    const sgm_filter_params *filters, const uint8_t *left,
+                                    const uint8_t *right, const uint8_t *left_bgr,
+                                    const uint8_t *right_bgr, const uint8_t *valid_left,
+                                    const uint8_t *valid_right, int w, int h, float *disparity,
+                                    float *dl, float *dr, sgm_stats *stats, uint8_t *reasons,
+                                    uint8_t *boundary) {
+  if (!ctx || !left || !right || !left_bgr || !right_bgr || !disparity)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "Color-distance census requires interleaved BGR inputs");
+  if ((valid_left == nullptr) != (valid_right == nullptr))
+    return fail(SGM_ERR_INVALID_ARGUMENT, "Invalid stereo mask dimensions");
+  std::lock_guard<std::mutex> g(ctx->lock);
+  int rc = validate_params(params, w, h);
+  if (rc) return rc;
+  if (filters && (rc = validate_filters(filters))) return rc;
+  int rows;
+  if ((rc = pick_plan(ctx, params, w, h, &rows))) return rc;
+  (void)hipSetDevice(ctx->device);
+  PairBuffers B;
+  if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows))) return rc;
+  ctx->last_streamed = rows > 0;
+  const size_t n = (size_t)w * h;
+  if ((rc = ctx->buffers["bgr0"].ensure(ctx, 3 * n, "bgr0"))) return rc;
+  if ((rc = ctx->buffers["bgr1"].ensure(ctx, 3 * n, "bgr1"))) return rc;
+  uint8_t *bgr0 = (uint8_t *)ctx->buffers["bgr0"].ptr, *bgr1 = (uint8_t *)ctx->buffers["bgr1"].ptr;
+  B.bgr[0] = bgr0;
+  B.bgr[1] = bgr1;
+  if ((rc = copy_h2d(ctx, B.gray[0], left, n))) return rc;
+  if ((rc = copy_h2d(ctx, B.gray[1], right, n))) return rc;
+  if ((rc = copy_h2d(ctx, bgr0, left_bgr, 3 * n))) return rc;
+  if ((rc = copy_h2d(ctx, bgr1, right_bgr, 3 * n))) return rc;
+  const uint8_t *vl = nullptr, *vr = nullptr;
+  if (valid_left) {
+    if ((rc = copy_h2d(ctx, B.valid[0], valid_left, n))) return rc;
+    if ((rc = copy_h2d(ctx, B.valid[1], valid_right, n))) return rc;
+    vl = B.valid[0];
+    vr = B.valid[1];
+  }
+  if ((rc = run_pair(ctx, params, w, h, B, B.gray[0], B.gray[1], vl, vr, B.checked, B.reasons_out,
+                     B.boundary_out, B.stats)))
+    return rc;
+  float *work = B.checked;
+  if (filters && (rc = run_filters(ctx, params, filters, w, h, B, work))) return rc;
+  unsigned long long counters[16];
+  int error = 0;
+  if ((rc = copy_d2h(ctx, disparity, work, n * 4))) return rc;
+  if (dl && (rc = copy_d2h(ctx, dl, B.disp[0], n * 4))) return rc;
+  if (dr && (rc = copy_d2h(ctx, dr, B.disp[1], n * 4))) return rc;
+  if (reasons && (rc = copy_d2h(ctx, reasons, B.reasons_out, n))) return rc;
+  if (boundary && (rc = copy_d2h(ctx, boundary, B.boundary_out, n))) return rc;
+  if ((rc = copy_d2h(ctx, counters, B.stats, sizeof counters))) return rc;
+  if (filters && (rc = copy_d2h(ctx, &error, ctx->buffers["cc_error"].ptr, sizeof error))) return rc;
+  if ((rc = finish(ctx))) return rc;
+  if (error) return fail(SGM_ERR_HIP, "speckle component search did not converge");
+  if (stats) {
+    stats_from_device(counters, stats);
+    if (filters) {
+      stats->post_median_lr_rejected = (int64_t)counters[7];
+      stats->photometric_rejected = (int64_t)counters[8];
+      stats->speckle_rejected = (int64_t)counters[9];
+    }
+  }
+  ctx->last_w = w;
+  ctx->last_h = h;
+  ctx->last_params = *params;
   return SGM_OK;
 }
 

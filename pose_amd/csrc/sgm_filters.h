@@ -21,6 +21,7 @@ struct FilterArgs {
   int bx, by;            // reciprocity border (census radius per axis)
   double lr_threshold;
   int corr_r;            // photometric window radius
+  int channels;          // interleaved channels of the photometric rasters (1 or 3)
   double correlation;
   int speckle_size, speckle_conn;
   double speckle_range;
@@ -102,7 +103,8 @@ __device__ __forceinline__ void add_counts(unsigned long long *stats, int f0, un
 }
 
 // CensusSgm.reciprocal + CensusSgm.zncc with the photometric threshold
-// (CensusSgm.java:224-257, 602-644); one channel.
+// (CensusSgm.java:224-257, 602-644); `left` / `right` hold F.channels
+// interleaved channels (gray, or BGR in colour-distance mode).
 static __global__ __launch_bounds__(256) void photo_kernel(float *__restrict__ disp,
                                                            const float *__restrict__ dr,
                                                            const uint8_t *__restrict__ left,
@@ -137,23 +139,27 @@ static __global__ __launch_bounds__(256) void photo_kernel(float *__restrict__ d
         double score = -1.0;
         if (!(x - r < 0 || x + r >= F.w || y - r < 0 || y + r >= F.h || bxi - r < 0 ||
               bxi + r + 1 >= F.w)) {
-          const int cnt = (2 * r + 1) * (2 * r + 1);
-          double sa = 0, sb = 0, saa = 0, sbb = 0, sab = 0;
-          for (int dy = -r; dy <= r; dy++)
-            for (int dx = -r; dx <= r; dx++) {
-              const size_t ai = (size_t)(y + dy) * F.w + x + dx;
-              const size_t bi = (size_t)(y + dy) * F.w + bxi + dx;
-              const double av = left[ai];
-              const double bv = (1 - fraction) * right[bi] + fraction * right[bi + 1];
-              sa += av;
-              sb += bv;
-              saa += av * av;
-              sbb += bv * bv;
-              sab += av * bv;
-            }
-          const double cov = sab - sa * sb / cnt, va = saa - sa * sa / cnt,
-                       vb = sbb - sb * sb / cnt;
-          if (!(va < cnt * 4 || vb < cnt * 4)) {
+          const int cnt = (2 * r + 1) * (2 * r + 1), ch = F.channels;
+          double cov = 0, va = 0, vb = 0;
+          for (int c = 0; c < ch; c++) {
+            double sa = 0, sb = 0, saa = 0, sbb = 0, sab = 0;
+            for (int dy = -r; dy <= r; dy++)
+              for (int dx = -r; dx <= r; dx++) {
+                const size_t ai = ((size_t)(y + dy) * F.w + x + dx) * ch + c;
+                const size_t bi = ((size_t)(y + dy) * F.w + bxi + dx) * ch + c;
+                const double av = left[ai];
+                const double bv = (1 - fraction) * right[bi] + fraction * right[bi + ch];
+                sa += av;
+                sb += bv;
+                saa += av * av;
+                sbb += bv * bv;
+                sab += av * bv;
+              }
+            cov += sab - sa * sb / cnt;
+            va += saa - sa * sa / cnt;
+            vb += sbb - sb * sb / cnt;
+          }
+          if (!(va < ch * cnt * 4 || vb < ch * cnt * 4)) {
             const double prod = va * vb;
             score = cov / sqrt(prod > 1e-12 ? prod : 1e-12);
           }

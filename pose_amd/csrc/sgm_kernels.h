@@ -144,6 +144,88 @@ __global__ __launch_bounds__(256) void census_kernel(CensusJobs jobs, int ww, in
 }
 
 // ---------------------------------------------------------------------------
+// Colour-distance census (CensusSgm.colorCensus, CensusSgm.java:302-332) from
+// interleaved BGR, rectangular window: bit k (window order, centre skipped) is
+// set where the Euclidean colour distance to the centre is >= the mean
+// distance over all ww*wh cells (the centre's zero included); a pixel is
+// textured when its largest distance is >= 4.  Distances, their sum and the
+// mean are double, summed in window order; the second pass recomputes each
+// (correctly rounded) square root instead of keeping up to 120 of them.
+// Overwrites the descriptors, the texture&valid mask and the mask byte of gm
+// that census_kernel wrote from the gray raster.
+struct ColorJob {
+  const uint8_t *bgr;       // interleaved B, G, R
+  const uint8_t *gray;      // gray raster of the same image (gm plane)
+  const uint8_t *valid;     // optional validity mask (0 = invalid)
+  uint64_t *desc;
+  uint8_t *mask;
+  uint16_t *gm;
+  int w, h;
+};
+
+struct ColorJobs {
+  ColorJob job[2];
+};
+
+template <int WORDS>
+__global__ __launch_bounds__(256) void color_census_kernel(ColorJobs jobs, int ww, int wh) {
+  const ColorJob J = jobs.job[blockIdx.z];
+  const int rx = ww >> 1, ry = wh >> 1;
+  const int x0 = blockIdx.x * kCensusTx, y0 = blockIdx.y * kCensusTy;
+  if (x0 >= J.w || y0 >= J.h) return;
+  __shared__ uint8_t tile[kCensusTy + 10][(kCensusTx + 10) * 3 + 2];
+  const int tw = kCensusTx + 2 * rx, th = kCensusTy + 2 * ry;
+  const int tid = threadIdx.y * kCensusTx + threadIdx.x;
+  for (int i = tid; i < tw * th * 3; i += 256) {
+    const int ty = i / (tw * 3), r = i - ty * tw * 3;
+    const int gx = x0 - rx + r / 3, gy = y0 - ry + ty;
+    tile[ty][r] = (gx >= 0 && gx < J.w && gy >= 0 && gy < J.h)
+                      ? J.bgr[((size_t)gy * J.w + gx) * 3 + r % 3]
+                      : 0;
+  }
+  __syncthreads();
+  const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+  if (x >= J.w || y >= J.h) return;
+  const size_t n = (size_t)J.w * J.h, p = (size_t)y * J.w + x;
+  const bool interior = x >= rx && x < J.w - rx && y >= ry && y < J.h - ry;
+  uint64_t d0 = 0, d1 = 0;
+  bool textured = false;
+  if (interior) {
+    const uint8_t *c = &tile[threadIdx.y + ry][(threadIdx.x + rx) * 3];
+    const int cb = c[0], cg = c[1], cr = c[2];
+    auto distance = [&](int yy, int xx) {
+      const uint8_t *q = &tile[threadIdx.y + yy][(threadIdx.x + xx) * 3];
+      const int db = q[0] - cb, dg = q[1] - cg, dr = q[2] - cr;
+      return sqrt((double)(db * db + dg * dg + dr * dr));
+    };
+    double total = 0, maximum = 0;
+    for (int yy = 0; yy < wh; yy++)
+      for (int xx = 0; xx < ww; xx++) {
+        if (yy == ry && xx == rx) continue;
+        const double d = distance(yy, xx);
+        total += d;
+        maximum = d > maximum ? d : maximum;
+      }
+    const double mean = total / (double)(ww * wh);
+    textured = maximum >= 4;
+    int bit = 0;
+    for (int yy = 0; yy < wh; yy++)
+      for (int xx = 0; xx < ww; xx++) {
+        if (yy == ry && xx == rx) continue;
+        const uint64_t m = (uint64_t)(distance(yy, xx) >= mean) << (bit & 63);
+        if (WORDS == 1 || bit < 64) d0 |= m;
+        else d1 |= m;
+        bit++;
+      }
+  }
+  const uint8_t m = interior && textured && (J.valid == nullptr || J.valid[p] != 0);
+  J.desc[p] = WORDS == 1 ? (d0 | ((uint64_t)m << 63)) : d0;
+  if (WORDS > 1) J.desc[n + p] = d1;
+  J.mask[p] = m;
+  if (J.gm) J.gm[p] = (uint16_t)(J.gray[p] | (m << 8));
+}
+
+// ---------------------------------------------------------------------------
 // Cost volume (StereoCost.value, StereoCost.java:50-90; validity test of the
 // reference's `costs` kernel, GpuStereo.java:706-731).  One thread = 4
 // consecutive disparities of one pixel -> one 8-byte coalesced store.

@@ -108,7 +108,7 @@ class SgmRequestInfo(ctypes.Structure):
 EXPORTS = [
     "sgm_device_count", "sgm_device_info_get", "sgm_open", "sgm_close", "sgm_plan_get",
     "sgm_disparity_limit", "sgm_match", "sgm_match_pair", "sgm_match_pair_device",
-    "sgm_match_pair_filtered",
+    "sgm_match_pair_filtered", "sgm_match_pair_color",
     "sgm_set_keep_volumes", "sgm_read_volumes", "sgm_set_cost_volume", "sgm_set_plan_mode",
     "sgm_synchronize",
     "sgm_release_buffers", "sgm_set_profiling",
@@ -154,6 +154,8 @@ def lib():
             L.sgm_match_pair_device.argtypes = [P, pp, P, P, P, P, I, I, P, P, P, P, P, P]
             L.sgm_match_pair_filtered.argtypes = [P, pp, ctypes.POINTER(SgmFilterParams), P, P, P,
                                                   P, I, I, P, ps, P]
+            L.sgm_match_pair_color.argtypes = [P, pp, ctypes.POINTER(SgmFilterParams), P, P, P,
+                                               P, P, P, I, I, P, P, P, ps, P, P]
             L.sgm_set_keep_volumes.argtypes = [P, I]
             L.sgm_set_cost_volume.argtypes = [P, I]
             L.sgm_set_plan_mode.argtypes = [P, I, I]
@@ -242,6 +244,7 @@ class Options:
         self.gpuDevice = 0
         self.gpuMemoryMb = 0
         self.gpuSgm = "auto"        # auto | resident | streamed (Options.java:32)
+        self.censusMode = "gray"    # gray | color-distance (Options.java:31)
         self.gpuStripRows = 64      # Options.java:39
         self.rawPenalties = False  # p1/p2 already in raw scaled units
         # post-L-R filters (Options.java:19-23,67,76)
@@ -438,6 +441,31 @@ class Context:
                                             _p(left), _p(right), _p(vl), _p(vr), w, h, _p(disp),
                                             ctypes.byref(st), _p(reasons)))
         return dict(disparity=disp, stats=st.as_dict(), reasons=reasons)
+
+    def match_pair_color(self, params, left, right, left_bgr, right_bgr, filters=None,
+                         valid_left=None, valid_right=None):
+        """--census-mode color-distance: gray (h, w) and interleaved BGR (h, w, 3)
+        rasters.  Without filters the L-R-validated map, else the final map."""
+        left = np.ascontiguousarray(left, np.uint8)
+        right = np.ascontiguousarray(right, np.uint8)
+        h, w = left.shape
+        lb = np.ascontiguousarray(left_bgr, np.uint8)
+        rb = np.ascontiguousarray(right_bgr, np.uint8)
+        if lb.size != 3 * w * h or rb.size != lb.size:
+            raise ValueError("Color-distance census requires interleaved BGR inputs")
+        n = w * h
+        vl = None if valid_left is None else np.ascontiguousarray(valid_left, np.uint8)
+        vr = None if valid_right is None else np.ascontiguousarray(valid_right, np.uint8)
+        disp, dl, dr = (np.empty(n, np.float32) for _ in range(3))
+        reasons = np.empty(n, np.uint8)
+        boundary = np.empty(n, np.uint8)
+        st = SgmStats()
+        fp = None if filters is None else ctypes.byref(filters)
+        check(lib().sgm_match_pair_color(self.h, ctypes.byref(params), fp, _p(left), _p(right),
+                                         _p(lb), _p(rb), _p(vl), _p(vr), w, h, _p(disp), _p(dl),
+                                         _p(dr), ctypes.byref(st), _p(reasons), _p(boundary)))
+        return dict(disparity=disp, checked=disp, dl=dl, dr=dr, stats=st.as_dict(),
+                    reasons=reasons, boundary=boundary)
 
     def match(self, params, gray_a, gray_b, mask_a, mask_b, w, h, reverse, stats=None,
               reasons=None, boundary=None):
@@ -670,8 +698,11 @@ class CensusSgm:
     def __init__(self, options):
         self.options = options
 
-    def compute(self, left, right, width, height, validLeft=None, validRight=None):
+    def compute(self, left, right, width, height, validLeft=None, validRight=None,
+                leftBgr=None, rightBgr=None):
         o = self.options
+        if o.censusMode not in ("gray", "color-distance"):
+            raise ValueError("--census-mode must be gray or color-distance")
         ww, wh = o.window()
         left = np.ascontiguousarray(left, np.uint8).reshape(height, width)
         right = np.ascontiguousarray(right, np.uint8).reshape(height, width)
@@ -683,8 +714,15 @@ class CensusSgm:
             raise RuntimeError("this engine has no CPU SGM path (compute=cpu)")
         try:
             ctx = GpuSgm.configured(o)
-            r = ctx.match_pair_filtered(params_from_options(o), filters_from_options(o), left,
-                                        right, validLeft, validRight)
+            if o.censusMode == "color-distance":
+                # CensusSgm.java:109-116
+                if leftBgr is None or rightBgr is None:
+                    raise ValueError("Color-distance census requires interleaved BGR inputs")
+                r = ctx.match_pair_color(params_from_options(o), left, right, leftBgr, rightBgr,
+                                         filters_from_options(o), validLeft, validRight)
+            else:
+                r = ctx.match_pair_filtered(params_from_options(o), filters_from_options(o),
+                                            left, right, validLeft, validRight)
         except (RuntimeError, OSError) as e:
             GpuSgm.closeSession()
             GpuSgm.failure(o, e)

@@ -34,3 +34,22 @@ def subpixel_texture(w, h, d):
         v = np.clip(v, 0, 255)
         return (np.floor(v) + ((v - np.floor(v)) >= 0.5)).astype(np.uint8)
     return tex(x, y), tex(x + d, y)
+
+
+def bgr_gray(bgr):
+    """Integer luma of interleaved BGR (0.114 B + 0.587 G + 0.299 R, rounded):
+    the gray rasters that accompany colour inputs."""
+    b = bgr.astype(np.int32)
+    return ((29 * b[..., 0] + 150 * b[..., 1] + 77 * b[..., 2] + 128) >> 8).astype(np.uint8)
+
+
+def color_pair(w, h, shift, seed):
+    """Random colour texture, smoothed a little, and its copy shifted by `shift`
+    pixels (right[y][x] = left[y][min(w-1, x+shift)]) with mild noise."""
+    rng = np.random.default_rng(seed)
+    base = rng.integers(0, 256, (h + 2, w + 2, 3)).astype(np.int32)
+    left = ((base[1:-1, 1:-1] * 2 + base[:-2, 1:-1] + base[2:, 1:-1]) // 4).astype(np.uint8)
+    idx = np.minimum(np.arange(w) + shift, w - 1)
+    right = left[:, idx].astype(np.int32) + rng.integers(-2, 3, (h, w, 3))
+    right = np.clip(right, 0, 255).astype(np.uint8)
+    return left, right
