@@ -229,6 +229,8 @@ class Options:
         self.censusHeight = 0
         self.disparities = 128
         self.minDisparity = 0
+        self.disparitiesExplicit = False      # --disparities given (Options.java)
+        self.minDisparityExplicit = False     # --min-disparity given
         self.matchingCost = "census"
         self.p1 = 4.0
         self.p2 = 24.0
