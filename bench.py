@@ -190,8 +190,8 @@ def run_streamed(sgm, W, H, D, P, d_left, d_right, rows=64, steps=2):
     return {"workload": "plain pair 1600x1067 D=256 under the streamed plan, strips of %d rows" % rows,
             "ms_per_pair": round(t * 1e3, 3), "gupdates_per_s": round(upd / t / 1e9, 3),
             "kernel_ms": prof, "device_bytes": plan["streamed_bytes"],
-            "host_aggregate_bytes_per_direction": plan["host_bytes"],
-            "pcie_bytes_per_pair": 2 * 2 * plan["host_bytes"]}
+            "host_aggregate_bytes": plan["host_bytes"],
+            "pcie_bytes_per_pair": 2 * plan["host_bytes"]}
 
 
 def run_zncc(sgm, steps=5):

@@ -191,20 +191,22 @@ int64_t pair_bytes(const sgm_params *p, int w, int h) {
 }
 
 // Streamed plan (GpuStereo.java:56-83): whole-image inputs and outputs of both
-// images, the cost (uint16) and aggregate volumes of one strip, and the
-// ping-pong carry rows of up to three vertical/diagonal directions per pass.
+// images, the cost (uint16) volume of one strip and two strip aggregate slots
+// (one computed while the other moves over PCIe), and the ping-pong carry rows
+// of up to three vertical/diagonal directions per pass.
 int64_t streamed_pair_bytes(const sgm_params *p, int w, int h, int rows) {
   const int64_t n = (int64_t)w * h, Dp = padded_disparities(p->disparities);
   const int sb = std::max(aggregate_bytes(p), 2);
   const int64_t inputs = direction_bytes(p, w, h, false) - n * Dp * sb;
-  const int64_t strip = (int64_t)w * rows * Dp * (2 + sb);
-  const int64_t carries = 2 * 3 * (int64_t)w * Dp * 4;
+  // both matching directions: strip cost + two aggregate slots each, carry rows each
+  const int64_t strip = 2 * (int64_t)w * rows * Dp * (2 + 2 * sb);
+  const int64_t carries = 2 * 2 * 3 * (int64_t)w * Dp * 4;
   return 2 * inputs + strip + carries + n * (2 + 2 + 2 + 4) + 256;
 }
 
-// The aggregate of one matching direction, parked between the two passes.
+// The aggregates of both matching directions, parked between the two passes.
 int64_t streamed_host_bytes(const sgm_params *p, int w, int h) {
-  return (int64_t)w * h * padded_disparities(p->disparities) * std::max(aggregate_bytes(p), 2);
+  return 2 * (int64_t)w * h * padded_disparities(p->disparities) * std::max(aggregate_bytes(p), 2);
 }
 
 // Largest strip height (gpuStripRows, halved until it fits) of the streamed
@@ -285,6 +287,12 @@ extern "C" int sgm_close(sgm_context *ctx) {
     (void)hipEventDestroy(pe.start);
     (void)hipEventDestroy(pe.stop);
   }
+  if (ctx->copy_stream) {
+    (void)hipStreamSynchronize(ctx->copy_stream);
+    (void)hipStreamDestroy(ctx->copy_stream);
+  }
+  for (hipEvent_t e : ctx->strip_events)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return SGM_OK;
@@ -719,13 +727,14 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
   B.host_sum = nullptr;
   if (rows) {
     const size_t cells = (size_t)w * rows * Dp;
-    ENSURE("strip_cost", cells * 2);
-    ENSURE("strip_sum", cells * sb);
-    ENSURE("carry", 2 * 3 * (size_t)w * Dp * 4);
+    // up to two matching directions per strip launch
+    ENSURE("strip_cost", 2 * cells * 2);
+    ENSURE("strip_sum", 2 * 2 * cells * sb);
+    ENSURE("carry", 2 * 2 * 3 * (size_t)w * Dp * 4);
     B.strip_cost = (uint16_t *)ctx->buffers["strip_cost"].ptr;
     B.strip_sum = ctx->buffers["strip_sum"].ptr;
     B.carry = (int *)ctx->buffers["carry"].ptr;
-    if ((rc = ctx->host_buffers["sum"].ensure(n * Dp * sb, "streamed aggregate"))) return rc;
+    if ((rc = ctx->host_buffers["sum"].ensure(2 * n * Dp * sb, "streamed aggregate"))) return rc;
     B.host_sum = (uint8_t *)ctx->host_buffers["sum"].ptr;
   }
 #undef ENSURE
@@ -740,7 +749,7 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
 }
 
 template <int R, typename SumT>
-void launch_strip_t(sgm_context *ctx, const StripJob &J, const PathArgs &A, int phase) {
+void launch_strip_t(sgm_context *ctx, const StripJobs &J, const PathArgs &A, int phase) {
   constexpr int waves = 4;
   const unsigned blocks = (unsigned)((A.total_lines + waves - 1) / waves);
   if (phase == PHASE_FIRST)
@@ -751,7 +760,7 @@ void launch_strip_t(sgm_context *ctx, const StripJob &J, const PathArgs &A, int 
     strip_kernel<R, SumT, PHASE_LAST><<<blocks, waves * 64, 0, ctx->stream>>>(J, A);
 }
 
-int launch_strip(sgm_context *ctx, int R, int sum_bytes, const StripJob &J, const PathArgs &A,
+int launch_strip(sgm_context *ctx, int R, int sum_bytes, const StripJobs &J, const PathArgs &A,
                  int phase) {
   ProfileScope ps(ctx, "strip");
 #define CASE_STRIP(RR, T)                                                   \
@@ -799,12 +808,30 @@ int run_directions_streamed(sgm_context *ctx, const sgm_params *p, int w, int h,
     if (dirs[dir][1] >= 0) down[ndown++] = dir;
     else up[nup++] = dir;
   }
+  // per matching direction i of the call: strip cost i, aggregate slots (slot, i),
+  // six carry rows, the host aggregate i
   const size_t carry_row = (size_t)w * Dp;  // ints
+  const size_t cells = (size_t)w * rows * Dp, slot_bytes = cells * sb;
+  const size_t host_bytes = (size_t)w * h * Dp * sb;
+  hipError_t e = hipSuccess;
+  if (!ctx->copy_stream) e = hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking);
+  for (int k = 0; k < 4 && e == hipSuccess; k++)
+    if (!ctx->strip_events[k])
+      e = hipEventCreateWithFlags(&ctx->strip_events[k], hipEventDisableTiming);
+  if (e != hipSuccess) return hip_fail(e, "streamed plan stream/events");
+  hipStream_t cs = ctx->copy_stream;
+  hipEvent_t *computed = ctx->strip_events, *copied = ctx->strip_events + 2;
+  // both slots start free: earlier work on the compute stream is all they wait for
+  for (int k = 0; k < 4; k++)
+    if ((e = hipEventRecord(ctx->strip_events[k], ctx->stream)) != hipSuccess)
+      return hip_fail(e, "hipEventRecord");
+  CostJobs cj;
+  memset(&cj, 0, sizeof cj);
+  StripJobs sj;
+  memset(&sj, 0, sizeof sj);
   for (int i = 0; i < nd; i++) {
     const int d = slots[i], o = 1 - d;
-    CostJobs cj;
-    memset(&cj, 0, sizeof cj);
-    CostJob &C = cj.job[0];
+    CostJob &C = cj.job[i];
     C.desc_a = B.desc[d];
     C.desc_b = B.desc[o];
     C.mask_a = B.mask[d];
@@ -815,17 +842,15 @@ int run_directions_streamed(sgm_context *ctx, const sgm_params *p, int w, int h,
     C.sum2_a = B.wsum2[d];
     C.sum_b = B.wsum[o];
     C.sum2_b = B.wsum2[o];
-    C.cost = B.strip_cost;
+    C.cost = B.strip_cost + i * cells;
     C.w = w;
     C.h = h;
     C.D = D;
     C.Dp = Dp;
     C.minD = p->min_disparity;
     C.reverse = d;
-    StripJob J;
-    memset(&J, 0, sizeof J);
-    J.cost = B.strip_cost;
-    J.sum = B.strip_sum;
+    StripJob &J = sj.job[i];
+    J.cost = C.cost;
     J.mask = B.mask[d];
     J.gray = B.gray[d];
     J.disp = B.disp[d];
@@ -839,44 +864,71 @@ int run_directions_streamed(sgm_context *ctx, const sgm_params *p, int w, int h,
     J.minD = p->min_disparity;
     J.rx = p->window_w / 2;
     J.ry = p->window_h / 2;
-    const int passes = nup ? 2 : 1;
-    for (int pass = 0; pass < passes; pass++) {
-      const int *list = pass == 0 ? down : up;
-      const int count = pass == 0 ? ndown : nup;
-      for (int ord = 0; ord < strips; ord++) {
-        const int s = pass == 0 ? ord : strips - 1 - ord;
-        const int y0 = s * rows, act = std::min(rows, h - y0);
-        const size_t bytes = (size_t)w * act * Dp * sb;
-        uint8_t *host = B.host_sum + (size_t)y0 * w * Dp * sb;
-        C.y0 = y0;
-        if ((rc = launch_cost(ctx, cj, 1, w, h, Dp, p, act, true))) return rc;
-        if (pass == 1) {
-          hipError_t e = hipMemcpyAsync(B.strip_sum, host, bytes, hipMemcpyHostToDevice, ctx->stream);
-          if (e != hipSuccess) return hip_fail(e, "strip aggregate upload");
-        }
-        J.y0 = y0;
-        J.rows = act;
-        int carried = 0;
-        for (int t = 0; t < count; t++) {
-          const int dir = list[t];
-          const bool first = pass == 0 && t == 0, last = pass == passes - 1 && t == count - 1;
-          const int phase = first ? PHASE_FIRST : last ? PHASE_LAST : PHASE_MID;
-          PathArgs A = path_args(p, dir, 1, 0, 0);
-          A.total_lines = segments_of(A.cls, w, act);
+  }
+  auto dev_slot = [&](int slot, int i) {
+    return (uint8_t *)B.strip_sum + ((size_t)slot * nd + i) * slot_bytes;
+  };
+  const int passes = nup ? 2 : 1;
+  for (int pass = 0; pass < passes; pass++) {
+    const int *list = pass == 0 ? down : up;
+    const int count = pass == 0 ? ndown : nup;
+    for (int ord = 0; ord < strips; ord++) {
+      const int s = pass == 0 ? ord : strips - 1 - ord, slot = ord & 1;
+      const int y0 = s * rows, act = std::min(rows, h - y0);
+      const size_t bytes = (size_t)w * act * Dp * sb, hoff = (size_t)y0 * w * Dp * sb;
+      if (pass == 1) {
+        // upload on the copy stream once the slot's previous strip is done;
+        // it overlaps the sweeps of the strip before
+        if ((e = hipStreamWaitEvent(cs, computed[slot], 0)) != hipSuccess)
+          return hip_fail(e, "hipStreamWaitEvent");
+        for (int i = 0; i < nd; i++)
+          if ((e = hipMemcpyAsync(dev_slot(slot, i), B.host_sum + i * host_bytes + hoff, bytes,
+                                  hipMemcpyHostToDevice, cs)) != hipSuccess)
+            return hip_fail(e, "strip aggregate upload");
+        if ((e = hipEventRecord(copied[slot], cs)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+      }
+      // the slot is free (pass 0: its last download is done) or filled (pass 1)
+      if ((e = hipStreamWaitEvent(ctx->stream, copied[slot], 0)) != hipSuccess)
+        return hip_fail(e, "hipStreamWaitEvent");
+      for (int i = 0; i < nd; i++) cj.job[i].y0 = y0;
+      if ((rc = launch_cost(ctx, cj, nd, w, h, Dp, p, act, true))) return rc;
+      for (int i = 0; i < nd; i++) {
+        sj.job[i].sum = dev_slot(slot, i);
+        sj.job[i].y0 = y0;
+        sj.job[i].rows = act;
+      }
+      int carried = 0;
+      for (int t = 0; t < count; t++) {
+        const int dir = list[t];
+        const bool first = pass == 0 && t == 0, last = pass == passes - 1 && t == count - 1;
+        const int phase = first ? PHASE_FIRST : last ? PHASE_LAST : PHASE_MID;
+        PathArgs A = path_args(p, dir, nd, 0, 0);
+        sj.split = segments_of(A.cls, w, act);
+        A.total_lines = nd * sj.split;
+        for (int i = 0; i < nd; i++) {
+          StripJob &J = sj.job[i];
           J.carry_in = J.carry_out = nullptr;
           if (A.dy != 0) {
-            int *in = B.carry + ((size_t)(ord & 1) * 3 + carried) * carry_row;
-            int *out = B.carry + ((size_t)((ord + 1) & 1) * 3 + carried) * carry_row;
-            J.carry_in = ord > 0 ? in : nullptr;
-            J.carry_out = ord < strips - 1 ? out : nullptr;
-            carried++;
+            int *base = B.carry + (size_t)i * 6 * carry_row;
+            J.carry_in = ord > 0 ? base + ((size_t)(ord & 1) * 3 + carried) * carry_row : nullptr;
+            J.carry_out =
+                ord < strips - 1 ? base + ((size_t)((ord + 1) & 1) * 3 + carried) * carry_row : nullptr;
           }
-          if ((rc = launch_strip(ctx, R, sb, J, A, phase))) return rc;
         }
-        if (pass == 0 && passes == 2) {
-          hipError_t e = hipMemcpyAsync(host, B.strip_sum, bytes, hipMemcpyDeviceToHost, ctx->stream);
-          if (e != hipSuccess) return hip_fail(e, "strip aggregate download");
-        }
+        if (A.dy != 0) carried++;
+        if ((rc = launch_strip(ctx, R, sb, sj, A, phase))) return rc;
+      }
+      if ((e = hipEventRecord(computed[slot], ctx->stream)) != hipSuccess)
+        return hip_fail(e, "hipEventRecord");
+      if (pass == 0 && passes == 2) {
+        // download on the copy stream, overlapping the sweeps of the next strip
+        if ((e = hipStreamWaitEvent(cs, computed[slot], 0)) != hipSuccess)
+          return hip_fail(e, "hipStreamWaitEvent");
+        for (int i = 0; i < nd; i++)
+          if ((e = hipMemcpyAsync(B.host_sum + i * host_bytes + hoff, dev_slot(slot, i), bytes,
+                                  hipMemcpyDeviceToHost, cs)) != hipSuccess)
+            return hip_fail(e, "strip aggregate download");
+        if ((e = hipEventRecord(copied[slot], cs)) != hipSuccess) return hip_fail(e, "hipEventRecord");
       }
     }
   }

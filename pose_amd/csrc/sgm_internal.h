@@ -96,6 +96,10 @@ struct sgm_context {
   int plan_mode = 0;       // SGM_PLAN_*
   int strip_rows = 64;     // Options.gpuStripRows default
   bool last_streamed = false;
+  // streamed plan: copy stream and the events of the two strip aggregate slots
+  // (0-1: slot computed, 2-3: slot copied), created on first use
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t strip_events[4] = {nullptr, nullptr, nullptr, nullptr};
   bool keep_volumes = false;
   int volume_policy = 0;  // SGM_VOLUME_*
   bool profiling = false;

@@ -229,15 +229,23 @@ __device__ __forceinline__ void strip_segment(const StripJob &J, const PathArgs 
   }
 }
 
+// The strip of up to two matching directions in one launch: lines [0, split)
+// belong to job 0, the rest to job 1.
+struct StripJobs {
+  StripJob job[2];
+  int split;
+};
+
 template <int R, typename SumT, int PHASE>
-__global__ __launch_bounds__(256) void strip_kernel(StripJob J, PathArgs A) {
+__global__ __launch_bounds__(256) void strip_kernel(StripJobs jobs, PathArgs A) {
   __shared__ int p2_table[256];
   build_p2_table(p2_table, A);
   __syncthreads();
   const int waves = blockDim.x >> 6;
   const int line = __builtin_amdgcn_readfirstlane(blockIdx.x * waves + (int)(threadIdx.x >> 6));
   if (line >= A.total_lines) return;
-  strip_segment<R, SumT, PHASE>(J, A, line, p2_table);
+  if (line < jobs.split) strip_segment<R, SumT, PHASE>(jobs.job[0], A, line, p2_table);
+  else strip_segment<R, SumT, PHASE>(jobs.job[1], A, line - jobs.split, p2_table);
 }
 
 }  // namespace sgm
