@@ -135,6 +135,17 @@ int sgm_match(sgm_context *ctx, const sgm_params *params, const uint8_t *gray_a,
               int height, int reverse, float *disp, sgm_stats *stats, uint8_t *reasons,
               uint8_t *boundary);
 
+/* GpuSgm.match with the caller's census descriptors `a` / `b` (GpuSgm.java:267,
+ * the long[][] arguments) instead of a census of gray_a / gray_b: word-major
+ * uint64 planes desc[word * width*height + p], words = (ww*wh - 1 + 63) / 64,
+ * e.g. CensusSgm.colorCensus descriptors (--census-mode color-distance,
+ * CensusSgm.java:109-121).  Everything else as sgm_match. */
+int sgm_match_desc(sgm_context *ctx, const sgm_params *params, const uint64_t *desc_a,
+                   const uint64_t *desc_b, const uint8_t *gray_a, const uint8_t *gray_b,
+                   const uint8_t *mask_a, const uint8_t *mask_b, int width, int height,
+                   int reverse, float *disp, sgm_stats *stats, uint8_t *reasons,
+                   uint8_t *boundary);
+
 /* CensusSgm.computeRecorded up to 02_left_right_validated (CensusSgm.java:45-202):
  * census + texture masks on device, both matching directions, device L-R check.
  * valid_left/valid_right, dl, dr, reasons, boundary may be NULL. *stats is

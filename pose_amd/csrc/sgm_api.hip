@@ -1408,13 +1408,17 @@ extern "C" int sgm_match_pair_device(sgm_context *ctx, const sgm_params *params,
   return SGM_OK;
 }
 
-extern "C" int sgm_match(sgm_context *ctx, const sgm_params *params, const uint8_t *gray_a,
-                         const uint8_t *gray_b, const uint8_t *mask_a, const uint8_t *mask_b, int w,
-                         int h, int reverse, float *disp, sgm_stats *stats, uint8_t *reasons,
-                         uint8_t *boundary) {
-  // GpuSgm.match (GpuSgm.java:267-303) for one matching direction with the
-  // caller's texture&valid masks.  Internally the reference image is slot 0
-  // and the target slot 1 with the direction's `reverse` flag.
+namespace {
+
+// GpuSgm.match (GpuSgm.java:267-303) for one matching direction with the
+// caller's texture&valid masks, census descriptors computed from the gray
+// rasters or (desc_a / desc_b non-null) the caller's word-major planes.
+// Internally the reference image is slot 0 and the target slot 1 with the
+// direction's `reverse` flag.
+int match_one(sgm_context *ctx, const sgm_params *params, const uint64_t *desc_a,
+              const uint64_t *desc_b, const uint8_t *gray_a, const uint8_t *gray_b,
+              const uint8_t *mask_a, const uint8_t *mask_b, int w, int h, int reverse, float *disp,
+              sgm_stats *stats, uint8_t *reasons, uint8_t *boundary) {
   if (!ctx || !gray_a || !gray_b || !mask_a || !mask_b || !disp)
     return fail(SGM_ERR_INVALID_ARGUMENT, "null argument");
   std::lock_guard<std::mutex> g(ctx->lock);
@@ -1455,6 +1459,16 @@ extern "C" int sgm_match(sgm_context *ctx, const sgm_params *params, const uint8
     J.h = h;
   }
   if ((rc = launch_census(ctx, cj, 2, w, h, params))) return rc;
+  if (desc_a) {
+    const int words = words_of(params);
+    if ((rc = copy_h2d(ctx, B.desc[ref], desc_a, n * 8 * words))) return rc;
+    if ((rc = copy_h2d(ctx, B.desc[tgt], desc_b, n * 8 * words))) return rc;
+    if (words == 1)
+      for (int d = 0; d < 2; d++) {
+        desc_flag_kernel<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(B.desc[d], B.mask[d], n);
+        if ((rc = launch_check("descriptor flag kernel"))) return rc;
+      }
+  }
   rc = run_directions(ctx, params, w, h, B, &ref, 1, ref, reasons ? B.reasons : nullptr,
                       boundary ? B.boundary : nullptr);
   if (rc == SGM_OK) {
@@ -1488,6 +1502,26 @@ extern "C" int sgm_match(sgm_context *ctx, const sgm_params *params, const uint8
   ctx->last_h = h;
   ctx->last_params = *params;
   return SGM_OK;
+}
+
+}  // namespace
+
+extern "C" int sgm_match(sgm_context *ctx, const sgm_params *params, const uint8_t *gray_a,
+                         const uint8_t *gray_b, const uint8_t *mask_a, const uint8_t *mask_b, int w,
+                         int h, int reverse, float *disp, sgm_stats *stats, uint8_t *reasons,
+                         uint8_t *boundary) {
+  return match_one(ctx, params, nullptr, nullptr, gray_a, gray_b, mask_a, mask_b, w, h, reverse,
+                   disp, stats, reasons, boundary);
+}
+
+extern "C" int sgm_match_desc(sgm_context *ctx, const sgm_params *params, const uint64_t *desc_a,
+                              const uint64_t *desc_b, const uint8_t *gray_a, const uint8_t *gray_b,
+                              const uint8_t *mask_a, const uint8_t *mask_b, int w, int h,
+                              int reverse, float *disp, sgm_stats *stats, uint8_t *reasons,
+                              uint8_t *boundary) {
+  if (!desc_a || !desc_b) return fail(SGM_ERR_INVALID_ARGUMENT, "null descriptors");
+  return match_one(ctx, params, desc_a, desc_b, gray_a, gray_b, mask_a, mask_b, w, h, reverse,
+                   disp, stats, reasons, boundary);
 }
 
 extern "C" int sgm_read_volumes(sgm_context *ctx, int direction, uint16_t *cost, uint32_t *aggregate) {

@@ -225,6 +225,15 @@ __global__ __launch_bounds__(256) void color_census_kernel(ColorJobs jobs, int w
   if (J.gm) J.gm[p] = (uint16_t)(J.gray[p] | (m << 8));
 }
 
+// Caller-supplied one-word descriptors (census bits <= 62): bit 63 carries the
+// texture&valid flag the sweeps test, as census_kernel writes it.
+static __global__ __launch_bounds__(256) void desc_flag_kernel(uint64_t *desc, const uint8_t *mask,
+                                                               size_t n) {
+  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  desc[p] = (desc[p] & 0x7FFFFFFFFFFFFFFFull) | ((uint64_t)(mask[p] != 0) << 63);
+}
+
 // ---------------------------------------------------------------------------
 // Cost volume (StereoCost.value, StereoCost.java:50-90; validity test of the
 // reference's `costs` kernel, GpuStereo.java:706-731).  One thread = 4

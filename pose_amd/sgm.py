@@ -107,7 +107,7 @@ class SgmRequestInfo(ctypes.Structure):
 
 EXPORTS = [
     "sgm_device_count", "sgm_device_info_get", "sgm_open", "sgm_close", "sgm_plan_get",
-    "sgm_disparity_limit", "sgm_match", "sgm_match_pair", "sgm_match_pair_device",
+    "sgm_disparity_limit", "sgm_match", "sgm_match_desc", "sgm_match_pair", "sgm_match_pair_device",
     "sgm_match_pair_filtered", "sgm_match_pair_color",
     "sgm_set_keep_volumes", "sgm_read_volumes", "sgm_set_cost_volume", "sgm_set_plan_mode",
     "sgm_synchronize",
@@ -150,6 +150,7 @@ def lib():
             L.sgm_plan_get.argtypes = [P, pp, I, I, ctypes.POINTER(SgmPlan)]
             L.sgm_disparity_limit.argtypes = [P, pp, I, I, ctypes.POINTER(I)]
             L.sgm_match.argtypes = [P, pp, P, P, P, P, I, I, I, P, ps, P, P]
+            L.sgm_match_desc.argtypes = [P, pp, P, P, P, P, P, P, I, I, I, P, ps, P, P]
             L.sgm_match_pair.argtypes = [P, pp, P, P, P, P, I, I, P, P, P, ps, P, P]
             L.sgm_match_pair_device.argtypes = [P, pp, P, P, P, P, I, I, P, P, P, P, P, P]
             L.sgm_match_pair_filtered.argtypes = [P, pp, ctypes.POINTER(SgmFilterParams), P, P, P,
@@ -469,6 +470,21 @@ class Context:
         return dict(disparity=disp, checked=disp, dl=dl, dr=dr, stats=st.as_dict(),
                     reasons=reasons, boundary=boundary)
 
+    def match_desc(self, params, desc_a, desc_b, gray_a, gray_b, mask_a, mask_b, w, h, reverse,
+                   stats=None, reasons=None, boundary=None):
+        """sgm_match with the caller's word-major uint64 descriptor planes."""
+        words = (params.window_w * params.window_h - 1 + 63) // 64
+        da = np.ascontiguousarray(desc_a, np.uint64).reshape(-1)
+        db = np.ascontiguousarray(desc_b, np.uint64).reshape(-1)
+        if da.size != words * w * h or db.size != da.size:
+            raise ValueError("descriptor planes must hold %d words per pixel" % words)
+        disp = np.empty(w * h, np.float32)
+        st = SgmStats() if stats is None else stats
+        check(lib().sgm_match_desc(self.h, ctypes.byref(params), _p(da), _p(db), _p(gray_a),
+                                   _p(gray_b), _p(mask_a), _p(mask_b), w, h, int(reverse),
+                                   _p(disp), ctypes.byref(st), _p(reasons), _p(boundary)))
+        return disp, st
+
     def match(self, params, gray_a, gray_b, mask_a, mask_b, w, h, reverse, stats=None,
               reasons=None, boundary=None):
         disp = np.empty(w * h, np.float32)
@@ -676,10 +692,14 @@ class GpuSgm:
             st = SgmStats()
             codes = None if reasons is None else np.array(reasons, np.uint8, copy=True)
             bnd = None if searchBoundary is None else np.array(searchBoundary, np.uint8, copy=True)
-            disp, st = ctx.match(params_from_options(o), np.ascontiguousarray(grayA, np.uint8),
-                                 np.ascontiguousarray(grayB, np.uint8),
-                                 np.ascontiguousarray(ta, np.uint8),
-                                 np.ascontiguousarray(tb, np.uint8), w, h, reverse, st, codes, bnd)
+            g = [np.ascontiguousarray(v, np.uint8) for v in (grayA, grayB, ta, tb)]
+            if o.censusMode == "color-distance":
+                # the caller's colour-distance descriptors (CensusSgm.java:109-121)
+                disp, st = ctx.match_desc(params_from_options(o), np.asarray(a, np.uint64),
+                                          np.asarray(b, np.uint64), *g, w, h, reverse, st, codes,
+                                          bnd)
+            else:
+                disp, st = ctx.match(params_from_options(o), *g, w, h, reverse, st, codes, bnd)
             stats.add_match(st.as_dict())
             if codes is not None:
                 reasons[:] = codes

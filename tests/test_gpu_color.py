@@ -87,3 +87,55 @@ def test_census_sgm_mirror_color_mode():
     with pytest.raises(ValueError):
         sgm.CensusSgm(o).compute(left, right, 160, 90)
     sgm.GpuSgm.closeSession()
+
+
+@pytest.mark.parametrize("window", [(5, 5), (9, 7), (11, 11)])
+def test_match_with_caller_descriptors(gpu_context, window):
+    """sgm_match_desc = GpuSgm.match with the caller's long[][] descriptors: the
+    colour-distance planes and textures computed on the host give the oracle's
+    single-direction result in both matching directions."""
+    ww, wh = window
+    w, h = 150, 70
+    lb, rb = color_pair(w, h, 12, seed=ww * wh)
+    a, b = bgr_gray(lb), bgr_gray(rb)
+    op = O.params(window_w=ww, window_h=wh, disparities=33, min_disparity=-2)
+    p = to_sgm(op)
+    da, ta = O.color_census(lb, w, h, ww, wh)
+    db, tb = O.color_census(rb, w, h, ww, wh)
+    for reverse in (0, 1):
+        A, B, TA, TB, DA, DB = (a, b, ta, tb, da, db) if not reverse else (b, a, tb, ta, db, da)
+        print("descriptors", window, "reverse", reverse, flush=True)
+        disp_o, st_o, r_o, b_o = O.match(op, DA, DB, A.reshape(-1), B.reshape(-1), TA, TB, w, h,
+                                         reverse)
+        reasons = np.ones(w * h, np.uint8)
+        boundary = np.zeros(w * h, np.uint8)
+        disp, st = gpu_context.match_desc(p, DA, DB, A, B, TA, TB, w, h, reverse, None, reasons,
+                                          boundary)
+        assert np.array_equal(bits(disp), bits(disp_o))
+        assert np.array_equal(reasons, r_o) and np.array_equal(boundary, b_o)
+        for k in ("lower_boundary_rejected", "full_interval_pixels", "boundary_rejected",
+                  "upper_boundary_rejected", "ambiguity_rejected", "selected"):
+            assert getattr(st, k) == st_o[k], k
+    # gray census planes through the descriptor entry equal the gray entry
+    dga, dgb = O.census(a, w, h, ww, wh), O.census(b, w, h, ww, wh)
+    tga, tgb = O.texture(a, w, h, ww, wh), O.texture(b, w, h, ww, wh)
+    d1, _ = gpu_context.match_desc(p, dga, dgb, a, b, tga, tgb, w, h, 0)
+    d2, _ = gpu_context.match(p, a, b, tga, tgb, w, h, 0)
+    assert np.array_equal(bits(d1), bits(d2))
+
+
+def test_gpusgm_match_color_mode_uses_caller_descriptors():
+    w, h = 160, 80
+    lb, rb = color_pair(w, h, 9, seed=77)
+    a, b = bgr_gray(lb), bgr_gray(rb)
+    o = sgm.Options(census=5, disparities=24, compute="gpu", censusMode="color-distance")
+    da, ta = O.color_census(lb, w, h, 5, 5)
+    db, tb = O.color_census(rb, w, h, 5, 5)
+    st = sgm.Stats()
+    disp = sgm.GpuSgm.match(o, da, db, a, b, ta, tb, w, h, False, st, None, None)
+    op = O.params(window_w=5, window_h=5, disparities=24, p1=sgm.StereoCost.p1(o),
+                  p2=sgm.StereoCost.p2(o))
+    want, st_o, _, _ = O.match(op, da, db, a.reshape(-1), b.reshape(-1), ta, tb, w, h, 0)
+    assert np.array_equal(bits(disp), bits(want))
+    assert st.selected == st_o["selected"]
+    sgm.GpuSgm.closeSession()
