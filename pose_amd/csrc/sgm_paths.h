@@ -807,7 +807,11 @@ __device__ __forceinline__ void census_costs(const uint64_t *tw, uint32_t alo, u
   for (int i = 0; i < R / 2; i++) c[i] = (uint32_t)cv[2 * i] | ((uint32_t)cv[2 * i + 1] << 16);
 }
 
-template <int R, int PHASE, int CM, int PFCAP = 64>
+// SLIDE_UP: direction of the sliding target window of a horizontal computed-cost
+// line, fixed at compile time (1: towards higher x indices of the window, 0:
+// lower) so the window rotates by register renaming in the unrolled steps; -1
+// where the line has no sliding window.
+template <int R, int PHASE, int CM, int PFCAP = 64, int SLIDE_UP = -1>
 struct PkSweep {
   static constexpr int NP = R / 2;
   static constexpr int PF = pk_prefetch_depth<R>() < PFCAP ? pk_prefetch_depth<R>() : PFCAP;
@@ -957,7 +961,9 @@ struct PkSweep {
   }
 
   __device__ __forceinline__ void slide_window(uint32_t elo, uint32_t ehi) {
-    const bool up = CM == CM_FUSED ? !J.reverse : ((A.dx > 0) != (J.reverse != 0));
+    const bool up = SLIDE_UP >= 0 ? SLIDE_UP == 1
+                    : CM == CM_FUSED ? !J.reverse
+                                     : ((A.dx > 0) != (J.reverse != 0));
     if (!up) {
       const uint32_t ilo = (uint32_t)dpp_shl1((int)(uint32_t)win[0], (int)elo);
       const uint32_t ihi = (uint32_t)dpp_shl1((int)(uint32_t)(win[0] >> 32), (int)ehi);
@@ -1156,6 +1162,17 @@ template <int R, int PHASE, int CM, int PFCAP = 64>
 __device__ __forceinline__ void sweep_line_pk(const PathJob &J, const PathArgs &A, int line,
                                               const int *p2_table) {
   static_assert(R % 2 == 0, "packed sweep needs an even lane group");
+  // the fused first sweep runs a copy of its loop per slide direction
+  if constexpr (CM == CM_FUSED) {
+    if (!J.reverse) {
+      PkSweep<R, PHASE, CM, PFCAP, 1> sw(J, A, line, p2_table);
+      sw.run();
+    } else {
+      PkSweep<R, PHASE, CM, PFCAP, 0> sw(J, A, line, p2_table);
+      sw.run();
+    }
+    return;
+  }
   PkSweep<R, PHASE, CM, PFCAP> sw(J, A, line, p2_table);
   sw.run();
 }
