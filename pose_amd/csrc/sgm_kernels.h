@@ -50,6 +50,26 @@ __device__ __forceinline__ int wave_min(int v) {
   return __builtin_amdgcn_readlane(v, 63);
 }
 
+// Wave minima of a (signed) and b (unsigned) at once: the two reductions are
+// interleaved step by step, so each DPP operation has an independent one
+// between it and the operation it reads.  Both results are uniform.
+__device__ __forceinline__ void wave_min_and_min_u(int &a, unsigned &b) {
+  int v = a, u = (int)b, t;
+#define SGM_MIN2_STEP(ctrl, rmask)                                                \
+  v = min(v, __builtin_amdgcn_update_dpp(0x7FFFFFFF, v, ctrl, rmask, 0xF, false)); \
+  t = __builtin_amdgcn_update_dpp(-1, u, ctrl, rmask, 0xF, false);                  \
+  u = (int)min((unsigned)u, (unsigned)t);
+  SGM_MIN2_STEP(0xB1, 0xF)
+  SGM_MIN2_STEP(0x4E, 0xF)
+  SGM_MIN2_STEP(0x141, 0xF)
+  SGM_MIN2_STEP(0x140, 0xF)
+  SGM_MIN2_STEP(0x142, 0xA)
+  SGM_MIN2_STEP(0x143, 0xC)
+#undef SGM_MIN2_STEP
+  a = __builtin_amdgcn_readlane(v, 63);
+  b = (unsigned)__builtin_amdgcn_readlane(u, 63);
+}
+
 __device__ __forceinline__ unsigned wave_min_u(unsigned u) {
   int v = (int)u;
   // unsigned min via the int builtins: compare as unsigned

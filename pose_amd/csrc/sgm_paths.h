@@ -1031,16 +1031,26 @@ struct PkSweep {
       L[i] = (v & keep[i]) | infm[i];
       lm = pk_min(lm, L[i]);
     }
-    const int lmin = (int)min(lm & 0xFFFFu, lm >> 16);
+    int lmin = (int)min(lm & 0xFFFFu, lm >> 16);
     uint32_t S[NP];
 #pragma unroll
     for (int i = 0; i < NP; i++) S[i] = PHASE == PHASE_FIRST ? L[i] : pk_add(sb[slot][i], L[i]);
     bstore_pairs<NP>(S, make_rsrc(st_s, (uint32_t)sum_store_bytes), loff);
     st_s += sstep;
-    minp = wave_min(lmin);
+    if constexpr (PHASE == PHASE_LAST) {
+      // the winner key reduces beside the path minimum, interleaved
+      uint32_t Sm[NP];
+      unsigned key = masked_key(S, c, Sm);
+      wave_min_and_min_u(lmin, key);
+      minp = lmin;
 #pragma unroll
-    for (int i = 0; i < NP; i++) prev[i] = L[i];
-    if constexpr (PHASE == PHASE_LAST) select(S, c, s);
+      for (int i = 0; i < NP; i++) prev[i] = L[i];
+      select(Sm, key, s);
+    } else {
+      minp = wave_min(lmin);
+#pragma unroll
+      for (int i = 0; i < NP; i++) prev[i] = L[i];
+    }
   }
 
   // Pair i (uniform) of the lane group, selected without dynamic register indexing.
@@ -1051,9 +1061,26 @@ struct PkSweep {
     return r;
   }
 
+  // Pair i (uniform) of lane `src` (uniform): every pair register is read from
+  // that lane and the scalar picks one (an indexed pick on the vector side is
+  // turned into an LDS round trip by the compiler).
+  __device__ __forceinline__ static uint32_t lane_pair(const uint32_t *v, int i, int src) {
+    if constexpr (NP <= 4) {
+      uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)v[0], src);
+#pragma unroll
+      for (int j = 1; j < NP; j++) {
+        const uint32_t t = (uint32_t)__builtin_amdgcn_readlane((int)v[j], src);
+        r = i == j ? t : r;
+      }
+      return r;
+    } else {
+      return (uint32_t)__builtin_amdgcn_readlane((int)pick_pair(v, i), src);
+    }
+  }
+
   // Masked aggregate of disparity k (uniform, 0 <= k < D): 0xFFFF when invalid.
   __device__ __forceinline__ static uint32_t cell(const uint32_t *Sm, int k) {
-    const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)pick_pair(Sm, (k % R) >> 1), k / R);
+    const uint32_t w = lane_pair(Sm, (k % R) >> 1, k / R);
     return (k & 1) ? (w >> 16) : (w & 0xFFFFu);
   }
 
@@ -1062,8 +1089,10 @@ struct PkSweep {
   // the winner and the validity of both range ends; CensusSgm.java:460-515),
   // on packed pairs: invalid cells (cost == invalid code, or k >= D) are masked
   // to 0xFFFF, above every aggregate.
-  __device__ __forceinline__ void select(const uint32_t *S, const uint32_t *c, int s) {
-    uint32_t Sm[NP];
+  // Lane-local part of the winner search: masked aggregates and the smallest
+  // (aggregate << 12 | k) key of the lane.
+  __device__ __forceinline__ unsigned masked_key(const uint32_t *S, const uint32_t *c,
+                                                 uint32_t *Sm) const {
     unsigned key = 0xFFFFFFFFu;
 #pragma unroll
     for (int i = 0; i < NP; i++) {
@@ -1073,7 +1102,11 @@ struct PkSweep {
       const unsigned k1 = ((Sm[i] >> 16) << 12) | (kpk[i] >> 16);
       key = min(key, min(k0, k1));
     }
-    key = wave_min_u(key);
+    return key;
+  }
+
+  // Sm, key: masked aggregates and the wave's smallest key (masked_key reduced).
+  __device__ __forceinline__ void select(const uint32_t *Sm, unsigned key, int s) {
     const bool any = key < (0xFFFFu << 12);
     const int best = (int)(key & 0xFFFu);
     // second best: cells with |k - best| <= 1 masked to 0xFFFF
@@ -1092,8 +1125,7 @@ struct PkSweep {
       const uint32_t before = best > 0 ? cell(Sm, best - 1) : 0xFFFFu;
       const uint32_t after = best + 1 < J.D ? cell(Sm, best + 1) : 0xFFFFu;
       const uint32_t v0 = (uint32_t)__builtin_amdgcn_readlane((int)Sm[0], 0) & 0xFFFFu;
-      const uint32_t wl =
-          (uint32_t)__builtin_amdgcn_readlane((int)pick_pair(Sm, last_pair), last_lane);
+      const uint32_t wl = lane_pair(Sm, last_pair, last_lane);
       const uint32_t vl = last_half ? (wl >> 16) : (wl & 0xFFFFu);
       const uint32_t flags = (v0 != 0xFFFFu ? 1u : 0u) | (vl != 0xFFFFu ? 2u : 0u);
       f0 = (uint32_t)(best + 1) | (flags << 13);
