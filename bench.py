@@ -368,6 +368,7 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the served-vs-plain leg")
     ap.add_argument("--no-zncc", action="store_true", help="skip the FP32 ZNCC leg")
     ap.add_argument("--no-streamed", action="store_true", help="skip the strip-streamed leg")
+    ap.add_argument("--no-filtered", action="store_true", help="skip the post-L-R filter leg")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals, pairs/s")
     ap.add_argument("--serve-requests", type=int, default=60)
     args = ap.parse_args()
@@ -459,7 +460,7 @@ def main():
                              census_weight=500, ad_truncation=30, gradient_truncation=60,
                              reserved=0, uniqueness=0.03, lr_threshold=1.0)
     zncc = None if args.no_zncc else run_zncc(sgm)
-    filtered = run_filtered(sgm, left, right, W, H)
+    filtered = None if args.no_filtered else run_filtered(sgm, left, right, W, H)
     batched = None if args.no_batched else run_batched(sgm, params_for)
     served = None if args.no_served else run_served(sgm, params_for, rate=args.serve_rate,
                                                     count=args.serve_requests)
