@@ -567,8 +567,10 @@ static int launch_path_any(sgm_context *ctx, int R, int sum_bytes, int mode, con
 #define CASE_PK(RR)                                                         \
   case RR:                                                                  \
     if (mode == 2) launch_path_t<RR, uint16_t, 2>(ctx, jobs, A, phase, waves); \
+    else if (mode == (2 | MODE_NT)) launch_path_t<RR, uint16_t, 2 | MODE_NT>(ctx, jobs, A, phase, waves); \
     else if (mode == 3) launch_path_t<RR, uint16_t, 3>(ctx, jobs, A, phase, waves); \
     else if (mode == 4) launch_path_t<RR, uint16_t, 4>(ctx, jobs, A, phase, waves); \
+    else if (mode == (4 | MODE_NT)) launch_path_t<RR, uint16_t, 4 | MODE_NT>(ctx, jobs, A, phase, waves); \
     else launch_path_t<RR, uint16_t, 1>(ctx, jobs, A, phase, waves);        \
     break;
   if (sum_bytes == 4) {
@@ -581,7 +583,7 @@ static int launch_path_any(sgm_context *ctx, int R, int sum_bytes, int mode, con
       CASE_PK(2) CASE_PK(4) CASE_PK(8)
 #define CASE_PK_LOAD(RR)                                                    \
   case RR:                                                                  \
-    if (mode == 4) launch_path_t<RR, uint16_t, 4>(ctx, jobs, A, phase, waves); \
+    if ((mode & 7) == 4) launch_path_t<RR, uint16_t, 4>(ctx, jobs, A, phase, waves); \
     else launch_path_t<RR, uint16_t, 1>(ctx, jobs, A, phase, waves);        \
     break;
       CASE_PK_LOAD(16) CASE_PK_LOAD(32) CASE_PK_LOAD(64)
@@ -948,7 +950,12 @@ int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuff
   const int sb = aggregate_bytes(p);
   const bool recompute = recompute_cost(ctx, p, w, h);
   const bool fused = !recompute && fused_cost(p);
-  const int mode = recompute ? 3 : sb == 2 ? (cost_u8(p) ? 4 : 1) : 0;
+  int mode = recompute ? 3 : sb == 2 ? (cost_u8(p) ? 4 : 1) : 0;
+  // stream the uint8 cost and uint16 aggregate volumes past the caches when
+  // their rows are whole 64-byte lines and they far exceed the 256 MB MALL
+  // (measured: +5% at 1600x1067 D=256, -28% at 2000x1500 D=100)
+  const bool nt = mode == 4 && Dp % 64 == 0 && (int64_t)nd * w * h * Dp * 3 > (int64_t)768 << 20;
+  if (nt) mode |= MODE_NT;
   if (!fused && !recompute) {
     CostJobs cj;
     memset(&cj, 0, sizeof cj);
@@ -1010,7 +1017,7 @@ int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuff
     PathArgs A = path_args(p, dir, nd, 0, ctx->keep_volumes ? 1 : 0);
     A.total_lines = nd * lines_of(A.cls, w, h);
     A.split = lines_of(A.cls, w, h);
-    const int m = (dir == 0 && fused) ? 2 : mode;
+    const int m = (dir == 0 && fused) ? (2 | (mode & MODE_NT)) : mode;
     if ((rc = launch_path(ctx, R, sb, m, pj, A, phase, waves))) return rc;
   }
   return SGM_OK;

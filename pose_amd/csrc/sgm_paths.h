@@ -585,23 +585,25 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, ui
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
 }
 
-// Cache policy of the cost / aggregate volume accesses: each cell is read and
-// written once per sweep and the volumes are far larger than the caches, so
-// they stream (nt: the SLC bit position of the buffer instructions' aux field).
-constexpr int kVolumePolicy = 2;
+// Cache policy of the cost / aggregate volume accesses (the aux field of the
+// buffer instructions): POL_NT (nt, the SLC bit position) streams the volumes
+// past the caches, POL_DEFAULT keeps the normal policy.  Streaming pays when a
+// volume is far larger than the MALL and its rows are whole cache lines; short
+// or unaligned rows and MALL-sized volumes run faster with the default policy.
+enum { POL_DEFAULT = 0, POL_NT = 2 };
 
-template <int NP>
+template <int NP, int POL = POL_DEFAULT>
 __device__ __forceinline__ void bload_pairs(uint32_t *dst, __amdgpu_buffer_rsrc_t r, int off) {
   if constexpr (NP == 1) {
-    dst[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, kVolumePolicy);
+    dst[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, POL);
   } else if constexpr (NP == 2) {
-    const v2u32 t = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, kVolumePolicy);
+    const v2u32 t = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, POL);
     dst[0] = t.x;
     dst[1] = t.y;
   } else {
 #pragma unroll
     for (int i = 0; i < NP / 4; i++) {
-      const v4u32 t = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16 * i, 0, kVolumePolicy);
+      const v4u32 t = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16 * i, 0, POL);
       dst[4 * i] = t.x;
       dst[4 * i + 1] = t.y;
       dst[4 * i + 2] = t.z;
@@ -610,15 +612,15 @@ __device__ __forceinline__ void bload_pairs(uint32_t *dst, __amdgpu_buffer_rsrc_
   }
 }
 
-template <int NP>
+template <int NP, int POL = POL_DEFAULT>
 __device__ __forceinline__ void bstore_pairs(const uint32_t *src, __amdgpu_buffer_rsrc_t r, int off) {
   if constexpr (NP == 1) {
-    __builtin_amdgcn_raw_buffer_store_b32(src[0], r, off, 0, kVolumePolicy);
+    __builtin_amdgcn_raw_buffer_store_b32(src[0], r, off, 0, POL);
   } else if constexpr (NP == 2) {
     v2u32 t;
     t.x = src[0];
     t.y = src[1];
-    __builtin_amdgcn_raw_buffer_store_b64(t, r, off, 0, kVolumePolicy);
+    __builtin_amdgcn_raw_buffer_store_b64(t, r, off, 0, POL);
   } else {
 #pragma unroll
     for (int i = 0; i < NP / 4; i++) {
@@ -627,7 +629,7 @@ __device__ __forceinline__ void bstore_pairs(const uint32_t *src, __amdgpu_buffe
       t.y = src[4 * i + 1];
       t.z = src[4 * i + 2];
       t.w = src[4 * i + 3];
-      __builtin_amdgcn_raw_buffer_store_b128(t, r, off + 16 * i, 0, kVolumePolicy);
+      __builtin_amdgcn_raw_buffer_store_b128(t, r, off + 16 * i, 0, POL);
     }
   }
 }
@@ -668,25 +670,25 @@ __device__ __forceinline__ uint64_t fetch_target(const PathJob &J, __amdgpu_buff
 enum { CM_LOAD = 0, CM_FUSED = 1, CM_RECOMPUTE = 2, CM_LOAD8 = 3 };
 
 // R bytes per lane through a buffer resource (R = 2 .. 64, byte offset `off`).
-template <int R>
+template <int R, int POL = POL_DEFAULT>
 __device__ __forceinline__ void bload_bytes(uint32_t *dst, __amdgpu_buffer_rsrc_t r, int off) {
   if constexpr (R == 2) {
-    dst[0] = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, kVolumePolicy);
+    dst[0] = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, POL);
   } else if constexpr (R == 4) {
-    dst[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, kVolumePolicy);
+    dst[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, POL);
   } else {
-    bload_pairs<R / 4>(dst, r, off);
+    bload_pairs<R / 4, POL>(dst, r, off);
   }
 }
 
-template <int R>
+template <int R, int POL = POL_DEFAULT>
 __device__ __forceinline__ void bstore_bytes(const uint32_t *src, __amdgpu_buffer_rsrc_t r, int off) {
   if constexpr (R == 2) {
-    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)src[0], r, off, 0, kVolumePolicy);
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)src[0], r, off, 0, POL);
   } else if constexpr (R == 4) {
-    __builtin_amdgcn_raw_buffer_store_b32(src[0], r, off, 0, kVolumePolicy);
+    __builtin_amdgcn_raw_buffer_store_b32(src[0], r, off, 0, POL);
   } else {
-    bstore_pairs<R / 4>(src, r, off);
+    bstore_pairs<R / 4, POL>(src, r, off);
   }
 }
 
@@ -811,9 +813,10 @@ __device__ __forceinline__ void census_costs(const uint64_t *tw, uint32_t alo, u
 // line, fixed at compile time (1: towards higher x indices of the window, 0:
 // lower) so the window rotates by register renaming in the unrolled steps; -1
 // where the line has no sliding window.
-template <int R, int PHASE, int CM, int PFCAP = 64, int SLIDE_UP = -1>
+template <int R, int PHASE, int CM, int PFCAP = 64, int SLIDE_UP = -1, bool NT = false>
 struct PkSweep {
   static constexpr int NP = R / 2;
+  static constexpr int POL = NT ? POL_NT : POL_DEFAULT;
   static constexpr int PF = pk_prefetch_depth<R>() < PFCAP ? pk_prefetch_depth<R>() : PFCAP;
   static constexpr bool kComputed = CM == CM_FUSED || CM == CM_RECOMPUTE;
   static constexpr bool kLoaded = CM == CM_LOAD || CM == CM_LOAD8;
@@ -929,13 +932,13 @@ struct PkSweep {
   __device__ __forceinline__ void load_step(int slot, int s) {
     const bool in = s < len;
     if constexpr (CM == CM_LOAD)
-      bload_pairs<NP>(cb[slot], make_rsrc(ld_c, in ? row_bytes : 0u), loff);
+      bload_pairs<NP, POL>(cb[slot], make_rsrc(ld_c, in ? row_bytes : 0u), loff);
     if constexpr (CM == CM_LOAD8)
-      bload_bytes<R>(cb[slot], make_rsrc(ld_c, in ? cost_row_bytes : 0u), boff);
+      bload_bytes<R, POL>(cb[slot], make_rsrc(ld_c, in ? cost_row_bytes : 0u), boff);
     if constexpr (CM == CM_RECOMPUTE)
       if (!slide) load_targets(dsc[slot], s);
     if constexpr (PHASE != PHASE_FIRST)
-      bload_pairs<NP>(sb[slot], make_rsrc(ld_s, in ? row_bytes : 0u), loff);
+      bload_pairs<NP, POL>(sb[slot], make_rsrc(ld_s, in ? row_bytes : 0u), loff);
     ld_c += cstep;
     ld_s += sstep;
   }
@@ -1000,7 +1003,7 @@ struct PkSweep {
       if constexpr (CM == CM_FUSED) {
         uint32_t raw[(R + 3) / 4];
         narrow_costs<NP>(c, raw);
-        bstore_bytes<R>(raw, make_rsrc(st_c, cost_row_bytes), boff);
+        bstore_bytes<R, POL>(raw, make_rsrc(st_c, cost_row_bytes), boff);
         st_c += cstep;
       }
     } else if constexpr (CM == CM_LOAD8) {
@@ -1035,7 +1038,7 @@ struct PkSweep {
     uint32_t S[NP];
 #pragma unroll
     for (int i = 0; i < NP; i++) S[i] = PHASE == PHASE_FIRST ? L[i] : pk_add(sb[slot][i], L[i]);
-    bstore_pairs<NP>(S, make_rsrc(st_s, (uint32_t)sum_store_bytes), loff);
+    bstore_pairs<NP, POL>(S, make_rsrc(st_s, (uint32_t)sum_store_bytes), loff);
     st_s += sstep;
     if constexpr (PHASE == PHASE_LAST) {
       // the winner key reduces beside the path minimum, interleaved
@@ -1190,22 +1193,22 @@ struct PkSweep {
   }
 };
 
-template <int R, int PHASE, int CM, int PFCAP = 64>
+template <int R, int PHASE, int CM, int PFCAP = 64, bool NT = false>
 __device__ __forceinline__ void sweep_line_pk(const PathJob &J, const PathArgs &A, int line,
                                               const int *p2_table) {
   static_assert(R % 2 == 0, "packed sweep needs an even lane group");
   // the fused first sweep runs a copy of its loop per slide direction
   if constexpr (CM == CM_FUSED) {
     if (!J.reverse) {
-      PkSweep<R, PHASE, CM, PFCAP, 1> sw(J, A, line, p2_table);
+      PkSweep<R, PHASE, CM, PFCAP, 1, NT> sw(J, A, line, p2_table);
       sw.run();
     } else {
-      PkSweep<R, PHASE, CM, PFCAP, 0> sw(J, A, line, p2_table);
+      PkSweep<R, PHASE, CM, PFCAP, 0, NT> sw(J, A, line, p2_table);
       sw.run();
     }
     return;
   }
-  PkSweep<R, PHASE, CM, PFCAP> sw(J, A, line, p2_table);
+  PkSweep<R, PHASE, CM, PFCAP, -1, NT> sw(J, A, line, p2_table);
   sw.run();
 }
 
@@ -1242,15 +1245,20 @@ struct PathJobs {
 // MODE: 0 generic int sweep; 1 packed u16 reading the uint16 cost volume; 2
 // packed u16 first sweep with the census cost fused in (writes the uint8 cost
 // volume); 3 packed u16 with the census cost recomputed in the sweep (no cost
-// volume); 4 packed u16 reading the uint8 cost volume.
+// volume); 4 packed u16 reading the uint8 cost volume.  MODE_NT (added to 2 or
+// 4): the volume accesses stream (nontemporal policy).
+constexpr int MODE_NT = 8;
+
 template <int R, typename SumT, int PHASE, int MODE>
 __device__ __forceinline__ void sweep_dispatch(const PathJob &J, const PathArgs &A, int line,
                                                const int *p2_table) {
-  if constexpr (MODE == 0) sweep_line<R, SumT, PHASE>(J, A, line, p2_table);
-  else if constexpr (MODE == 1) sweep_line_pk<R, PHASE, CM_LOAD>(J, A, line, p2_table);
-  else if constexpr (MODE == 2) sweep_line_pk<R, PHASE, CM_FUSED>(J, A, line, p2_table);
-  else if constexpr (MODE == 3) sweep_line_pk<R, PHASE, CM_RECOMPUTE>(J, A, line, p2_table);
-  else sweep_line_pk<R, PHASE, CM_LOAD8>(J, A, line, p2_table);
+  constexpr int M = MODE & 7;
+  constexpr bool NT = (MODE & MODE_NT) != 0;
+  if constexpr (M == 0) sweep_line<R, SumT, PHASE>(J, A, line, p2_table);
+  else if constexpr (M == 1) sweep_line_pk<R, PHASE, CM_LOAD>(J, A, line, p2_table);
+  else if constexpr (M == 2) sweep_line_pk<R, PHASE, CM_FUSED, 64, NT>(J, A, line, p2_table);
+  else if constexpr (M == 3) sweep_line_pk<R, PHASE, CM_RECOMPUTE>(J, A, line, p2_table);
+  else sweep_line_pk<R, PHASE, CM_LOAD8, 64, NT>(J, A, line, p2_table);
 }
 
 template <int R, typename SumT, int PHASE, int MODE>
