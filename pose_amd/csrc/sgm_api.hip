@@ -571,6 +571,7 @@ static int launch_path_any(sgm_context *ctx, int R, int sum_bytes, int mode, con
     else if (mode == 3) launch_path_t<RR, uint16_t, 3>(ctx, jobs, A, phase, waves); \
     else if (mode == 4) launch_path_t<RR, uint16_t, 4>(ctx, jobs, A, phase, waves); \
     else if (mode == (4 | MODE_NT)) launch_path_t<RR, uint16_t, 4 | MODE_NT>(ctx, jobs, A, phase, waves); \
+    else if (mode == (1 | MODE_NT)) launch_path_t<RR, uint16_t, 1 | MODE_NT>(ctx, jobs, A, phase, waves); \
     else launch_path_t<RR, uint16_t, 1>(ctx, jobs, A, phase, waves);        \
     break;
   if (sum_bytes == 4) {
@@ -954,7 +955,9 @@ int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuff
   // stream the uint8 cost and uint16 aggregate volumes past the caches when
   // their rows are whole 64-byte lines and they far exceed the 256 MB MALL
   // (measured: +5% at 1600x1067 D=256, -28% at 2000x1500 D=100)
-  const bool nt = mode == 4 && Dp % 64 == 0 && (int64_t)nd * w * h * Dp * 3 > (int64_t)768 << 20;
+  const int vol_bytes = mode == 4 ? 3 : 4;  // cost + aggregate bytes per cell
+  const bool nt = (mode == 4 || mode == 1) && Dp % 64 == 0 && R <= 8 &&
+                  (int64_t)nd * w * h * Dp * vol_bytes > (int64_t)768 << 20;
   if (nt) mode |= MODE_NT;
   if (!fused && !recompute) {
     CostJobs cj;

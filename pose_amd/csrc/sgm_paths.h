@@ -1245,8 +1245,8 @@ struct PathJobs {
 // MODE: 0 generic int sweep; 1 packed u16 reading the uint16 cost volume; 2
 // packed u16 first sweep with the census cost fused in (writes the uint8 cost
 // volume); 3 packed u16 with the census cost recomputed in the sweep (no cost
-// volume); 4 packed u16 reading the uint8 cost volume.  MODE_NT (added to 2 or
-// 4): the volume accesses stream (nontemporal policy).
+// volume); 4 packed u16 reading the uint8 cost volume.  MODE_NT (added to 1, 2
+// or 4): the volume accesses stream (nontemporal policy).
 constexpr int MODE_NT = 8;
 
 template <int R, typename SumT, int PHASE, int MODE>
@@ -1255,7 +1255,7 @@ __device__ __forceinline__ void sweep_dispatch(const PathJob &J, const PathArgs 
   constexpr int M = MODE & 7;
   constexpr bool NT = (MODE & MODE_NT) != 0;
   if constexpr (M == 0) sweep_line<R, SumT, PHASE>(J, A, line, p2_table);
-  else if constexpr (M == 1) sweep_line_pk<R, PHASE, CM_LOAD>(J, A, line, p2_table);
+  else if constexpr (M == 1) sweep_line_pk<R, PHASE, CM_LOAD, 64, NT>(J, A, line, p2_table);
   else if constexpr (M == 2) sweep_line_pk<R, PHASE, CM_FUSED, 64, NT>(J, A, line, p2_table);
   else if constexpr (M == 3) sweep_line_pk<R, PHASE, CM_RECOMPUTE>(J, A, line, p2_table);
   else sweep_line_pk<R, PHASE, CM_LOAD8, 64, NT>(J, A, line, p2_table);
