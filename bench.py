@@ -143,7 +143,7 @@ def run_filtered(sgm, left, right, W, H, steps=3):
     p = workload_params(sgm)
     f = sgm.SgmFilterParams(median=3, median_classic=0, median_edge_threshold=24,
                             correlation_window=7, speckle_size=50, speckle_connectivity=8,
-                            reserved0=0, reserved1=0, median_max_change=0.5, correlation=0.55,
+                            subpixel_refinement=0, reserved1=0, median_max_change=0.5, correlation=0.55,
                             speckle_range=1.0, reserved2=0.0)
     ctx = sgm.Context(0)
     r = ctx.match_pair_filtered(p, f, left, right)

@@ -169,7 +169,8 @@ typedef struct sgm_filter_params {
   int32_t correlation_window;
   int32_t speckle_size;
   int32_t speckle_connectivity; /* 4 or 8 */
-  int32_t reserved0, reserved1;
+  int32_t subpixel_refinement;  /* 1: PhotometricRefinement before the L-R check */
+  int32_t reserved1;
   double median_max_change;
   double correlation;
   double speckle_range;

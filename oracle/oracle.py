@@ -33,7 +33,7 @@ class OracleFilter(ctypes.Structure):
         ("median", ctypes.c_int32), ("median_classic", ctypes.c_int32),
         ("median_edge_threshold", ctypes.c_int32), ("correlation_window", ctypes.c_int32),
         ("speckle_size", ctypes.c_int32), ("speckle_connectivity", ctypes.c_int32),
-        ("reserved0", ctypes.c_int32), ("reserved1", ctypes.c_int32),
+        ("subpixel_refinement", ctypes.c_int32), ("reserved1", ctypes.c_int32),
         ("median_max_change", ctypes.c_double), ("correlation", ctypes.c_double),
         ("speckle_range", ctypes.c_double), ("reserved2", ctypes.c_double),
     ]
@@ -44,7 +44,7 @@ def filters(**kw):
     (Options.java:19-23,67,76): edge-preserving 3x3 median, max change 0.5, edge
     threshold 24, 7x7 photometric ZNCC >= 0.55, speckles < 50 px, range 1, 8-connected."""
     d = dict(median=3, median_classic=0, median_edge_threshold=24, correlation_window=7,
-             speckle_size=50, speckle_connectivity=8, reserved0=0, reserved1=0,
+             speckle_size=50, speckle_connectivity=8, subpixel_refinement=0, reserved1=0,
              median_max_change=0.5, correlation=0.55, speckle_range=1.0, reserved2=0.0)
     d.update(kw)
     return OracleFilter(**d)
@@ -99,6 +99,9 @@ def lib():
         L.oracle_compute_color.argtypes = [ctypes.POINTER(OracleParams), P, P, P, P, P, P,
                                            ctypes.c_int, ctypes.c_int, P, P, P,
                                            ctypes.POINTER(OracleStats), P, P]
+        L.oracle_compute_ex.argtypes = [ctypes.POINTER(OracleParams), ctypes.POINTER(OracleFilter),
+                                        P, P, P, P, P, P, ctypes.c_int, ctypes.c_int, P, P, P,
+                                        ctypes.POINTER(OracleStats), P, P]
         L.oracle_postfilter_ch.argtypes = [ctypes.POINTER(OracleParams),
                                            ctypes.POINTER(OracleFilter), P, P, P, ctypes.c_int,
                                            P, ctypes.c_int, ctypes.c_int, P,
@@ -214,6 +217,26 @@ def compute_color(p, left, right, left_bgr, right_bgr, w, h, valid_left=None, va
                 boundary=boundary)
 
 
+def compute_ex(p, f, left, right, w, h, valid_left=None, valid_right=None, left_bgr=None,
+               right_bgr=None):
+    """compute() / compute_color() with the filter options' subpixel refinement
+    (PhotometricRefinement before the L-R check when f.subpixel_refinement)."""
+    n = w * h
+    arrs = [None if a is None else np.ascontiguousarray(a, np.uint8).reshape(-1)
+            for a in (left, right, left_bgr, right_bgr, valid_left, valid_right)]
+    checked, dl, dr = (np.empty(n, np.float32) for _ in range(3))
+    reasons = np.empty(n, np.uint8)
+    boundary = np.empty(n, np.uint8)
+    st = OracleStats()
+    rc = lib().oracle_compute_ex(ctypes.byref(p), ctypes.byref(f), *(_ptr(a) for a in arrs), w, h,
+                                 _ptr(checked), _ptr(dl), _ptr(dr), ctypes.byref(st),
+                                 _ptr(reasons), _ptr(boundary))
+    if rc != 0:
+        raise ValueError("oracle_compute_ex failed: %d" % rc)
+    return dict(checked=checked, dl=dl, dr=dr, stats=st.as_dict(), reasons=reasons,
+                boundary=boundary)
+
+
 def postfilter(p, f, left, right, dr, checked, w, h, stats=None, reasons=None, photo=None):
     """Median, reciprocity + photometric ZNCC and speckles on a copy of `checked`.
     ``photo`` = (left_bgr, right_bgr) scores the photometric test over three
@@ -256,12 +279,8 @@ def compute_filtered(p, f, left, right, w, h, valid_left=None, valid_right=None,
                      right_bgr=None):
     """CensusSgm.computeRecorded through 05_final_stereo (colour-distance census
     and three-channel photometric test when the BGR rasters are given)."""
-    if left_bgr is None:
-        r = compute(p, left, right, w, h, valid_left, valid_right)
-        photo = None
-    else:
-        r = compute_color(p, left, right, left_bgr, right_bgr, w, h, valid_left, valid_right)
-        photo = (left_bgr, right_bgr)
+    r = compute_ex(p, f, left, right, w, h, valid_left, valid_right, left_bgr, right_bgr)
+    photo = None if left_bgr is None else (left_bgr, right_bgr)
     out, st, rs = postfilter(p, f, left, right, r["dr"], r["checked"], w, h, r["stats"],
                              r["reasons"], photo=photo)
     return dict(r, disparity=out, stats=st, reasons=rs)

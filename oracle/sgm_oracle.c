@@ -348,8 +348,11 @@ int oracle_match(const oracle_params *o, const uint64_t *a, const uint64_t *b,
 
 static int isfinite_f(float v) { return isfinite(v); }
 
-static int compute_pair(const oracle_params *o, const uint8_t *left, const uint8_t *right,
-                        const uint8_t *left_bgr, const uint8_t *right_bgr,
+static int64_t refine(float *disparity, const uint8_t *a, const uint8_t *b, int w, int h,
+                      int reverse, const oracle_params *o, const oracle_filter *f);
+
+static int compute_pair(const oracle_params *o, const oracle_filter *f, const uint8_t *left,
+                        const uint8_t *right, const uint8_t *left_bgr, const uint8_t *right_bgr,
                         const uint8_t *valid_left, const uint8_t *valid_right, int width,
                         int height, float *checked, float *dl_out, float *dr_out,
                         oracle_stats *stats, uint8_t *reasons, uint8_t *searchBoundary) {
@@ -382,6 +385,11 @@ static int compute_pair(const oracle_params *o, const uint8_t *left, const uint8
   if (rc == 0)
     rc = oracle_match(o, b, a, right, left, tb, ta, width, height, 1, dr, &discarded, NULL, NULL,
                       NULL, NULL);
+  if (rc == 0 && f && f->subpixel_refinement) {
+    /* --subpixel-refinement zncc (CensusSgm.java:168-170), on the gray rasters */
+    refine(dl, left, right, width, height, 0, o, f);
+    refine(dr, right, left, width, height, 1, o, f);
+  }
   if (rc == 0) {
     /* Left-right check (CensusSgm.java:181-201). */
     int rx = o->window_w / 2, ry = o->window_h / 2;
@@ -419,8 +427,18 @@ int oracle_compute(const oracle_params *o, const uint8_t *left, const uint8_t *r
                    const uint8_t *valid_left, const uint8_t *valid_right, int width, int height,
                    float *checked, float *dl, float *dr, oracle_stats *stats, uint8_t *reasons,
                    uint8_t *boundary) {
-  return compute_pair(o, left, right, NULL, NULL, valid_left, valid_right, width, height,
+  return compute_pair(o, NULL, left, right, NULL, NULL, valid_left, valid_right, width, height,
                       checked, dl, dr, stats, reasons, boundary);
+}
+
+int oracle_compute_ex(const oracle_params *o, const oracle_filter *f, const uint8_t *left,
+                      const uint8_t *right, const uint8_t *left_bgr, const uint8_t *right_bgr,
+                      const uint8_t *valid_left, const uint8_t *valid_right, int width,
+                      int height, float *checked, float *dl, float *dr, oracle_stats *stats,
+                      uint8_t *reasons, uint8_t *boundary) {
+  if ((left_bgr == NULL) != (right_bgr == NULL)) return -1;
+  return compute_pair(o, f, left, right, left_bgr, right_bgr, valid_left, valid_right, width,
+                      height, checked, dl, dr, stats, reasons, boundary);
 }
 
 int oracle_compute_color(const oracle_params *o, const uint8_t *left, const uint8_t *right,
@@ -429,7 +447,7 @@ int oracle_compute_color(const oracle_params *o, const uint8_t *left, const uint
                          int height, float *checked, float *dl, float *dr, oracle_stats *stats,
                          uint8_t *reasons, uint8_t *boundary) {
   if (!left_bgr || !right_bgr) return -1;
-  return compute_pair(o, left, right, left_bgr, right_bgr, valid_left, valid_right, width,
+  return compute_pair(o, NULL, left, right, left_bgr, right_bgr, valid_left, valid_right, width,
                       height, checked, dl, dr, stats, reasons, boundary);
 }
 
@@ -533,6 +551,67 @@ static double photo_zncc(const uint8_t *a, const uint8_t *b, int w, int h, int x
   if (var_a < channels * n * 4 || var_b < channels * n * 4) return -1;
   double prod = var_a * var_b;
   return covariance / sqrt(prod > 1e-12 ? prod : 1e-12);
+}
+
+/* PhotometricRefinement.refine (PhotometricRefinement.java:5-61): quarter-step
+ * ZNCC search around each disparity away from depth edges, a parabolic
+ * eighth-step refinement, and acceptance of clear improvements only. */
+static int64_t refine(float *disparity, const uint8_t *a, const uint8_t *b, int w, int h,
+                      int reverse, const oracle_params *o, const oracle_filter *f) {
+  size_t n = (size_t)w * h;
+  float *original = (float *)malloc(sizeof(float) * n);
+  memcpy(original, disparity, sizeof(float) * n);
+  int64_t changed = 0;
+  int r = f->correlation_window / 2;
+  for (int y = r; y < h - r; y++)
+    for (int x = r; x < w - r; x++) {
+      size_t p = (size_t)y * w + x;
+      double d = original[p];
+      if (!isfinite(d)) continue;
+      int edge = 0;
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          float q = original[p + (ptrdiff_t)dy * w + dx];
+          if (isfinite_f(q) && fabs((double)q - d) > 1.5) edge = 1;
+        }
+      if (edge) continue;
+      double initial = photo_zncc(a, b, w, h, x, y, reverse ? -d : d, r, 1), best = initial,
+             chosen = d;
+      int slot = -1;
+      double scores[7];
+      for (int k = 0; k < 7; k++) {
+        double candidate = d + (k - 3) * .25;
+        if (candidate <= o->min_disparity || candidate >= o->min_disparity + o->disparities - 1) {
+          scores[k] = -1;
+          continue;
+        }
+        scores[k] = photo_zncc(a, b, w, h, x, y, reverse ? -candidate : candidate, r, 1);
+        if (scores[k] > best) {
+          best = scores[k];
+          chosen = candidate;
+          slot = k;
+        }
+      }
+      if (slot > 0 && slot < 6 && scores[slot - 1] > -1 && scores[slot + 1] > -1) {
+        double curvature = 2 * best - scores[slot - 1] - scores[slot + 1];
+        if (curvature > 1e-8) {
+          double offset = .125 * (scores[slot + 1] - scores[slot - 1]) / curvature;
+          offset = fmax(-.125, fmin(.125, offset));
+          double candidate = chosen + offset,
+                 score = photo_zncc(a, b, w, h, x, y, reverse ? -candidate : candidate, r, 1);
+          if (score > best) {
+            best = score;
+            chosen = candidate;
+          }
+        }
+      }
+      if (best >= fmax(.5, f->correlation) && best > initial + .002 && fabs(chosen - d) <= .75) {
+        disparity[p] = (float)chosen;
+        changed++;
+      }
+    }
+  free(original);
+  return changed;
 }
 
 /* CensusSgm.removeSpeckles (CensusSgm.java:651-681). */

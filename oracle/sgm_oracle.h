@@ -114,7 +114,8 @@ typedef struct {
   int32_t correlation_window;   /* photometric ZNCC window (odd) */
   int32_t speckle_size;         /* minimum component size; <= 1 disables */
   int32_t speckle_connectivity; /* 4 or 8 */
-  int32_t reserved0, reserved1;
+  int32_t subpixel_refinement;  /* 1: PhotometricRefinement before the L-R check */
+  int32_t reserved1;
   double median_max_change;
   double correlation;           /* photometric threshold: score < correlation rejects */
   double speckle_range;
@@ -138,6 +139,16 @@ int oracle_postfilter_ch(const oracle_params *o, const oracle_filter *f, const u
                          const uint8_t *photo_left, const uint8_t *photo_right, int channels,
                          const float *dr, int w, int h, float *checked, oracle_stats *stats,
                          uint8_t *reasons);
+
+/* compute_pair with options: f (may be NULL) with subpixel_refinement set runs
+ * PhotometricRefinement (PhotometricRefinement.java:5-61) on both matching
+ * directions before the L-R check (CensusSgm.java:168-170); left_bgr /
+ * right_bgr (both or neither) select the colour-distance census. */
+int oracle_compute_ex(const oracle_params *o, const oracle_filter *f, const uint8_t *left,
+                      const uint8_t *right, const uint8_t *left_bgr, const uint8_t *right_bgr,
+                      const uint8_t *valid_left, const uint8_t *valid_right, int w, int h,
+                      float *checked, float *dl, float *dr, oracle_stats *stats,
+                      uint8_t *reasons, uint8_t *boundary);
 
 /* CensusSgm.zncc (CensusSgm.java:614-644) over `channels` interleaved channels:
  * the photometric score. */

@@ -384,6 +384,7 @@ extern "C" int sgm_release_buffers(sgm_context *ctx) {
   std::lock_guard<std::mutex> g(ctx->lock);
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
   for (auto &kv : ctx->buffers) kv.second.release(ctx);
   ctx->buffers.clear();
   for (auto &kv : ctx->host_buffers) kv.second.release();
@@ -666,6 +667,8 @@ struct PairBuffers {
   unsigned long long *stats;
   // colour-distance census: interleaved BGR of both images on the device, or null
   const uint8_t *bgr[2];
+  // filter options whose subpixel_refinement runs before the L-R check, or null
+  const sgm_filter_params *refine;
   // streamed plan (strip_rows > 0): one strip's volumes, carry rows, host aggregate
   int strip_rows;
   uint16_t *strip_cost;
@@ -723,6 +726,7 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
   ENSURE("stats", 16 * 8);
   ENSURE("code", n);
   B.bgr[0] = B.bgr[1] = nullptr;
+  B.refine = nullptr;
   B.strip_rows = rows;
   B.strip_cost = nullptr;
   B.strip_sum = nullptr;
@@ -1078,6 +1082,29 @@ int run_pair(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuffers &B
   }
   static const int both[2] = {0, 1};
   if ((rc = run_directions(ctx, p, w, h, B, both, 2, 0, B.reasons, B.boundary))) return rc;
+  if (B.refine && B.refine->subpixel_refinement) {
+    // PhotometricRefinement of both directions on the gray rasters (CensusSgm.java:168-170)
+    const size_t n = (size_t)w * h;
+    if ((rc = ctx->buffers["refine0"].ensure(ctx, n * 4, "refine0"))) return rc;
+    if ((rc = ctx->buffers["refine1"].ensure(ctx, n * 4, "refine1"))) return rc;
+    for (int d = 0; d < 2; d++) {
+      float *orig = (float *)ctx->buffers[d ? "refine1" : "refine0"].ptr;
+      hipError_t e = hipMemcpyAsync(orig, B.disp[d], n * 4, hipMemcpyDeviceToDevice, ctx->stream);
+      if (e != hipSuccess) return hip_fail(e, "refinement copy");
+      RefineArgs G;
+      G.w = w;
+      G.h = h;
+      G.r = B.refine->correlation_window / 2;
+      G.min_d = p->min_disparity;
+      G.count = p->disparities;
+      G.reverse = d;
+      G.correlation = B.refine->correlation;
+      ProfileScope ps(ctx, "refine");
+      refine_kernel<<<dim3((unsigned)((w + 255) / 256), (unsigned)h), 256, 0, ctx->stream>>>(
+          orig, B.disp[d], B.gray[d], B.gray[1 - d], G);
+      if ((rc = launch_check("refinement kernel"))) return rc;
+    }
+  }
   LrArgs L;
   L.code = B.code;
   L.dl = B.disp[0];
@@ -1195,6 +1222,10 @@ int validate_filters(const sgm_filter_params *f) {
     return fail(SGM_ERR_INVALID_ARGUMENT, "speckle connectivity must be 4 or 8");
   if (!(f->median_max_change >= 0) || !(f->speckle_range >= 0))
     return fail(SGM_ERR_INVALID_ARGUMENT, "median max change and speckle range must be >= 0");
+  if (f->subpixel_refinement != 0 && f->subpixel_refinement != 1)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "--subpixel-refinement must be none (0) or zncc (1)");
+  if (f->subpixel_refinement && f->correlation_window < 3)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "subpixel refinement needs a correlation window >= 3");
   return SGM_OK;
 }
 
@@ -1290,6 +1321,7 @@ extern "C" int sgm_match_pair_filtered(sgm_context *ctx, const sgm_params *param
     vl = B.valid[0];
     vr = B.valid[1];
   }
+  B.refine = filters;
   if ((rc = run_pair(ctx, params, w, h, B, B.gray[0], B.gray[1], vl, vr, B.checked, B.reasons_out,
                      B.boundary_out, B.stats)))
     return rc;
@@ -1350,6 +1382,7 @@ extern "C" int sgm_match_pair_color(sgm_context *ctx, const sgm_params *params,
     vl = B.valid[0];
     vr = B.valid[1];
   }
+  B.refine = filters;
   if ((rc = run_pair(ctx, params, w, h, B, B.gray[0], B.gray[1], vl, vr, B.checked, B.reasons_out,
                      B.boundary_out, B.stats)))
     return rc;

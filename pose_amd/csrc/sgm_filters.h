@@ -102,6 +102,42 @@ __device__ __forceinline__ void add_counts(unsigned long long *stats, int f0, un
   }
 }
 
+// CensusSgm.zncc (CensusSgm.java:614-644): the photometric score of pixel
+// (x, y) at a subpixel disparity over `ch` interleaved channels, in double in
+// the reference's summation order; -1 outside the image or on flat windows.
+__device__ __forceinline__ double photo_score(const uint8_t *__restrict__ left,
+                                              const uint8_t *__restrict__ right, int w, int h,
+                                              int x, int y, double disparity, int r, int ch) {
+  const double target = (double)x - disparity;
+  const int bxi = (int)floor(target);
+  const double fraction = target - bxi;
+  if (x - r < 0 || x + r >= w || y - r < 0 || y + r >= h || bxi - r < 0 || bxi + r + 1 >= w)
+    return -1.0;
+  const int cnt = (2 * r + 1) * (2 * r + 1);
+  double cov = 0, va = 0, vb = 0;
+  for (int c = 0; c < ch; c++) {
+    double sa = 0, sb = 0, saa = 0, sbb = 0, sab = 0;
+    for (int dy = -r; dy <= r; dy++)
+      for (int dx = -r; dx <= r; dx++) {
+        const size_t ai = ((size_t)(y + dy) * w + x + dx) * ch + c;
+        const size_t bi = ((size_t)(y + dy) * w + bxi + dx) * ch + c;
+        const double av = left[ai];
+        const double bv = (1 - fraction) * right[bi] + fraction * right[bi + ch];
+        sa += av;
+        sb += bv;
+        saa += av * av;
+        sbb += bv * bv;
+        sab += av * bv;
+      }
+    cov += sab - sa * sb / cnt;
+    va += saa - sa * sa / cnt;
+    vb += sbb - sb * sb / cnt;
+  }
+  if (va < ch * cnt * 4 || vb < ch * cnt * 4) return -1.0;
+  const double prod = va * vb;
+  return cov / sqrt(prod > 1e-12 ? prod : 1e-12);
+}
+
 // CensusSgm.reciprocal + CensusSgm.zncc with the photometric threshold
 // (CensusSgm.java:224-257, 602-644); `left` / `right` hold F.channels
 // interleaved channels (gray, or BGR in colour-distance mode).
@@ -132,38 +168,8 @@ static __global__ __launch_bounds__(256) void photo_kernel(float *__restrict__ d
         lr_rej = 1;
         if (reasons) reasons[p] = 5;
       } else {
-        const int r = F.corr_r;
-        const double target = (double)x - (double)d;
-        const int bxi = (int)floor(target);
-        const double fraction = target - bxi;
-        double score = -1.0;
-        if (!(x - r < 0 || x + r >= F.w || y - r < 0 || y + r >= F.h || bxi - r < 0 ||
-              bxi + r + 1 >= F.w)) {
-          const int cnt = (2 * r + 1) * (2 * r + 1), ch = F.channels;
-          double cov = 0, va = 0, vb = 0;
-          for (int c = 0; c < ch; c++) {
-            double sa = 0, sb = 0, saa = 0, sbb = 0, sab = 0;
-            for (int dy = -r; dy <= r; dy++)
-              for (int dx = -r; dx <= r; dx++) {
-                const size_t ai = ((size_t)(y + dy) * F.w + x + dx) * ch + c;
-                const size_t bi = ((size_t)(y + dy) * F.w + bxi + dx) * ch + c;
-                const double av = left[ai];
-                const double bv = (1 - fraction) * right[bi] + fraction * right[bi + ch];
-                sa += av;
-                sb += bv;
-                saa += av * av;
-                sbb += bv * bv;
-                sab += av * bv;
-              }
-            cov += sab - sa * sb / cnt;
-            va += saa - sa * sa / cnt;
-            vb += sbb - sb * sb / cnt;
-          }
-          if (!(va < ch * cnt * 4 || vb < ch * cnt * 4)) {
-            const double prod = va * vb;
-            score = cov / sqrt(prod > 1e-12 ? prod : 1e-12);
-          }
-        }
+        const double score =
+            photo_score(left, right, F.w, F.h, x, y, (double)d, F.corr_r, F.channels);
         if (score < F.correlation) {
           disp[p] = __builtin_nanf("");
           photo_rej = 1;
@@ -173,6 +179,69 @@ static __global__ __launch_bounds__(256) void photo_kernel(float *__restrict__ d
     }
   }
   add_counts(stats, 7, lr_rej, 8, photo_rej);
+}
+
+// PhotometricRefinement.refine (PhotometricRefinement.java:5-61) of one
+// matching direction (--subpixel-refinement zncc, applied before the L-R check,
+// CensusSgm.java:168-170): away from depth edges (a finite 3x3 neighbour more
+// than 1.5 away), the ZNCC of the gray windows at the disparity and at seven
+// quarter steps around it inside the search range, a parabolic eighth-step
+// refinement around the best, and the best kept only if it is a clear
+// improvement.  Reads the unrefined map `orig`, writes `disp`.
+struct RefineArgs {
+  int w, h, r, min_d, count, reverse;
+  double correlation;
+};
+
+static __global__ __launch_bounds__(256) void refine_kernel(const float *__restrict__ orig,
+                                                            float *__restrict__ disp,
+                                                            const uint8_t *__restrict__ a,
+                                                            const uint8_t *__restrict__ b,
+                                                            RefineArgs G) {
+  const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+  const int r = G.r;
+  if (x < r || x >= G.w - r || y < r || y >= G.h - r) return;
+  const size_t p = (size_t)y * G.w + x;
+  const double d = orig[p];
+  if (!isfinite(d)) return;
+  for (int dy = -1; dy <= 1; dy++)
+    for (int dx = -1; dx <= 1; dx++) {
+      const float q = orig[p + (ptrdiff_t)dy * G.w + dx];
+      if (isfinite(q) && fabs((double)q - d) > 1.5) return;  // depth edge
+    }
+  const double sgn = G.reverse ? -1.0 : 1.0;
+  const double initial = photo_score(a, b, G.w, G.h, x, y, sgn * d, r, 1);
+  double best = initial, chosen = d;
+  int slot = -1;
+  double scores[7];
+  for (int k = 0; k < 7; k++) {
+    const double candidate = d + (k - 3) * .25;
+    if (candidate <= G.min_d || candidate >= G.min_d + G.count - 1) {
+      scores[k] = -1;
+      continue;
+    }
+    scores[k] = photo_score(a, b, G.w, G.h, x, y, sgn * candidate, r, 1);
+    if (scores[k] > best) {
+      best = scores[k];
+      chosen = candidate;
+      slot = k;
+    }
+  }
+  if (slot > 0 && slot < 6 && scores[slot - 1] > -1 && scores[slot + 1] > -1) {
+    const double curvature = 2 * best - scores[slot - 1] - scores[slot + 1];
+    if (curvature > 1e-8) {
+      double offset = .125 * (scores[slot + 1] - scores[slot - 1]) / curvature;
+      offset = fmax(-.125, fmin(.125, offset));
+      const double candidate = chosen + offset;
+      const double score = photo_score(a, b, G.w, G.h, x, y, sgn * candidate, r, 1);
+      if (score > best) {
+        best = score;
+        chosen = candidate;
+      }
+    }
+  }
+  if (best >= fmax(.5, G.correlation) && best > initial + .002 && fabs(chosen - d) <= .75)
+    disp[p] = (float)chosen;
 }
 
 // ---- speckles: connected components of 4/8-connected pixels whose values

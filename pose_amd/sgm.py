@@ -58,7 +58,7 @@ class SgmFilterParams(ctypes.Structure):
         ("median", ctypes.c_int32), ("median_classic", ctypes.c_int32),
         ("median_edge_threshold", ctypes.c_int32), ("correlation_window", ctypes.c_int32),
         ("speckle_size", ctypes.c_int32), ("speckle_connectivity", ctypes.c_int32),
-        ("reserved0", ctypes.c_int32), ("reserved1", ctypes.c_int32),
+        ("subpixel_refinement", ctypes.c_int32), ("reserved1", ctypes.c_int32),
         ("median_max_change", ctypes.c_double), ("correlation", ctypes.c_double),
         ("speckle_range", ctypes.c_double), ("reserved2", ctypes.c_double),
     ]
@@ -260,6 +260,7 @@ class Options:
         self.speckleSize = 50
         self.speckleRange = 1.0
         self.speckleConnectivity = 8
+        self.subpixelRefinement = "none"   # none | zncc (Options.java:37)
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise AttributeError("unknown option " + k)
@@ -315,11 +316,14 @@ def filters_from_options(o):
     """sgm_filter_params from the reference's option names."""
     if o.medianMode not in ("edge-preserving", "classic"):
         raise ValueError("--median-mode must be edge-preserving or classic")
+    if o.subpixelRefinement not in ("none", "zncc"):
+        raise ValueError("--subpixel-refinement must be none or zncc")
     return SgmFilterParams(
         median=o.median, median_classic=1 if o.medianMode == "classic" else 0,
         median_edge_threshold=o.medianEdgeThreshold, correlation_window=o.correlationWindow,
         speckle_size=o.speckleSize, speckle_connectivity=4 if o.speckleConnectivity == 4 else 8,
-        reserved0=0, reserved1=0, median_max_change=o.medianMaxChange,
+        subpixel_refinement=1 if o.subpixelRefinement == "zncc" else 0, reserved1=0,
+        median_max_change=o.medianMaxChange,
         correlation=o.correlation, speckle_range=o.speckleRange, reserved2=0.0)
 
 

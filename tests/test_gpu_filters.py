@@ -33,6 +33,11 @@ CASES = [
                                              median_max_change=1.5, speckle_size=0)),
     (240, 130, 64, 0, (9, 7), 0, False, dict(correlation=-2.0, speckle_range=0.5,
                                              correlation_window=5)),
+    # --subpixel-refinement zncc (PhotometricRefinement before the L-R check)
+    (160, 100, 48, 0, (5, 5), 0, False, dict(subpixel_refinement=1)),
+    (200, 120, 64, -4, (9, 7), 1, True, dict(subpixel_refinement=1, correlation=0.3)),
+    (180, 90, 40, 2, (7, 7), 0, False, dict(subpixel_refinement=1, correlation_window=5,
+                                             median=1, speckle_size=0)),
 ]
 
 

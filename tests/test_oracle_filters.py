@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from pose_amd import synth
 
 
 def np_median_valid(a, window):
@@ -224,3 +225,92 @@ def test_reciprocity_rejects_with_reason_5():
     out = out.reshape(h, w)
     assert out[5, 10] == 3.0 and np.isnan(out[6, 10])
     assert st["post_median_lr_rejected"] == 1 and rs.reshape(h, w)[6, 10] == 5
+
+
+def _py_zncc(a, b, w, h, x, y, disparity, r):
+    """CensusSgm.zncc (CensusSgm.java:614-644), one channel, plain Python."""
+    target = x - disparity
+    bx = math.floor(target)
+    fraction = target - bx
+    if x - r < 0 or x + r >= w or y - r < 0 or y + r >= h or bx - r < 0 or bx + r + 1 >= w:
+        return -1.0
+    n = (2 * r + 1) ** 2
+    sa = sb = saa = sbb = sab = 0.0
+    for dy in range(-r, r + 1):
+        for dx in range(-r, r + 1):
+            av = float(a[(y + dy) * w + x + dx])
+            bi = (y + dy) * w + bx + dx
+            bv = (1 - fraction) * float(b[bi]) + fraction * float(b[bi + 1])
+            sa += av
+            sb += bv
+            saa += av * av
+            sbb += bv * bv
+            sab += av * bv
+    cov = 0.0 + (sab - sa * sb / n)
+    va = 0.0 + (saa - sa * sa / n)
+    vb = 0.0 + (sbb - sb * sb / n)
+    if va < n * 4 or vb < n * 4:
+        return -1.0
+    return cov / math.sqrt(max(1e-12, va * vb))
+
+
+def _py_refine(disp, a, b, w, h, reverse, min_d, count, window, correlation):
+    """PhotometricRefinement.refine (PhotometricRefinement.java:5-61), plain Python."""
+    out = np.array(disp, np.float32, copy=True)
+    r = window // 2
+    for y in range(r, h - r):
+        for x in range(r, w - r):
+            p = y * w + x
+            d = float(disp[p])
+            if not math.isfinite(d):
+                continue
+            if any(math.isfinite(float(disp[p + dy * w + dx])) and abs(float(disp[p + dy * w + dx]) - d) > 1.5
+                   for dy in (-1, 0, 1) for dx in (-1, 0, 1)):
+                continue
+            sgn = -1 if reverse else 1
+            initial = _py_zncc(a, b, w, h, x, y, sgn * d, r)
+            best, chosen, slot = initial, d, -1
+            scores = [0.0] * 7
+            for k in range(7):
+                c = d + (k - 3) * .25
+                if c <= min_d or c >= min_d + count - 1:
+                    scores[k] = -1.0
+                    continue
+                scores[k] = _py_zncc(a, b, w, h, x, y, sgn * c, r)
+                if scores[k] > best:
+                    best, chosen, slot = scores[k], c, k
+            if 0 < slot < 6 and scores[slot - 1] > -1 and scores[slot + 1] > -1:
+                curvature = 2 * best - scores[slot - 1] - scores[slot + 1]
+                if curvature > 1e-8:
+                    offset = .125 * (scores[slot + 1] - scores[slot - 1]) / curvature
+                    offset = max(-.125, min(.125, offset))
+                    c = chosen + offset
+                    score = _py_zncc(a, b, w, h, x, y, sgn * c, r)
+                    if score > best:
+                        best, chosen = score, c
+            if best >= max(.5, correlation) and best > initial + .002 and abs(chosen - d) <= .75:
+                out[p] = np.float32(chosen)
+    return out
+
+
+def test_subpixel_refinement_matches_python_restatement():
+    """--subpixel-refinement zncc: the oracle refines both matching directions
+    before the L-R check exactly as an independent Python restatement of
+    PhotometricRefinement.refine does from the unrefined maps."""
+    w, h = 44, 22
+    left, right, _ = synth.stereo_pair(w, h, 12, 0, seed=9)
+    p = O.params(disparities=12, lr_threshold=1.0)
+    base = O.compute(p, left, right, w, h)
+    f = O.filters(subpixel_refinement=1, correlation=0.3)
+    ref = O.compute_ex(p, f, left, right, w, h)
+    a, b = left.reshape(-1), right.reshape(-1)
+    want_l = _py_refine(base["dl"], a, b, w, h, False, 0, 12, f.correlation_window, f.correlation)
+    want_r = _py_refine(base["dr"], b, a, w, h, True, 0, 12, f.correlation_window, f.correlation)
+    assert np.array_equal(ref["dl"].view(np.uint32), want_l.view(np.uint32))
+    assert np.array_equal(ref["dr"].view(np.uint32), want_r.view(np.uint32))
+    moved = np.isfinite(base["dl"]) & (ref["dl"] != base["dl"])
+    assert moved.sum() > 0
+    assert np.all(np.abs(ref["dl"][moved] - base["dl"][moved]) <= .75)
+    # refinement off: identical to the plain computation
+    off = O.compute_ex(p, O.filters(), left, right, w, h)
+    assert np.array_equal(off["checked"].view(np.uint32), base["checked"].view(np.uint32))
