@@ -278,8 +278,9 @@ def run_served(sgm, params_for, rate=20.0, count=60, budget=250e9, max_batch=8, 
     out = {"workload": "served stream: 70%% batched-distribution pairs, 25%% 1600x1067 D=256, "
                        "5%% 6000x4000 D=512; census 9x7, P1=10 P2=120, 8 paths; host buffers "
                        "in/out; Poisson %.0f pairs/s (%d requests) and a saturated burst "
-                       "(%d requests at t=0); server budget %.0f GB, max batch %d, %d streams"
-                       % (rate, count, count, budget / 1e9, max_batch, streams),
+                       "(%d requests at t=0); server budget %.0f GB, %d streams; batching server "
+                       "(max batch %d) and lane server (max batch 1, plain kernels per lane)"
+                       % (rate, count, count, budget / 1e9, streams, max_batch),
            "input_generation_s": round(t_gen, 2), "distinct_pairs": len(images)}
     for name, trace in traces.items():
         updates = sum(w * h * D * 8 * 2 for _, _, (w, h, D, md, s) in trace)
@@ -306,43 +307,56 @@ def run_served(sgm, params_for, rate=20.0, count=60, budget=250e9, max_batch=8, 
             plain_out.append(r["checked"])
         plain_span = time.perf_counter() - t0 - trace[0][0]
         ctx.close()
-        # (b) server: a submitter thread replays the arrivals
-        ctx = sgm.Context(0, budget_bytes=int(budget) + (8 << 30))
-        srv = sgm.Server(ctx, params_for(64, 0), budget_bytes=int(budget), max_batch=max_batch,
-                         streams=streams)
-        tickets = [None] * len(trace)
-        def submit():
-            t1 = time.perf_counter()
-            for i, (ta, _, (w, h, D, md, s)) in enumerate(trace):
-                now = time.perf_counter() - t1
-                if ta > now:
-                    time.sleep(ta - now)
-                tickets[i] = srv.submit(*images[s], D, md)
-        th = threading.Thread(target=submit)
-        th.start()
-        lat, first, last, same, batches = [], None, 0.0, True, set()
-        for i in range(len(trace)):
-            while tickets[i] is None:
-                time.sleep(0.0005)
-            got = srv.wait(tickets[i], timeout=600.0)
-            info = got["info"]
-            lat.append(info["finished_s"] - info["arrival_s"])
-            first = info["arrival_s"] if first is None else min(first, info["arrival_s"])
-            last = max(last, info["finished_s"])
-            batches.add(info["batch_id"])
-            same &= bool(np.array_equal(got["checked"].view(np.uint32),
-                                        plain_out[i].view(np.uint32)))
-        th.join()
-        srv.close()
-        ctx.close()
-        span = last - first
+        # (b) servers: a submitter thread replays the arrivals; the batching
+        # server (persistent batch kernel) and the lane server (max batch 1:
+        # plain kernels per pair on per-lane streams)
+        def replay(mb, streams_):
+            ctx = sgm.Context(0, budget_bytes=int(budget) + (8 << 30))
+            srv = sgm.Server(ctx, params_for(64, 0), budget_bytes=int(budget), max_batch=mb,
+                             streams=streams_)
+            if mb == 1:
+                # lanes allocate on first use: warm every lane on the largest request
+                warm = [srv.submit(*images[big[4]], big[2], big[3]) for _ in range(streams_)]
+                for t in warm:
+                    srv.wait(t, timeout=600.0)
+            tickets = [None] * len(trace)
+
+            def submit():
+                t1 = time.perf_counter()
+                for i, (ta, _, (w, h, D, md, s)) in enumerate(trace):
+                    now = time.perf_counter() - t1
+                    if ta > now:
+                        time.sleep(ta - now)
+                    tickets[i] = srv.submit(*images[s], D, md)
+            th = threading.Thread(target=submit)
+            th.start()
+            lat, first, last, same, batches = [], None, 0.0, True, set()
+            for i in range(len(trace)):
+                while tickets[i] is None:
+                    time.sleep(0.0005)
+                got = srv.wait(tickets[i], timeout=600.0)
+                info = got["info"]
+                lat.append(info["finished_s"] - info["arrival_s"])
+                first = info["arrival_s"] if first is None else min(first, info["arrival_s"])
+                last = max(last, info["finished_s"])
+                batches.add(info["batch_id"])
+                same &= bool(np.array_equal(got["checked"].view(np.uint32),
+                                            plain_out[i].view(np.uint32)))
+            th.join()
+            srv.close()
+            ctx.close()
+            span = last - first
+            return span, {"p50_ms": round(pct(lat, 0.5) * 1e3, 2),
+                          "p99_ms": round(pct(lat, 0.99) * 1e3, 2),
+                          "pairs_per_s": round(len(trace) / span, 3),
+                          "gupdates_per_s": round(updates / span / 1e9, 3),
+                          "batches": len(batches), "bit_exact_vs_plain": same}
+        span, served = replay(max_batch, streams)
+        span_l, lanes = replay(1, streams)
         out[name] = {
             "requests": len(trace), "classes": classes, "cell_updates": updates,
-            "served": {"p50_ms": round(pct(lat, 0.5) * 1e3, 2),
-                       "p99_ms": round(pct(lat, 0.99) * 1e3, 2),
-                       "pairs_per_s": round(len(trace) / span, 3),
-                       "gupdates_per_s": round(updates / span / 1e9, 3),
-                       "batches": len(batches)},
+            "served": served,
+            "served_lanes": lanes,
             "plain": {"p50_ms": round(pct(plain_lat, 0.5) * 1e3, 2),
                       "p99_ms": round(pct(plain_lat, 0.99) * 1e3, 2),
                       "pairs_per_s": round(len(trace) / plain_span, 3),
@@ -351,7 +365,8 @@ def run_served(sgm, params_for, rate=20.0, count=60, budget=250e9, max_batch=8, 
                                           for c, v in service.items()},
                       "max_service_ms": {c: round(max(v) * 1e3, 2) for c, v in service.items()}},
             "speedup_vs_plain": round(plain_span / span, 3),
-            "bit_exact_vs_plain": same,
+            "lanes_speedup_vs_plain": round(plain_span / span_l, 3),
+            "bit_exact_vs_plain": served["bit_exact_vs_plain"] and lanes["bit_exact_vs_plain"],
         }
         print("served leg %s done" % name, file=sys.stderr, flush=True)
     return out

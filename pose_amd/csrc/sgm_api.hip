@@ -1646,3 +1646,81 @@ extern "C" int sgm_memcpy(sgm_context *ctx, void *dst, const void *src, size_t b
   e = hipStreamSynchronize(ctx->stream);
   return e == hipSuccess ? SGM_OK : hip_fail(e, "sgm_memcpy synchronize");
 }
+
+// ---------------------------------------------------------------------------
+// Asynchronous plain pair for the serving lanes: sgm_match_pair enqueued on the
+// context's stream through pinned staging buffers (so neither the uploads nor
+// the downloads block the calling thread), completed by lane_collect once the
+// stream has passed the work.
+namespace sgm {
+
+int lane_enqueue(sgm_context *ctx, const sgm_params *base, const sgm_pair &pr) {
+  if (!ctx || !pr.left || !pr.right || !pr.checked)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "null argument");
+  if ((pr.valid_left == nullptr) != (pr.valid_right == nullptr))
+    return fail(SGM_ERR_INVALID_ARGUMENT, "Invalid stereo mask dimensions");
+  sgm_params p = *base;
+  p.disparities = pr.disparities;
+  p.min_disparity = pr.min_disparity;
+  const int w = pr.width, h = pr.height;
+  std::lock_guard<std::mutex> g(ctx->lock);
+  int rc = validate_params(&p, w, h);
+  if (rc) return rc;
+  int rows;
+  if ((rc = pick_plan(ctx, &p, w, h, &rows))) return rc;
+  (void)hipSetDevice(ctx->device);
+  PairBuffers B;
+  if ((rc = ensure_pair_buffers(ctx, &p, w, h, B, rows))) return rc;
+  ctx->last_streamed = rows > 0;
+  const size_t n = (size_t)w * h;
+  if ((rc = ctx->host_buffers["lane_in"].ensure(4 * n, "lane inputs"))) return rc;
+  if ((rc = ctx->host_buffers["lane_out"].ensure(14 * n + 128, "lane outputs"))) return rc;
+  uint8_t *in = (uint8_t *)ctx->host_buffers["lane_in"].ptr;
+  uint8_t *out = (uint8_t *)ctx->host_buffers["lane_out"].ptr;
+  memcpy(in, pr.left, n);
+  memcpy(in + n, pr.right, n);
+  if ((rc = copy_h2d(ctx, B.gray[0], in, n))) return rc;
+  if ((rc = copy_h2d(ctx, B.gray[1], in + n, n))) return rc;
+  const uint8_t *vl = nullptr, *vr = nullptr;
+  if (pr.valid_left) {
+    memcpy(in + 2 * n, pr.valid_left, n);
+    memcpy(in + 3 * n, pr.valid_right, n);
+    if ((rc = copy_h2d(ctx, B.valid[0], in + 2 * n, n))) return rc;
+    if ((rc = copy_h2d(ctx, B.valid[1], in + 3 * n, n))) return rc;
+    vl = B.valid[0];
+    vr = B.valid[1];
+  }
+  if ((rc = run_pair(ctx, &p, w, h, B, B.gray[0], B.gray[1], vl, vr, B.checked, B.reasons_out,
+                     B.boundary_out, B.stats)))
+    return rc;
+  // staging layout: checked 4n | dl 4n | dr 4n | reasons n | boundary n | counters 128
+  if ((rc = copy_d2h(ctx, out, B.checked, n * 4))) return rc;
+  if (pr.dl && (rc = copy_d2h(ctx, out + 4 * n, B.disp[0], n * 4))) return rc;
+  if (pr.dr && (rc = copy_d2h(ctx, out + 8 * n, B.disp[1], n * 4))) return rc;
+  if (pr.reasons && (rc = copy_d2h(ctx, out + 12 * n, B.reasons_out, n))) return rc;
+  if (pr.boundary && (rc = copy_d2h(ctx, out + 13 * n, B.boundary_out, n))) return rc;
+  if ((rc = copy_d2h(ctx, out + 14 * n, B.stats, 128))) return rc;
+  ctx->last_w = w;
+  ctx->last_h = h;
+  ctx->last_params = p;
+  return SGM_OK;
+}
+
+int lane_collect(sgm_context *ctx, const sgm_pair &pr, sgm_stats *stats) {
+  std::lock_guard<std::mutex> g(ctx->lock);
+  int rc;
+  if ((rc = finish(ctx))) return rc;  // the stream has already passed the work
+  const size_t n = (size_t)pr.width * pr.height;
+  const uint8_t *out = (const uint8_t *)ctx->host_buffers["lane_out"].ptr;
+  memcpy(pr.checked, out, n * 4);
+  if (pr.dl) memcpy(pr.dl, out + 4 * n, n * 4);
+  if (pr.dr) memcpy(pr.dr, out + 8 * n, n * 4);
+  if (pr.reasons) memcpy(pr.reasons, out + 12 * n, n);
+  if (pr.boundary) memcpy(pr.boundary, out + 13 * n, n);
+  unsigned long long counters[16];
+  memcpy(counters, out + 14 * n, sizeof counters);
+  if (stats) stats_from_device(counters, stats);
+  return SGM_OK;
+}
+
+}  // namespace sgm

@@ -55,6 +55,9 @@ int64_t pair_bytes(const sgm_params *p, int w, int h);
 int64_t pair_bytes(const sgm_params *p, int w, int h, bool cost_volume);
 int lines_of(int cls, int w, int h);
 void collect_profile(sgm_context *ctx);
+// plain pair on the context's stream without waiting (serving lanes)
+int lane_enqueue(sgm_context *ctx, const sgm_params *base, const sgm_pair &pr);
+int lane_collect(sgm_context *ctx, const sgm_pair &pr, sgm_stats *stats);
 
 struct ProfileScope {
   sgm_context *ctx;

@@ -6,8 +6,11 @@
 // batch (persistent work-queue kernel) on that stream, and keeps up to
 // `streams` batches in flight, so requests that arrive while others run
 // overlap with them on the device.  Volumes stay device-resident for the life
-// of their batch.  Neither mode has a counterpart in the reference, which
-// matches pairs sequentially (DenseReconstruction.java:64-72).
+// of their batch.  With max_batch == 1 each slot is a lane instead: a context
+// of its own that runs every admitted pair through the plain per-pair kernels
+// on its stream, the lanes overlapping on the device, admission by the device
+// memory the lanes hold.  Neither mode has a counterpart in the reference,
+// which matches pairs sequentially (DenseReconstruction.java:64-72).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -89,6 +92,7 @@ struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   sgm_batch *batch = nullptr;
+  sgm_context *lane = nullptr;  // lane mode: the slot's own context (and stream)
   std::vector<Request *> reqs;
   size_t off = 0, bytes = 0;
   bool busy = false;
@@ -100,6 +104,7 @@ struct sgm_server {
   sgm_context *ctx = nullptr;
   sgm_params params{};
   int max_batch = 16;
+  bool lanes = false;  // max_batch == 1: plain per-pair lanes
   void *region = nullptr;
   size_t region_bytes = 0;
   Pool pool;
@@ -117,6 +122,7 @@ struct sgm_server {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   void run();
+  bool admit_lane(Slot &s);
 };
 
 // Smallest device footprint of a request served alone (admission test).
@@ -135,8 +141,13 @@ void sgm_server::run() {
       if (!s.busy) continue;
       if (hipEventQuery(s.done) != hipSuccess) continue;
       lk.unlock();
-      const int rc = batch_finish(s.batch);
-      sgm_batch_destroy(s.batch);
+      int rc;
+      if (lanes) {
+        rc = lane_collect(s.lane, s.reqs[0]->pair, &s.reqs[0]->stats);
+      } else {
+        rc = batch_finish(s.batch);
+        sgm_batch_destroy(s.batch);
+      }
       lk.lock();
       const double t = now();
       for (Request *r : s.reqs) {
@@ -144,7 +155,7 @@ void sgm_server::run() {
         r->status = rc == SGM_OK ? 0 : rc;
         if (r->pair.stats) *r->pair.stats = r->stats;
       }
-      pool.release(s.off, s.bytes);
+      if (!lanes) pool.release(s.off, s.bytes);
       s.busy = false;
       s.batch = nullptr;
       s.reqs.clear();
@@ -155,9 +166,13 @@ void sgm_server::run() {
       for (Slot &s : slots) any |= s.busy;
       if (!any) break;
     }
-    // admit and launch: one batch per free stream slot
+    // admit and launch: one batch per free stream slot (lanes: one pair)
     for (Slot &s : slots) {
       if (s.busy || pending.empty()) continue;
+      if (lanes) {
+        if (!admit_lane(s)) break;  // wait for memory to free up
+        continue;
+      }
       // A request whose footprint exceeds an eighth of the pool is batched
       // alone when it reaches the head of the queue; the batches formed
       // before that take the small requests behind it in arrival order, so
@@ -255,6 +270,54 @@ void sgm_server::run() {
   }
 }
 
+// Lane mode: the pair at the head of the queue on free lane `s` when the
+// device memory all lanes hold, with this lane grown to the pair's footprint,
+// stays within the budget; cached buffers of idle lanes are released first if
+// that makes it fit.  Called and returning with `mu` held.
+bool sgm_server::admit_lane(Slot &s) {
+  Request *r = pending.front();
+  const int64_t need = (int64_t)r->bytes;
+  auto committed = [&]() {
+    int64_t c = 0;
+    for (Slot &o : slots)
+      if (&o != &s) c += o.lane->allocated;
+    return c + std::max(s.lane->allocated, need);
+  };
+  if (committed() > (int64_t)region_bytes) {
+    for (Slot &o : slots)
+      if (!o.busy && &o != &s && o.lane->allocated) (void)sgm_release_buffers(o.lane);
+    if (committed() > (int64_t)region_bytes && s.lane->allocated > need)
+      (void)sgm_release_buffers(s.lane);
+    if (committed() > (int64_t)region_bytes) return false;
+  }
+  pending.pop_front();
+  const double t = now();
+  r->admitted = t;
+  r->batch_id = next_batch++;
+  s.reqs.assign(1, r);
+  s.busy = true;
+  std::unique_lock<std::mutex> lk(mu, std::adopt_lock);
+  lk.unlock();
+  int rc = lane_enqueue(s.lane, &params, r->pair);
+  if (rc == SGM_OK) {
+    hipError_t e = hipEventRecord(s.done, s.lane->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "hipEventRecord");
+  }
+  lk.lock();
+  lk.release();
+  const double ts = now();
+  if (rc != SGM_OK) {
+    r->status = rc;
+    r->finished = ts;
+    s.busy = false;
+    s.reqs.clear();
+    cv_done.notify_all();
+    return true;
+  }
+  r->started = ts;
+  return true;
+}
+
 extern "C" int sgm_server_create(sgm_context *ctx, const sgm_params *params, int64_t budget_bytes,
                                  int max_batch, int streams, sgm_server **out) {
   if (!ctx || !params || !out || streams < 1 || streams > 8 || max_batch < 1)
@@ -271,16 +334,28 @@ extern "C" int sgm_server_create(sgm_context *ctx, const sgm_params *params, int
   s->ctx = ctx;
   s->params = *params;
   s->max_batch = max_batch;
+  s->lanes = max_batch == 1;
   s->region_bytes = (size_t)bytes;
-  rc = context_alloc(ctx, &s->region, s->region_bytes, "serving pool");
-  if (rc) {
-    delete s;
-    return rc;
+  if (!s->lanes) {
+    rc = context_alloc(ctx, &s->region, s->region_bytes, "serving pool");
+    if (rc) {
+      delete s;
+      return rc;
+    }
+    s->pool.reset(s->region_bytes);
   }
-  s->pool.reset(s->region_bytes);
   s->slots.resize(streams);
   for (Slot &sl : s->slots) {
-    hipError_t e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+    hipError_t e = hipSuccess;
+    if (s->lanes) {
+      // each lane is a context of its own: buffers, stream, autotuned shapes
+      if ((rc = sgm_open(ctx->device, bytes, &sl.lane))) break;
+      sl.lane->tuned_waves = ctx->tuned_waves;
+      sl.lane->volume_policy = ctx->volume_policy;
+      sl.stream = sl.lane->stream;
+    } else {
+      e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+    }
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
     if (e != hipSuccess) {
       rc = hip_fail(e, "server stream");
@@ -290,10 +365,13 @@ extern "C" int sgm_server_create(sgm_context *ctx, const sgm_params *params, int
   if (rc) {
     for (Slot &sl : s->slots) {
       if (sl.done) (void)hipEventDestroy(sl.done);
-      if (sl.stream) (void)hipStreamDestroy(sl.stream);
+      if (sl.lane) (void)sgm_close(sl.lane);
+      else if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
-    (void)hipFree(s->region);
-    ctx->allocated -= s->region_bytes;
+    if (s->region) {
+      (void)hipFree(s->region);
+      ctx->allocated -= s->region_bytes;
+    }
     delete s;
     return rc;
   }
@@ -378,10 +456,11 @@ extern "C" int sgm_server_destroy(sgm_server *srv) {
   (void)hipSetDevice(srv->ctx->device);
   for (Slot &sl : srv->slots) {
     if (sl.done) (void)hipEventDestroy(sl.done);
-    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    if (sl.lane) (void)sgm_close(sl.lane);
+    else if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }
   for (auto &kv : srv->all) delete kv.second;
-  {
+  if (srv->region) {
     std::lock_guard<std::mutex> g(srv->ctx->lock);
     (void)hipFree(srv->region);
     srv->ctx->allocated -= srv->region_bytes;

@@ -269,3 +269,58 @@ def test_autotune_writes_and_loads(gpu_context):
             assert np.array_equal(bits(r["checked"]), bits(want["checked"]))
         finally:
             ctx.close()
+
+
+def test_server_lanes_results_equal_alone(gpu_context):
+    """max_batch 1: each slot is a lane running the plain kernels on its own
+    stream; requests overlap across lanes and each result equals its solo run."""
+    ctx = gpu_context
+    pairs = uneven_pairs(8, 21, wrange=(120, 600), drange=(16, 300))
+    params = to_sgm(O.params(**dict(CENSUS97, disparities=64)))
+    alone = []
+    for pr in pairs:
+        p = to_sgm(O.params(**dict(CENSUS97, disparities=pr["disparities"],
+                                   min_disparity=pr["min_disparity"])))
+        alone.append(ctx.match_pair(p, pr["left"], pr["right"]))
+    srv = sgm.Server(ctx, params, budget_bytes=2 << 30, max_batch=1, streams=3)
+    try:
+        tickets = [srv.submit(pr["left"], pr["right"], pr["disparities"], pr["min_disparity"])
+                   for pr in pairs]
+        spans = []
+        for i, t in enumerate(tickets):
+            got = srv.wait(t, timeout=60.0)
+            for k in ("checked", "dl", "dr"):
+                assert np.array_equal(bits(got[k]), bits(alone[i][k])), (i, k)
+            for k in ("selected", "boundary_rejected", "ambiguity_rejected", "lr_rejected", "valid",
+                      "lower_boundary_rejected", "full_interval_pixels"):
+                assert got["stats"][k] == alone[i]["stats"][k], (i, k)
+            info = got["info"]
+            assert info["arrival_s"] <= info["admitted_s"] <= info["started_s"] <= info["finished_s"]
+            spans.append((info["started_s"], info["finished_s"]))
+        # some requests ran at the same time on different lanes
+        assert any(a[0] < b[1] and b[0] < a[1] for i, a in enumerate(spans) for b in spans[i + 1:])
+    finally:
+        srv.close()
+
+
+def test_server_lanes_large_request_after_small_ones(gpu_context):
+    """Lanes hold their buffers between requests; a large request behind small
+    ones on a tight budget (idle lanes give their buffers back when the sum
+    would not fit) is admitted and served exactly."""
+    ctx = gpu_context
+    small = uneven_pairs(4, 31, wrange=(300, 500), drange=(100, 200))
+    big_l, big_r, _ = synth.stereo_pair(900, 700, 256, 0, seed=78)
+    params = to_sgm(O.params(**dict(CENSUS97, disparities=64)))
+    pbig = to_sgm(O.params(**dict(CENSUS97, disparities=256)))
+    want = ctx.match_pair(pbig, big_l, big_r)
+    budget = int(ctx.plan(pbig, 900, 700)["pair_bytes"] * 1.2)
+    srv = sgm.Server(ctx, params, budget_bytes=budget, max_batch=1, streams=2)
+    try:
+        ts = [srv.submit(pr["left"], pr["right"], pr["disparities"], pr["min_disparity"])
+              for pr in small]
+        for t in ts:
+            srv.wait(t, timeout=60.0)
+        got = srv.wait(srv.submit(big_l, big_r, 256, 0), timeout=60.0)
+        assert np.array_equal(bits(got["checked"]), bits(want["checked"]))
+    finally:
+        srv.close()
