@@ -386,6 +386,7 @@ def main():
     ap.add_argument("--no-filtered", action="store_true", help="skip the post-L-R filter leg")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals, pairs/s")
     ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--serve-streams", type=int, default=3, help="server streams / lanes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this path runs on a single MI355X; "
@@ -478,7 +479,8 @@ def main():
     filtered = None if args.no_filtered else run_filtered(sgm, left, right, W, H)
     batched = None if args.no_batched else run_batched(sgm, params_for)
     served = None if args.no_served else run_served(sgm, params_for, rate=args.serve_rate,
-                                                    count=args.serve_requests)
+                                                    count=args.serve_requests,
+                                                    streams=args.serve_streams)
 
     cpu = None
     if not args.no_cpu:

@@ -326,7 +326,9 @@ typedef struct sgm_request_info {
 
 /* budget_bytes bounds the device footprint of admitted requests (cost +
  * aggregate volumes and per-pair buffers); max_batch caps the pairs launched
- * together; streams is the number of batches that may run concurrently. */
+ * together; streams is the number of batches that may run concurrently.
+ * max_batch == 1 selects lanes: each of the `streams` lanes is a plain context
+ * of its own, and an admitted pair runs the plain path's kernels on it. */
 int sgm_server_create(sgm_context *ctx, const sgm_params *params, int64_t budget_bytes,
                       int max_batch, int streams, sgm_server **out);
 /* Submit one pair (host buffers, owned by the caller until the request
