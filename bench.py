@@ -194,6 +194,42 @@ def run_streamed(sgm, W, H, D, P, d_left, d_right, rows=64, steps=2):
             "pcie_bytes_per_pair": 2 * plan["host_bytes"]}
 
 
+def run_esgm(sgm, W, H, D, P, d_left, d_right, rows=64, pool=2, steps=2):
+    """The benchmark pair under the compact plan (proposal and sum sweeps per
+    path over strips of `rows` rows, certification, exact verification of the
+    uncertified pixels): inputs in HBM; the device footprint holds no volume."""
+    params = workload_params(sgm)
+    ctx = sgm.Context(0)
+    ctx.plan_mode(sgm.PLAN_ESGM, rows)
+    ctx.esgm_candidates(pool)
+    n = W * H
+    out = ctx.alloc(4 * n)
+    ctx.match_pair_device(params, d_left, d_right, W, H, out)
+    ctx.synchronize()
+    ctx.set_profiling(True)
+    ctx.profile()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.match_pair_device(params, d_left, d_right, W, H, out)
+    ctx.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    prof = {k: round(v["total_ms"] / steps, 3)
+            for k, v in ctx.profile()["device_kernel_milliseconds"].items()}
+    ctx.set_profiling(False)
+    counts = ctx.esgm_counts()
+    plan = ctx.plan(params, W, H)
+    out.free()
+    ctx.close()
+    upd = W * H * D * P * 2
+    verified = counts["left"][1] + counts["right"][1]
+    return {"workload": "plain pair 1600x1067 D=256 under the compact (esgm) plan, %d proposals "
+                        "per path, strips of %d rows" % (plan["esgm_candidates"], plan["esgm_rows"]),
+            "ms_per_pair": round(t * 1e3, 3), "gupdates_per_s": round(upd / t / 1e9, 3),
+            "kernel_ms": prof, "device_bytes": plan["esgm_bytes"],
+            "resident_device_bytes": plan["pair_bytes"],
+            "verified_fraction": round(verified / (2.0 * n), 4)}
+
+
 def run_zncc(sgm, steps=5):
     """The FP32 ZNCC workload: 1600x1067, 7x7 windows, D=192, P1=12, P2=150, gain
     field 0.8-1.2 on the right image; inputs in HBM; uint16 cost volume."""
@@ -383,6 +419,7 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the served-vs-plain leg")
     ap.add_argument("--no-zncc", action="store_true", help="skip the FP32 ZNCC leg")
     ap.add_argument("--no-streamed", action="store_true", help="skip the strip-streamed leg")
+    ap.add_argument("--no-esgm", action="store_true", help="skip the compact (esgm) leg")
     ap.add_argument("--no-filtered", action="store_true", help="skip the post-L-R filter leg")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals, pairs/s")
     ap.add_argument("--serve-requests", type=int, default=60)
@@ -466,6 +503,7 @@ def main():
         traffic = json.load(open(tfile)).get("bytes_per_launch")
 
     streamed = None if args.no_streamed else run_streamed(sgm, W, H, D, P, d_left, d_right)
+    esgm = None if args.no_esgm else run_esgm(sgm, W, H, D, P, d_left, d_right)
     for b in (d_left, d_right, d_out, d_stats):
         b.free()
     ctx.close()
@@ -518,6 +556,7 @@ def main():
         "cpu_baseline": cpu,
         "filtered": filtered,
         "streamed": streamed,
+        "esgm": esgm,
         "zncc": zncc,
         "batched": batched,
         "served": served,

@@ -104,6 +104,10 @@ typedef struct sgm_plan {
   int32_t strip_rows;           /* rows per strip of the streamed plan (0 if none fits) */
   int64_t streamed_bytes;       /* device footprint of a pair under the streamed plan */
   int64_t host_bytes;           /* pinned host memory of the streamed aggregate */
+  /* Compact branch (GpuCandidateStereo.plan, GpuCandidateStereo.java:19-47). */
+  int32_t esgm_candidates;      /* proposals per path that fit (0: no compact plan) */
+  int32_t esgm_rows;            /* strip rows of the compact plan */
+  int64_t esgm_bytes;           /* device footprint of a pair under the compact plan */
 } sgm_plan;
 
 typedef struct sgm_context sgm_context;
@@ -244,7 +248,24 @@ int sgm_read_volumes(sgm_context *ctx, int direction, uint16_t *cost, uint32_t *
 #define SGM_PLAN_AUTO 0
 #define SGM_PLAN_RESIDENT 1
 #define SGM_PLAN_STREAMED 2
+/* ESGM (--gpu-sgm esgm; GpuCandidateStereo.execute, GpuCandidateStereo.java:
+ * 49-171): no aggregate volume.  Two strip sweeps per path keep a few
+ * separated proposals per path and the exact aggregate around them plus a
+ * lower bound elsewhere; pixels whose outcome these settle are certified, the
+ * rest are verified exactly over the full range.  Results equal the other
+ * plans.  Gray Census with 16-bit aggregates, 3 <= D <= 4096, 4 or 8 paths;
+ * other parameters or the colour / descriptor entry points fail with
+ * SGM_ERR_NO_PLAN. */
+#define SGM_PLAN_ESGM 3
 int sgm_set_plan_mode(sgm_context *ctx, int mode, int strip_rows);
+/* --esgm-candidates (Options.java:35,781-782): proposals per path, 1..4; the
+ * plan lowers it until the pair fits the budget. */
+int sgm_set_esgm_candidates(sgm_context *ctx, int candidates);
+/* Outcome of the last compact run ("Compact SGM ... certified pixels ...
+ * require exact verification", GpuCandidateStereo.java:103-111): counts[0..1]
+ * certified / verified pixels of the left-reference direction, counts[2..3]
+ * of the right-reference direction (zeros for a direction not run). */
+int sgm_esgm_counts(sgm_context *ctx, int64_t counts[4]);
 
 /* Cost-volume policy for 1-word Census with uint16 aggregates (no counterpart
  * in the reference, whose GpuStereo.plan drops to the streamed plan when the

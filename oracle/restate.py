@@ -268,3 +268,87 @@ def match(p, ga, gb, ta, tb, reverse):
     S = aggregate(p, path_volumes(p, C, ga, ta), sum_bytes)
     disp, reasons, boundary, st = select(p, summary(C, S), p["window_w"] // 2, p["window_h"] // 2)
     return disp, reasons, boundary, st, C, S
+
+
+def esgm_summary(p, C, vols, S, pool):
+    """Compact proposals and certification of GpuCandidateStereo (the COLLECT
+    hook, GpuCandidateStereo.java:413-441; esgm_select :456-478; certification
+    in execute :82-102) over the full per-path volumes.  Returns the compact
+    winner summary and the per-pixel `trusted` mask."""
+    h, w, D = C.shape
+    valid = C != 512
+    big = np.int64(1) << 60
+    k = np.arange(D, dtype=np.int64)[None, None, :]
+    cands = []
+    for L in vols:
+        chosen = []
+        for _ in range(pool):
+            ok = valid.copy()
+            for c in chosen:
+                ok &= np.abs(k - c[..., None]) > 1
+            key = np.where(ok, L * (D + 1) + k, big).min(axis=2)
+            chosen.append(np.where(key == big, 65535, key % (D + 1)))
+        cands += chosen
+    cand = np.stack(cands, axis=2)  # (h, w, Cn)
+    Cn = cand.shape[2]
+    Sw = S & 0xFFFF
+    comp = np.zeros((h, w, Cn, 3), np.int64)
+    for j in range(Cn):
+        for o in range(3):
+            kk = cand[..., j] + o - 1
+            inside = (kk >= 0) & (kk < D)
+            comp[..., j, o] = np.where(inside, np.take_along_axis(Sw, np.clip(kk, 0, D - 1)[..., None], 2)[..., 0], 0)
+    covered = np.zeros((h, w, D), bool)
+    for j in range(Cn):
+        covered |= np.abs(k - cand[..., j][..., None]) <= 1
+    outside = np.zeros((h, w), np.int64)
+    for L in vols:
+        m = np.where(valid & ~covered, L, big).min(axis=2)
+        outside = np.where(m == big, 65535, np.minimum(65535, outside + m))
+    # winner over the proposals
+    best = np.full((h, w), -1, np.int64)
+    value = np.full((h, w), big, np.int64)
+    owner = np.full((h, w), -1, np.int64)
+    for j in range(Cn):
+        kj, vj = cand[..., j], comp[..., j, 1]
+        take = (kj < D) & ((vj < value) | ((vj == value) & ((best < 0) | (kj < best))))
+        best = np.where(take, kj, best)
+        value = np.where(take, vj, value)
+        owner = np.where(take, j, owner)
+    def valid_at(kk):
+        inside = (kk >= 0) & (kk < D)
+        return inside & np.take_along_axis(valid, np.clip(kk, 0, D - 1)[..., None], 2)[..., 0]
+    has = owner >= 0
+    own = np.clip(owner, 0, Cn - 1)
+    at = np.take_along_axis(comp, own[..., None, None], 2)[..., 0, :]
+    before = np.where(has & valid_at(best - 1), at[..., 0], -1)
+    after = np.where(has & valid_at(best + 1), at[..., 2], -1)
+    second = np.full((h, w), big, np.int64)
+    for j in range(Cn):
+        for o in range(3):
+            kk = cand[..., j] + o - 1
+            use = has & (np.abs(kk - best) > 1) & valid_at(kk)
+            second = np.where(use, np.minimum(second, comp[..., j, o]), second)
+    flags = valid[..., 0].astype(np.int64) | (valid[..., D - 1].astype(np.int64) << 1)
+    lower = outside
+    winner = (lower > value) & ((before < 0) | (before > value)) & ((after < 0) | (after > value)) \
+        & (second > value)
+    margin = np.maximum(1.0, value * p["uniqueness"])
+    trusted = ~has | (winner & ((best == 0) | (best == D - 1) | (second - value.astype(np.float64) <= margin)
+                                | ((lower - value) > margin)))
+    return (best, value, second, before, after, flags), trusted
+
+
+def esgm_match(p, ga, gb, ta, tb, reverse, pool):
+    """GpuCandidateStereo.execute: certified pixels take the compact summary,
+    the others the exact one (the verification pass).  Returns (disp, reasons,
+    boundary, stats, trusted)."""
+    da, db = census(ga, p["window_w"], p["window_h"]), census(gb, p["window_w"], p["window_h"])
+    C = cost_volume(p, ga, gb, da, db, ta, tb, reverse)
+    vols = path_volumes(p, C, ga, ta)
+    S = aggregate(p, vols, 2)
+    compact, trusted = esgm_summary(p, C, vols, S, pool)
+    exact = summary(C, S)
+    merged = tuple(np.where(trusted, c, e) for c, e in zip(compact, exact))
+    disp, reasons, boundary, st = select(p, merged, p["window_w"] // 2, p["window_h"] // 2)
+    return disp, reasons, boundary, st, trusted

@@ -21,6 +21,7 @@
 #include "sgm_internal.h"
 #include "sgm_filters.h"
 #include "sgm_strips.h"
+#include "sgm_compact.h"
 
 using namespace sgm;
 
@@ -217,6 +218,50 @@ int streamed_rows(sgm_context *ctx, const sgm_params *p, int w, int h) {
   return 0;
 }
 
+// Compact plan (GpuCandidateStereo.plan / deviceBytes, GpuCandidateStereo.java:
+// 19-47, 174-182): whole-image inputs and outputs of both images, per matching
+// direction the proposals (2 B), their sums (6 B) per proposal, the off-proposal
+// bound, verification slot, pixel list and winner summary (2 + 4 + 4 + 12 B per
+// pixel), one strip's uint16 cost volume and two carry rows; plus one pixel of
+// verification per direction (the group size grows with the budget left).
+int64_t esgm_pair_bytes(const sgm_params *p, int w, int h, int rows, int pool) {
+  const int64_t n = (int64_t)w * h, Dp = padded_disparities(p->disparities);
+  const int64_t C = (int64_t)p->directions * pool;
+  const int sb = std::max(aggregate_bytes(p), 2);
+  const int64_t inputs = direction_bytes(p, w, h, false) - n * Dp * sb;
+  const int64_t per = n * (C * 8 + 2 + 4 + 4 + 12) + (int64_t)w * rows * Dp * 2 + 2 * (int64_t)w * Dp * 4;
+  return 2 * inputs + 2 * per + n * (2 + 2 + 2 + 4) + 2 * Dp * 2 + 512;
+}
+
+// Why the compact plan cannot run these parameters (GpuCandidateStereo.java:
+// 19-30, 198-216), or nullptr.
+const char *esgm_unsupported(const sgm_params *p, int w, int h) {
+  if (p->cost != SGM_COST_CENSUS) return "the compact plan takes Census costs only";
+  if (aggregate_bytes(p) != 2) return "the compact plan needs bounded 16-bit aggregates";
+  if (aggregate_bound(p) * (p->disparities + 1LL) >= 0x7FFFFFFFLL)
+    return "aggregate bound times the range exceeds 32 bits";
+  if (p->disparities < 3 || p->disparities > 4096) return "the compact plan takes 3..4096 disparities";
+  if (p->directions != 4 && p->directions != 8) return "the compact plan takes 4 or 8 paths";
+  if (w < p->window_w + 2 || h < p->window_h + 2) return "Invalid research stereo raster dimensions";
+  return nullptr;
+}
+
+// GpuCandidateStereo.plan: proposals per path from the configured count down
+// to 1, strip rows from gpuStripRows halving, first combination that fits.
+int esgm_plan(sgm_context *ctx, const sgm_params *p, int w, int h, int *rows, int *pool) {
+  *rows = 0;
+  *pool = 0;
+  if (esgm_unsupported(p, w, h)) return SGM_OK;
+  for (int q = ctx->esgm_candidates; q >= 1; q--)
+    for (int r = std::min(h, ctx->strip_rows); r >= 1; r /= 2)
+      if (esgm_pair_bytes(p, w, h, r, q) <= ctx->budget) {
+        *rows = r;
+        *pool = q;
+        return SGM_OK;
+      }
+  return SGM_OK;
+}
+
 }  // namespace sgm
 
 // ---------------------------------------------------------------------------
@@ -319,17 +364,38 @@ extern "C" int sgm_plan_get(sgm_context *ctx, const sgm_params *params, int w, i
   out->streamed = ctx->plan_mode == SGM_PLAN_STREAMED ? out->strip_rows > 0
                   : ctx->plan_mode == SGM_PLAN_AUTO   ? !out->resident && out->strip_rows > 0
                                                       : 0;
+  int rows, pool = 0;
+  esgm_plan(ctx, params, w, h, &rows, &pool);
+  out->esgm_candidates = pool;
+  out->esgm_rows = rows;
+  out->esgm_bytes = pool ? esgm_pair_bytes(params, w, h, rows, pool) : 0;
   return SGM_OK;
 }
 
 extern "C" int sgm_set_plan_mode(sgm_context *ctx, int mode, int strip_rows) {
   if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
-  if (mode < SGM_PLAN_AUTO || mode > SGM_PLAN_STREAMED)
-    return fail(SGM_ERR_INVALID_ARGUMENT, "plan mode %d not in 0..2", mode);
+  if (mode < SGM_PLAN_AUTO || mode > SGM_PLAN_ESGM)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "plan mode %d not in 0..3", mode);
   if (strip_rows < 1) return fail(SGM_ERR_INVALID_ARGUMENT, "--gpu-strip-rows must be positive");
   std::lock_guard<std::mutex> g(ctx->lock);
   ctx->plan_mode = mode;
   ctx->strip_rows = strip_rows;
+  return SGM_OK;
+}
+
+extern "C" int sgm_set_esgm_candidates(sgm_context *ctx, int candidates) {
+  if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
+  if (candidates < 1 || candidates > 4)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "--esgm-candidates must be 1..4");
+  std::lock_guard<std::mutex> g(ctx->lock);
+  ctx->esgm_candidates = candidates;
+  return SGM_OK;
+}
+
+extern "C" int sgm_esgm_counts(sgm_context *ctx, int64_t counts[4]) {
+  if (!ctx || !counts) return fail(SGM_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> g(ctx->lock);
+  for (int i = 0; i < 4; i++) counts[i] = ctx->esgm_last[i];
   return SGM_OK;
 }
 
@@ -675,11 +741,18 @@ struct PairBuffers {
   void *strip_sum;
   int *carry;
   uint8_t *host_sum;
+  // compact plan (esgm_pool > 0): proposals, their sums and the per-pixel
+  // bound / slot / pixel list / summary of both matching directions
+  int esgm_pool, esgm_rows;
+  uint16_t *e_cand, *e_sums, *e_outside, *e_strip_cost;
+  int *e_slot, *e_pixels, *e_count, *e_carry;
+  uint32_t *e_summary;
 };
 
-// Resident (rows = 0) or strip-streamed (rows > 0) buffers of a pair.
+// Resident (rows = 0), strip-streamed (rows > 0) or compact (pool > 0, strips
+// of `rows` rows) buffers of a pair.
 int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuffers &B,
-                        int rows = 0) {
+                        int rows = 0, int pool = 0) {
   const size_t n = (size_t)w * h, Dp = padded_disparities(p->disparities);
   const int sb = aggregate_bytes(p), words = words_of(p);
   const bool zncc = p->cost == SGM_COST_ZNCC || p->cost == SGM_COST_ZNCC_FP32;
@@ -727,12 +800,38 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
   ENSURE("code", n);
   B.bgr[0] = B.bgr[1] = nullptr;
   B.refine = nullptr;
-  B.strip_rows = rows;
+  B.strip_rows = pool ? 0 : rows;
   B.strip_cost = nullptr;
   B.strip_sum = nullptr;
   B.carry = nullptr;
   B.host_sum = nullptr;
-  if (rows) {
+  B.esgm_pool = pool;
+  B.esgm_rows = pool ? rows : 0;
+  B.e_cand = B.e_sums = B.e_outside = B.e_strip_cost = nullptr;
+  B.e_slot = B.e_pixels = B.e_count = B.e_carry = nullptr;
+  B.e_summary = nullptr;
+  if (pool) {
+    const size_t C = (size_t)p->directions * pool;
+    ENSURE("esgm_cand", 2 * n * C * 2);
+    ENSURE("esgm_sums", 2 * n * C * 6);
+    ENSURE("esgm_outside", 2 * n * 2);
+    ENSURE("esgm_slot", 2 * n * 4);
+    ENSURE("esgm_pixels", 2 * n * 4);
+    ENSURE("esgm_summary", 2 * n * 12);
+    ENSURE("esgm_strip_cost", 2 * (size_t)w * rows * Dp * 2);
+    ENSURE("esgm_carry", 2 * 2 * (size_t)w * Dp * 4);
+    ENSURE("esgm_count", 256);
+    B.e_cand = (uint16_t *)ctx->buffers["esgm_cand"].ptr;
+    B.e_sums = (uint16_t *)ctx->buffers["esgm_sums"].ptr;
+    B.e_outside = (uint16_t *)ctx->buffers["esgm_outside"].ptr;
+    B.e_slot = (int *)ctx->buffers["esgm_slot"].ptr;
+    B.e_pixels = (int *)ctx->buffers["esgm_pixels"].ptr;
+    B.e_summary = (uint32_t *)ctx->buffers["esgm_summary"].ptr;
+    B.e_strip_cost = (uint16_t *)ctx->buffers["esgm_strip_cost"].ptr;
+    B.e_carry = (int *)ctx->buffers["esgm_carry"].ptr;
+    B.e_count = (int *)ctx->buffers["esgm_count"].ptr;
+  }
+  if (rows && !pool) {
     const size_t cells = (size_t)w * rows * Dp;
     // up to two matching directions per strip launch
     ENSURE("strip_cost", 2 * cells * 2);
@@ -942,12 +1041,229 @@ int run_directions_streamed(sgm_context *ctx, const sgm_params *p, int w, int h,
   return SGM_OK;
 }
 
+template <int R, int PH>
+void launch_compact_t(sgm_context *ctx, const CompactJobs &J, const PathArgs &A,
+                      const CompactArgs &E, int Dp) {
+  // two Dp-int LDS rows per wave, at most 32 KiB per workgroup
+  const int waves = Dp <= 1024 ? 4 : Dp <= 2048 ? 2 : 1;
+  const unsigned blocks = (unsigned)((A.total_lines + waves - 1) / waves);
+  const size_t lds = (size_t)waves * 2 * Dp * sizeof(int);
+  compact_kernel<R, PH><<<blocks, waves * 64, lds, ctx->stream>>>(J, A, E);
+}
+
+int launch_compact(sgm_context *ctx, int R, int phase, const CompactJobs &J, const PathArgs &A,
+                   const CompactArgs &E, int Dp) {
+  ProfileScope ps(ctx, "compact");
+#define CASE_COMPACT(RR)                                                        \
+  case RR:                                                                      \
+    if (phase == ESGM_CANDIDATES)                                               \
+      launch_compact_t<RR, ESGM_CANDIDATES>(ctx, J, A, E, Dp);                  \
+    else if (phase == ESGM_SUMS)                                                \
+      launch_compact_t<RR, ESGM_SUMS>(ctx, J, A, E, Dp);                        \
+    else                                                                        \
+      launch_compact_t<RR, ESGM_VERIFY>(ctx, J, A, E, Dp);                      \
+    break;
+  switch (R) {
+    CASE_COMPACT(2) CASE_COMPACT(4) CASE_COMPACT(8) CASE_COMPACT(16) CASE_COMPACT(32)
+    CASE_COMPACT(64)
+    default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
+  }
+#undef CASE_COMPACT
+  return launch_check("compact sweep kernel");
+}
+
+int launch_esgm_verify(sgm_context *ctx, int R, const SummaryJobs &J, int slots) {
+  ProfileScope ps(ctx, "esgm verify");
+  const dim3 grid((unsigned)((slots + 3) / 4), 2);
+  switch (R) {
+    case 2: esgm_verify_kernel<2><<<grid, 256, 0, ctx->stream>>>(J); break;
+    case 4: esgm_verify_kernel<4><<<grid, 256, 0, ctx->stream>>>(J); break;
+    case 8: esgm_verify_kernel<8><<<grid, 256, 0, ctx->stream>>>(J); break;
+    case 16: esgm_verify_kernel<16><<<grid, 256, 0, ctx->stream>>>(J); break;
+    case 32: esgm_verify_kernel<32><<<grid, 256, 0, ctx->stream>>>(J); break;
+    case 64: esgm_verify_kernel<64><<<grid, 256, 0, ctx->stream>>>(J); break;
+    default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
+  }
+  return launch_check("compact verification kernel");
+}
+
+// Compact plan for `nd` matching directions (GpuCandidateStereo.execute/run,
+// GpuCandidateStereo.java:49-171, 190-335): proposal sweeps and sum sweeps
+// over every path and strip, certification, exact verification of the rest in
+// groups that fit the budget, then selection of every pixel from its summary.
+// Same contract as run_directions.
+int run_directions_esgm(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuffers &B,
+                        const int *slots, int nd, int out_slot, uint8_t *reasons,
+                        uint8_t *boundary) {
+  int rc;
+  const int D = p->disparities, Dp = padded_disparities(D), R = path_lanes(p);
+  const int rows = B.esgm_rows, pool = B.esgm_pool, C = p->directions * pool;
+  const int strips = (h + rows - 1) / rows;
+  const size_t n = (size_t)w * h, cells = (size_t)w * rows * Dp, carry_row = (size_t)w * Dp;
+  CostJobs cj;
+  memset(&cj, 0, sizeof cj);
+  CompactJobs kj;
+  memset(&kj, 0, sizeof kj);
+  SummaryJobs sj;
+  memset(&sj, 0, sizeof sj);
+  for (int i = 0; i < nd; i++) {
+    const int d = slots[i], o = 1 - d;
+    CostJob &Cj = cj.job[i];
+    Cj.desc_a = B.desc[d];
+    Cj.desc_b = B.desc[o];
+    Cj.mask_a = B.mask[d];
+    Cj.mask_b = B.mask[o];
+    Cj.gray_a = B.gray[d];
+    Cj.gray_b = B.gray[o];
+    Cj.cost = B.e_strip_cost + i * cells;
+    Cj.w = w;
+    Cj.h = h;
+    Cj.D = D;
+    Cj.Dp = Dp;
+    Cj.minD = p->min_disparity;
+    Cj.reverse = d;
+    CompactJob &K = kj.job[i];
+    K.cost = Cj.cost;
+    K.mask = B.mask[d];
+    K.gray = B.gray[d];
+    K.cand = B.e_cand + i * n * C;
+    K.sums = B.e_sums + i * n * 3 * C;
+    K.outside = B.e_outside + i * n;
+    K.slot = B.e_slot + i * n;
+    K.w = w;
+    K.h = h;
+    K.D = D;
+    K.Dp = Dp;
+    SummaryJob &S = sj.job[i];
+    S.cand = K.cand;
+    S.sums = K.sums;
+    S.outside = K.outside;
+    S.mask_a = B.mask[d];
+    S.mask_b = B.mask[o];
+    S.summary = B.e_summary + i * 3 * n;
+    S.slot = B.e_slot + i * n;
+    S.pixels = B.e_pixels + i * n;
+    S.count = B.e_count + i;
+    S.w = w;
+    S.h = h;
+    S.D = D;
+    S.Dp = Dp;
+    S.minD = p->min_disparity;
+    S.rx = p->window_w / 2;
+    S.ry = p->window_h / 2;
+    S.reverse = d;
+  }
+  if (nd == 1) sj.job[1] = sj.job[0], sj.job[1].hi = 0;
+  auto sweep = [&](int phase) -> int {
+    for (int dir = 0; dir < p->directions; dir++) {
+      PathArgs A = path_args(p, dir, nd, 0, 0);
+      const CompactArgs E{C, pool, dir};
+      for (int ord = 0; ord < strips; ord++) {
+        const int s = A.dy < 0 ? strips - 1 - ord : ord;
+        const int y0 = s * rows, act = std::min(rows, h - y0);
+        for (int i = 0; i < nd; i++) cj.job[i].y0 = y0;
+        int rc2;
+        if ((rc2 = launch_cost(ctx, cj, nd, w, h, Dp, p, act, true))) return rc2;
+        kj.split = segments_of(A.cls, w, act);
+        A.total_lines = nd * kj.split;
+        for (int i = 0; i < nd; i++) {
+          CompactJob &K = kj.job[i];
+          K.y0 = y0;
+          K.rows = act;
+          K.carry_in = K.carry_out = nullptr;
+          if (A.dy != 0) {
+            int *base = B.e_carry + (size_t)i * 2 * carry_row;
+            K.carry_in = ord > 0 ? base + (size_t)(ord & 1) * carry_row : nullptr;
+            K.carry_out = ord < strips - 1 ? base + (size_t)((ord + 1) & 1) * carry_row : nullptr;
+          }
+        }
+        if ((rc2 = launch_compact(ctx, R, phase, kj, A, E, Dp))) return rc2;
+      }
+    }
+    return SGM_OK;
+  };
+  // every proposal entry is written by the proposal sweeps; sums and bounds accumulate
+  if ((rc = launch_fill_u8(ctx, (uint8_t *)B.e_sums, 0, (size_t)nd * n * 3 * C * 2))) return rc;
+  if ((rc = launch_fill_u8(ctx, (uint8_t *)B.e_outside, 0, (size_t)nd * n * 2))) return rc;
+  if ((rc = launch_fill_u8(ctx, (uint8_t *)B.e_count, 0, 8))) return rc;
+  if ((rc = sweep(ESGM_CANDIDATES))) return rc;
+  if ((rc = sweep(ESGM_SUMS))) return rc;
+  {
+    ProfileScope ps(ctx, "esgm summary");
+    esgm_summary_kernel<<<dim3((unsigned)((n + 255) / 256), nd), 256, 0, ctx->stream>>>(
+        sj, C, p->uniqueness);
+    if ((rc = launch_check("compact summary kernel"))) return rc;
+  }
+  int counts[2] = {0, 0};
+  hipError_t e = hipMemcpyAsync(counts, B.e_count, nd * sizeof(int), hipMemcpyDeviceToHost,
+                                ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(e, "compact verification count");
+  for (int k = 0; k < 4; k++) ctx->esgm_last[k] = 0;
+  int most = 0;
+  for (int i = 0; i < nd; i++) {
+    ctx->esgm_last[2 * slots[i]] = (int64_t)n - counts[i];
+    ctx->esgm_last[2 * slots[i] + 1] = counts[i];
+    most = std::max(most, counts[i]);
+  }
+  if (most > 0) {
+    // verification groups: the budget left by the compact plan, at least one pixel
+    const int64_t room = ctx->budget - esgm_pair_bytes(p, w, h, rows, pool) + 2 * Dp * 2;
+    int64_t group = std::max<int64_t>(1, room / ((int64_t)nd * Dp * 2));
+    group = std::min<int64_t>(group, most);
+    const size_t vbytes = (size_t)nd * group * Dp * 2;
+    if ((rc = ctx->buffers["esgm_verify"].ensure(ctx, vbytes, "esgm_verify"))) return rc;
+    uint16_t *verify = (uint16_t *)ctx->buffers["esgm_verify"].ptr;
+    for (int i = 0; i < nd; i++) {
+      kj.job[i].verify = verify + (size_t)i * group * Dp;
+      sj.job[i].verify = kj.job[i].verify;
+    }
+    for (int64_t lo = 0; lo < most; lo += group) {
+      const int hi = (int)std::min<int64_t>(most, lo + group);
+      if ((rc = launch_fill_u8(ctx, (uint8_t *)verify, 0, vbytes))) return rc;
+      for (int i = 0; i < nd; i++) {
+        kj.job[i].lo = (int)lo;
+        kj.job[i].hi = std::min(hi, counts[i]);
+        sj.job[i].lo = (int)lo;
+        sj.job[i].hi = std::min(hi, counts[i]);
+      }
+      if (nd == 1) sj.job[1].hi = 0;
+      if ((rc = sweep(ESGM_VERIFY))) return rc;
+      if ((rc = launch_esgm_verify(ctx, R, sj, hi - (int)lo))) return rc;
+    }
+  }
+  SelectJobs xj;
+  memset(&xj, 0, sizeof xj);
+  for (int i = 0; i < nd; i++) {
+    const int d = slots[i];
+    SelectJob &X = xj.job[i];
+    X.summary = B.e_summary + i * 3 * n;
+    X.out.disp = B.disp[d];
+    X.out.reasons = d == out_slot ? reasons : nullptr;
+    X.out.boundary = d == out_slot ? boundary : nullptr;
+    X.out.code = d == out_slot ? B.code : nullptr;
+    X.out.w = w;
+    X.out.h = h;
+    X.out.rx = p->window_w / 2;
+    X.out.ry = p->window_h / 2;
+    X.out.D = D;
+    X.out.minD = p->min_disparity;
+  }
+  ProfileScope ps(ctx, "esgm select");
+  const PathArgs A = path_args(p, 0, nd, 0, 0);
+  const unsigned chunks = (unsigned)((w + 63) / 64);
+  esgm_select_kernel<<<dim3((chunks + 3) / 4, h, nd), 256, 0, ctx->stream>>>(xj, A);
+  return launch_check("compact selection kernel");
+}
+
 // Cost + path sweeps + fused selection for `nd` matching directions
 // (d = 0: left reference, target x - d; d = 1: right reference, target x + d).
 // The selection codes of slot `out_slot` go to B.code (counted later), its
 // reason / boundary updates to `reasons` / `boundary`.
 int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuffers &B,
                    const int *slots, int nd, int out_slot, uint8_t *reasons, uint8_t *boundary) {
+  if (B.esgm_pool)
+    return run_directions_esgm(ctx, p, w, h, B, slots, nd, out_slot, reasons, boundary);
   if (B.strip_rows)
     return run_directions_streamed(ctx, p, w, h, B, slots, nd, out_slot, reasons, boundary);
   int rc;
@@ -1154,8 +1470,23 @@ int finish(sgm_context *ctx) {
 
 // GpuStereo.plan (GpuStereo.java:33-84) under the context's plan mode: *rows =
 // 0 for the resident plan, else the strip height of the streamed plan.
-int pick_plan(sgm_context *ctx, const sgm_params *p, int w, int h, int *rows) {
+int pick_plan(sgm_context *ctx, const sgm_params *p, int w, int h, int *rows,
+              int *pool = nullptr) {
   *rows = 0;
+  if (pool) *pool = 0;
+  if (ctx->plan_mode == SGM_PLAN_ESGM) {
+    // GpuStereo.execute (GpuStereo.java:185-197): an explicit esgm request
+    // that no compact plan serves is an error
+    const char *why = pool ? esgm_unsupported(p, w, h) : "the compact plan takes gray Census only";
+    if (!why) {
+      esgm_plan(ctx, p, w, h, rows, pool);
+      if (*pool) return SGM_OK;
+      why = "its buffers exceed the budget";
+    }
+    return fail(SGM_ERR_NO_PLAN,
+                "No compact Census/16-bit GPU plan fits (%s); use --gpu-sgm streamed for exact "
+                "recomputation", why);
+  }
   const bool fits = pair_bytes(p, w, h) <= ctx->budget;
   if (ctx->plan_mode != SGM_PLAN_STREAMED && fits) return SGM_OK;
   if (ctx->plan_mode != SGM_PLAN_RESIDENT) *rows = streamed_rows(ctx, p, w, h);
@@ -1176,12 +1507,12 @@ extern "C" int sgm_match_pair(sgm_context *ctx, const sgm_params *params, const 
   std::lock_guard<std::mutex> g(ctx->lock);
   int rc = validate_params(params, w, h);
   if (rc) return rc;
-  int rows;
-  if ((rc = pick_plan(ctx, params, w, h, &rows))) return rc;
+  int rows, pool = 0;
+  if ((rc = pick_plan(ctx, params, w, h, &rows, &pool))) return rc;
   (void)hipSetDevice(ctx->device);
   PairBuffers B;
-  if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows))) return rc;
-  ctx->last_streamed = rows > 0;
+  if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows, pool))) return rc;
+  ctx->last_streamed = rows > 0 && !pool;
   const size_t n = (size_t)w * h;
   if ((rc = copy_h2d(ctx, B.gray[0], left, n))) return rc;
   if ((rc = copy_h2d(ctx, B.gray[1], right, n))) return rc;
@@ -1305,12 +1636,12 @@ extern "C" int sgm_match_pair_filtered(sgm_context *ctx, const sgm_params *param
   int rc = validate_params(params, w, h);
   if (rc) return rc;
   if ((rc = validate_filters(filters))) return rc;
-  int rows;
-  if ((rc = pick_plan(ctx, params, w, h, &rows))) return rc;
+  int rows, pool = 0;
+  if ((rc = pick_plan(ctx, params, w, h, &rows, &pool))) return rc;
   (void)hipSetDevice(ctx->device);
   PairBuffers B;
-  if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows))) return rc;
-  ctx->last_streamed = rows > 0;
+  if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows, pool))) return rc;
+  ctx->last_streamed = rows > 0 && !pool;
   const size_t n = (size_t)w * h;
   if ((rc = copy_h2d(ctx, B.gray[0], left, n))) return rc;
   if ((rc = copy_h2d(ctx, B.gray[1], right, n))) return rc;
@@ -1424,12 +1755,12 @@ extern "C" int sgm_match_pair_device(sgm_context *ctx, const sgm_params *params,
   std::lock_guard<std::mutex> g(ctx->lock);
   int rc = validate_params(params, w, h);
   if (rc) return rc;
-  int rows;
-  if ((rc = pick_plan(ctx, params, w, h, &rows))) return rc;
+  int rows, pool = 0;
+  if ((rc = pick_plan(ctx, params, w, h, &rows, &pool))) return rc;
   (void)hipSetDevice(ctx->device);
   PairBuffers B;
-  if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows))) return rc;
-  ctx->last_streamed = rows > 0;
+  if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows, pool))) return rc;
+  ctx->last_streamed = rows > 0 && !pool;
   const size_t n = (size_t)w * h;
   unsigned long long *stats = d_stats ? (unsigned long long *)d_stats : B.stats;
   if ((rc = run_pair(ctx, params, w, h, B, (const uint8_t *)d_left, (const uint8_t *)d_right,
@@ -1467,12 +1798,12 @@ int match_one(sgm_context *ctx, const sgm_params *params, const uint64_t *desc_a
   std::lock_guard<std::mutex> g(ctx->lock);
   int rc = validate_params(params, w, h);
   if (rc) return rc;
-  int rows;
-  if ((rc = pick_plan(ctx, params, w, h, &rows))) return rc;
+  int rows, pool = 0;
+  if ((rc = pick_plan(ctx, params, w, h, &rows, desc_a ? nullptr : &pool))) return rc;
   (void)hipSetDevice(ctx->device);
   PairBuffers B;
-  if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows))) return rc;
-  ctx->last_streamed = rows > 0;
+  if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows, pool))) return rc;
+  ctx->last_streamed = rows > 0 && !pool;
   const size_t n = (size_t)w * h;
   // slot d = reverse: the reference image sits in the slot whose cost job has J.reverse = reverse.
   const int ref = reverse ? 1 : 0, tgt = 1 - ref;

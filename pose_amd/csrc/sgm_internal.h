@@ -76,6 +76,9 @@ int launch_cost(sgm_context *ctx, const CostJobs &jobs, int njobs, int w, int h,
 int64_t streamed_pair_bytes(const sgm_params *p, int w, int h, int rows);
 int64_t streamed_host_bytes(const sgm_params *p, int w, int h);
 int streamed_rows(sgm_context *ctx, const sgm_params *p, int w, int h);
+int64_t esgm_pair_bytes(const sgm_params *p, int w, int h, int rows, int pool);
+const char *esgm_unsupported(const sgm_params *p, int w, int h);
+int esgm_plan(sgm_context *ctx, const sgm_params *p, int w, int h, int *rows, int *pool);
 CostArgs cost_args(const sgm_params *p);
 PathArgs path_args(const sgm_params *p, int dir, int njobs, int total_lines, int keep_sum);
 int launch_path(sgm_context *ctx, int R, int sum_bytes, int mode, const PathJobs &jobs,
@@ -99,6 +102,8 @@ struct sgm_context {
   int plan_mode = 0;       // SGM_PLAN_*
   int strip_rows = 64;     // Options.gpuStripRows default
   bool last_streamed = false;
+  int esgm_candidates = 2;            // Options.esgmCandidates default
+  int64_t esgm_last[4] = {0, 0, 0, 0};  // certified / verified pixels per matching direction
   // streamed plan: copy stream and the events of the two strip aggregate slots
   // (0-1: slot computed, 2-3: slot copied), created on first use
   hipStream_t copy_stream = nullptr;

@@ -27,9 +27,10 @@ ERRORS = {
 }
 
 VOLUME_AUTO, VOLUME_RESIDENT, VOLUME_RECOMPUTE = 0, 1, 2
-# aggregation plan: --gpu-sgm auto|resident|streamed (Options.java:32)
-PLAN_AUTO, PLAN_RESIDENT, PLAN_STREAMED = 0, 1, 2
-PLAN_MODES = {"auto": PLAN_AUTO, "resident": PLAN_RESIDENT, "streamed": PLAN_STREAMED}
+# aggregation plan: --gpu-sgm auto|resident|streamed|esgm (Options.java:32,780)
+PLAN_AUTO, PLAN_RESIDENT, PLAN_STREAMED, PLAN_ESGM = 0, 1, 2, 3
+PLAN_MODES = {"auto": PLAN_AUTO, "resident": PLAN_RESIDENT, "streamed": PLAN_STREAMED,
+              "esgm": PLAN_ESGM}
 COSTS = {"census": 0, "ad-census": 1, "census-gradient": 2, "zncc": 3, "zncc-fp32": 4}
 
 
@@ -85,7 +86,9 @@ class SgmPlan(ctypes.Structure):
                 ("device_bytes", ctypes.c_int64), ("pair_bytes", ctypes.c_int64),
                 ("budget_bytes", ctypes.c_int64), ("aggregate_bound", ctypes.c_int64),
                 ("streamed", ctypes.c_int32), ("strip_rows", ctypes.c_int32),
-                ("streamed_bytes", ctypes.c_int64), ("host_bytes", ctypes.c_int64)]
+                ("streamed_bytes", ctypes.c_int64), ("host_bytes", ctypes.c_int64),
+                ("esgm_candidates", ctypes.c_int32), ("esgm_rows", ctypes.c_int32),
+                ("esgm_bytes", ctypes.c_int64)]
 
 
 class SgmPair(ctypes.Structure):
@@ -110,7 +113,7 @@ EXPORTS = [
     "sgm_disparity_limit", "sgm_match", "sgm_match_desc", "sgm_match_pair", "sgm_match_pair_device",
     "sgm_match_pair_filtered", "sgm_match_pair_color",
     "sgm_set_keep_volumes", "sgm_read_volumes", "sgm_set_cost_volume", "sgm_set_plan_mode",
-    "sgm_synchronize",
+    "sgm_set_esgm_candidates", "sgm_esgm_counts", "sgm_synchronize",
     "sgm_release_buffers", "sgm_set_profiling",
     "sgm_profile_json", "sgm_stream", "sgm_debug_fail_allocations", "sgm_last_error",
     "sgm_device_alloc", "sgm_device_free", "sgm_memcpy", "sgm_device_copy",
@@ -160,6 +163,8 @@ def lib():
             L.sgm_set_keep_volumes.argtypes = [P, I]
             L.sgm_set_cost_volume.argtypes = [P, I]
             L.sgm_set_plan_mode.argtypes = [P, I, I]
+            L.sgm_set_esgm_candidates.argtypes = [P, I]
+            L.sgm_esgm_counts.argtypes = [P, ctypes.POINTER(I64)]
             L.sgm_read_volumes.argtypes = [P, I, P, P]
             L.sgm_synchronize.argtypes = [P]
             L.sgm_release_buffers.argtypes = [P]
@@ -246,7 +251,8 @@ class Options:
         self.compute = "auto"
         self.gpuDevice = 0
         self.gpuMemoryMb = 0
-        self.gpuSgm = "auto"        # auto | resident | streamed (Options.java:32)
+        self.gpuSgm = "auto"        # auto | resident | streamed | esgm (Options.java:32)
+        self.esgmCandidates = 2     # Options.java:35
         self.censusMode = "gray"    # gray | color-distance (Options.java:31)
         self.gpuStripRows = 64      # Options.java:39
         self.rawPenalties = False  # p1/p2 already in raw scaled units
@@ -514,10 +520,22 @@ class Context:
         check(lib().sgm_set_cost_volume(self.h, int(policy)))
 
     def plan_mode(self, mode, strip_rows=64):
-        """Aggregation plan: PLAN_AUTO, PLAN_RESIDENT or PLAN_STREAMED (or the
-        reference's names "auto", "resident", "streamed") and gpuStripRows."""
+        """Aggregation plan: PLAN_AUTO, PLAN_RESIDENT, PLAN_STREAMED or PLAN_ESGM
+        (or the reference's names "auto", "resident", "streamed", "esgm") and
+        gpuStripRows."""
         mode = PLAN_MODES[mode] if isinstance(mode, str) else int(mode)
         check(lib().sgm_set_plan_mode(self.h, mode, int(strip_rows)))
+
+    def esgm_candidates(self, count):
+        """--esgm-candidates: proposals per path of the compact plan (1..4)."""
+        check(lib().sgm_set_esgm_candidates(self.h, int(count)))
+
+    def esgm_counts(self):
+        """Certified and verified pixels of the last compact run:
+        {"left": (certified, verified), "right": (certified, verified)}."""
+        out = (ctypes.c_int64 * 4)()
+        check(lib().sgm_esgm_counts(self.h, out))
+        return {"left": (out[0], out[1]), "right": (out[2], out[3])}
 
     def read_volumes(self, direction, n, D):
         cost = np.empty(n * D, np.uint16)
@@ -635,9 +653,12 @@ class GpuSgm:
     def configured(o):
         """The session with the options' plan mode (--gpu-sgm, --gpu-strip-rows)."""
         if o.gpuSgm not in PLAN_MODES:
-            raise ValueError("--gpu-sgm must be auto, resident or streamed")
+            raise ValueError("--gpu-sgm must be auto, resident, streamed or esgm")
+        if not 1 <= o.esgmCandidates <= 4:
+            raise ValueError("--esgm-candidates must be 1..4")
         s = GpuSgm.session(o)
         s.plan_mode(o.gpuSgm, o.gpuStripRows)
+        s.esgm_candidates(o.esgmCandidates)
         return s
 
     @staticmethod
