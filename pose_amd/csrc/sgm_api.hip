@@ -222,14 +222,15 @@ int streamed_rows(sgm_context *ctx, const sgm_params *p, int w, int h) {
 // 19-47, 174-182): whole-image inputs and outputs of both images, per matching
 // direction the proposals (2 B), their sums (6 B) per proposal, the off-proposal
 // bound, verification slot, pixel list and winner summary (2 + 4 + 4 + 12 B per
-// pixel), one strip's uint16 cost volume and two carry rows; plus one pixel of
+// pixel), one strip's uint16 cost volume and six carry rows (three carried
+// paths per pass, ping-pong); plus one pixel of
 // verification per direction (the group size grows with the budget left).
 int64_t esgm_pair_bytes(const sgm_params *p, int w, int h, int rows, int pool) {
   const int64_t n = (int64_t)w * h, Dp = padded_disparities(p->disparities);
   const int64_t C = (int64_t)p->directions * pool;
   const int sb = std::max(aggregate_bytes(p), 2);
   const int64_t inputs = direction_bytes(p, w, h, false) - n * Dp * sb;
-  const int64_t per = n * (C * 8 + 2 + 4 + 4 + 12) + (int64_t)w * rows * Dp * 2 + 2 * (int64_t)w * Dp * 4;
+  const int64_t per = n * (C * 8 + 2 + 4 + 4 + 12) + (int64_t)w * rows * Dp * 2 + 6 * (int64_t)w * Dp * 4;
   return 2 * inputs + 2 * per + n * (2 + 2 + 2 + 4) + 2 * Dp * 2 + 512;
 }
 
@@ -819,7 +820,7 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     ENSURE("esgm_pixels", 2 * n * 4);
     ENSURE("esgm_summary", 2 * n * 12);
     ENSURE("esgm_strip_cost", 2 * (size_t)w * rows * Dp * 2);
-    ENSURE("esgm_carry", 2 * 2 * (size_t)w * Dp * 4);
+    ENSURE("esgm_carry", 2 * 6 * (size_t)w * Dp * 4);
     ENSURE("esgm_count", 256);
     B.e_cand = (uint16_t *)ctx->buffers["esgm_cand"].ptr;
     B.e_sums = (uint16_t *)ctx->buffers["esgm_sums"].ptr;
@@ -1154,30 +1155,44 @@ int run_directions_esgm(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     S.reverse = d;
   }
   if (nd == 1) sj.job[1] = sj.job[0], sj.job[1].hi = 0;
+  // pass 0 runs the paths with dy >= 0 top-down, pass 1 the others bottom-up;
+  // up to three carried paths per pass, each with a ping-pong pair of carry rows
+  static const int dirs[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
   auto sweep = [&](int phase) -> int {
-    for (int dir = 0; dir < p->directions; dir++) {
-      PathArgs A = path_args(p, dir, nd, 0, 0);
-      const CompactArgs E{C, pool, dir};
+    for (int pass = 0; pass < 2; pass++) {
       for (int ord = 0; ord < strips; ord++) {
-        const int s = A.dy < 0 ? strips - 1 - ord : ord;
+        const int s = pass ? strips - 1 - ord : ord;
         const int y0 = s * rows, act = std::min(rows, h - y0);
-        for (int i = 0; i < nd; i++) cj.job[i].y0 = y0;
-        int rc2;
-        if ((rc2 = launch_cost(ctx, cj, nd, w, h, Dp, p, act, true))) return rc2;
-        kj.split = segments_of(A.cls, w, act);
-        A.total_lines = nd * kj.split;
-        for (int i = 0; i < nd; i++) {
-          CompactJob &K = kj.job[i];
-          K.y0 = y0;
-          K.rows = act;
-          K.carry_in = K.carry_out = nullptr;
-          if (A.dy != 0) {
-            int *base = B.e_carry + (size_t)i * 2 * carry_row;
-            K.carry_in = ord > 0 ? base + (size_t)(ord & 1) * carry_row : nullptr;
-            K.carry_out = ord < strips - 1 ? base + (size_t)((ord + 1) & 1) * carry_row : nullptr;
+        bool costed = false;
+        int carried = 0;
+        for (int dir = 0; dir < p->directions; dir++) {
+          if ((dirs[dir][1] < 0) != (pass == 1)) continue;
+          int rc2;
+          if (!costed) {
+            for (int i = 0; i < nd; i++) cj.job[i].y0 = y0;
+            if ((rc2 = launch_cost(ctx, cj, nd, w, h, Dp, p, act, true))) return rc2;
+            costed = true;
           }
+          PathArgs A = path_args(p, dir, nd, 0, 0);
+          const CompactArgs E{C, pool, dir};
+          kj.split = segments_of(A.cls, w, act);
+          A.total_lines = nd * kj.split;
+          for (int i = 0; i < nd; i++) {
+            CompactJob &K = kj.job[i];
+            K.y0 = y0;
+            K.rows = act;
+            K.carry_in = K.carry_out = nullptr;
+            if (A.dy != 0) {
+              int *base = B.e_carry + (size_t)i * 6 * carry_row;
+              K.carry_in = ord > 0 ? base + ((size_t)(ord & 1) * 3 + carried) * carry_row : nullptr;
+              K.carry_out = ord < strips - 1
+                                ? base + ((size_t)((ord + 1) & 1) * 3 + carried) * carry_row
+                                : nullptr;
+            }
+          }
+          if (A.dy != 0) carried++;
+          if ((rc2 = launch_compact(ctx, R, phase, kj, A, E, Dp))) return rc2;
         }
-        if ((rc2 = launch_compact(ctx, R, phase, kj, A, E, Dp))) return rc2;
       }
     }
     return SGM_OK;

@@ -15,8 +15,10 @@
 // selected map, reasons and counters are therefore those of ordinary SGM.
 //
 // Sweeps run over horizontal strips with int32 path carries at the seams
-// (sgm_strips.h), so the device holds one strip's cost volume at a time.  One
-// wave64 per line segment; lane l holds disparities l*R .. l*R+R-1.
+// (sgm_strips.h), so the device holds one strip's cost volume at a time: a
+// top-down pass over the strips runs the paths with dy >= 0, a bottom-up pass
+// the others, each strip's cost computed once per pass.  One wave64 per line
+// segment; lane l holds disparities l*R .. l*R+R-1.
 #pragma once
 
 #include "sgm_strips.h"
@@ -76,11 +78,29 @@ __device__ __forceinline__ void compact_segment(const CompactJob &J, const PathA
   }
 
   CostVec<R> cb[PF];
+  // sum pass: the proposals, the 3C sums (two lane chunks) and the bound of
+  // the pixel PF steps ahead are loaded with its costs
+  int pcand[PF], psum0[PF], psum1[PF], pout[PF];
 #pragma unroll
-  for (int j = 0; j < PF; j++) cb[j] = CostVec<R>{};
+  for (int j = 0; j < PF; j++) {
+    cb[j] = CostVec<R>{};
+    pcand[j] = 65535;
+    psum0[j] = psum1[j] = pout[j] = 0;
+  }
+  auto prefetch = [&](int j, ptrdiff_t pp) {
+    if (lane_mem) cb[j].load(cost + pp * Dp + kbase);
+    if constexpr (PH == ESGM_SUMS) {
+      const size_t g = (size_t)(img + pp);
+      const uint16_t *row = J.sums + g * (size_t)(3 * C);
+      pcand[j] = lane < C ? (int)J.cand[g * C + lane] : 65535;
+      psum0[j] = lane < 3 * C ? (int)row[lane] : 0;
+      psum1[j] = lane + 64 < 3 * C ? (int)row[lane + 64] : 0;
+      pout[j] = J.outside[g];
+    }
+  };
 #pragma unroll
   for (int j = 0; j < PF; j++)
-    if (j < len && lane_mem) cb[j].load(cost + (p0 + (ptrdiff_t)j * step_off) * Dp + kbase);
+    if (j < len) prefetch(j, p0 + (ptrdiff_t)j * step_off);
 
   int prev[R];
 #pragma unroll
@@ -159,8 +179,8 @@ __device__ __forceinline__ void compact_segment(const CompactJob &J, const PathA
         L[j] = v;
         lmin = min(lmin, v);
       }
-      if (s + PF < len && lane_mem)
-        cb[jj].load(cost + (pp + (ptrdiff_t)PF * step_off) * Dp + kbase);
+      const int prop = pcand[jj], sum0 = psum0[jj], sum1 = psum1[jj], prior = pout[jj];
+      if (s + PF < len) prefetch(jj, pp + (ptrdiff_t)PF * step_off);
       minp = wave_min(lmin);
 #pragma unroll
       for (int j = 0; j < R; j++) prev[j] = L[j];
@@ -195,11 +215,10 @@ __device__ __forceinline__ void compact_segment(const CompactJob &J, const PathA
 #pragma unroll
         for (int j = 0; j < R; j++)
           if (lane_mem) lds_l[kbase + j] = L[j];
-        const int cv = lane < C ? (int)J.cand[gp * C + lane] : 65535;
-        if (cv < D) {
+        if (prop < D) {
 #pragma unroll
           for (int o = -1; o <= 1; o++) {
-            const int kk = cv + o;
+            const int kk = prop + o;
             if (kk >= 0 && kk < D) lds_cov[kk] = s;
           }
         }
@@ -207,18 +226,18 @@ __device__ __forceinline__ void compact_segment(const CompactJob &J, const PathA
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // entry t of the pixel's 3C sums: proposal t / 3, offset t % 3 - 1
-        const int prop0 = __builtin_amdgcn_ds_bpermute((lane / 3) * 4, cv);
-        const int prop1 = __builtin_amdgcn_ds_bpermute(((lane + 64) / 3) * 4, cv);
+        const int prop0 = __builtin_amdgcn_ds_bpermute((lane / 3) * 4, prop);
+        const int prop1 = __builtin_amdgcn_ds_bpermute(((lane + 64) / 3) * 4, prop);
         uint16_t *row = J.sums + gp * (size_t)(3 * C);
         {
           const int t = lane;
           const int k = prop0 + t % 3 - 1;
-          if (t < 3 * C && k >= 0 && k < D) row[t] = (uint16_t)(row[t] + lds_l[k]);
+          if (t < 3 * C && k >= 0 && k < D) row[t] = (uint16_t)(sum0 + lds_l[k]);
         }
         {
           const int t = lane + 64;
           const int k = prop1 + t % 3 - 1;
-          if (t < 3 * C && k >= 0 && k < D) row[t] = (uint16_t)(row[t] + lds_l[k]);
+          if (t < 3 * C && k >= 0 && k < D) row[t] = (uint16_t)(sum1 + lds_l[k]);
         }
         int lower = 0x7FFFFFFF;
 #pragma unroll
@@ -227,10 +246,8 @@ __device__ __forceinline__ void compact_segment(const CompactJob &J, const PathA
           if (k < D && c[j] != A.inv_code && lds_cov[k] != s) lower = min(lower, L[j]);
         }
         lower = wave_min(lower);
-        if (lane == 0) {
-          const int prior = J.outside[gp];
+        if (lane == 0)
           J.outside[gp] = (uint16_t)(lower == 0x7FFFFFFF ? 65535 : min(65535, prior + lower));
-        }
         // the next step rewrites the LDS only after every lane has read it
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
