@@ -1042,34 +1042,47 @@ int run_directions_streamed(sgm_context *ctx, const sgm_params *p, int w, int h,
   return SGM_OK;
 }
 
-template <int R, int PH>
+template <int R, int PH, bool HREC>
 void launch_compact_t(sgm_context *ctx, const CompactJobs &J, const PathArgs &A,
                       const CompactArgs &E, int Dp) {
   // two Dp-int LDS rows per wave, at most 32 KiB per workgroup
   const int waves = Dp <= 1024 ? 4 : Dp <= 2048 ? 2 : 1;
   const unsigned blocks = (unsigned)((A.total_lines + waves - 1) / waves);
   const size_t lds = (size_t)waves * 2 * Dp * sizeof(int);
-  compact_kernel<R, PH><<<blocks, waves * 64, lds, ctx->stream>>>(J, A, E);
+  compact_kernel<R, PH, HREC><<<blocks, waves * 64, lds, ctx->stream>>>(J, A, E);
 }
 
-int launch_compact(sgm_context *ctx, int R, int phase, const CompactJobs &J, const PathArgs &A,
-                   const CompactArgs &E, int Dp) {
-  ProfileScope ps(ctx, "compact");
-#define CASE_COMPACT(RR)                                                        \
-  case RR:                                                                      \
-    if (phase == ESGM_CANDIDATES)                                               \
-      launch_compact_t<RR, ESGM_CANDIDATES>(ctx, J, A, E, Dp);                  \
-    else if (phase == ESGM_SUMS)                                                \
-      launch_compact_t<RR, ESGM_SUMS>(ctx, J, A, E, Dp);                        \
-    else                                                                        \
-      launch_compact_t<RR, ESGM_VERIFY>(ctx, J, A, E, Dp);                      \
-    break;
-  switch (R) {
-    CASE_COMPACT(2) CASE_COMPACT(4) CASE_COMPACT(8) CASE_COMPACT(16) CASE_COMPACT(32)
-    CASE_COMPACT(64)
-    default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
+template <int R, bool HREC>
+void launch_compact_phase(sgm_context *ctx, int phase, const CompactJobs &J, const PathArgs &A,
+                          const CompactArgs &E, int Dp) {
+  if (phase == ESGM_CANDIDATES) launch_compact_t<R, ESGM_CANDIDATES, HREC>(ctx, J, A, E, Dp);
+  else if (phase == ESGM_SUMS) launch_compact_t<R, ESGM_SUMS, HREC>(ctx, J, A, E, Dp);
+  else launch_compact_t<R, ESGM_VERIFY, HREC>(ctx, J, A, E, Dp);
+}
+
+// hrec: whole-image horizontal lines with the census cost computed on the fly
+// (1-word windows, R <= 8); otherwise a strip of the cost volume.
+int launch_compact(sgm_context *ctx, int R, int phase, bool hrec, const CompactJobs &J,
+                   const PathArgs &A, const CompactArgs &E, int Dp) {
+  ProfileScope ps(ctx, hrec ? "compact horizontal" : "compact");
+  if (hrec) {
+    switch (R) {
+      case 2: launch_compact_phase<2, true>(ctx, phase, J, A, E, Dp); break;
+      case 4: launch_compact_phase<4, true>(ctx, phase, J, A, E, Dp); break;
+      case 8: launch_compact_phase<8, true>(ctx, phase, J, A, E, Dp); break;
+      default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
+    }
+  } else {
+    switch (R) {
+      case 2: launch_compact_phase<2, false>(ctx, phase, J, A, E, Dp); break;
+      case 4: launch_compact_phase<4, false>(ctx, phase, J, A, E, Dp); break;
+      case 8: launch_compact_phase<8, false>(ctx, phase, J, A, E, Dp); break;
+      case 16: launch_compact_phase<16, false>(ctx, phase, J, A, E, Dp); break;
+      case 32: launch_compact_phase<32, false>(ctx, phase, J, A, E, Dp); break;
+      case 64: launch_compact_phase<64, false>(ctx, phase, J, A, E, Dp); break;
+      default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
+    }
   }
-#undef CASE_COMPACT
   return launch_check("compact sweep kernel");
 }
 
@@ -1135,6 +1148,12 @@ int run_directions_esgm(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     K.h = h;
     K.D = D;
     K.Dp = Dp;
+    K.desc_a = B.desc[d];
+    K.desc_b = B.desc[o];
+    K.minD = p->min_disparity;
+    K.reverse = d;
+    K.rx = p->window_w / 2;
+    K.ry = p->window_h / 2;
     SummaryJob &S = sj.job[i];
     S.cand = K.cand;
     S.sums = K.sums;
@@ -1155,10 +1174,30 @@ int run_directions_esgm(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     S.reverse = d;
   }
   if (nd == 1) sj.job[1] = sj.job[0], sj.job[1].hi = 0;
-  // pass 0 runs the paths with dy >= 0 top-down, pass 1 the others bottom-up;
-  // up to three carried paths per pass, each with a ping-pong pair of carry rows
+  // With 1-word windows the horizontal paths run over the whole image at once
+  // (2H lines instead of 2 x strip rows per launch), computing their census
+  // costs on the fly.  The strip passes: pass 0 runs the (other) paths with
+  // dy >= 0 top-down, pass 1 the paths with dy < 0 bottom-up; up to three
+  // carried paths per pass, each with a ping-pong pair of carry rows.
   static const int dirs[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
+  const bool hrec = words_of(p) == 1 && R <= 8;
   auto sweep = [&](int phase) -> int {
+    if (hrec) {
+      for (int dir = 0; dir < 2; dir++) {
+        PathArgs A = path_args(p, dir, nd, 0, 0);
+        const CompactArgs E{C, pool, dir};
+        kj.split = h;
+        A.total_lines = nd * h;
+        for (int i = 0; i < nd; i++) {
+          CompactJob &K = kj.job[i];
+          K.y0 = 0;
+          K.rows = h;
+          K.carry_in = K.carry_out = nullptr;
+        }
+        int rc2;
+        if ((rc2 = launch_compact(ctx, R, phase, true, kj, A, E, Dp))) return rc2;
+      }
+    }
     for (int pass = 0; pass < 2; pass++) {
       for (int ord = 0; ord < strips; ord++) {
         const int s = pass ? strips - 1 - ord : ord;
@@ -1167,6 +1206,7 @@ int run_directions_esgm(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
         int carried = 0;
         for (int dir = 0; dir < p->directions; dir++) {
           if ((dirs[dir][1] < 0) != (pass == 1)) continue;
+          if (hrec && dirs[dir][1] == 0) continue;
           int rc2;
           if (!costed) {
             for (int i = 0; i < nd; i++) cj.job[i].y0 = y0;
@@ -1191,7 +1231,7 @@ int run_directions_esgm(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
             }
           }
           if (A.dy != 0) carried++;
-          if ((rc2 = launch_compact(ctx, R, phase, kj, A, E, Dp))) return rc2;
+          if ((rc2 = launch_compact(ctx, R, phase, false, kj, A, E, Dp))) return rc2;
         }
       }
     }

@@ -40,6 +40,11 @@ struct CompactJob {
   uint16_t *verify;          // [slot - lo][Dp] exact aggregates of the group's pixels
   int lo, hi;                // verification slots of this group
   int w, h, D, Dp, y0, rows;
+  // horizontal lines over the whole image with the census cost computed on
+  // the fly (1-word windows): reference / target descriptors (bit 63 = the
+  // texture & valid flag of the pixel), the disparity mapping and the window
+  const uint64_t *desc_a, *desc_b;
+  int minD, reverse, rx, ry;
 };
 
 struct CompactJobs {
@@ -55,7 +60,7 @@ struct CompactArgs {
 
 // The LDS of one wave: the path costs of the current pixel and the coverage
 // stamps of the proposals, Dp ints each.
-template <int R, int PH>
+template <int R, int PH, bool HREC>
 __device__ __forceinline__ void compact_segment(const CompactJob &J, const PathArgs &A,
                                                 const CompactArgs &E, int line,
                                                 const int *p2_table, int *lds_l, int *lds_cov) {
@@ -81,14 +86,34 @@ __device__ __forceinline__ void compact_segment(const CompactJob &J, const PathA
   // sum pass: the proposals, the 3C sums (two lane chunks) and the bound of
   // the pixel PF steps ahead are loaded with its costs
   int pcand[PF], psum0[PF], psum1[PF], pout[PF];
+  // HREC: target descriptors of the pixel PF steps ahead (index clamped into
+  // the row, in-range bits in pok)
+  uint64_t pdesc[HREC ? PF : 1][HREC ? R : 1];
+  uint32_t pok[HREC ? PF : 1];
 #pragma unroll
   for (int j = 0; j < PF; j++) {
     cb[j] = CostVec<R>{};
     pcand[j] = 65535;
     psum0[j] = psum1[j] = pout[j] = 0;
   }
-  auto prefetch = [&](int j, ptrdiff_t pp) {
-    if (lane_mem) cb[j].load(cost + pp * Dp + kbase);
+  auto prefetch = [&](int j, int st) {
+    const ptrdiff_t pp = p0 + (ptrdiff_t)st * step_off;
+    if constexpr (HREC) {
+      const int xs = x + st * A.dx;
+      const size_t rowb = (size_t)y * J.w;
+      uint32_t ok = 0;
+#pragma unroll
+      for (int jr = 0; jr < R; jr++) {
+        const int d = J.minD + kbase + jr;
+        const int t = J.reverse ? xs + d : xs - d;
+        const bool in = kbase + jr < D && t >= J.rx && t < J.w - J.rx;
+        ok |= in ? 1u << jr : 0u;
+        pdesc[j][jr] = J.desc_b[rowb + (size_t)min(max(t, 0), J.w - 1)];
+      }
+      pok[j] = ok;
+    } else {
+      if (lane_mem) cb[j].load(cost + pp * Dp + kbase);
+    }
     if constexpr (PH == ESGM_SUMS) {
       const size_t g = (size_t)(img + pp);
       const uint16_t *row = J.sums + g * (size_t)(3 * C);
@@ -100,7 +125,7 @@ __device__ __forceinline__ void compact_segment(const CompactJob &J, const PathA
   };
 #pragma unroll
   for (int j = 0; j < PF; j++)
-    if (j < len) prefetch(j, p0 + (ptrdiff_t)j * step_off);
+    if (j < len) prefetch(j, j);
 
   int prev[R];
 #pragma unroll
@@ -123,12 +148,17 @@ __device__ __forceinline__ void compact_segment(const CompactJob &J, const PathA
       minp = wave_min(lm);
     }
   }
-  LineChunk gch, mch, sch;
+  LineChunk gch, mch, sch, alo, ahi;
   if (lane < len) {
     const ptrdiff_t pl = img + p0 + (ptrdiff_t)lane * step_off;
     gch.next = J.gray[pl];
     mch.next = J.mask[pl];
     if (PH == ESGM_VERIFY) sch.next = J.slot[pl];
+    if (HREC) {
+      const uint64_t a = J.desc_a[pl];
+      alo.next = (int)(uint32_t)a;
+      ahi.next = (int)(uint32_t)(a >> 32);
+    }
   }
 
   for (int s0 = 0; s0 < len; s0 += PF) {
@@ -142,12 +172,19 @@ __device__ __forceinline__ void compact_segment(const CompactJob &J, const PathA
         gch.cur = gch.next;
         mch.cur = mch.next;
         sch.cur = sch.next;
+        alo.cur = alo.next;
+        ahi.cur = ahi.next;
         const int si = s + 64 + lane;
         if (si < len) {
           const ptrdiff_t pl = img + p0 + (ptrdiff_t)si * step_off;
           gch.next = J.gray[pl];
           mch.next = J.mask[pl];
           if (PH == ESGM_VERIFY) sch.next = J.slot[pl];
+          if (HREC) {
+            const uint64_t a = J.desc_a[pl];
+            alo.next = (int)(uint32_t)a;
+            ahi.next = (int)(uint32_t)(a >> 32);
+          }
         }
       }
       const int g_cur = __builtin_amdgcn_readlane(gch.cur, s & 63);
@@ -158,8 +195,26 @@ __device__ __forceinline__ void compact_segment(const CompactJob &J, const PathA
       m_prev = m_cur;
 
       int c[R], L[R];
+      if constexpr (HREC) {
+        // census Hamming cost with the `costs` kernel's validity test
+        // (GpuStereo.java:706-719): reference pixel interior and flagged,
+        // target inside [rx, w - rx) and flagged; 512 otherwise
+        const uint32_t a_lo = (uint32_t)__builtin_amdgcn_readlane(alo.cur, s & 63);
+        const uint32_t a_hi = (uint32_t)__builtin_amdgcn_readlane(ahi.cur, s & 63);
+        const int xs = x + s * A.dx;
+        const bool pix = xs >= J.rx && xs < J.w - J.rx && y >= J.ry && y < J.h - J.ry && m_cur;
 #pragma unroll
-      for (int j = 0; j < R; j++) c[j] = cb[jj].get(j);
+        for (int j = 0; j < R; j++) {
+          const uint64_t e = pdesc[jj][j];
+          const uint32_t e_lo = (uint32_t)e, e_hi = (uint32_t)(e >> 32);
+          const int ham = __builtin_popcount(e_lo ^ a_lo) +
+                          __builtin_popcount((e_hi ^ a_hi) & 0x7FFFFFFFu);
+          c[j] = pix && ((pok[jj] >> j) & 1u) && (e_hi >> 31) ? ham : A.inv_code;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < R; j++) c[j] = cb[jj].get(j);
+      }
       const int left_nb = dpp_shr1(prev[R - 1], kInf);
       const int right_nb = dpp_shl1(prev[0], kInf);
       int lmin = kInf;
@@ -180,7 +235,7 @@ __device__ __forceinline__ void compact_segment(const CompactJob &J, const PathA
         lmin = min(lmin, v);
       }
       const int prop = pcand[jj], sum0 = psum0[jj], sum1 = psum1[jj], prior = pout[jj];
-      if (s + PF < len) prefetch(jj, pp + (ptrdiff_t)PF * step_off);
+      if (s + PF < len) prefetch(jj, s + PF);
       minp = wave_min(lmin);
 #pragma unroll
       for (int j = 0; j < R; j++) prev[j] = L[j];
@@ -277,7 +332,7 @@ __device__ __forceinline__ void compact_segment(const CompactJob &J, const PathA
 }
 
 // Dynamic LDS: 2 * Dp ints per wave.
-template <int R, int PH>
+template <int R, int PH, bool HREC>
 __global__ __launch_bounds__(256) void compact_kernel(CompactJobs jobs, PathArgs A, CompactArgs E) {
   __shared__ int p2_table[256];
   extern __shared__ int compact_lds[];
@@ -288,8 +343,10 @@ __global__ __launch_bounds__(256) void compact_kernel(CompactJobs jobs, PathArgs
   if (line >= A.total_lines) return;
   const int Dp = jobs.job[0].Dp;
   int *lds_l = compact_lds + (size_t)wave * 2 * Dp, *lds_cov = lds_l + Dp;
-  if (line < jobs.split) compact_segment<R, PH>(jobs.job[0], A, E, line, p2_table, lds_l, lds_cov);
-  else compact_segment<R, PH>(jobs.job[1], A, E, line - jobs.split, p2_table, lds_l, lds_cov);
+  if (line < jobs.split)
+    compact_segment<R, PH, HREC>(jobs.job[0], A, E, line, p2_table, lds_l, lds_cov);
+  else
+    compact_segment<R, PH, HREC>(jobs.job[1], A, E, line - jobs.split, p2_table, lds_l, lds_cov);
 }
 
 // ---------------------------------------------------------------------------

@@ -44,17 +44,18 @@ def test_upgrade_fixture_esgm(gpu_context, directions):
     assert certified > 0 and verified > 0
 
 
-@pytest.mark.parametrize("case", range(5))
+@pytest.mark.parametrize("case", range(6))
 def test_esgm_equals_resident(gpu_context, case):
     """Lane groups R = 2..16, signed ranges, rectangular windows, edge-adaptive
-    P2 on and off."""
-    D = [3, 64, 130, 300, 700][case]
-    w, h = [60, 150, 170, 220, 760][case], [30, 61, 68, 47, 40][case]
+    P2 on and off; horizontal paths with on-the-fly census costs (1-word
+    windows, R <= 8) and from the strip cost volume (R = 16, 2-word 11x11)."""
+    D = [3, 64, 130, 300, 700, 96][case]
+    w, h = [60, 150, 170, 220, 760, 140][case], [30, 61, 68, 47, 40, 52][case]
     left, right, _ = synth.stereo_pair(w, h, D, -4 if case % 2 else 0, seed=400 + case)
-    p = sgm.SgmParams(window_w=[3, 9, 5, 7, 9][case], window_h=[3, 7, 5, 5, 7][case],
+    p = sgm.SgmParams(window_w=[3, 9, 5, 7, 9, 11][case], window_h=[3, 7, 5, 5, 7, 11][case],
                       disparities=D, min_disparity=-4 if case % 2 else 0, cost=0,
-                      p1=[4, 8, 10, 8, 10][case], p2=[30, 96, 120, 96, 120][case],
-                      edge_scale=[0, 32, 32, 0, 32][case], directions=[4, 8, 8, 4, 8][case],
+                      p1=[4, 8, 10, 8, 10, 20][case], p2=[30, 96, 120, 96, 120, 120][case],
+                      edge_scale=[0, 32, 32, 0, 32, 32][case], directions=[4, 8, 8, 4, 8, 8][case],
                       aggregate_bytes=0, census_weight=500, ad_truncation=30,
                       gradient_truncation=60, reserved=0, uniqueness=0.03, lr_threshold=1.0)
     want = gpu_context.match_pair(p, left, right)
