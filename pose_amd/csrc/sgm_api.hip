@@ -225,12 +225,17 @@ int streamed_rows(sgm_context *ctx, const sgm_params *p, int w, int h) {
 // pixel), one strip's uint16 cost volume and six carry rows (three carried
 // paths per pass, ping-pong); plus one pixel of
 // verification per direction (the group size grows with the budget left).
+// With 1-word windows and R <= 8 the sweeps compute their census costs on the
+// fly over the whole image: no strip cost volume, no carries.
+bool esgm_computed(const sgm_params *p) { return words_of(p) == 1 && path_lanes(p) <= 8; }
+
 int64_t esgm_pair_bytes(const sgm_params *p, int w, int h, int rows, int pool) {
   const int64_t n = (int64_t)w * h, Dp = padded_disparities(p->disparities);
   const int64_t C = (int64_t)p->directions * pool;
   const int sb = std::max(aggregate_bytes(p), 2);
   const int64_t inputs = direction_bytes(p, w, h, false) - n * Dp * sb;
-  const int64_t per = n * (C * 8 + 2 + 4 + 4 + 12) + (int64_t)w * rows * Dp * 2 + 6 * (int64_t)w * Dp * 4;
+  const int64_t strip = esgm_computed(p) ? 0 : (int64_t)w * rows * Dp * 2 + 6 * (int64_t)w * Dp * 4;
+  const int64_t per = n * (C * 8 + 2 + 4 + 4 + 12) + strip;
   return 2 * inputs + 2 * per + n * (2 + 2 + 2 + 4) + 2 * Dp * 2 + 512;
 }
 
@@ -248,13 +253,15 @@ const char *esgm_unsupported(const sgm_params *p, int w, int h) {
 }
 
 // GpuCandidateStereo.plan: proposals per path from the configured count down
-// to 1, strip rows from gpuStripRows halving, first combination that fits.
+// to 1, strip rows from gpuStripRows halving, first combination that fits
+// (rows = h where the costs are computed on the fly: no strips).
 int esgm_plan(sgm_context *ctx, const sgm_params *p, int w, int h, int *rows, int *pool) {
   *rows = 0;
   *pool = 0;
   if (esgm_unsupported(p, w, h)) return SGM_OK;
+  const int top = esgm_computed(p) ? h : std::min(h, ctx->strip_rows);
   for (int q = ctx->esgm_candidates; q >= 1; q--)
-    for (int r = std::min(h, ctx->strip_rows); r >= 1; r /= 2)
+    for (int r = top; r >= 1; r = esgm_computed(p) ? 0 : r / 2)
       if (esgm_pair_bytes(p, w, h, r, q) <= ctx->budget) {
         *rows = r;
         *pool = q;
@@ -819,8 +826,9 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     ENSURE("esgm_slot", 2 * n * 4);
     ENSURE("esgm_pixels", 2 * n * 4);
     ENSURE("esgm_summary", 2 * n * 12);
-    ENSURE("esgm_strip_cost", 2 * (size_t)w * rows * Dp * 2);
-    ENSURE("esgm_carry", 2 * 6 * (size_t)w * Dp * 4);
+    const bool computed = esgm_computed(p);
+    ENSURE("esgm_strip_cost", computed ? 8 : 2 * (size_t)w * rows * Dp * 2);
+    ENSURE("esgm_carry", computed ? 8 : 2 * 6 * (size_t)w * Dp * 4);
     ENSURE("esgm_count", 256);
     B.e_cand = (uint16_t *)ctx->buffers["esgm_cand"].ptr;
     B.e_sums = (uint16_t *)ctx->buffers["esgm_sums"].ptr;
@@ -1060,11 +1068,14 @@ void launch_compact_phase(sgm_context *ctx, int phase, const CompactJobs &J, con
   else launch_compact_t<R, ESGM_VERIFY, HREC>(ctx, J, A, E, Dp);
 }
 
-// hrec: whole-image horizontal lines with the census cost computed on the fly
-// (1-word windows, R <= 8); otherwise a strip of the cost volume.
+// hrec: whole-image lines with the census cost computed on the fly (1-word
+// windows, R <= 8); otherwise lines of a strip of the cost volume.
 int launch_compact(sgm_context *ctx, int R, int phase, bool hrec, const CompactJobs &J,
                    const PathArgs &A, const CompactArgs &E, int Dp) {
-  ProfileScope ps(ctx, hrec ? "compact horizontal" : "compact");
+  static const char *names[2][3] = {
+      {"compact proposals (strips)", "compact sums (strips)", "compact verify (strips)"},
+      {"compact proposals", "compact sums", "compact verify"}};
+  ProfileScope ps(ctx, names[hrec ? 1 : 0][phase]);
   if (hrec) {
     switch (R) {
       case 2: launch_compact_phase<2, true>(ctx, phase, J, A, E, Dp); break;
@@ -1113,7 +1124,9 @@ int run_directions_esgm(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
   const int D = p->disparities, Dp = padded_disparities(D), R = path_lanes(p);
   const int rows = B.esgm_rows, pool = B.esgm_pool, C = p->directions * pool;
   const int strips = (h + rows - 1) / rows;
-  const size_t n = (size_t)w * h, cells = (size_t)w * rows * Dp, carry_row = (size_t)w * Dp;
+  const bool computed = esgm_computed(p);
+  const size_t n = (size_t)w * h, cells = computed ? 0 : (size_t)w * rows * Dp;
+  const size_t carry_row = (size_t)w * Dp;
   CostJobs cj;
   memset(&cj, 0, sizeof cj);
   CompactJobs kj;
@@ -1174,20 +1187,19 @@ int run_directions_esgm(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     S.reverse = d;
   }
   if (nd == 1) sj.job[1] = sj.job[0], sj.job[1].hi = 0;
-  // With 1-word windows the horizontal paths run over the whole image at once
-  // (2H lines instead of 2 x strip rows per launch), computing their census
-  // costs on the fly.  The strip passes: pass 0 runs the (other) paths with
-  // dy >= 0 top-down, pass 1 the paths with dy < 0 bottom-up; up to three
-  // carried paths per pass, each with a ping-pong pair of carry rows.
+  // With 1-word windows and R <= 8 every path runs over the whole image in one
+  // launch, computing its census costs on the fly.  Otherwise strip passes:
+  // pass 0 runs the paths with dy >= 0 top-down, pass 1 the paths with dy < 0
+  // bottom-up; up to three carried paths per pass, each with a ping-pong pair
+  // of carry rows.
   static const int dirs[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
-  const bool hrec = words_of(p) == 1 && R <= 8;
   auto sweep = [&](int phase) -> int {
-    if (hrec) {
-      for (int dir = 0; dir < 2; dir++) {
+    if (computed) {
+      for (int dir = 0; dir < p->directions; dir++) {
         PathArgs A = path_args(p, dir, nd, 0, 0);
         const CompactArgs E{C, pool, dir};
-        kj.split = h;
-        A.total_lines = nd * h;
+        kj.split = lines_of(A.cls, w, h);
+        A.total_lines = nd * kj.split;
         for (int i = 0; i < nd; i++) {
           CompactJob &K = kj.job[i];
           K.y0 = 0;
@@ -1197,6 +1209,7 @@ int run_directions_esgm(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
         int rc2;
         if ((rc2 = launch_compact(ctx, R, phase, true, kj, A, E, Dp))) return rc2;
       }
+      return SGM_OK;
     }
     for (int pass = 0; pass < 2; pass++) {
       for (int ord = 0; ord < strips; ord++) {
@@ -1206,7 +1219,6 @@ int run_directions_esgm(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
         int carried = 0;
         for (int dir = 0; dir < p->directions; dir++) {
           if ((dirs[dir][1] < 0) != (pass == 1)) continue;
-          if (hrec && dirs[dir][1] == 0) continue;
           int rc2;
           if (!costed) {
             for (int i = 0; i < nd; i++) cj.job[i].y0 = y0;

@@ -14,11 +14,13 @@
 // pass that accumulates the full aggregate vector of only those pixels.  The
 // selected map, reasons and counters are therefore those of ordinary SGM.
 //
-// Sweeps run over horizontal strips with int32 path carries at the seams
-// (sgm_strips.h), so the device holds one strip's cost volume at a time: a
-// top-down pass over the strips runs the paths with dy >= 0, a bottom-up pass
-// the others, each strip's cost computed once per pass.  One wave64 per line
-// segment; lane l holds disparities l*R .. l*R+R-1.
+// With 1-word census windows and R <= 8 every path sweeps the whole image and
+// computes its census costs on the fly from the descriptors (HREC): no cost
+// volume at all.  Otherwise sweeps run over horizontal strips with int32 path
+// carries at the seams (sgm_strips.h), so the device holds one strip's cost
+// volume at a time: a top-down pass over the strips runs the paths with
+// dy >= 0, a bottom-up pass the others, each strip's cost computed once per
+// pass.  One wave64 per line (segment); lane l holds disparities l*R .. l*R+R-1.
 #pragma once
 
 #include "sgm_strips.h"
@@ -40,9 +42,9 @@ struct CompactJob {
   uint16_t *verify;          // [slot - lo][Dp] exact aggregates of the group's pixels
   int lo, hi;                // verification slots of this group
   int w, h, D, Dp, y0, rows;
-  // horizontal lines over the whole image with the census cost computed on
-  // the fly (1-word windows): reference / target descriptors (bit 63 = the
-  // texture & valid flag of the pixel), the disparity mapping and the window
+  // whole-image lines with the census cost computed on the fly (1-word
+  // windows): reference / target descriptors (bit 63 = the texture & valid
+  // flag of the pixel), the disparity mapping and the window
   const uint64_t *desc_a, *desc_b;
   int minD, reverse, rx, ry;
 };
@@ -86,8 +88,8 @@ __device__ __forceinline__ void compact_segment(const CompactJob &J, const PathA
   // sum pass: the proposals, the 3C sums (two lane chunks) and the bound of
   // the pixel PF steps ahead are loaded with its costs
   int pcand[PF], psum0[PF], psum1[PF], pout[PF];
-  // HREC: target descriptors of the pixel PF steps ahead (index clamped into
-  // the row, in-range bits in pok)
+  // HREC (costs computed on the fly): target descriptors of the pixel PF
+  // steps ahead (index clamped into its row, in-range bits in pok)
   uint64_t pdesc[HREC ? PF : 1][HREC ? R : 1];
   uint32_t pok[HREC ? PF : 1];
 #pragma unroll
@@ -99,8 +101,8 @@ __device__ __forceinline__ void compact_segment(const CompactJob &J, const PathA
   auto prefetch = [&](int j, int st) {
     const ptrdiff_t pp = p0 + (ptrdiff_t)st * step_off;
     if constexpr (HREC) {
-      const int xs = x + st * A.dx;
-      const size_t rowb = (size_t)y * J.w;
+      const int xs = x + st * A.dx, ys = y + st * A.dy;
+      const size_t rowb = (size_t)ys * J.w;
       uint32_t ok = 0;
 #pragma unroll
       for (int jr = 0; jr < R; jr++) {
@@ -201,8 +203,8 @@ __device__ __forceinline__ void compact_segment(const CompactJob &J, const PathA
         // target inside [rx, w - rx) and flagged; 512 otherwise
         const uint32_t a_lo = (uint32_t)__builtin_amdgcn_readlane(alo.cur, s & 63);
         const uint32_t a_hi = (uint32_t)__builtin_amdgcn_readlane(ahi.cur, s & 63);
-        const int xs = x + s * A.dx;
-        const bool pix = xs >= J.rx && xs < J.w - J.rx && y >= J.ry && y < J.h - J.ry && m_cur;
+        const int xs = x + s * A.dx, ys = y + s * A.dy;
+        const bool pix = xs >= J.rx && xs < J.w - J.rx && ys >= J.ry && ys < J.h - J.ry && m_cur;
 #pragma unroll
         for (int j = 0; j < R; j++) {
           const uint64_t e = pdesc[jj][j];
