@@ -417,8 +417,12 @@ extern "C" int sgm_disparity_limit(sgm_context *ctx, const sgm_params *params, i
     int mid = (low + high + 1) / 2;
     q.disparities = mid;
     sgm_plan plan;
+    // under the compact plan: the largest range it serves (its verification
+    // runs inside its own footprint)
     bool ok = mid >= 3 && sgm_plan_get(ctx, &q, w, h, &plan) == SGM_OK &&
-              ((plan.resident && ctx->plan_mode != SGM_PLAN_STREAMED) || plan.streamed);
+              (ctx->plan_mode == SGM_PLAN_ESGM
+                   ? plan.esgm_candidates > 0
+                   : (plan.resident && ctx->plan_mode != SGM_PLAN_STREAMED) || plan.streamed);
     if (ok) low = mid;
     else high = mid - 1;
   }

@@ -86,6 +86,19 @@ def test_verification_in_small_groups():
         got, counts = esgm(ctx, p, left, right, rows=64, pool=2, what="small groups")
         assert counts["left"][1] > 6 and counts["right"][1] > 6
         assert_same(got, want, "small verification groups")
+        # GpuStereo.limit under the compact plan: the largest range it serves
+        ctx.plan_mode(sgm.PLAN_ESGM)
+        try:
+            lim = ctx.disparity_limit(p, 120, 50)
+            assert lim >= 48
+            q = sgm.SgmParams(**{f: getattr(p, f) for f, _ in sgm.SgmParams._fields_})
+            q.disparities = lim
+            assert ctx.plan(q, 120, 50)["esgm_candidates"] >= 1
+            if lim < 4096:
+                q.disparities = lim + 1
+                assert ctx.plan(q, 120, 50)["esgm_candidates"] == 0
+        finally:
+            ctx.plan_mode(sgm.PLAN_AUTO)
         # a tighter budget: the plan drops to one proposal per path
         # (GpuCandidateStereo.plan); costs are computed on the fly, no strips
         cut = 120 * 50 * 2 * 8 * 8
