@@ -170,3 +170,24 @@ def test_unsupported_parameters_are_refused(gpu_context):
     with pytest.raises(sgm.SgmError) as e:
         gpu_context.esgm_candidates(5)
     assert e.value.code == -1
+
+
+def test_census_sgm_mirror_with_esgm_options():
+    """CensusSgm.compute with --gpu-sgm esgm / --esgm-candidates equals the
+    default plan through the reference option names; a bad candidate count is
+    refused as Options.java:781-782 does."""
+    left, right, _ = synth.stereo_pair(200, 120, 48, 0, seed=5)
+    base = dict(census=5, disparities=48, p1=4, p2=24, compute="gpu", lrThreshold=1.0,
+                speckleSize=30)
+    try:
+        want = sgm.CensusSgm(sgm.Options(**base)).compute(left, right, 200, 120)
+        for pool in (1, 4):
+            print("census sgm esgm", pool, flush=True)
+            got = sgm.CensusSgm(sgm.Options(gpuSgm="esgm", esgmCandidates=pool, **base)).compute(
+                left, right, 200, 120)
+            assert np.array_equal(got.disparity.view(np.uint32), want.disparity.view(np.uint32))
+            assert got.stats.csv() == want.stats.csv()
+        with pytest.raises(ValueError):
+            sgm.GpuSgm.configured(sgm.Options(gpuSgm="esgm", esgmCandidates=5, **base))
+    finally:
+        sgm.GpuSgm.closeSession()
