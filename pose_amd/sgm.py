@@ -737,6 +737,70 @@ class GpuSgm:
             return None
 
 
+class GpuStereo:
+    """Planning and reporting of GpuStereo (GpuStereo.java:33-168) over
+    sgm_plan_get / sgm_disparity_limit, under the options' plan mode."""
+
+    @staticmethod
+    def plan(w, h, o):
+        return GpuSgm.configured(o).plan(params_from_options(o), w, h)
+
+    @staticmethod
+    def limit(w, h, o):
+        return GpuSgm.configured(o).disparity_limit(params_from_options(o), w, h)
+
+    @staticmethod
+    def report(w, h, o):
+        """GpuStereo.report (GpuStereo.java:95-168): the storage the pair runs
+        with and its footprint, with the reference's keys.  This engine's plans
+        differ in form: volumes are flat allocations (no banks), the streamed
+        plan moves the aggregate over PCIe once per pair (one read, one write),
+        and the compact plan verifies on the device within its own footprint
+        (no host aggregate).  None for compute=cpu; {"unavailable": message}
+        when planning fails."""
+        if o.compute == "cpu":
+            return None
+        try:
+            p = GpuStereo.plan(w, h, o)
+            compact = o.gpuSgm == "esgm"
+            if compact and not p["esgm_candidates"]:
+                raise SgmError(-4, "No compact Census/16-bit GPU plan fits")
+            streamed = not compact and bool(p["streamed"])
+            if not compact and not streamed and not p["resident"]:
+                raise SgmError(-4, "No GPU SGM memory plan fits the requested range")
+            sb = p["aggregate_bytes"]
+            computed = compact and p["esgm_rows"] == h
+            strips = 1 if not (compact or streamed) else -(-h // (p["esgm_rows"] if compact else p["strip_rows"]))
+            return {
+                "storage": ("compact_with_sparse_exact_verification" if compact
+                            else "streamed_ram" if streamed else "resident"),
+                "rows": p["esgm_rows"] if compact else p["strip_rows"] if streamed else h,
+                "bank_bytes": 0,
+                "estimated_device_bytes": (p["esgm_bytes"] if compact else p["streamed_bytes"]
+                                           if streamed else p["pair_bytes"]),
+                "estimated_host_bytes": p["host_bytes"] if streamed else 0,
+                "host_reserve_bytes": 0,
+                "aggregate_bits": 8 * sb,
+                "aggregate_proved_bound": p["aggregate_bound"],
+                # cost volume builds per matching direction: once (resident),
+                # once per strip pass (streamed), none when the compact sweeps
+                # compute their costs (3 passes x paths otherwise)
+                "cost_generation_passes": (0 if computed else 3 * 2 if compact
+                                           else 2 if streamed else 1),
+                "cost_generation_passes_are_upper_bound": False,
+                "aggregate_volume_reads": 1 if streamed else 0,
+                "aggregate_volume_writes": 1 if streamed else 0,
+                "aggregate_disk_bytes": 0,
+                "aggregate_ram_bytes": p["host_bytes"] if streamed else 0,
+                "verification_block_rows": h if compact else None,
+                "recomputation_blocks": strips if compact else 1,
+                "candidates_per_path": p["esgm_candidates"] if compact else None,
+                "candidate_device_bytes": p["esgm_bytes"] if compact else 0,
+            }
+        except (RuntimeError, OSError, ValueError) as e:
+            return {"unavailable": str(e)}
+
+
 class CensusSgm:
     """CensusSgm.compute (CensusSgm.java:18-280) through the device path: matching,
     the left-right check and the post-L-R filters (median, reciprocity and

@@ -191,3 +191,37 @@ def test_census_sgm_mirror_with_esgm_options():
             sgm.GpuSgm.configured(sgm.Options(gpuSgm="esgm", esgmCandidates=5, **base))
     finally:
         sgm.GpuSgm.closeSession()
+
+
+def test_gpu_stereo_report_keys_and_storage():
+    """GpuStereo.report (GpuStereo.java:95-168): the reference's keys, the
+    storage each plan mode selects, footprints from the plan."""
+    o = sgm.Options(census=9, censusWidth=9, censusHeight=7, disparities=96, p1=8, p2=96,
+                    rawPenalties=True, compute="gpu")
+    keys = {"storage", "rows", "bank_bytes", "estimated_device_bytes", "estimated_host_bytes",
+            "host_reserve_bytes", "aggregate_bits", "aggregate_proved_bound",
+            "cost_generation_passes", "cost_generation_passes_are_upper_bound",
+            "aggregate_volume_reads", "aggregate_volume_writes", "aggregate_disk_bytes",
+            "aggregate_ram_bytes", "verification_block_rows", "recomputation_blocks",
+            "candidates_per_path", "candidate_device_bytes"}
+    try:
+        seen = {}
+        for mode in ("auto", "streamed", "esgm"):
+            o.gpuSgm = mode
+            r = sgm.GpuStereo.report(640, 480, o)
+            assert set(r) == keys, set(r) ^ keys
+            seen[mode] = r
+        assert seen["auto"]["storage"] == "resident"
+        assert seen["streamed"]["storage"] == "streamed_ram"
+        assert seen["streamed"]["aggregate_ram_bytes"] == 2 * 640 * 480 * 96 * 2
+        assert seen["esgm"]["storage"] == "compact_with_sparse_exact_verification"
+        assert seen["esgm"]["candidates_per_path"] == 2
+        assert seen["esgm"]["estimated_device_bytes"] < seen["auto"]["estimated_device_bytes"]
+        assert seen["auto"]["aggregate_bits"] == 16
+        assert sgm.GpuStereo.limit(640, 480, o) >= 96
+        o.gpuSgm, o.matchingCost = "esgm", "ad-census"
+        assert "unavailable" in sgm.GpuStereo.report(640, 480, o)
+        o.compute = "cpu"
+        assert sgm.GpuStereo.report(640, 480, o) is None
+    finally:
+        sgm.GpuSgm.closeSession()
