@@ -1190,7 +1190,11 @@ int run_directions_esgm(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     S.ry = p->window_h / 2;
     S.reverse = d;
   }
-  if (nd == 1) sj.job[1] = sj.job[0], sj.job[1].hi = 0;
+  if (nd == 1) {
+    // the verification grid always spans two jobs: the second one is empty
+    sj.job[1] = sj.job[0];
+    sj.job[1].hi = 0;
+  }
   // With 1-word windows and R <= 8 every path runs over the whole image in one
   // launch, computing its census costs on the fly.  Otherwise strip passes:
   // pass 0 runs the paths with dy >= 0 top-down, pass 1 the paths with dy < 0
