@@ -47,8 +47,7 @@ extern "C" {
  * the library applies P2 = max(P1 + 1, P2) (StereoCost.java:21-23). */
 typedef struct sgm_params {
   int32_t window_w, window_h;   /* census window; the reference uses census x census */
-  int32_t disparities;          /* D, 3..4096 in GpuStereo.java:43-44; this build
-                                   refuses D > 2048 (64 disparities per lane) */
+  int32_t disparities;          /* D, 3..4096 (GpuStereo.java:43-44) */
   int32_t min_disparity;        /* signed */
   int32_t cost;                 /* SGM_COST_* */
   int32_t p1, p2;               /* raw penalties */
@@ -254,7 +253,7 @@ int sgm_read_volumes(sgm_context *ctx, int direction, uint16_t *cost, uint32_t *
  * separated proposals per path and the exact aggregate around them plus a
  * lower bound elsewhere; pixels whose outcome these settle are certified, the
  * rest are verified exactly over the full range.  Results equal the other
- * plans.  Gray Census with 16-bit aggregates, 3 <= D <= 2048, 4 or 8 paths;
+ * plans.  Gray Census with 16-bit aggregates, 3 <= D <= 4096, 4 or 8 paths;
  * other parameters or the colour / descriptor entry points fail with
  * SGM_ERR_NO_PLAN. */
 #define SGM_PLAN_ESGM 3

@@ -107,13 +107,6 @@ int validate_params(const sgm_params *p, int w, int h) {
                 p->window_w, p->window_h);
   if (p->disparities < 3 || p->disparities > 4096)
     return fail(SGM_ERR_INVALID_ARGUMENT, "disparities must be 3..4096 (got %d)", p->disparities);
-  // Ranges above 2048 need 64 disparities per lane; that sweep variant took a
-  // device memory fault at D = 3000 whose cause is not found yet, so it is
-  // refused rather than run.  sgm_disparity_limit reports at most 2048.
-  if (p->disparities > 2048)
-    return fail(SGM_ERR_INVALID_ARGUMENT,
-                "disparity ranges above 2048 (64 per lane) are disabled in this build (got %d)",
-                p->disparities);
   if (p->cost < 0 || p->cost > 4) return fail(SGM_ERR_INVALID_ARGUMENT, "unknown matching cost %d", p->cost);
   if (p->directions != 4 && p->directions != 8)
     return fail(SGM_ERR_INVALID_ARGUMENT, "directions must be 4 or 8 (got %d)", p->directions);

@@ -258,7 +258,10 @@ __device__ __forceinline__ FlushArgs make_flush_args(const PathJob &J, const Pat
 __device__ __forceinline__ void select_flush(const FlushArgs &F, const WinnerView &d, int count,
                                              int base, int x0, int y0) {
   const int lane = lane_id();
-  if (lane >= count) return;
+  // Every lane of F is read while the whole wave is active: after the lanes
+  // >= count have left, their part of F is dead to the compiler, which may
+  // restore a spilled F for the remaining lanes only (seen with 64 disparities
+  // per lane, where F lives in an AGPR across the sweep).
   const int Jw = F.get(FA_W), Jh = F.get(FA_H), Jrx = F.get(FA_RX), Jry = F.get(FA_RY);
   const int JD = F.get(FA_D), JminD = F.get(FA_MIND), Adx = F.get(FA_DX), Ady = F.get(FA_DY);
   const double uniq = __builtin_bit_cast(
@@ -267,6 +270,7 @@ __device__ __forceinline__ void select_flush(const FlushArgs &F, const WinnerVie
   uint8_t *codes = F.ptr<uint8_t>(FA_CODE);
   uint8_t *reasons = F.ptr<uint8_t>(FA_REASONS);
   uint8_t *boundary = F.ptr<uint8_t>(FA_BOUNDARY);
+  if (lane >= count) return;
   const int s = base + lane;
   const int px = x0 + s * Adx, py = y0 + s * Ady;
   const size_t pp = (size_t)py * Jw + px;

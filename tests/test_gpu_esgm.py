@@ -94,7 +94,7 @@ def test_verification_in_small_groups():
             q = sgm.SgmParams(**{f: getattr(p, f) for f, _ in sgm.SgmParams._fields_})
             q.disparities = lim
             assert ctx.plan(q, 120, 50)["esgm_candidates"] >= 1
-            if lim < 2048:
+            if lim < 4096:
                 q.disparities = lim + 1
                 assert ctx.plan(q, 120, 50)["esgm_candidates"] == 0
         finally:

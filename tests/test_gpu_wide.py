@@ -1,7 +1,8 @@
-"""Very wide disparity ranges: lane groups R = 32 and 64 (D > 1024; the
-reference's GpuStereo.plan allows up to 4096, GpuStereo.java:43-47) through the
-resident, strip-streamed and compact plans, bit-exact to the oracle; ranges
-above 2048 are refused (test_ranges_above_2048_are_refused)."""
+"""Very wide disparity ranges: lane groups R = 32 and 64 (D > 1024, up to the
+4096 of GpuStereo.plan, GpuStereo.java:43-47) through the resident,
+strip-streamed and compact plans, bit-exact to the oracle.  D = 3000 with 12
+rows also covers last-sweep flushes of fewer than 18 steps while the flush
+arguments are spilled (64 disparities per lane)."""
 import pytest
 
 from oracle import oracle as O
@@ -12,9 +13,7 @@ from tests.test_gpu_parity import assert_same, to_sgm
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("D", [1500, pytest.param(3000, marks=pytest.mark.skip(
-    reason="D > 2048 (64 disparities per lane) took a device memory fault in the resident "
-           "sweeps; the cause is not found and the library refuses such ranges"))])
+@pytest.mark.parametrize("D", [1500, 3000])
 def test_wide_ranges_all_plans(gpu_context, D):
     w, h = D + 120, 12
     a, b = shifted(500 + D, w, h, 37)
@@ -34,9 +33,8 @@ def test_wide_ranges_all_plans(gpu_context, D):
         assert_same(got, want, "plan %d D=%d" % (mode, D))
 
 
-def test_ranges_above_2048_are_refused(gpu_context):
-    a, b = shifted(7, 2300, 12, 37)
-    with pytest.raises(sgm.SgmError) as e:
-        gpu_context.match_pair(to_sgm(O.params(disparities=2100)), a, b)
-    assert e.value.code == -1 and "2048" in str(e.value)
-    assert gpu_context.disparity_limit(to_sgm(O.params(disparities=64)), 2300, 12) == 2048
+
+def test_disparity_limit_reaches_4096(gpu_context):
+    """GpuStereo.limit's binary search (GpuStereo.java:86-94) tops out at 4096
+    when memory allows."""
+    assert gpu_context.disparity_limit(to_sgm(O.params(disparities=64)), 4200, 12) == 4096
