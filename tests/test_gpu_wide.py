@@ -13,24 +13,29 @@ from tests.test_gpu_parity import assert_same, to_sgm
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("D", [1500, 3000])
-def test_wide_ranges_all_plans(gpu_context, D):
+@pytest.mark.parametrize("D,agg", [(1500, 0), (3000, 0), (3000, 4)])
+def test_wide_ranges_all_plans(gpu_context, D, agg):
+    """agg 4: 32-bit aggregates (the generic sweep); the compact plan refuses them."""
     w, h = D + 120, 12
     a, b = shifted(500 + D, w, h, 37)
-    op = O.params(disparities=D, min_disparity=-20)
+    op = O.params(disparities=D, min_disparity=-20, aggregate_bytes=agg)
     print("wide D", D, "oracle", flush=True)
     want = O.compute(op, a, b, w, h)
     p = to_sgm(op)
     print("wide D", D, "resident", flush=True)
     assert_same(gpu_context.match_pair(p, a, b), want, "resident D=%d" % D)
     for mode, rows in ((sgm.PLAN_STREAMED, 5), (sgm.PLAN_ESGM, 5)):
-        print("wide D", D, "plan", mode, flush=True)
+        print("wide D", D, "agg", agg, "plan", mode, flush=True)
         gpu_context.plan_mode(mode, rows)
         try:
+            if mode == sgm.PLAN_ESGM and agg == 4:
+                with pytest.raises(sgm.SgmError):
+                    gpu_context.match_pair(p, a, b)
+                continue
             got = gpu_context.match_pair(p, a, b)
         finally:
             gpu_context.plan_mode(sgm.PLAN_AUTO)
-        assert_same(got, want, "plan %d D=%d" % (mode, D))
+        assert_same(got, want, "plan %d D=%d agg %d" % (mode, D, agg))
 
 
 
