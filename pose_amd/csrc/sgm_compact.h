@@ -372,14 +372,22 @@ struct SummaryJobs {
 };
 
 // Cost validity of (p, k): the cost kernel's test without the cost
-// (GpuCandidateStereo.java:452-455 esgm_valid).
-__device__ __forceinline__ bool esgm_valid(const SummaryJob &J, int x, int y, int k) {
+// (GpuCandidateStereo.java:452-455 esgm_valid), split into the reference
+// pixel's part (interior, texture & valid; once per pixel) and the target's.
+__device__ __forceinline__ bool esgm_pixel_ok(const SummaryJob &J, int x, int y) {
+  return x >= J.rx && x < J.w - J.rx && y >= J.ry && y < J.h - J.ry &&
+         J.mask_a[(size_t)y * J.w + x];
+}
+
+__device__ __forceinline__ bool esgm_target_ok(const SummaryJob &J, int x, int y, int k) {
   if (k < 0 || k >= J.D) return false;
-  if (x < J.rx || x >= J.w - J.rx || y < J.ry || y >= J.h - J.ry) return false;
-  if (!J.mask_a[(size_t)y * J.w + x]) return false;
   const int d = J.minD + k;
   const long long t = (long long)x + (J.reverse ? (long long)d : -(long long)d);
   return t >= J.rx && t < J.w - J.rx && J.mask_b[(size_t)y * J.w + (size_t)t];
+}
+
+__device__ __forceinline__ bool esgm_valid(const SummaryJob &J, int x, int y, int k) {
+  return esgm_pixel_ok(J, x, y) && esgm_target_ok(J, x, y, k);
 }
 
 __device__ __forceinline__ void put_summary(uint32_t *out, int best, int flags, int value,
@@ -409,15 +417,23 @@ __global__ __launch_bounds__(256) void esgm_summary_kernel(SummaryJobs jobs, int
         owner = j;
       }
     }
-    if (owner >= 0) {
-      if (esgm_valid(J, x, y, best - 1)) before = sums[3 * owner];
-      if (esgm_valid(J, x, y, best + 1)) after = sums[3 * owner + 2];
-      for (int j = 0; j < 3 * C; j++) {
-        const int k = (int)cand[j / 3] + j % 3 - 1;
-        if (abs(k - best) > 1 && esgm_valid(J, x, y, k)) second = min(second, (int)sums[j]);
+    const bool pix = esgm_pixel_ok(J, x, y);
+    if (owner >= 0 && pix) {
+      if (esgm_target_ok(J, x, y, best - 1)) before = sums[3 * owner];
+      if (esgm_target_ok(J, x, y, best + 1)) after = sums[3 * owner + 2];
+      for (int j = 0; j < C; j++) {
+        const int kc = (int)cand[j];
+#pragma unroll
+        for (int o = 0; o < 3; o++) {
+          const int k = kc + o - 1;
+          if (abs(k - best) > 1 && esgm_target_ok(J, x, y, k))
+            second = min(second, (int)sums[3 * j + o]);
+        }
       }
     }
-    const int flags = (esgm_valid(J, x, y, 0) ? 1 : 0) | (esgm_valid(J, x, y, J.D - 1) ? 2 : 0);
+    const int flags = pix ? (esgm_target_ok(J, x, y, 0) ? 1 : 0) |
+                                (esgm_target_ok(J, x, y, J.D - 1) ? 2 : 0)
+                          : 0;
     put_summary(J.summary + 3 * p, best, flags, value, second, before, after);
     if (best >= 0) {
       const long long lower = J.outside[p];
