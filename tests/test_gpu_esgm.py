@@ -8,7 +8,7 @@ import pytest
 
 from oracle import oracle as O
 from pose_amd import sgm, synth
-from tests.cases import upgrade_fixture
+from tests.cases import shifted, upgrade_fixture
 from tests.test_gpu_parity import assert_same, to_sgm
 
 pytestmark = pytest.mark.gpu
@@ -225,3 +225,25 @@ def test_gpu_stereo_report_keys_and_storage():
         assert sgm.GpuStereo.report(640, 480, o) is None
     finally:
         sgm.GpuSgm.closeSession()
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_esgm_edge_cases_against_oracle(gpu_context, case):
+    """Ranges wider than the image (most targets outside it), large negative
+    minimum disparities, heavy validity masks and an untextured image: the
+    compact plan equals the oracle bit for bit."""
+    rng = np.random.default_rng(900 + case)
+    w, h = [60, 90, 70][case], [40, 33, 30][case]
+    D, md = [130, 48, 40][case], [0, -60, -5][case]
+    a, b = shifted(910 + case, w, h, 5)
+    if case == 2:
+        a = np.full((h, w), 77, np.uint8)  # constant image: every cost invalid
+    ma = (rng.random((h, w)) > [0.1, 0.5, 0.0][case]).astype(np.uint8) * 255
+    mb = (rng.random((h, w)) > [0.1, 0.5, 0.0][case]).astype(np.uint8) * 255
+    op = O.params(window_w=5, window_h=5, disparities=D, min_disparity=md,
+                  directions=[8, 4, 8][case])
+    want = O.compute(op, a, b, w, h, ma, mb)
+    for pool in (1, 3):
+        got, counts = esgm(gpu_context, to_sgm(op), a, b, ma, mb, 64, pool, "edge %d" % case)
+        assert_same(got, want, "edge case %d pool %d" % (case, pool))
+        assert sum(counts["left"]) == w * h
