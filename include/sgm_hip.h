@@ -208,7 +208,9 @@ int sgm_match_pair_color(sgm_context *ctx, const sgm_params *params,
 
 /* Same as sgm_match_pair with every buffer in device memory (no transfers, no
  * host synchronisation).  d_stats (device, sgm_stats) may be NULL.  Work is
- * enqueued on the context's stream; call sgm_synchronize before reading. */
+ * enqueued on the context's stream; call sgm_synchronize before reading.
+ * Under SGM_PLAN_ESGM the call waits once for its own stream mid-way: the
+ * number of pixels to verify sizes the verification groups. */
 int sgm_match_pair_device(sgm_context *ctx, const sgm_params *params, const void *d_left,
                           const void *d_right, const void *d_valid_left,
                           const void *d_valid_right, int width, int height, void *d_checked,
