@@ -217,6 +217,27 @@ int sgm_match_pair_device(sgm_context *ctx, const sgm_params *params, const void
                           void *d_dl, void *d_dr, void *d_stats, void *d_reasons,
                           void *d_boundary);
 
+/* sgm_match_pair_device without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during the
+ * call (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results as sgm_match_pair_device under
+ * the resident plan (the plan mode is ignored), with every intermediate buffer
+ * (volumes, descriptors, masks, maps, counters) carved from a fixed workspace
+ * of SGM_NOALLOC_WORKSPACE_BYTES that the library holds on each device for the
+ * life of the process; the census cost is recomputed per sweep when the cost
+ * volume would not fit.  A pair that needs more workspace (about
+ * 640x480 with 64 disparities at most with a cost volume) fails with
+ * SGM_ERR_INVALID_ARGUMENT; so does a context with sgm_set_keep_volumes on.
+ * Every call on one device shares that device's workspace: calls that
+ * overlap on one device (from two contexts or threads) race and must be
+ * ordered by the caller. */
+#define SGM_NOALLOC_WORKSPACE_BYTES ((size_t)128 << 20)
+int sgm_match_pair_device_noalloc(sgm_context *ctx, const sgm_params *params,
+                                  const void *d_left, const void *d_right,
+                                  const void *d_valid_left, const void *d_valid_right, int width,
+                                  int height, void *d_checked, void *d_dl, void *d_dr,
+                                  void *d_stats, void *d_reasons, void *d_boundary);
+
 /* Device memory for the *_device entry points (callers without their own HIP
  * allocator, e.g. a JVM through FFM).  sgm_memcpy is ordered on the context's
  * stream and returns after the copy completes; kind: 0 host->device,

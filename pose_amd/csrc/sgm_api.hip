@@ -25,6 +25,47 @@
 
 using namespace sgm;
 
+// Fixed workspace of sgm_match_pair_device_noalloc: every buffer of a resident
+// pair is carved from this array, so that entry point allocates nothing.  Each
+// device holds its own copy for the life of the process.  In host code the
+// name `scratch` is only a stand-in for the symbol; device code gets the
+// per-device address that sgm_open looks up (scratch_address).
+constexpr size_t kScratchBytes = SGM_NOALLOC_WORKSPACE_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_lock;
+void *g_scratch_ptr[kMaxScratchDevices];
+bool g_scratch_done[kMaxScratchDevices];
+hipError_t g_scratch_error[kMaxScratchDevices];
+
+// Device address of `scratch` on `device` (which must be current), looked up
+// once per device.
+hipError_t scratch_setup(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> g(g_scratch_lock);
+  if (!g_scratch_done[device]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      ptr = nullptr;
+    }
+    g_scratch_ptr[device] = ptr;
+    g_scratch_error[device] = e;
+    g_scratch_done[device] = true;
+  }
+  return g_scratch_error[device];
+}
+
+void *scratch_address(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> g(g_scratch_lock);
+  return g_scratch_done[device] ? g_scratch_ptr[device] : nullptr;
+}
+}  // namespace
+
 namespace sgm {
 
 thread_local std::string g_last_error;
@@ -315,6 +356,9 @@ extern "C" int sgm_open(int device, int64_t budget_bytes, sgm_context **out) {
   if (rc) return rc;
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  // the no-allocation entry point's workspace on this device; a failed lookup
+  // only disables that entry point (it reports the error)
+  (void)scratch_setup(device);
   sgm_context *ctx = new sgm_context();
   ctx->device = device;
   ctx->info = info;
@@ -759,6 +803,9 @@ struct PairBuffers {
   uint16_t *e_cand, *e_sums, *e_outside, *e_strip_cost;
   int *e_slot, *e_pixels, *e_count, *e_carry;
   uint32_t *e_summary;
+  // resident plan without a cost volume: the census cost is recomputed in
+  // every sweep
+  bool recompute;
 };
 
 // Resident (rows = 0), strip-streamed (rows > 0) or compact (pool > 0, strips
@@ -800,6 +847,7 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     B.wsum2[d] = zncc ? (int32_t *)ctx->buffers["wsum2" + s].ptr : nullptr;
     B.cost[d] = rows || recompute_cost(ctx, p, w, h) ? nullptr
                                                      : (uint16_t *)ctx->buffers["cost" + s].ptr;
+    B.recompute = !rows && recompute_cost(ctx, p, w, h);
     B.sum[d] = rows ? nullptr : ctx->buffers["sum" + s].ptr;
     B.disp[d] = (float *)ctx->buffers["disp" + s].ptr;
   }
@@ -1344,7 +1392,7 @@ int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuff
   int rc;
   const int D = p->disparities, Dp = padded_disparities(D), R = path_lanes(p);
   const int sb = aggregate_bytes(p);
-  const bool recompute = recompute_cost(ctx, p, w, h);
+  const bool recompute = B.recompute;
   const bool fused = !recompute && fused_cost(p);
   int mode = recompute ? 3 : sb == 2 ? (cost_u8(p) ? 4 : 1) : 0;
   // stream the uint8 cost and uint16 aggregate volumes past the caches when
@@ -1836,6 +1884,103 @@ extern "C" int sgm_match_pair_device(sgm_context *ctx, const sgm_params *params,
   PairBuffers B;
   if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows, pool))) return rc;
   ctx->last_streamed = rows > 0 && !pool;
+  const size_t n = (size_t)w * h;
+  unsigned long long *stats = d_stats ? (unsigned long long *)d_stats : B.stats;
+  if ((rc = run_pair(ctx, params, w, h, B, (const uint8_t *)d_left, (const uint8_t *)d_right,
+                     (const uint8_t *)d_valid_left, (const uint8_t *)d_valid_right,
+                     (float *)d_checked, d_reasons ? (uint8_t *)d_reasons : B.reasons_out,
+                     d_boundary ? (uint8_t *)d_boundary : B.boundary_out, stats)))
+    return rc;
+  if (d_dl) {
+    hipError_t e = hipMemcpyAsync(d_dl, B.disp[0], n * 4, hipMemcpyDeviceToDevice, ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "device copy");
+  }
+  if (d_dr) {
+    hipError_t e = hipMemcpyAsync(d_dr, B.disp[1], n * 4, hipMemcpyDeviceToDevice, ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "device copy");
+  }
+  ctx->last_w = w;
+  ctx->last_h = h;
+  ctx->last_params = *params;
+  return SGM_OK;
+}
+
+namespace {
+
+// The resident plan's pair buffers laid out back to back (256-byte aligned)
+// from `base` (nullptr: only count).  Returns the bytes the layout takes.  The
+// caller's valid masks are read in place, so no valid copies are laid out.
+size_t carve_pair_buffers(const sgm_params *p, int w, int h, bool cost_volume, uint8_t *base,
+                          PairBuffers &B) {
+  const size_t n = (size_t)w * h, Dp = padded_disparities(p->disparities);
+  const int sb = aggregate_bytes(p), words = words_of(p);
+  const bool zncc = p->cost == SGM_COST_ZNCC || p->cost == SGM_COST_ZNCC_FP32;
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> void * {
+    void *q = base ? base + off : nullptr;
+    off += (bytes + 255) & ~(size_t)255;
+    return q;
+  };
+  memset(&B, 0, sizeof B);
+  for (int d = 0; d < 2; d++) {
+    B.gray[d] = (uint8_t *)take(n);
+    B.valid[d] = nullptr;
+    B.mask[d] = (uint8_t *)take(n);
+    B.gm[d] = (uint16_t *)take(2 * n);
+    B.desc[d] = (uint64_t *)take(8 * words * n);
+    B.wsum[d] = zncc ? (int32_t *)take(4 * n) : nullptr;
+    B.wsum2[d] = zncc ? (int32_t *)take(4 * n) : nullptr;
+    B.cost[d] = cost_volume ? (uint16_t *)take(n * Dp * (cost_u8(p) ? 1 : 2)) : nullptr;
+    B.sum[d] = take(n * Dp * sb);
+    B.disp[d] = (float *)take(4 * n);
+  }
+  B.checked = (float *)take(4 * n);
+  B.reasons = (uint8_t *)take(n);
+  B.boundary = (uint8_t *)take(n);
+  B.reasons_out = (uint8_t *)take(n);
+  B.boundary_out = (uint8_t *)take(n);
+  B.stats = (unsigned long long *)take(16 * 8);
+  B.code = (uint8_t *)take(n);
+  B.recompute = !cost_volume;
+  return off;
+}
+
+}  // namespace
+
+extern "C" int sgm_match_pair_device_noalloc(sgm_context *ctx, const sgm_params *params,
+                                             const void *d_left, const void *d_right,
+                                             const void *d_valid_left, const void *d_valid_right,
+                                             int w, int h, void *d_checked, void *d_dl, void *d_dr,
+                                             void *d_stats, void *d_reasons, void *d_boundary) {
+  if (!ctx || !d_left || !d_right || !d_checked) return fail(SGM_ERR_INVALID_ARGUMENT, "null argument");
+  if ((d_valid_left == nullptr) != (d_valid_right == nullptr))
+    return fail(SGM_ERR_INVALID_ARGUMENT, "Invalid stereo mask dimensions");
+  std::lock_guard<std::mutex> g(ctx->lock);
+  int rc = validate_params(params, w, h);
+  if (rc) return rc;
+  if (ctx->keep_volumes)
+    return fail(SGM_ERR_INVALID_ARGUMENT,
+                "sgm_match_pair_device_noalloc keeps no volumes (disable sgm_set_keep_volumes)");
+  uint8_t *base = (uint8_t *)scratch_address(ctx->device);
+  if (!base)
+    return fail(SGM_ERR_HIP, "the no-allocation workspace of device %d is unavailable (HIP %d)",
+                ctx->device, (int)g_scratch_error[ctx->device < kMaxScratchDevices ? ctx->device : 0]);
+  PairBuffers B;
+  // the resident cost volume when it fits, else census costs recomputed per sweep
+  bool cost_volume = !(census_recompute_possible(params) && ctx->volume_policy == SGM_VOLUME_RECOMPUTE);
+  size_t need = carve_pair_buffers(params, w, h, cost_volume, nullptr, B);
+  if (need > kScratchBytes && cost_volume && census_recompute_possible(params)) {
+    cost_volume = false;
+    need = carve_pair_buffers(params, w, h, cost_volume, nullptr, B);
+  }
+  if (need > kScratchBytes)
+    return fail(SGM_ERR_INVALID_ARGUMENT,
+                "a %dx%d pair with %d disparities needs %zu bytes of workspace; "
+                "sgm_match_pair_device_noalloc holds at most %zu (use sgm_match_pair_device)",
+                w, h, params->disparities, need, kScratchBytes);
+  carve_pair_buffers(params, w, h, cost_volume, base, B);
+  (void)hipSetDevice(ctx->device);
+  ctx->last_streamed = false;
   const size_t n = (size_t)w * h;
   unsigned long long *stats = d_stats ? (unsigned long long *)d_stats : B.stats;
   if ((rc = run_pair(ctx, params, w, h, B, (const uint8_t *)d_left, (const uint8_t *)d_right,

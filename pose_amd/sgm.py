@@ -29,6 +29,8 @@ ERRORS = {
 VOLUME_AUTO, VOLUME_RESIDENT, VOLUME_RECOMPUTE = 0, 1, 2
 # aggregation plan: --gpu-sgm auto|resident|streamed|esgm (Options.java:32,780)
 PLAN_AUTO, PLAN_RESIDENT, PLAN_STREAMED, PLAN_ESGM = 0, 1, 2, 3
+# fixed per-device workspace of match_pair_device_noalloc (SGM_NOALLOC_WORKSPACE_BYTES)
+NOALLOC_WORKSPACE_BYTES = 128 << 20
 PLAN_MODES = {"auto": PLAN_AUTO, "resident": PLAN_RESIDENT, "streamed": PLAN_STREAMED,
               "esgm": PLAN_ESGM}
 COSTS = {"census": 0, "ad-census": 1, "census-gradient": 2, "zncc": 3, "zncc-fp32": 4}
@@ -111,6 +113,7 @@ class SgmRequestInfo(ctypes.Structure):
 EXPORTS = [
     "sgm_device_count", "sgm_device_info_get", "sgm_open", "sgm_close", "sgm_plan_get",
     "sgm_disparity_limit", "sgm_match", "sgm_match_desc", "sgm_match_pair", "sgm_match_pair_device",
+    "sgm_match_pair_device_noalloc",
     "sgm_match_pair_filtered", "sgm_match_pair_color",
     "sgm_set_keep_volumes", "sgm_read_volumes", "sgm_set_cost_volume", "sgm_set_plan_mode",
     "sgm_set_esgm_candidates", "sgm_esgm_counts", "sgm_synchronize",
@@ -156,6 +159,7 @@ def lib():
             L.sgm_match_desc.argtypes = [P, pp, P, P, P, P, P, P, I, I, I, P, ps, P, P]
             L.sgm_match_pair.argtypes = [P, pp, P, P, P, P, I, I, P, P, P, ps, P, P]
             L.sgm_match_pair_device.argtypes = [P, pp, P, P, P, P, I, I, P, P, P, P, P, P]
+            L.sgm_match_pair_device_noalloc.argtypes = [P, pp, P, P, P, P, I, I, P, P, P, P, P, P]
             L.sgm_match_pair_filtered.argtypes = [P, pp, ctypes.POINTER(SgmFilterParams), P, P, P,
                                                   P, I, I, P, ps, P]
             L.sgm_match_pair_color.argtypes = [P, pp, ctypes.POINTER(SgmFilterParams), P, P, P,
@@ -511,6 +515,17 @@ class Context:
                                           _p(d_valid_left), _p(d_valid_right), w, h,
                                           _p(d_checked), _p(d_dl), _p(d_dr), _p(d_stats),
                                           _p(d_reasons), _p(d_boundary)))
+
+    def match_pair_device_noalloc(self, params, d_left, d_right, w, h, d_checked,
+                                  d_valid_left=None, d_valid_right=None, d_dl=None, d_dr=None,
+                                  d_stats=None, d_reasons=None, d_boundary=None):
+        """match_pair_device in the library's fixed per-device workspace: no
+        device allocation during the call (pairs up to NOALLOC_WORKSPACE_BYTES)."""
+        check(lib().sgm_match_pair_device_noalloc(self.h, ctypes.byref(params), _p(d_left),
+                                                  _p(d_right), _p(d_valid_left),
+                                                  _p(d_valid_right), w, h, _p(d_checked),
+                                                  _p(d_dl), _p(d_dr), _p(d_stats),
+                                                  _p(d_reasons), _p(d_boundary)))
 
     def keep_volumes(self, keep=True):
         check(lib().sgm_set_keep_volumes(self.h, int(keep)))
