@@ -106,6 +106,13 @@ def lib():
                                            ctypes.POINTER(OracleFilter), P, P, P, ctypes.c_int,
                                            P, ctypes.c_int, ctypes.c_int, P,
                                            ctypes.POINTER(OracleStats), P]
+        L.oracle_match_parallel.argtypes = [ctypes.POINTER(OracleParams), P, P, P, P, P, P,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_int, P,
+                                            ctypes.POINTER(OracleStats), P, P, ctypes.c_int]
+        L.oracle_compute_parallel.argtypes = [ctypes.POINTER(OracleParams), P, P, P, P,
+                                              ctypes.c_int, ctypes.c_int, P, P, P,
+                                              ctypes.POINTER(OracleStats), P, P, ctypes.c_int]
+        L.oracle_threads.restype = ctypes.c_int
         L.oracle_round_float.argtypes = [ctypes.c_float]
         L.oracle_round_float.restype = ctypes.c_int32
         L.oracle_round_double.argtypes = [ctypes.c_double]
@@ -196,6 +203,35 @@ def compute(p, left, right, w, h, valid_left=None, valid_right=None):
         raise ValueError("oracle_compute failed: %d" % rc)
     return dict(checked=checked, dl=dl, dr=dr, stats=st.as_dict(), reasons=reasons,
                 boundary=boundary)
+
+
+def compute_parallel(p, left, right, w, h, valid_left=None, valid_right=None, threads=0):
+    """compute() split over host threads (OpenMP, threads <= 0: all): the same
+    results bit for bit, for the full-size workloads."""
+    n = w * h
+    left = np.ascontiguousarray(left, np.uint8).reshape(-1)
+    right = np.ascontiguousarray(right, np.uint8).reshape(-1)
+    vl = None if valid_left is None else np.ascontiguousarray(valid_left, np.uint8).reshape(-1)
+    vr = None if valid_right is None else np.ascontiguousarray(valid_right, np.uint8).reshape(-1)
+    checked = np.empty(n, np.float32)
+    dl = np.empty(n, np.float32)
+    dr = np.empty(n, np.float32)
+    reasons = np.empty(n, np.uint8)
+    boundary = np.empty(n, np.uint8)
+    st = OracleStats()
+    rc = lib().oracle_compute_parallel(ctypes.byref(p), _ptr(left), _ptr(right), _ptr(vl),
+                                       _ptr(vr), w, h, _ptr(checked), _ptr(dl), _ptr(dr),
+                                       ctypes.byref(st), _ptr(reasons), _ptr(boundary),
+                                       int(threads))
+    if rc != 0:
+        raise ValueError("oracle_compute_parallel failed: %d" % rc)
+    return dict(checked=checked, dl=dl, dr=dr, stats=st.as_dict(), reasons=reasons,
+                boundary=boundary)
+
+
+def threads():
+    """Host threads a parallel oracle call uses."""
+    return int(lib().oracle_threads())
 
 
 def compute_color(p, left, right, left_bgr, right_bgr, w, h, valid_left=None, valid_right=None):

@@ -18,6 +18,9 @@
 #include "sgm_oracle.h"
 
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stddef.h>
 #include <stdlib.h>
 #include <string.h>
@@ -714,4 +717,293 @@ int oracle_postfilter_ch(const oracle_params *o, const oracle_filter *f, const u
     free(before);
   }
   return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* The same restatement split over host threads (OpenMP).                    */
+/*                                                                            */
+/* oracle_match's arithmetic, loop for loop, with the work divided where the  */
+/* reference's data dependences allow it:                                     */
+/*   cost fill (CensusSgm.java:405-420)   every cell independent: rows split  */
+/*   path recurrence (:421-459)           a horizontal path's rows are        */
+/*       independent (each keeps its own previous-pixel vector); for the      */
+/*       other paths a pixel reads only the previous row, so each row's       */
+/*       pixels are split and the rows run in path order                      */
+/*   WTA / subpixel (:460-515)            every pixel independent; counters   */
+/*       summed                                                               */
+/* Each cell's aggregate receives the same integer terms in the same path     */
+/* order, so results equal oracle_match bit for bit (checked by the tests).   */
+
+static void set_threads(int threads) {
+#ifdef _OPENMP
+  omp_set_num_threads(threads > 0 ? threads : omp_get_num_procs());
+#else
+  (void)threads;
+#endif
+}
+
+int oracle_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* One pixel step of the recurrence: L(p, .) from the predecessor's vector
+ * `prev` (NULL: no predecessor), added into the aggregate. */
+static inline void path_step(const uint16_t *costs, uint16_t *agg16,
+                             int32_t *agg32, const int *prev, int *out, size_t base, int nd,
+                             int p1, int localP2, int value_mask) {
+  int minPrev = 0x7FFFFFFF;
+  if (prev)
+    for (int k = 0; k < nd; k++)
+      if (prev[k] < minPrev) minPrev = prev[k];
+  for (int k = 0; k < nd; k++) {
+    int value = costs[base + k] & value_mask;
+    if (prev) {
+      int transition = prev[k];
+      if (minPrev + localP2 < transition) transition = minPrev + localP2;
+      if (k > 0 && prev[k - 1] + p1 < transition) transition = prev[k - 1] + p1;
+      if (k + 1 < nd && prev[k + 1] + p1 < transition) transition = prev[k + 1] + p1;
+      value += transition - minPrev;
+    }
+    out[k] = value;
+    if (agg16) agg16[base + k] = (uint16_t)(agg16[base + k] + (uint32_t)value);
+    else agg32[base + k] = (int32_t)((uint32_t)agg32[base + k] + (uint32_t)value);
+  }
+}
+
+int oracle_match_parallel(const oracle_params *o, const uint64_t *a, const uint64_t *b,
+                          const uint8_t *grayA, const uint8_t *grayB, const uint8_t *texturedA,
+                          const uint8_t *texturedB, int width, int height, int reverse,
+                          float *result, oracle_stats *stats, uint8_t *reasons,
+                          uint8_t *searchBoundary, int threads) {
+  int nd = o->disparities, rx = o->window_w / 2, ry = o->window_h / 2;
+  int words = oracle_words(o->window_w, o->window_h);
+  size_t count = (size_t)width * height, cells = count * (size_t)nd;
+  int sum_bytes = oracle_aggregate_bytes(o);
+  if (sum_bytes < 0 || nd < 1 || o->directions < 1 || o->directions > 8) return -1;
+  set_threads(threads);
+  int invalid = o->cost == OR_ZNCC1024 ? 0x8800 : INVALID_COST;
+  int value_mask = o->cost == OR_ZNCC1024 ? 0x7FFF : 0xFFFF;
+  uint16_t *costs = (uint16_t *)malloc(cells * sizeof(uint16_t));
+  uint16_t *agg16 = NULL;
+  int32_t *agg32 = NULL;
+  if (sum_bytes == 2) agg16 = (uint16_t *)malloc(cells * sizeof(uint16_t));
+  else agg32 = (int32_t *)malloc(cells * sizeof(int32_t));
+  int *previous = (int *)malloc(sizeof(int) * (size_t)width * nd);
+  int *row = (int *)malloc(sizeof(int) * (size_t)width * nd);
+  if (!costs || (!agg16 && !agg32) || !previous || !row) {
+    free(costs);
+    free(agg16);
+    free(agg32);
+    free(previous);
+    free(row);
+    return -2;
+  }
+  /* cost fill (CensusSgm.java:405-420), zeroed aggregates */
+#pragma omp parallel for schedule(dynamic, 2)
+  for (int y = 0; y < height; y++) {
+    size_t first = (size_t)y * width * nd, last = first + (size_t)width * nd;
+    for (size_t i = first; i < last; i++) costs[i] = (uint16_t)invalid;
+    if (agg16) memset(agg16 + first, 0, (last - first) * sizeof(uint16_t));
+    else memset(agg32 + first, 0, (last - first) * sizeof(int32_t));
+    if (y < ry || y >= height - ry) continue;
+    for (int x = rx; x < width - rx; x++) {
+      size_t p = (size_t)y * width + x, base = p * nd;
+      if (!texturedA[p]) continue;
+      for (int k = 0; k < nd; k++) {
+        int disparity = o->min_disparity + k;
+        int64_t target = (int64_t)x + (reverse ? disparity : -(int64_t)disparity);
+        if (target < rx || target >= width - rx) continue;
+        size_t q = (size_t)y * width + (size_t)target;
+        if (!texturedB[q]) continue;
+        costs[base + k] = (uint16_t)cost_value(o, a, b, grayA, grayB, p, q, width, count, words);
+      }
+    }
+  }
+  /* path recurrence (CensusSgm.java:421-459), directions in the reference order */
+  static const int paths[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1},
+                                  {1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
+  int p1 = o->p1, p2 = oracle_effective_p2(o);
+  for (int path = 0; path < o->directions; path++) {
+    int dx = paths[path][0], dy = paths[path][1];
+    int yStart = dy < 0 ? height - 1 : 0, yStep = dy < 0 ? -1 : 1;
+    int xStart = dx < 0 ? width - 1 : 0, xStep = dx < 0 ? -1 : 1;
+    if (dy == 0) {
+#pragma omp parallel
+      {
+        int *line = (int *)malloc(sizeof(int) * 2 * (size_t)nd);
+#pragma omp for schedule(dynamic, 4)
+        for (int y = 0; y < height; y++)
+          for (int xi = 0, x = xStart; xi < width; xi++, x += xStep) {
+            int px = x - dx;
+            int has = px >= 0 && px < width && texturedA[(size_t)y * width + px];
+            int *out = line + (size_t)(xi & 1) * nd, *prev = line + (size_t)((xi + 1) & 1) * nd;
+            int localP2 = has ? edge_p2(p1, p2, o->edge_scale, grayA, (size_t)y * width + x,
+                                        (size_t)y * width + px)
+                              : p2;
+            path_step(costs, agg16, agg32, has ? prev : NULL, out,
+                      ((size_t)y * width + x) * nd, nd, p1, localP2, value_mask);
+          }
+        free(line);
+      }
+      continue;
+    }
+    for (int yi = 0, y = yStart; yi < height; yi++, y += yStep) {
+      int py = y - dy;
+#pragma omp parallel for schedule(static)
+      for (int x = 0; x < width; x++) {
+        int px = x - dx;
+        int has = px >= 0 && px < width && py >= 0 && py < height &&
+                  texturedA[(size_t)py * width + px];
+        int localP2 = has ? edge_p2(p1, p2, o->edge_scale, grayA, (size_t)y * width + x,
+                                    (size_t)py * width + px)
+                          : p2;
+        path_step(costs, agg16, agg32, has ? previous + (size_t)px * nd : NULL,
+                  row + (size_t)x * nd, ((size_t)y * width + x) * nd, nd, p1, localP2,
+                  value_mask);
+      }
+      int *swap = previous;
+      previous = row;
+      row = swap;
+    }
+  }
+#define AGG(i) (agg16 ? (int)agg16[i] : (int)agg32[i])
+  /* WTA, boundary, uniqueness, subpixel (CensusSgm.java:460-515) */
+  int64_t full_n = 0, boundary_n = 0, lower_n = 0, upper_n = 0, ambiguity_n = 0, selected_n = 0;
+#pragma omp parallel for schedule(dynamic, 4) \
+    reduction(+ : full_n, boundary_n, lower_n, upper_n, ambiguity_n, selected_n)
+  for (int y = 0; y < height; y++) {
+    for (int x = 0; x < width; x++) result[(size_t)y * width + x] = NAN;
+    if (y < ry || y >= height - ry) continue;
+    for (int x = rx; x < width - rx; x++) {
+      size_t pixel = (size_t)y * width + x, base = pixel * nd;
+      int best = -1, bestCost = 0x7FFFFFFF;
+      for (int k = 0; k < nd; k++)
+        if (costs[base + k] != invalid && AGG(base + k) < bestCost) {
+          best = k;
+          bestCost = AGG(base + k);
+        }
+      if (best < 0) continue;
+      int full = costs[base] != invalid && costs[base + nd - 1] != invalid;
+      if (full) {
+        full_n++;
+        if (searchBoundary) searchBoundary[pixel] = 4;
+      }
+      if (best == 0 || best == nd - 1) {
+        boundary_n++;
+        if (reasons) reasons[pixel] = 2;
+        if (full && best == 0) {
+          lower_n++;
+          if (searchBoundary) searchBoundary[pixel] |= 1;
+        }
+        if (full && best == nd - 1) {
+          upper_n++;
+          if (searchBoundary) searchBoundary[pixel] |= 2;
+        }
+        continue;
+      }
+      int second = 0x7FFFFFFF;
+      for (int k = 0; k < nd; k++)
+        if (abs(k - best) > 1 && costs[base + k] != invalid && AGG(base + k) < second)
+          second = AGG(base + k);
+      double allowed = (double)bestCost * o->uniqueness;
+      if (!(allowed >= 1.0)) allowed = 1.0;
+      if (second != 0x7FFFFFFF && (double)(second - bestCost) <= allowed) {
+        ambiguity_n++;
+        if (reasons) reasons[pixel] = 3;
+        continue;
+      }
+      double offset = 0;
+      if (best > 0 && best + 1 < nd && costs[base + best - 1] != invalid &&
+          costs[base + best + 1] != invalid) {
+        double before = AGG(base + best - 1), after = AGG(base + best + 1);
+        double curvature = before + after - 2.0 * bestCost;
+        if (curvature > 0) {
+          double t = .5 * (before - after) / curvature;
+          t = t < .5 ? t : .5;
+          offset = t > -.5 ? t : -.5;
+        }
+      }
+      result[pixel] = (float)((double)(o->min_disparity + best) + offset);
+      selected_n++;
+      if (reasons) reasons[pixel] = 0;
+    }
+  }
+#undef AGG
+  stats->full_interval_pixels += full_n;
+  stats->boundary_rejected += boundary_n;
+  stats->lower_boundary_rejected += lower_n;
+  stats->upper_boundary_rejected += upper_n;
+  stats->ambiguity_rejected += ambiguity_n;
+  stats->selected += selected_n;
+  free(costs);
+  free(agg16);
+  free(agg32);
+  free(previous);
+  free(row);
+  return 0;
+}
+
+/* oracle_compute through oracle_match_parallel (gray census, no filters). */
+int oracle_compute_parallel(const oracle_params *o, const uint8_t *left, const uint8_t *right,
+                            const uint8_t *valid_left, const uint8_t *valid_right, int width,
+                            int height, float *checked, float *dl_out, float *dr_out,
+                            oracle_stats *stats, uint8_t *reasons, uint8_t *searchBoundary,
+                            int threads) {
+  if (width < o->window_w + 2 || height < o->window_h + 2) return -1;
+  size_t n = (size_t)width * height;
+  int words = oracle_words(o->window_w, o->window_h);
+  uint64_t *a = (uint64_t *)malloc(sizeof(uint64_t) * n * words);
+  uint64_t *b = (uint64_t *)malloc(sizeof(uint64_t) * n * words);
+  uint8_t *ta = (uint8_t *)malloc(n), *tb = (uint8_t *)malloc(n);
+  float *dl = dl_out ? dl_out : (float *)malloc(sizeof(float) * n);
+  float *dr = dr_out ? dr_out : (float *)malloc(sizeof(float) * n);
+  if (!a || !b || !ta || !tb || !dl || !dr) return -2;
+  if (reasons) memset(reasons, 1, n);
+  if (searchBoundary) memset(searchBoundary, 0, n);
+  oracle_census(left, width, height, o->window_w, o->window_h, a);
+  oracle_census(right, width, height, o->window_w, o->window_h, b);
+  oracle_texture(left, valid_left, width, height, o->window_w, o->window_h, ta);
+  oracle_texture(right, valid_right, width, height, o->window_w, o->window_h, tb);
+  oracle_stats discarded;
+  memset(&discarded, 0, sizeof discarded);
+  int rc = oracle_match_parallel(o, a, b, left, right, ta, tb, width, height, 0, dl, stats,
+                                 reasons, searchBoundary, threads);
+  if (rc == 0)
+    rc = oracle_match_parallel(o, b, a, right, left, tb, ta, width, height, 1, dr, &discarded,
+                               NULL, NULL, threads);
+  if (rc == 0) {
+    /* left-right check (CensusSgm.java:181-201) */
+    int rx = o->window_w / 2, ry = o->window_h / 2;
+    for (size_t i = 0; i < n; i++) checked[i] = NAN;
+    for (int y = ry; y < height - ry; y++)
+      for (int x = rx; x < width - rx; x++) {
+        size_t p = (size_t)y * width + x;
+        if (!isfinite_f(dl[p])) continue;
+        int xr = oracle_round_float((float)x - dl[p]);
+        if (xr < rx || xr >= width - rx) {
+          if (reasons) reasons[p] = 4;
+          continue;
+        }
+        float other = dr[(size_t)y * width + xr];
+        if (isfinite_f(other) && (double)fabsf(dl[p] - other) <= o->lr_threshold)
+          checked[p] = dl[p];
+        else {
+          stats->lr_rejected++;
+          if (reasons) reasons[p] = 4;
+        }
+      }
+    for (size_t i = 0; i < n; i++)
+      if (isfinite_f(checked[i])) stats->valid++;
+  }
+  free(a);
+  free(b);
+  free(ta);
+  free(tb);
+  if (!dl_out) free(dl);
+  if (!dr_out) free(dr);
+  return rc;
 }

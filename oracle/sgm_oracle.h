@@ -105,6 +105,19 @@ int oracle_compute_color(const oracle_params *o, const uint8_t *left, const uint
                          float *checked, float *dl, float *dr, oracle_stats *stats,
                          uint8_t *reasons, uint8_t *boundary);
 
+/* oracle_match / oracle_compute split over host threads (OpenMP; threads
+ * <= 0: all available).  Same results bit for bit; gray census, no filters.
+ * oracle_threads() is the number of threads a parallel region gets. */
+int oracle_match_parallel(const oracle_params *o, const uint64_t *da, const uint64_t *db,
+                          const uint8_t *ga, const uint8_t *gb, const uint8_t *ta,
+                          const uint8_t *tb, int w, int h, int reverse, float *disp,
+                          oracle_stats *stats, uint8_t *reasons, uint8_t *boundary, int threads);
+int oracle_compute_parallel(const oracle_params *o, const uint8_t *left, const uint8_t *right,
+                            const uint8_t *valid_left, const uint8_t *valid_right, int w, int h,
+                            float *checked, float *dl, float *dr, oracle_stats *stats,
+                            uint8_t *reasons, uint8_t *boundary, int threads);
+int oracle_threads(void);
+
 /* Post-L-R filters of CensusSgm.computeRecorded (CensusSgm.java:203-271);
  * same layout as sgm_filter_params (include/sgm_hip.h). */
 typedef struct {
