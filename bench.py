@@ -10,9 +10,11 @@ Prints one JSON line: the metric is SGM cell-updates per second
 and the CPU oracle's rate on the host beside it.
 """
 import argparse
+import hashlib
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -30,7 +32,7 @@ def workload_params(sgm):
 
 
 def path_bytes_per_step(w, h, D, paths, cost_bytes=1):
-    """Algorithmic HBM bytes of the path sweeps for one pair (both directions),
+    """HBM bytes this build's path sweeps move for one pair (both directions),
     uint16 aggregates S and a cost volume C of `cost_bytes` per cell: the first
     sweep computes the census cost and writes C and S (1 + 2 B/cell), middle
     sweeps read C, read S and write S (1 + 2 + 2 B/cell), the last sweep reads C
@@ -42,24 +44,80 @@ def path_bytes_per_step(w, h, D, paths, cost_bytes=1):
     return cells * (first + mid * (paths - 2) + last)
 
 
-def cpu_baseline(w, D, rows=256, trials=1):
-    """The C oracle (single thread) on a bounded strip of the same workload."""
-    import numpy as np
-    from oracle import oracle as O
-    from pose_amd import synth
-    left, right, _ = synth.stereo_pair(w, rows, D, 0, seed=2)
-    op = O.params(window_w=9, window_h=7, disparities=D, p1=8, p2=96, lr_threshold=1.0)
-    O.compute(op, left[:64], right[:64], w, 64)  # warm-up
+def algorithmic_bytes_per_step(w, h, D, paths):
+    """The problem's algorithmic HBM bytes for one pair (both directions) with
+    uint16 cost and aggregate volumes: cost-build write 2 B, P cost reads 2P B,
+    P-1 aggregate reads 2(P-1) B, P aggregate writes 2P B, less the final
+    aggregate write that a selection fused into the last sweep does not make:
+    6P - 2 = 46 B per cell at 8 paths."""
+    return 2 * w * h * D * (6 * paths - 2)
+
+
+# Sources whose change invalidates the committed HBM traffic measurement of the
+# path sweeps (profiles/path_traffic.json carries the hash it was measured at).
+TRAFFIC_SOURCES = ["pose_amd/csrc/sgm_paths.h", "pose_amd/csrc/sgm_kernels.h",
+                   "pose_amd/csrc/sgm_internal.h", "pose_amd/csrc/sgm_api.hip"]
+
+
+def kernel_source_hash():
+    h = hashlib.sha256()
+    for f in TRAFFIC_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def cpu_info():
+    """Host CPU model (lscpu) and the processors this process may use."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"model": model, "nproc": len(os.sched_getaffinity(0))}
+
+
+def timed_trials(fn, trials=3):
+    """MatchingCostBenchmark protocol: one discarded warm-up, then `trials`
+    timed runs; the median."""
+    fn()
     times = []
     for _ in range(trials):
         t0 = time.perf_counter()
-        O.compute(op, left, right, w, rows)
+        fn()
         times.append(time.perf_counter() - t0)
-    t = sorted(times)[len(times) // 2]
-    updates = w * rows * D * 8 * 2
-    return dict(value=round(updates / t / 1e9, 4), unit="Gupdates/s", cores=1, kind="port",
-                sample="%dx%d strip of the 1600x1067 D=%d workload, both directions + L-R, "
-                       "C oracle single-threaded, %.2f s" % (w, rows, D, t))
+    return sorted(times)[len(times) // 2], times
+
+
+def cpu_baseline(w, h, D, rows=256, trials=3):
+    """The C oracle on the host: (a) one thread in the reference's loop order on
+    a `rows`-row strip of the workload, (b) split over every host core on the
+    whole pair; one warm-up and `trials` timed runs each, median."""
+    from oracle import oracle as O
+    from pose_amd import synth
+    left, right, _ = synth.stereo_pair(w, h, D, 0, seed=2)
+    op = O.params(window_w=9, window_h=7, disparities=D, p1=8, p2=96, lr_threshold=1.0)
+    t1, runs1 = timed_trials(lambda: O.compute(op, left[:rows], right[:rows], w, rows), trials)
+    single = w * rows * D * 8 * 2 / t1 / 1e9
+    threads = len(os.sched_getaffinity(0))
+    tn, runsn = timed_trials(lambda: O.compute_parallel(op, left, right, w, h, threads=threads),
+                             trials)
+    allcores = w * h * D * 8 * 2 / tn / 1e9
+    info = cpu_info()
+    return dict(value=round(allcores, 4), unit="Gupdates/s", cores=threads, kind="port",
+                sample="whole %dx%d D=%d pair, both directions + L-R, C oracle split over %d "
+                       "host threads (OpenMP); 1 warm-up + %d trials, median %.2f s"
+                       % (w, h, D, threads, trials, tn),
+                trials_s=[round(t, 3) for t in runsn], cpu_model=info["model"],
+                nproc=info["nproc"],
+                single_thread={"value": round(single, 4), "unit": "Gupdates/s", "cores": 1,
+                               "sample": "%dx%d strip of the same pair, C oracle in the "
+                                         "reference's loop order; 1 warm-up + %d trials, "
+                                         "median %.2f s" % (w, rows, trials, t1),
+                               "trials_s": [round(t, 3) for t in runs1]})
 
 
 def batch_workload(count=48, seed=4):
@@ -408,7 +466,72 @@ def run_served(sgm, params_for, rate=20.0, count=60, budget=250e9, max_batch=8, 
     return out
 
 
+def read_counter_csv(directory, counter):
+    """Per-dispatch values of one PMC counter from a rocprofv3 --pmc CSV run:
+    [(kernel name, value)] in dispatch order."""
+    import csv
+    import glob
+    rows = []
+    for f in glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Counter_Name") == counter:
+                    rows.append((int(r.get("Dispatch_Id", 0)), r["Kernel_Name"],
+                                 float(r["Counter_Value"])))
+    rows.sort()
+    return [(k, v) for _, k, v in rows]
+
+
+def write_traffic(fetch_dir, write_dir, copy_bytes=2 << 30, paths=8):
+    """profiles/path_traffic.json from two rocprofv3 passes over
+    `bench.py --steps 1 --warmup 1 --profile-steps 1 --no-...` (FETCH_SIZE,
+    then WRITE_SIZE, KiB per dispatch).  Each counter is scaled by what it
+    reads for the library's copy kernel (copy_bytes read + copy_bytes written
+    per launch); path launches are grouped by sweep phase (first / middle /
+    last), and the per-launch figure weights them as one pair does."""
+    W, H, D = 1600, 1067, 256
+    fetch, write = read_counter_csv(fetch_dir, "FETCH_SIZE"), read_counter_csv(write_dir, "WRITE_SIZE")
+
+    def mean_of(rows, pred):
+        v = [x for k, x in rows if pred(k)]
+        if not v:
+            raise SystemExit("no dispatches matched in the counter CSV")
+        return sum(v) / len(v)
+
+    is_copy = lambda k: "copy_kernel" in k
+    fetch_scale = copy_bytes / 1024.0 / mean_of(fetch, is_copy)
+    write_scale = copy_bytes / 1024.0 / mean_of(write, is_copy)
+    cells = 2 * W * H * D
+    alg = {"first": 3 * cells, "mid": 5 * cells, "last": 3 * cells}
+    out = {"workload": "1600x1067, census 9x7, D=256, 8 paths, both matching directions per launch",
+           "source_hash": kernel_source_hash(), "fetch_scale": round(fetch_scale, 4),
+           "write_scale": round(write_scale, 4),
+           "calibration": "copy_kernel: %d B read + %d B written per launch; FETCH_SIZE read %.0f "
+                          "KiB, WRITE_SIZE %.0f KiB" % (copy_bytes, copy_bytes,
+                                                        mean_of(fetch, is_copy),
+                                                        mean_of(write, is_copy)),
+           "per_launch": {}}
+    for phase, tag in (("first", ", 0, "), ("mid", ", 1, "), ("last", ", 2, ")):
+        pred = lambda k, tag=tag: "path_kernel" in k and tag in k
+        names = sorted({k for k, _ in fetch if pred(k)})
+        rd = mean_of(fetch, pred) * 1024 * fetch_scale
+        wr = mean_of(write, pred) * 1024 * write_scale
+        out["per_launch"][phase] = {"kernel": names, "read_bytes": int(rd), "write_bytes": int(wr),
+                                    "hbm_bytes": int(rd + wr), "algorithmic_bytes": alg[phase],
+                                    "ratio": round((rd + wr) / alg[phase], 4)}
+    pl = out["per_launch"]
+    out["bytes_per_launch"] = int((pl["first"]["hbm_bytes"] + (paths - 2) * pl["mid"]["hbm_bytes"]
+                                   + pl["last"]["hbm_bytes"]) / paths)
+    out["build_bytes_per_launch"] = int((alg["first"] + (paths - 2) * alg["mid"] + alg["last"]) / paths)
+    with open(os.path.join(ROOT, "profiles", "path_traffic.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out))
+
+
 def main():
+    if len(sys.argv) == 4 and sys.argv[1] == "--write-traffic":
+        write_traffic(sys.argv[2], sys.argv[3])
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -474,8 +597,10 @@ def main():
     ctx.set_profiling(False)
     path_ms = prof["path"]["total_ms"] / args.profile_steps
     path_launches = prof["path"]["launches"] / args.profile_steps
-    bytes_step = path_bytes_per_step(W, H, D, P)
-    achieved = bytes_step / (path_ms / 1e3) / 1e9
+    launch_s = path_ms / 1e3 / path_launches
+    alg_launch = algorithmic_bytes_per_step(W, H, D, P) / path_launches
+    build_launch = path_bytes_per_step(W, H, D, P) / path_launches
+    achieved = alg_launch / launch_s / 1e9
     kernels = {k: round(v["total_ms"] / args.profile_steps, 4) for k, v in prof.items()}
 
     # Achievable HBM bandwidth: the library's plain streaming copy kernel over
@@ -497,10 +622,15 @@ def main():
     hbm_copy = {"bytes_per_launch": 2 * copy_bytes, "ms": round(copy_ms, 4),
                 "gbps": round(2 * copy_bytes / (copy_ms / 1e3) / 1e9, 1)}
 
-    traffic = None
+    # HBM bytes per launch from the calibrated FETCH_SIZE / WRITE_SIZE passes,
+    # valid only for the kernel sources they were measured on
+    traffic, traffic_hash, source_hash = None, None, kernel_source_hash()
     tfile = os.path.join(ROOT, "profiles", "path_traffic.json")
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("bytes_per_launch")
+        tj = json.load(open(tfile))
+        traffic_hash = tj.get("source_hash")
+        if traffic_hash == source_hash:
+            traffic = tj.get("bytes_per_launch")
 
     streamed = None if args.no_streamed else run_streamed(sgm, W, H, D, P, d_left, d_right)
     esgm = None if args.no_esgm else run_esgm(sgm, W, H, D, P, d_left, d_right)
@@ -522,7 +652,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(W, D)
+        cpu = cpu_baseline(W, H, D)
 
     lat_ms = sorted(x * 1e3 for x in lat)
     line = {
@@ -551,7 +681,19 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
                      "traffic": traffic, "kernel": "sgm::path_kernel",
                      "launches_per_step": path_launches,
-                     "algorithmic_bytes_per_launch": bytes_step / max(1, path_launches),
+                     "launch_ms": round(launch_s * 1e3, 4),
+                     "algorithmic_bytes_per_launch": alg_launch,
+                     "algorithmic_model": "6P-2 B per cell (uint16 C and S, selection fused "
+                                          "into the last sweep), 2*W*H*D cells per pair",
+                     "build_bytes_per_launch": build_launch,
+                     "build_achieved": round(build_launch / launch_s / 1e9, 1),
+                     "build_frac": round(build_launch / launch_s / 1e9 / PEAK_HBM_GBPS, 4),
+                     "traffic_achieved": None if traffic is None
+                     else round(traffic / launch_s / 1e9, 1),
+                     "traffic_frac": None if traffic is None
+                     else round(traffic / launch_s / 1e9 / PEAK_HBM_GBPS, 4),
+                     "traffic_source_hash": traffic_hash, "source_hash": source_hash,
+                     "traffic_stale": traffic is None,
                      "copy_kernel": hbm_copy},
         "cpu_baseline": cpu,
         "filtered": filtered,
