@@ -366,6 +366,7 @@ typedef struct sgm_request_info {
   int64_t device_bytes;
   int32_t status;
   int32_t batch_id;
+  int64_t pool_bytes;           /* device memory the server's requests held once this one started */
 } sgm_request_info;
 
 /* budget_bytes bounds the device footprint of admitted requests (cost +

@@ -223,8 +223,9 @@ int64_t direction_bytes(const sgm_params *p, int w, int h) {
 
 int64_t pair_bytes(const sgm_params *p, int w, int h, bool cost_volume) {
   const int64_t n = (int64_t)w * h;
-  // both directions + input copies, validity, reasons/boundary scratch, checked, stats
-  return 2 * direction_bytes(p, w, h, cost_volume) + n * (2 + 2 + 2 + 4) + 256;
+  // both directions + validity copies (2n), reasons / boundary in and out and
+  // selection codes (5n), checked (4n), counters
+  return 2 * direction_bytes(p, w, h, cost_volume) + n * (2 + 5 + 4) + 256;
 }
 
 int64_t pair_bytes(const sgm_params *p, int w, int h) {
@@ -243,7 +244,7 @@ int64_t streamed_pair_bytes(const sgm_params *p, int w, int h, int rows) {
   // both matching directions: strip cost + two aggregate slots each, carry rows each
   const int64_t strip = 2 * (int64_t)w * rows * Dp * (2 + 2 * sb);
   const int64_t carries = 2 * 2 * 3 * (int64_t)w * Dp * 4;
-  return 2 * inputs + strip + carries + n * (2 + 2 + 2 + 4) + 256;
+  return 2 * inputs + strip + carries + n * (2 + 5 + 4) + 256;
 }
 
 // The aggregates of both matching directions, parked between the two passes.
@@ -277,7 +278,7 @@ int64_t esgm_pair_bytes(const sgm_params *p, int w, int h, int rows, int pool) {
   const int64_t inputs = direction_bytes(p, w, h, false) - n * Dp * sb;
   const int64_t strip = esgm_computed(p) ? 0 : (int64_t)w * rows * Dp * 2 + 6 * (int64_t)w * Dp * 4;
   const int64_t per = n * (C * 8 + 2 + 4 + 4 + 12) + strip;
-  return 2 * inputs + 2 * per + n * (2 + 2 + 2 + 4) + 2 * Dp * 2 + 512;
+  return 2 * inputs + 2 * per + n * (2 + 5 + 4) + 2 * Dp * 2 + 512;
 }
 
 // Why the compact plan cannot run these parameters (GpuCandidateStereo.java:

@@ -83,6 +83,7 @@ struct Request {
   sgm_pair pair;
   sgm_stats stats;
   size_t bytes = 0, off = 0;
+  int64_t pool_bytes = 0;  // device memory the server held once this request started
   double arrival = 0, admitted = 0, started = 0, finished = 0;
   int status = 1;  // 1 pending, 0 done, < 0 error
   int batch_id = -1;
@@ -258,7 +259,10 @@ void sgm_server::run() {
         continue;
       }
       s.batch = b;
-      for (Request *r : take) r->started = ts;
+      for (Request *r : take) {
+        r->started = ts;
+        r->pool_bytes = (int64_t)pool.used();
+      }
     }
     bool any_busy = false;
     for (Slot &s : slots) any_busy |= s.busy;
@@ -271,25 +275,45 @@ void sgm_server::run() {
 }
 
 // Lane mode: the pair at the head of the queue on free lane `s` when the
-// device memory all lanes hold, with this lane grown to the pair's footprint,
-// stays within the budget; cached buffers of idle lanes are released first if
-// that makes it fit.  Called and returning with `mu` held.
+// plain-path footprint it will allocate fits what the other lanes leave of
+// the budget; cached buffers of idle lanes are released first if that makes
+// it fit.  The lane's own budget is capped at that remainder, so its plan
+// (resident cost volume or recomputed census costs) is chosen against the
+// memory actually left, and its cached buffers are dropped first when they
+// and the new footprint together would pass it.  Called and returning with
+// `mu` held.
 bool sgm_server::admit_lane(Slot &s) {
   Request *r = pending.front();
-  const int64_t need = (int64_t)r->bytes;
-  auto committed = [&]() {
+  sgm_params q = params;
+  q.disparities = r->pair.disparities;
+  q.min_disparity = r->pair.min_disparity;
+  const int w = r->pair.width, h = r->pair.height;
+  auto others = [&]() {
     int64_t c = 0;
     for (Slot &o : slots)
       if (&o != &s) c += o.lane->allocated;
-    return c + std::max(s.lane->allocated, need);
+    return c;
   };
-  if (committed() > (int64_t)region_bytes) {
+  // the footprint of sgm_match_pair on a lane whose budget is `avail`
+  // (recompute_cost's rule: the cost volume stays when the pair fits with it)
+  auto need_for = [&](int64_t avail) {
+    const bool possible = census_recompute_possible(&q) && !s.lane->keep_volumes;
+    const int64_t with = pair_bytes(&q, w, h, true);
+    const int64_t without = possible ? pair_bytes(&q, w, h, false) : with;
+    if (s.lane->volume_policy == SGM_VOLUME_RECOMPUTE) return without;
+    if (s.lane->volume_policy == SGM_VOLUME_RESIDENT) return with;
+    return with <= avail ? with : without;
+  };
+  int64_t avail = (int64_t)region_bytes - others();
+  if (need_for(avail) > avail) {
     for (Slot &o : slots)
       if (!o.busy && &o != &s && o.lane->allocated) (void)sgm_release_buffers(o.lane);
-    if (committed() > (int64_t)region_bytes && s.lane->allocated > need)
-      (void)sgm_release_buffers(s.lane);
-    if (committed() > (int64_t)region_bytes) return false;
+    avail = (int64_t)region_bytes - others();
+    if (need_for(avail) > avail) return false;
   }
+  const int64_t need = need_for(avail);
+  if (s.lane->allocated && s.lane->allocated + need > avail) (void)sgm_release_buffers(s.lane);
+  s.lane->budget = avail;
   pending.pop_front();
   const double t = now();
   r->admitted = t;
@@ -315,6 +339,7 @@ bool sgm_server::admit_lane(Slot &s) {
     return true;
   }
   r->started = ts;
+  r->pool_bytes = others() + s.lane->allocated;
   return true;
 }
 
@@ -433,6 +458,7 @@ extern "C" int sgm_server_wait(sgm_server *srv, uint64_t ticket, double timeout_
     info->started_s = r->started;
     info->finished_s = r->finished;
     info->device_bytes = (int64_t)r->bytes;
+    info->pool_bytes = r->pool_bytes;
     info->status = r->status;
     info->batch_id = r->batch_id;
   }

@@ -107,7 +107,7 @@ class SgmRequestInfo(ctypes.Structure):
     _fields_ = [("arrival_s", ctypes.c_double), ("admitted_s", ctypes.c_double),
                 ("started_s", ctypes.c_double), ("finished_s", ctypes.c_double),
                 ("device_bytes", ctypes.c_int64), ("status", ctypes.c_int32),
-                ("batch_id", ctypes.c_int32)]
+                ("batch_id", ctypes.c_int32), ("pool_bytes", ctypes.c_int64)]
 
 
 EXPORTS = [

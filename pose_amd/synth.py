@@ -32,7 +32,10 @@ def _load():
 
 
 def stereo_pair(width, height, disparities, min_disparity=0, seed=0, noise=2.0, gain=False):
-    """Returns (left, right, disparity): uint8 (h, w) images and float32 plane disparities."""
+    """Returns (left, right, disparity): uint8 (h, w) images and float32 plane
+    disparities on the right image's pixel grid (right[y, x] shows the left
+    texture at x + disparity[y, x]): the truth of the right-reference map;
+    left_view() warps it to the left image."""
     left = np.empty((height, width), np.uint8)
     right = np.empty((height, width), np.uint8)
     disp = np.empty((height, width), np.float32)
@@ -56,4 +59,17 @@ def batch_pairs(count=48, seed=0, long_side=(800, 3000), disparities=(64, 384), 
         D = int(rng.integers(disparities[0], disparities[1] + 1))
         md = int(rng.integers(offset[0], offset[1] + 1))
         out.append((w, h, D, md, seed * 1000 + i))
+    return out
+
+
+def left_view(disparity):
+    """Left-image truth from the right-grid disparity of stereo_pair: each right
+    pixel x lands on left pixel round(x + d); left pixels no right pixel lands on
+    (occluded) are NaN."""
+    h, w = disparity.shape
+    out = np.full((h, w), np.nan, np.float32)
+    xs = np.arange(w)[None, :] + np.rint(disparity).astype(np.int64)
+    ys = np.broadcast_to(np.arange(h)[:, None], (h, w))
+    m = (xs >= 0) & (xs < w)
+    out[ys[m], xs[m]] = disparity[m]
     return out

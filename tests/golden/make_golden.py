@@ -4,7 +4,15 @@ The reference holds no SGM golden vectors (its only stored hash covers unshipped
 inputs), so these fixtures are produced by the pinned oracle on the reference's
 regenerated known-answer inputs and on seeded synthetic pairs.  Run from the repo
 root: python tests/golden/make_golden.py
+
+`--large` writes the full-size fixtures (tests/golden/large_*.json): pairs whose
+volumes exceed 4 GiB, too large to store as maps, so the fixture holds the
+generator arguments, SHA-256 hashes of the input rasters and of every output
+map, and the counters, all from the oracle split over host threads (which
+equals the single-threaded oracle bit for bit).
 """
+import hashlib
+import json
 import os
 import sys
 
@@ -42,7 +50,41 @@ def cases():
            dict(window_w=7, window_h=7, disparities=32, cost=4, p1=12, p2=150, lr_threshold=1.0))
 
 
+def large_cases():
+    # 6000x700 with 512 disparities: a uint16 aggregate volume of 4.3 GB per
+    # matching direction (pixel offsets past 2^32 bytes), census 9x7, the
+    # served stream's parameters
+    yield ("large_6000x700_census9x7_d512", dict(width=6000, height=700, disparities=512,
+                                                 min_disparity=0, seed=501),
+           dict(window_w=9, window_h=7, disparities=512, p1=10, p2=120, lr_threshold=1.0))
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def main_large():
+    for name, gen, kw in large_cases():
+        left, right, _ = synth.stereo_pair(gen["width"], gen["height"], gen["disparities"],
+                                           gen["min_disparity"], seed=gen["seed"])
+        p = O.params(**kw)
+        w, h = gen["width"], gen["height"]
+        res = O.compute_parallel(p, left, right, w, h)
+        out = {"generator": gen, "params": {f: getattr(p, f) for f, _ in O.OracleParams._fields_},
+               "input_sha256": {"left": sha(left), "right": sha(right)},
+               "output_sha256": {k: sha(res[k].view(np.uint32) if res[k].dtype == np.float32
+                                        else res[k]) for k in
+                                 ("checked", "dl", "dr", "reasons", "boundary")},
+               "stats": res["stats"]}
+        with open(os.path.join(HERE, name + ".json"), "w") as fh:
+            json.dump(out, fh, indent=1)
+        print(name, res["stats"]["valid"], "valid")
+
+
 def main():
+    if "--large" in sys.argv:
+        main_large()
+        return
     for name, left, right, vl, vr, kw in cases():
         p = O.params(**kw)
         h, w = left.shape

@@ -85,3 +85,40 @@ def test_pfm_round_trip_and_layout():
         w, h, back = dense.read_pfm(path)
     assert (w, h) == (7, 5)
     assert np.array_equal(back.view(np.uint32), d.view(np.uint32))
+
+
+def test_pair_range_regression_expansion_known_answers():
+    """PairRangeRegression.expansion (PairRangeRegression.java:158-208): the
+    reference's own assertions on DisparityRange.expansion, same inputs."""
+    o = opts()
+    current = opts(minDisparity=450, disparities=64)
+    st = stats(lower=100, full=1000)
+    nxt = dense.expansion(o, current, st, 1000, 0)
+    assert nxt is not None and nxt.minimum < 450 and nxt.maximum() >= 513   # lower pressure
+    st = stats(lower=0, upper=100, full=1000)
+    nxt = dense.expansion(o, current, st, 1000, 0)
+    assert nxt.minimum == 450 and nxt.maximum() > 513                      # upper pressure
+    st = stats(lower=100, upper=100, full=1000)
+    nxt = dense.expansion(o, current, st, 1000, 0)
+    assert nxt.minimum < 450 and nxt.maximum() > 513                       # both together
+    nxt = dense.expansion(o, current, st, 1000, 3)
+    assert nxt.minimum == 0 and nxt.maximum() == 999                       # whole-domain fallback
+    pinned = opts(minDisparity=450, minDisparityExplicit=True)
+    assert dense.expansion(pinned, current, stats(lower=100, upper=0, full=1000), 1000, 0) is None
+    fixed = opts(disparitiesExplicit=True)
+    assert dense.expansion(fixed, current, stats(lower=100, upper=100, full=1000), 1000, 0) is None
+    # incomplete border intervals (no full-interval pixels) cannot trigger a retry
+    border = dict(stats(full=0), boundary_rejected=100000)
+    assert dense.expansion(o, current, border, 1000, 0) is None
+    # sparse endpoint noise does not cause retries
+    assert dense.expansion(o, current, stats(lower=3, upper=3, full=1000), 1000, 0) is None
+
+
+def test_pair_range_regression_bounds_known_answers():
+    """PairRangeRegression.java:138-155: an unusable physical minimum and an
+    observed span over 4096 hypotheses fail instead of overflowing or
+    truncating."""
+    with pytest.raises(ValueError):
+        dense.bounds(999, 999, 1000, opts(minDisparity=999))
+    with pytest.raises(ValueError):
+        dense.bounds(0, 8000, 10000, opts())

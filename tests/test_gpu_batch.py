@@ -324,3 +324,42 @@ def test_server_lanes_large_request_after_small_ones(gpu_context):
         assert np.array_equal(bits(got["checked"]), bits(want["checked"]))
     finally:
         srv.close()
+
+
+def test_server_lanes_stay_within_budget_mixed_sizes(gpu_context):
+    """Lanes admit on the footprint the plain path will allocate, capped at
+    what the other lanes leave: with a budget below two resident 900x700 D=256
+    pairs, mixed sizes never make the lanes together hold more than the budget
+    (a lane recomputes census costs or waits), and every result equals its solo
+    run."""
+    ctx = gpu_context
+    small = uneven_pairs(9, 6, wrange=(300, 500), drange=(100, 200))
+    bigs = [synth.stereo_pair(900, 700, 256, 0, seed=90 + i)[:2] for i in range(3)]
+    pairs = []
+    for i, pr in enumerate(small):
+        pairs.append(pr)
+        if i % 2 == 0:
+            pairs.append(dict(left=bigs[i // 2][0], right=bigs[i // 2][1], disparities=256,
+                              min_disparity=0))
+    pbig = to_sgm(O.params(**dict(CENSUS97, disparities=256)))
+    plan = ctx.plan(pbig, 900, 700)
+    budget = int(plan["pair_bytes"] * 1.6)
+    alone = []
+    for pr in pairs:
+        p = to_sgm(O.params(**dict(CENSUS97, disparities=pr["disparities"],
+                                   min_disparity=pr["min_disparity"])))
+        alone.append(ctx.match_pair(p, pr["left"], pr["right"]))
+    params = to_sgm(O.params(**dict(CENSUS97, disparities=64)))
+    srv = sgm.Server(ctx, params, budget_bytes=budget, max_batch=1, streams=3)
+    try:
+        tickets = [srv.submit(pr["left"], pr["right"], pr["disparities"], pr["min_disparity"])
+                   for pr in pairs]
+        peak = 0
+        for i, t in enumerate(tickets):
+            got = srv.wait(t, timeout=120.0)
+            peak = max(peak, got["info"]["pool_bytes"])
+            for k in ("checked", "dl", "dr"):
+                assert np.array_equal(bits(got[k]), bits(alone[i][k])), (i, k)
+        assert 0 < peak <= budget, (peak, budget)
+    finally:
+        srv.close()

@@ -5,6 +5,7 @@ in one batch; each final result equals the plain path on the final interval."""
 import numpy as np
 import pytest
 
+from oracle import oracle as O
 from pose_amd import dense, sgm
 from tests.cases import shifted
 from tests.test_gpu_parity import bits
@@ -51,3 +52,60 @@ def test_explicit_range_is_not_expanded(gpu_context):
     a, b = shifted(300, 160, 40, 30)
     out = dense.search(gpu_context, o, [dict(left=a, right=b, minimum=0, count=16)])
     assert out[0]["stop_reason"] == "explicit" and len(out[0]["attempts"]) == 1
+
+
+def oracle_params(o, minimum, count):
+    p = sgm.params_from_options(o)
+    p.min_disparity, p.disparities = minimum, count
+    return O.params(**{f: getattr(p, f) for f, _ in O.OracleParams._fields_})
+
+
+def test_search_attempts_equal_the_oracle(gpu_context):
+    """Every attempt of the batched search is the oracle's match on that
+    attempt's interval: same boundary counters (which drive the expansion) and,
+    for the final attempt, the same L-R-validated map bit for bit."""
+    o = sgm.Options(census=5, lrThreshold=1.0)
+    specs = [(200, 60, 30, 0, 24), (170, 64, 45, 0, 16), (190, 56, 12, 20, 24)]
+    pairs = []
+    for i, (w, h, shift, lo, count) in enumerate(specs):
+        a, b = shifted(500 + i, w, h, shift)
+        pairs.append(dict(left=a, right=b, minimum=lo, count=count))
+    out = dense.search(gpu_context, o, pairs, graph=False)
+    retried = 0
+    for p, r in zip(pairs, out):
+        h, w = p["left"].shape
+        retried += len(r["attempts"]) > 1
+        for att in r["attempts"]:
+            print("attempt", w, h, att["min_disparity"], att["disparities"], flush=True)
+            want = O.compute(oracle_params(o, att["min_disparity"], att["disparities"]),
+                             p["left"], p["right"], w, h)
+            for k in ("lower_boundary_rejected", "upper_boundary_rejected",
+                      "full_interval_pixels"):
+                assert att[k] == want["stats"][k], (att, k)
+            assert att["final_valid"] == want["stats"]["valid"]
+        assert np.array_equal(bits(r["checked"]), bits(want["checked"]))
+        assert np.array_equal(r["reasons"], want["reasons"])
+        assert np.array_equal(r["boundary"], want["boundary"])
+    assert retried >= 2  # the lower and the upper expansion both ran
+
+
+@pytest.mark.parametrize("low", [True, False])
+def test_pair_range_regression_boundary_on_gpu(gpu_context, low):
+    """PairRangeRegression.boundary (PairRangeRegression.java:210-249): with a
+    clipped range the GPU's lower/upper/eligible counters equal the CPU's, and
+    the per-pixel boundary flags equal the counters."""
+    w, h = 160, 96
+    a, b = shifted(497, w, h, 8)
+    o = sgm.Options(minDisparity=8 if low else 3, disparities=17 if low else 6, speckleSize=0)
+    p = sgm.params_from_options(o)
+    got = gpu_context.match_pair(p, a, b)
+    want = O.compute(O.params(**{f: getattr(p, f) for f, _ in O.OracleParams._fields_}),
+                     a, b, w, h)
+    st = got["stats"]
+    for k in ("lower_boundary_rejected", "upper_boundary_rejected", "full_interval_pixels"):
+        assert st[k] == want["stats"][k], k
+    assert st["full_interval_pixels"] > 5000
+    assert dense.lower_clipped(st) if low else dense.upper_clipped(st)
+    flag = 1 if low else 2
+    hits = int(((got["boundary"] & flag) != 0).sum())
+    assert hits == (st["lower_boundary_rejected"] if low else st["upper_boundary_rejected"])
