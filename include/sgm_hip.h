@@ -31,6 +31,7 @@ extern "C" {
 #define SGM_ERR_REJECTED (-6)       /* serving admission refused the request */
 #define SGM_ERR_TIMEOUT (-7)
 #define SGM_ERR_NOT_FOUND (-8)
+#define SGM_ERR_INTERRUPTED (-9)    /* sgm_interrupt stopped the call (GpuStereo.java:350-351) */
 
 /* Matching costs: 0..3 are StereoCost.code (StereoCost.java:7-15); 3 is the
  * reference's exact integer/FP64 ZNCC.  4 is the FP32 ZNCC variant with cost
@@ -108,6 +109,13 @@ typedef struct sgm_plan {
   int32_t esgm_candidates;      /* proposals per path that fit (0: no compact plan) */
   int32_t esgm_rows;            /* strip rows of the compact plan */
   int64_t esgm_bytes;           /* device footprint of a pair under the compact plan */
+  /* Host storage of the streamed aggregate (GpuStereo.Plan.aggregateStorage,
+   * blockRows, aggregateBytes; GpuStereo.java:61-83). */
+  int32_t aggregate_storage;    /* SGM_AGGREGATE_RAM / _DISK / _RECOMPUTE of the streamed plan */
+  int32_t block_rows;           /* rows finished per recomputation block (h unless recompute) */
+  int64_t disk_bytes;           /* temporary file of the disk storage */
+  int32_t compact;              /* 1 if a pair runs the compact plan under the plan mode */
+  int32_t reserved;
 } sgm_plan;
 
 typedef struct sgm_context sgm_context;
@@ -281,6 +289,35 @@ int sgm_read_volumes(sgm_context *ctx, int direction, uint16_t *cost, uint32_t *
  * SGM_ERR_NO_PLAN. */
 #define SGM_PLAN_ESGM 3
 int sgm_set_plan_mode(sgm_context *ctx, int mode, int strip_rows);
+
+/* Host storage of the streamed plan's aggregate between its top-down and
+ * bottom-up pass (--gpu-aggregate, GpuStereo.plan, GpuStereo.java:56-83;
+ * GpuAggregate.java).  RAM: pinned host memory for the whole aggregate of both
+ * matching directions.  DISK: a temporary file in `directory` (removed as soon
+ * as it is open; free space is checked first), staged through two pinned strip
+ * slots.  RECOMPUTE: host memory for a block of rows only; blocks are finished
+ * one after another and each block's passes re-sweep the strips above and
+ * below it for the path state.  AUTO: RAM when the aggregate fits the host
+ * budget, else RECOMPUTE, and under SGM_PLAN_AUTO a pair that only fits with
+ * RECOMPUTE runs the compact plan when it supports the pair
+ * (GpuStereo.java:176-193); a RAM allocation that fails falls back to
+ * RECOMPUTE (GpuStereo.java:198-208).  host_budget_bytes bounds the host
+ * memory of the streamed plan (0: 65% of physical memory; Options.memoryMb).
+ * Results are identical under every storage. */
+#define SGM_AGGREGATE_AUTO 0
+#define SGM_AGGREGATE_RAM 1
+#define SGM_AGGREGATE_DISK 2
+#define SGM_AGGREGATE_RECOMPUTE 3
+int sgm_set_aggregate_storage(sgm_context *ctx, int storage, const char *directory,
+                              int64_t host_budget_bytes);
+
+/* Interrupt the context's running (or next) matching call from any thread:
+ * the call stops at its next strip, block or path launch, waits for the work
+ * already queued and fails with SGM_ERR_INTERRUPTED ("GPU SGM interrupted",
+ * the reference's per-strip Thread.isInterrupted check, GpuStereo.java:
+ * 350-351).  The request is consumed by the call that sees it; the context
+ * stays usable. */
+int sgm_interrupt(sgm_context *ctx);
 /* --esgm-candidates (Options.java:35,781-782): proposals per path, 1..4; the
  * plan lowers it until the pair fits the budget. */
 int sgm_set_esgm_candidates(sgm_context *ctx, int candidates);
@@ -313,6 +350,9 @@ void *sgm_stream(sgm_context *ctx);
 /* Host-forceable failure hook for tests of the fallback path: the next
  * `count` device allocations fail as if out of memory. */
 int sgm_debug_fail_allocations(sgm_context *ctx, int count);
+/* The same for pinned host allocations (the reference's AllocationFailure of
+ * a planned RAM aggregate, GpuAggregate.java:11-18). */
+int sgm_debug_fail_host_allocations(sgm_context *ctx, int count);
 
 /* Message of the last failing call on this thread. */
 const char *sgm_last_error(void);

@@ -17,6 +17,10 @@
 #include <string>
 #include <vector>
 
+#include <fcntl.h>
+#include <sys/statvfs.h>
+#include <unistd.h>
+
 #include "../../include/sgm_hip.h"
 #include "sgm_internal.h"
 #include "sgm_filters.h"
@@ -120,9 +124,14 @@ void Buffer::release(sgm_context *ctx) {
   cap = 0;
 }
 
-int HostBuffer::ensure(size_t bytes, const char *what) {
+int HostBuffer::ensure(size_t bytes, const char *what, int *fail_counter) {
   if (bytes <= cap && ptr) return SGM_OK;
   release();
+  if (fail_counter && *fail_counter > 0) {
+    (*fail_counter)--;
+    return fail(SGM_ERR_OUT_OF_MEMORY, "pinned host allocation (%zu MiB) for %s failed (simulated)",
+                bytes >> 20, what);
+  }
   hipError_t e = hipHostMalloc(&ptr, bytes ? bytes : 8, hipHostMallocDefault);
   if (e != hipSuccess) {
     (void)hipGetLastError();
@@ -252,12 +261,72 @@ int64_t streamed_host_bytes(const sgm_params *p, int w, int h) {
   return 2 * (int64_t)w * h * padded_disparities(p->disparities) * std::max(aggregate_bytes(p), 2);
 }
 
-// Largest strip height (gpuStripRows, halved until it fits) of the streamed
-// plan, 0 if none fits.
+// Host memory the streamed plan may use (GpuStereo.plan's hostBudget,
+// Options.memoryMb): the context's setting, else 65% of physical memory.
+int64_t host_budget(const sgm_context *ctx) {
+  if (ctx->host_budget > 0) return ctx->host_budget;
+  const long pages = sysconf(_SC_PHYS_PAGES), page = sysconf(_SC_PAGE_SIZE);
+  return pages > 0 && page > 0 ? (int64_t)pages * page / 100 * 65 : (int64_t)1 << 30;
+}
+
+// Free host memory (GpuStereo.availableRam): MemAvailable of /proc/meminfo.
+static int64_t available_ram() {
+  FILE *f = fopen("/proc/meminfo", "r");
+  if (!f) return INT64_MAX;
+  char line[256];
+  int64_t kb = -1;
+  while (fgets(line, sizeof line, f))
+    if (sscanf(line, "MemAvailable: %lld kB", (long long *)&kb) == 1) break;
+  fclose(f);
+  return kb > 0 ? kb * 1024 : INT64_MAX;
+}
+
+// GpuStereo.plan's streamed branch (GpuStereo.java:56-83) for aggregate
+// storage `storage` (SGM_AGGREGATE_*): the largest strip height (gpuStripRows,
+// halved) whose device footprint fits the budget and whose host side fits
+// the host budget.  RAM parks both directions' whole aggregates in pinned host
+// memory; DISK in a file, through two pinned strip slots per direction;
+// RECOMPUTE holds a block of as many strips as the host budget allows.  AUTO
+// is RAM when the aggregate fits the host budget (and three quarters of the
+// free memory), else RECOMPUTE.  rows = 0: no streamed plan fits.
+StreamedPlan streamed_plan(sgm_context *ctx, const sgm_params *p, int w, int h, int storage) {
+  StreamedPlan sp;
+  const int64_t Dp = padded_disparities(p->disparities), sb = std::max(aggregate_bytes(p), 2);
+  const int64_t full = 2 * (int64_t)w * h * Dp * sb, budget = host_budget(ctx);
+  const bool fits_ram = full <= budget && full <= available_ram() / 4 * 3;
+  for (int rows = std::min(h, ctx->strip_rows); rows >= 1; rows /= 2) {
+    const int64_t device = streamed_pair_bytes(p, w, h, rows);
+    if (device > ctx->budget) continue;
+    const int64_t stripe = 2 * (int64_t)w * rows * Dp * sb;  // one strip, both directions
+    sp.rows = rows;
+    sp.device_bytes = device;
+    sp.block_rows = h;
+    sp.disk_bytes = 0;
+    if (storage == SGM_AGGREGATE_RAM && !fits_ram) continue;
+    if (storage == SGM_AGGREGATE_DISK) {
+      if (2 * stripe > budget) continue;  // two pinned staging slots
+      sp.storage = SGM_AGGREGATE_DISK;
+      sp.host_bytes = 2 * stripe;
+      sp.disk_bytes = full;
+      return sp;
+    }
+    if (storage != SGM_AGGREGATE_RECOMPUTE && fits_ram) {
+      sp.storage = SGM_AGGREGATE_RAM;
+      sp.host_bytes = full;
+      return sp;
+    }
+    const int64_t stripes = budget / stripe;
+    if (stripes < 1) continue;
+    sp.storage = SGM_AGGREGATE_RECOMPUTE;
+    sp.block_rows = (int)std::min<int64_t>(h, stripes * rows);
+    sp.host_bytes = 2 * (int64_t)w * sp.block_rows * Dp * sb;
+    return sp;
+  }
+  return StreamedPlan();
+}
+
 int streamed_rows(sgm_context *ctx, const sgm_params *p, int w, int h) {
-  for (int rows = std::min(h, ctx->strip_rows); rows >= 1; rows /= 2)
-    if (streamed_pair_bytes(p, w, h, rows) <= ctx->budget) return rows;
-  return 0;
+  return streamed_plan(ctx, p, w, h, ctx->aggregate_storage).rows;
 }
 
 // Compact plan (GpuCandidateStereo.plan / deviceBytes, GpuCandidateStereo.java:
@@ -411,9 +480,13 @@ extern "C" int sgm_plan_get(sgm_context *ctx, const sgm_params *params, int w, i
   out->budget_bytes = ctx->budget;
   out->aggregate_bound = aggregate_bound(params);
   out->resident = out->pair_bytes <= ctx->budget;
-  out->strip_rows = streamed_rows(ctx, params, w, h);
-  out->streamed_bytes = out->strip_rows ? streamed_pair_bytes(params, w, h, out->strip_rows) : 0;
-  out->host_bytes = out->strip_rows ? streamed_host_bytes(params, w, h) : 0;
+  const StreamedPlan sp = streamed_plan(ctx, params, w, h, ctx->aggregate_storage);
+  out->strip_rows = sp.rows;
+  out->streamed_bytes = sp.device_bytes;
+  out->host_bytes = sp.host_bytes;
+  out->aggregate_storage = sp.rows ? sp.storage : 0;
+  out->block_rows = sp.rows ? sp.block_rows : 0;
+  out->disk_bytes = sp.disk_bytes;
   out->streamed = ctx->plan_mode == SGM_PLAN_STREAMED ? out->strip_rows > 0
                   : ctx->plan_mode == SGM_PLAN_AUTO   ? !out->resident && out->strip_rows > 0
                                                       : 0;
@@ -422,6 +495,13 @@ extern "C" int sgm_plan_get(sgm_context *ctx, const sgm_params *params, int w, i
   out->esgm_candidates = pool;
   out->esgm_rows = rows;
   out->esgm_bytes = pool ? esgm_pair_bytes(params, w, h, rows, pool) : 0;
+  // GpuStereo.execute (GpuStereo.java:176-193): the compact plan under
+  // --gpu-sgm esgm, or under auto when only a bounded recomputation would fit
+  out->compact = ctx->plan_mode == SGM_PLAN_ESGM ||
+                 (ctx->plan_mode == SGM_PLAN_AUTO && ctx->aggregate_storage == SGM_AGGREGATE_AUTO &&
+                  out->streamed && sp.storage == SGM_AGGREGATE_RECOMPUTE && pool > 0 &&
+                  !esgm_unsupported(params, w, h));
+  if (out->compact && ctx->plan_mode == SGM_PLAN_AUTO) out->streamed = 0;
   return SGM_OK;
 }
 
@@ -433,6 +513,32 @@ extern "C" int sgm_set_plan_mode(sgm_context *ctx, int mode, int strip_rows) {
   std::lock_guard<std::mutex> g(ctx->lock);
   ctx->plan_mode = mode;
   ctx->strip_rows = strip_rows;
+  return SGM_OK;
+}
+
+extern "C" int sgm_set_aggregate_storage(sgm_context *ctx, int storage, const char *directory,
+                                         int64_t host_budget_bytes) {
+  if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
+  if (storage < SGM_AGGREGATE_AUTO || storage > SGM_AGGREGATE_RECOMPUTE)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "aggregate storage %d not in 0..3", storage);
+  if (host_budget_bytes < 0) return fail(SGM_ERR_INVALID_ARGUMENT, "negative host budget");
+  std::lock_guard<std::mutex> g(ctx->lock);
+  ctx->aggregate_storage = storage;
+  if (directory && *directory) ctx->aggregate_dir = directory;
+  ctx->host_budget = host_budget_bytes;
+  return SGM_OK;
+}
+
+extern "C" int sgm_interrupt(sgm_context *ctx) {
+  if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
+  ctx->interrupt.store(1);
+  return SGM_OK;
+}
+
+extern "C" int sgm_debug_fail_host_allocations(sgm_context *ctx, int count) {
+  if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> g(ctx->lock);
+  ctx->fail_host_allocations = count;
   return SGM_OK;
 }
 
@@ -569,8 +675,14 @@ extern "C" int sgm_profile_json(sgm_context *ctx, char *buffer, size_t length) {
     first = false;
   }
   char b[256];
-  snprintf(b, sizeof b, "},\"allocated_bytes\":%lld,\"budget_bytes\":%lld}", (long long)ctx->allocated,
-           (long long)ctx->budget);
+  static const char *storages[4] = {"auto", "ram", "disk", "recompute"};
+  snprintf(b, sizeof b,
+           "},\"allocated_bytes\":%lld,\"budget_bytes\":%lld,\"last_plan\":{\"plan\":\"%s\","
+           "\"storage\":\"%s\",\"block_rows\":%d}}",
+           (long long)ctx->allocated, (long long)ctx->budget,
+           ctx->last_compact ? "compact" : ctx->last_streamed ? "streamed" : "resident",
+           ctx->last_streamed ? storages[ctx->splan.storage & 3] : "none",
+           ctx->last_streamed ? ctx->splan.block_rows : 0);
   s += b;
   if (s.size() + 1 > length) return fail(SGM_ERR_INVALID_ARGUMENT, "buffer too small (%zu)", s.size() + 1);
   memcpy(buffer, s.c_str(), s.size() + 1);
@@ -636,10 +748,25 @@ int launch_cost(sgm_context *ctx, const CostJobs &jobs, int njobs, int w, int h,
   if (force_u16) A.u8 = 0;
   const int gy = rows < 0 ? h : rows;
   if (p->cost == SGM_COST_ZNCC || p->cost == SGM_COST_ZNCC_FP32) {
-    const int ntx = (w + kZnccTX - 1) / kZnccTX, ntd = (Dp + kZnccTD - 1) / kZnccTD;
-    dim3 grid(ntx * ntd, gy, njobs);
-    zncc_cost_kernel<<<grid, 256, 0, ctx->stream>>>(jobs, A);
-    return launch_check("zncc cost kernel");
+    const int ntx = (w + kZnccTX - 1) / kZnccTX, ntd = (Dp + 63) / 64;
+    dim3 grid(ntx * ntd, (gy + kZnccBand - 1) / kZnccBand, njobs);
+    const bool out8 = A.u8 != 0;
+#define ZNCC_LAUNCH(RR)                                                                   \
+  case RR:                                                                                \
+    if (p->cost == SGM_COST_ZNCC_FP32)                                                    \
+      zncc_band_kernel<RR, 4, false><<<grid, 64, 0, ctx->stream>>>(jobs, A, gy);          \
+    else if (out8)                                                                        \
+      zncc_band_kernel<RR, 3, true><<<grid, 64, 0, ctx->stream>>>(jobs, A, gy);           \
+    else                                                                                  \
+      zncc_band_kernel<RR, 3, false><<<grid, 64, 0, ctx->stream>>>(jobs, A, gy);          \
+    break;
+    switch (p->window_w / 2) {
+      ZNCC_LAUNCH(1) ZNCC_LAUNCH(2) ZNCC_LAUNCH(3) ZNCC_LAUNCH(4) ZNCC_LAUNCH(5)
+      default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported ZNCC window");
+    }
+#undef ZNCC_LAUNCH
+    if (int rc = launch_check("zncc cost kernel")) return rc;
+    return SGM_OK;
   }
   const int quads = w * (Dp / 4);
   dim3 grid((quads + 255) / 256, gy, njobs);
@@ -797,7 +924,10 @@ struct PairBuffers {
   uint16_t *strip_cost;
   void *strip_sum;
   int *carry;
+  // host side of the streamed aggregate: the whole aggregate (RAM), a block of
+  // rows (RECOMPUTE) or two staging slots per direction (DISK)
   uint8_t *host_sum;
+  int storage, block_rows;
   // compact plan (esgm_pool > 0): proposals, their sums and the per-pixel
   // bound / slot / pixel list / summary of both matching directions
   int esgm_pool, esgm_rows;
@@ -893,8 +1023,35 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     B.e_carry = (int *)ctx->buffers["esgm_carry"].ptr;
     B.e_count = (int *)ctx->buffers["esgm_count"].ptr;
   }
+  B.storage = 0;
+  B.block_rows = h;
   if (rows && !pool) {
-    const size_t cells = (size_t)w * rows * Dp;
+    // host side first: a RAM aggregate that cannot be allocated falls back to
+    // bounded recomputation under --gpu-aggregate auto (GpuStereo.java:198-208)
+    StreamedPlan &sp = ctx->splan;
+    auto host_of = [&](const StreamedPlan &q) {
+      const size_t stripe = 2 * (size_t)w * q.rows * Dp * sb;  // both directions
+      return q.storage == SGM_AGGREGATE_RAM ? 2 * n * Dp * sb
+             : q.storage == SGM_AGGREGATE_RECOMPUTE ? 2 * (size_t)w * q.block_rows * Dp * sb
+                                                    : 2 * stripe;
+    };
+    rc = ctx->host_buffers["sum"].ensure(host_of(sp), "streamed aggregate",
+                                         &ctx->fail_host_allocations);
+    if (rc == SGM_ERR_OUT_OF_MEMORY && sp.storage == SGM_AGGREGATE_RAM &&
+        ctx->aggregate_storage == SGM_AGGREGATE_AUTO) {
+      const StreamedPlan retry = streamed_plan(ctx, p, w, h, SGM_AGGREGATE_RECOMPUTE);
+      if (retry.rows) {
+        sp = retry;
+        rc = ctx->host_buffers["sum"].ensure(host_of(sp), "streamed aggregate",
+                                             &ctx->fail_host_allocations);
+      }
+    }
+    if (rc) return rc;
+    B.host_sum = (uint8_t *)ctx->host_buffers["sum"].ptr;
+    B.storage = sp.storage;
+    B.block_rows = sp.block_rows;
+    B.strip_rows = sp.rows;
+    const size_t cells = (size_t)w * sp.rows * Dp;
     // up to two matching directions per strip launch
     ENSURE("strip_cost", 2 * cells * 2);
     ENSURE("strip_sum", 2 * 2 * cells * sb);
@@ -902,8 +1059,6 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     B.strip_cost = (uint16_t *)ctx->buffers["strip_cost"].ptr;
     B.strip_sum = ctx->buffers["strip_sum"].ptr;
     B.carry = (int *)ctx->buffers["carry"].ptr;
-    if ((rc = ctx->host_buffers["sum"].ensure(2 * n * Dp * sb, "streamed aggregate"))) return rc;
-    B.host_sum = (uint8_t *)ctx->host_buffers["sum"].ptr;
   }
 #undef ENSURE
   B.checked = (float *)ctx->buffers["checked"].ptr;
@@ -951,6 +1106,49 @@ int launch_strip(sgm_context *ctx, int R, int sum_bytes, const StripJobs &J, con
 #undef CASE_STRIP
   return launch_check("strip kernel");
 }
+
+// Temporary file of the disk aggregate storage (GpuAggregate.java:26-88): made
+// in the storage directory and unlinked at once, so it disappears when closed
+// (or when the process ends); written and read in whole strips.
+struct AggregateFile {
+  int fd = -1;
+  ~AggregateFile() {
+    if (fd >= 0) close(fd);
+  }
+  int open(const std::string &dir, int64_t bytes) {
+    struct statvfs sv;
+    if (statvfs(dir.c_str(), &sv) == 0) {
+      const int64_t avail = (int64_t)sv.f_bavail * (int64_t)sv.f_frsize;
+      if (avail > 0 && bytes + (16 << 20) > avail)
+        return fail(SGM_ERR_OUT_OF_MEMORY,
+                    "Streamed GPU SGM needs %lld MiB temporary aggregate storage; insufficient "
+                    "disk space", (long long)(bytes >> 20));
+    }
+    std::string path = dir + "/gpu-sgm-aggregate-XXXXXX";
+    std::vector<char> name(path.begin(), path.end());
+    name.push_back('\0');
+    fd = mkstemp(name.data());
+    if (fd < 0) return fail(SGM_ERR_INVALID_ARGUMENT, "cannot create a temporary aggregate in %s", dir.c_str());
+    unlink(name.data());
+    return SGM_OK;
+  }
+  int write(const uint8_t *src, size_t bytes, size_t off) {
+    for (size_t done = 0; done < bytes;) {
+      const ssize_t r = pwrite(fd, src + done, bytes - done, (off_t)(off + done));
+      if (r <= 0) return fail(SGM_ERR_HIP, "Streamed aggregate I/O made no progress");
+      done += (size_t)r;
+    }
+    return SGM_OK;
+  }
+  int read(uint8_t *dst, size_t bytes, size_t off) {
+    for (size_t done = 0; done < bytes;) {
+      const ssize_t r = pread(fd, dst + done, bytes - done, (off_t)(off + done));
+      if (r <= 0) return fail(SGM_ERR_HIP, "Incomplete streamed SGM aggregate");
+      done += (size_t)r;
+    }
+    return SGM_OK;
+  }
+};
 
 // Strip-streamed counterpart of run_directions (the streamed loop of
 // GpuStereo.executeTargets, GpuStereo.java:336-451).  The reference runs every
@@ -1037,67 +1235,134 @@ int run_directions_streamed(sgm_context *ctx, const sgm_params *p, int w, int h,
     return (uint8_t *)B.strip_sum + ((size_t)slot * nd + i) * slot_bytes;
   };
   const int passes = nup ? 2 : 1;
-  for (int pass = 0; pass < passes; pass++) {
-    const int *list = pass == 0 ? down : up;
-    const int count = pass == 0 ? ndown : nup;
-    for (int ord = 0; ord < strips; ord++) {
-      const int s = pass == 0 ? ord : strips - 1 - ord, slot = ord & 1;
-      const int y0 = s * rows, act = std::min(rows, h - y0);
-      const size_t bytes = (size_t)w * act * Dp * sb, hoff = (size_t)y0 * w * Dp * sb;
-      if (pass == 1) {
-        // upload on the copy stream once the slot's previous strip is done;
-        // it overlaps the sweeps of the strip before
-        if ((e = hipStreamWaitEvent(cs, computed[slot], 0)) != hipSuccess)
+  // Host side of the aggregate.  RAM: direction i's whole aggregate at
+  // host_sum + i * host_bytes.  RECOMPUTE: the current block's rows only.
+  // DISK: a temporary file (direction i at i * host_bytes), written and read
+  // through the pinned staging slots host_sum + (slot * nd + i) * slot_bytes.
+  const bool disk = B.storage == SGM_AGGREGATE_DISK;
+  const int block_rows = B.storage == SGM_AGGREGATE_RECOMPUTE ? B.block_rows : h;
+  const size_t dir_bytes = B.storage == SGM_AGGREGATE_RECOMPUTE
+                               ? (size_t)w * block_rows * Dp * sb : host_bytes;
+  AggregateFile file;
+  if (disk && passes == 2 && (rc = file.open(ctx->aggregate_dir, (int64_t)nd * host_bytes)))
+    return rc;
+  auto stage = [&](int slot, int i) { return B.host_sum + ((size_t)slot * nd + i) * slot_bytes; };
+  // pending disk writes of pass 0: the strip's slot and offset, written once
+  // its download is done
+  int pend_slot = -1;
+  size_t pend_off = 0, pend_bytes = 0;
+  auto flush_pending = [&]() -> int {
+    if (pend_slot < 0) return SGM_OK;
+    hipError_t e2 = hipEventSynchronize(copied[pend_slot]);
+    if (e2 != hipSuccess) return hip_fail(e2, "hipEventSynchronize");
+    for (int i = 0; i < nd; i++)
+      if (int r2 = file.write(stage(pend_slot, i), pend_bytes, (size_t)i * host_bytes + pend_off))
+        return r2;
+    pend_slot = -1;
+    return SGM_OK;
+  };
+  auto interrupted = [&]() {
+    if (!ctx->interrupt.exchange(0)) return false;
+    (void)hipStreamSynchronize(cs);
+    (void)hipStreamSynchronize(ctx->stream);
+    return true;
+  };
+  for (int b0 = 0; b0 < h; b0 += block_rows) {
+    const int b1 = std::min(h, b0 + block_rows);
+    const int s_first = b0 / rows, s_last = (b1 - 1) / rows;
+    for (int pass = 0; pass < passes; pass++) {
+      const int *list = pass == 0 ? down : up;
+      const int count = pass == 0 ? ndown : nup;
+      // strips of the pass: top-down to the block's last strip, bottom-up to
+      // its first; strips outside the block only carry the paths into it
+      const int nstrips = pass == 0 ? s_last + 1 : strips - s_first;
+      for (int ord = 0; ord < nstrips; ord++) {
+        if (interrupted()) return fail(SGM_ERR_INTERRUPTED, "GPU SGM interrupted");
+        const int s = pass == 0 ? ord : strips - 1 - ord, slot = ord & 1;
+        const bool target = s >= s_first && s <= s_last;
+        const int y0 = s * rows, act = std::min(rows, h - y0);
+        const size_t bytes = (size_t)w * act * Dp * sb;
+        const size_t hoff = (size_t)(y0 - (B.storage == SGM_AGGREGATE_RECOMPUTE ? b0 : 0)) * w * Dp * sb;
+        if (pass == 1 && target) {
+          // upload on the copy stream once the slot's previous strip is done;
+          // it overlaps the sweeps of the strip before
+          if (disk) {
+            // the staging slot is free once its previous upload is done
+            if ((e = hipEventSynchronize(copied[slot])) != hipSuccess)
+              return hip_fail(e, "hipEventSynchronize");
+            for (int i = 0; i < nd; i++)
+              if ((rc = file.read(stage(slot, i), bytes, (size_t)i * host_bytes + hoff))) return rc;
+          }
+          if ((e = hipStreamWaitEvent(cs, computed[slot], 0)) != hipSuccess)
+            return hip_fail(e, "hipStreamWaitEvent");
+          for (int i = 0; i < nd; i++) {
+            const uint8_t *src = disk ? stage(slot, i) : B.host_sum + i * dir_bytes + hoff;
+            if ((e = hipMemcpyAsync(dev_slot(slot, i), src, bytes, hipMemcpyHostToDevice, cs)) !=
+                hipSuccess)
+              return hip_fail(e, "strip aggregate upload");
+          }
+          if ((e = hipEventRecord(copied[slot], cs)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+        }
+        // the slot is free (pass 0: its last download is done) or filled (pass 1)
+        if ((e = hipStreamWaitEvent(ctx->stream, copied[slot], 0)) != hipSuccess)
           return hip_fail(e, "hipStreamWaitEvent");
-        for (int i = 0; i < nd; i++)
-          if ((e = hipMemcpyAsync(dev_slot(slot, i), B.host_sum + i * host_bytes + hoff, bytes,
-                                  hipMemcpyHostToDevice, cs)) != hipSuccess)
-            return hip_fail(e, "strip aggregate upload");
-        if ((e = hipEventRecord(copied[slot], cs)) != hipSuccess) return hip_fail(e, "hipEventRecord");
-      }
-      // the slot is free (pass 0: its last download is done) or filled (pass 1)
-      if ((e = hipStreamWaitEvent(ctx->stream, copied[slot], 0)) != hipSuccess)
-        return hip_fail(e, "hipStreamWaitEvent");
-      for (int i = 0; i < nd; i++) cj.job[i].y0 = y0;
-      if ((rc = launch_cost(ctx, cj, nd, w, h, Dp, p, act, true))) return rc;
-      for (int i = 0; i < nd; i++) {
-        sj.job[i].sum = dev_slot(slot, i);
-        sj.job[i].y0 = y0;
-        sj.job[i].rows = act;
-      }
-      int carried = 0;
-      for (int t = 0; t < count; t++) {
-        const int dir = list[t];
-        const bool first = pass == 0 && t == 0, last = pass == passes - 1 && t == count - 1;
-        const int phase = first ? PHASE_FIRST : last ? PHASE_LAST : PHASE_MID;
-        PathArgs A = path_args(p, dir, nd, 0, 0);
-        sj.split = segments_of(A.cls, w, act);
-        A.total_lines = nd * sj.split;
+        for (int i = 0; i < nd; i++) cj.job[i].y0 = y0;
+        if ((rc = launch_cost(ctx, cj, nd, w, h, Dp, p, act, true))) return rc;
         for (int i = 0; i < nd; i++) {
-          StripJob &J = sj.job[i];
-          J.carry_in = J.carry_out = nullptr;
-          if (A.dy != 0) {
-            int *base = B.carry + (size_t)i * 6 * carry_row;
-            J.carry_in = ord > 0 ? base + ((size_t)(ord & 1) * 3 + carried) * carry_row : nullptr;
-            J.carry_out =
-                ord < strips - 1 ? base + ((size_t)((ord + 1) & 1) * 3 + carried) * carry_row : nullptr;
+          sj.job[i].sum = dev_slot(slot, i);
+          sj.job[i].y0 = y0;
+          sj.job[i].rows = act;
+        }
+        int carried = 0;
+        for (int t = 0; t < count; t++) {
+          const int dir = list[t];
+          PathArgs A = path_args(p, dir, nd, 0, 0);
+          // outside the block only the paths that cross strips run, each
+          // starting its strip aggregate afresh (never read back)
+          if (!target && A.dy == 0) continue;
+          const bool first = pass == 0 && t == 0, last = pass == passes - 1 && t == count - 1;
+          const int phase = !target ? PHASE_FIRST : first ? PHASE_FIRST : last ? PHASE_LAST
+                                                                             : PHASE_MID;
+          sj.split = segments_of(A.cls, w, act);
+          A.total_lines = nd * sj.split;
+          for (int i = 0; i < nd; i++) {
+            StripJob &J = sj.job[i];
+            J.carry_in = J.carry_out = nullptr;
+            if (A.dy != 0) {
+              int *base = B.carry + (size_t)i * 6 * carry_row;
+              J.carry_in = ord > 0 ? base + ((size_t)(ord & 1) * 3 + carried) * carry_row : nullptr;
+              J.carry_out = ord < nstrips - 1
+                                ? base + ((size_t)((ord + 1) & 1) * 3 + carried) * carry_row
+                                : nullptr;
+            }
+          }
+          if (A.dy != 0) carried++;
+          if ((rc = launch_strip(ctx, R, sb, sj, A, phase))) return rc;
+        }
+        if ((e = hipEventRecord(computed[slot], ctx->stream)) != hipSuccess)
+          return hip_fail(e, "hipEventRecord");
+        if (pass == 0 && passes == 2 && target) {
+          // download on the copy stream, overlapping the sweeps of the next strip
+          if (disk && pend_slot == slot && (rc = flush_pending())) return rc;
+          if ((e = hipStreamWaitEvent(cs, computed[slot], 0)) != hipSuccess)
+            return hip_fail(e, "hipStreamWaitEvent");
+          for (int i = 0; i < nd; i++) {
+            uint8_t *dst = disk ? stage(slot, i) : B.host_sum + i * dir_bytes + hoff;
+            if ((e = hipMemcpyAsync(dst, dev_slot(slot, i), bytes, hipMemcpyDeviceToHost, cs)) !=
+                hipSuccess)
+              return hip_fail(e, "strip aggregate download");
+          }
+          if ((e = hipEventRecord(copied[slot], cs)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+          if (disk) {
+            // write the previous strip (other slot) while this one downloads
+            if ((rc = flush_pending())) return rc;
+            pend_slot = slot;
+            pend_off = hoff;
+            pend_bytes = bytes;
           }
         }
-        if (A.dy != 0) carried++;
-        if ((rc = launch_strip(ctx, R, sb, sj, A, phase))) return rc;
       }
-      if ((e = hipEventRecord(computed[slot], ctx->stream)) != hipSuccess)
-        return hip_fail(e, "hipEventRecord");
-      if (pass == 0 && passes == 2) {
-        // download on the copy stream, overlapping the sweeps of the next strip
-        if ((e = hipStreamWaitEvent(cs, computed[slot], 0)) != hipSuccess)
-          return hip_fail(e, "hipStreamWaitEvent");
-        for (int i = 0; i < nd; i++)
-          if ((e = hipMemcpyAsync(B.host_sum + i * host_bytes + hoff, dev_slot(slot, i), bytes,
-                                  hipMemcpyDeviceToHost, cs)) != hipSuccess)
-            return hip_fail(e, "strip aggregate download");
-        if ((e = hipEventRecord(copied[slot], cs)) != hipSuccess) return hip_fail(e, "hipEventRecord");
-      }
+      if (disk && pass == 0 && (rc = flush_pending())) return rc;
     }
   }
   return SGM_OK;
@@ -1250,9 +1515,15 @@ int run_directions_esgm(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
   // bottom-up; up to three carried paths per pass, each with a ping-pong pair
   // of carry rows.
   static const int dirs[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
+  auto interrupted = [&]() {
+    if (!ctx->interrupt.exchange(0)) return false;
+    (void)hipStreamSynchronize(ctx->stream);
+    return true;
+  };
   auto sweep = [&](int phase) -> int {
     if (computed) {
       for (int dir = 0; dir < p->directions; dir++) {
+        if (interrupted()) return fail(SGM_ERR_INTERRUPTED, "GPU SGM interrupted");
         PathArgs A = path_args(p, dir, nd, 0, 0);
         const CompactArgs E{C, pool, dir};
         kj.split = lines_of(A.cls, w, h);
@@ -1270,6 +1541,7 @@ int run_directions_esgm(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     }
     for (int pass = 0; pass < 2; pass++) {
       for (int ord = 0; ord < strips; ord++) {
+        if (interrupted()) return fail(SGM_ERR_INTERRUPTED, "GPU SGM interrupted");
         const int s = pass ? strips - 1 - ord : ord;
         const int y0 = s * rows, act = std::min(rows, h - y0);
         bool costed = false;
@@ -1460,6 +1732,10 @@ int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuff
   }
   const int waves = path_waves(ctx, R);
   for (int dir = 0; dir < p->directions; dir++) {
+    if (ctx->interrupt.exchange(0)) {
+      (void)hipStreamSynchronize(ctx->stream);
+      return fail(SGM_ERR_INTERRUPTED, "GPU SGM interrupted");
+    }
     const int phase = dir == 0 ? PHASE_FIRST : dir == p->directions - 1 ? PHASE_LAST : PHASE_MID;
     PathArgs A = path_args(p, dir, nd, 0, ctx->keep_volumes ? 1 : 0);
     A.total_lines = nd * lines_of(A.cls, w, h);
@@ -1613,7 +1889,22 @@ int pick_plan(sgm_context *ctx, const sgm_params *p, int w, int h, int *rows,
   }
   const bool fits = pair_bytes(p, w, h) <= ctx->budget;
   if (ctx->plan_mode != SGM_PLAN_STREAMED && fits) return SGM_OK;
-  if (ctx->plan_mode != SGM_PLAN_RESIDENT) *rows = streamed_rows(ctx, p, w, h);
+  if (ctx->plan_mode != SGM_PLAN_RESIDENT) {
+    ctx->splan = streamed_plan(ctx, p, w, h, ctx->aggregate_storage);
+    *rows = ctx->splan.rows;
+  }
+  if (*rows && ctx->plan_mode == SGM_PLAN_AUTO && ctx->aggregate_storage == SGM_AGGREGATE_AUTO &&
+      ctx->splan.storage == SGM_AGGREGATE_RECOMPUTE && pool && !esgm_unsupported(p, w, h)) {
+    // GpuStereo.execute (GpuStereo.java:176-184): auto with only a bounded
+    // recomputation left runs the compact plan when it serves the pair
+    int erows = 0, epool = 0;
+    esgm_plan(ctx, p, w, h, &erows, &epool);
+    if (epool) {
+      *rows = erows;
+      *pool = epool;
+      return SGM_OK;
+    }
+  }
   if (*rows) return SGM_OK;
   return fail(SGM_ERR_NO_PLAN, "No GPU SGM memory plan fits the requested range (%lld > %lld bytes)",
               (long long)pair_bytes(p, w, h), (long long)ctx->budget);
@@ -1637,6 +1928,7 @@ extern "C" int sgm_match_pair(sgm_context *ctx, const sgm_params *params, const 
   PairBuffers B;
   if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows, pool))) return rc;
   ctx->last_streamed = rows > 0 && !pool;
+  ctx->last_compact = pool > 0;
   const size_t n = (size_t)w * h;
   if ((rc = copy_h2d(ctx, B.gray[0], left, n))) return rc;
   if ((rc = copy_h2d(ctx, B.gray[1], right, n))) return rc;
@@ -1766,6 +2058,7 @@ extern "C" int sgm_match_pair_filtered(sgm_context *ctx, const sgm_params *param
   PairBuffers B;
   if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows, pool))) return rc;
   ctx->last_streamed = rows > 0 && !pool;
+  ctx->last_compact = pool > 0;
   const size_t n = (size_t)w * h;
   if ((rc = copy_h2d(ctx, B.gray[0], left, n))) return rc;
   if ((rc = copy_h2d(ctx, B.gray[1], right, n))) return rc;
@@ -1820,6 +2113,7 @@ extern "C" int sgm_match_pair_color(sgm_context *ctx, const sgm_params *params,
   PairBuffers B;
   if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows))) return rc;
   ctx->last_streamed = rows > 0;
+  ctx->last_compact = false;
   const size_t n = (size_t)w * h;
   if ((rc = ctx->buffers["bgr0"].ensure(ctx, 3 * n, "bgr0"))) return rc;
   if ((rc = ctx->buffers["bgr1"].ensure(ctx, 3 * n, "bgr1"))) return rc;
@@ -1885,6 +2179,7 @@ extern "C" int sgm_match_pair_device(sgm_context *ctx, const sgm_params *params,
   PairBuffers B;
   if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows, pool))) return rc;
   ctx->last_streamed = rows > 0 && !pool;
+  ctx->last_compact = pool > 0;
   const size_t n = (size_t)w * h;
   unsigned long long *stats = d_stats ? (unsigned long long *)d_stats : B.stats;
   if ((rc = run_pair(ctx, params, w, h, B, (const uint8_t *)d_left, (const uint8_t *)d_right,
@@ -1982,6 +2277,7 @@ extern "C" int sgm_match_pair_device_noalloc(sgm_context *ctx, const sgm_params 
   carve_pair_buffers(params, w, h, cost_volume, base, B);
   (void)hipSetDevice(ctx->device);
   ctx->last_streamed = false;
+  ctx->last_compact = false;
   const size_t n = (size_t)w * h;
   unsigned long long *stats = d_stats ? (unsigned long long *)d_stats : B.stats;
   if ((rc = run_pair(ctx, params, w, h, B, (const uint8_t *)d_left, (const uint8_t *)d_right,
@@ -2025,6 +2321,7 @@ int match_one(sgm_context *ctx, const sgm_params *params, const uint64_t *desc_a
   PairBuffers B;
   if ((rc = ensure_pair_buffers(ctx, params, w, h, B, rows, pool))) return rc;
   ctx->last_streamed = rows > 0 && !pool;
+  ctx->last_compact = pool > 0;
   const size_t n = (size_t)w * h;
   // slot d = reverse: the reference image sits in the slot whose cost job has J.reverse = reverse.
   const int ref = reverse ? 1 : 0, tgt = 1 - ref;
@@ -2224,6 +2521,7 @@ int lane_enqueue(sgm_context *ctx, const sgm_params *base, const sgm_pair &pr) {
   PairBuffers B;
   if ((rc = ensure_pair_buffers(ctx, &p, w, h, B, rows))) return rc;
   ctx->last_streamed = rows > 0;
+  ctx->last_compact = false;
   const size_t n = (size_t)w * h;
   if ((rc = ctx->host_buffers["lane_in"].ensure(4 * n, "lane inputs"))) return rc;
   if ((rc = ctx->host_buffers["lane_out"].ensure(14 * n + 128, "lane outputs"))) return rc;

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -22,12 +23,21 @@ struct Buffer {
   void release(sgm_context *ctx);
 };
 
-// Pinned host memory (hipHostMalloc), grown on demand.
+// Pinned host memory (hipHostMalloc), grown on demand.  `fail_counter`
+// (optional) simulates allocation failures while positive.
 struct HostBuffer {
   void *ptr = nullptr;
   size_t cap = 0;
-  int ensure(size_t bytes, const char *what);
+  int ensure(size_t bytes, const char *what, int *fail_counter = nullptr);
   void release();
+};
+
+// Streamed plan of a pair (GpuStereo.Plan, GpuStereo.java:12-20,56-83).
+struct StreamedPlan {
+  int rows = 0;        // strip rows (0: no streamed plan fits)
+  int storage = 0;     // SGM_AGGREGATE_RAM / _DISK / _RECOMPUTE
+  int block_rows = 0;  // rows per recomputation block (h for RAM and DISK)
+  int64_t device_bytes = 0, host_bytes = 0, disk_bytes = 0;
 };
 
 struct PendingEvent {
@@ -76,6 +86,8 @@ int launch_cost(sgm_context *ctx, const CostJobs &jobs, int njobs, int w, int h,
 int64_t streamed_pair_bytes(const sgm_params *p, int w, int h, int rows);
 int64_t streamed_host_bytes(const sgm_params *p, int w, int h);
 int streamed_rows(sgm_context *ctx, const sgm_params *p, int w, int h);
+StreamedPlan streamed_plan(sgm_context *ctx, const sgm_params *p, int w, int h, int storage);
+int64_t host_budget(const sgm_context *ctx);
 int64_t esgm_pair_bytes(const sgm_params *p, int w, int h, int rows, int pool);
 const char *esgm_unsupported(const sgm_params *p, int w, int h);
 int esgm_plan(sgm_context *ctx, const sgm_params *p, int w, int h, int *rows, int *pool);
@@ -112,6 +124,15 @@ struct sgm_context {
   int volume_policy = 0;  // SGM_VOLUME_*
   bool profiling = false;
   int fail_allocations = 0;
+  int fail_host_allocations = 0;
+  // streamed aggregate storage (SGM_AGGREGATE_*), its directory and host budget
+  int aggregate_storage = 0;
+  std::string aggregate_dir = "/tmp";
+  int64_t host_budget = 0;
+  // sgm_interrupt: set by any thread, consumed by the matching call that sees it
+  std::atomic<int> interrupt{0};
+  bool last_compact = false;
+  sgm::StreamedPlan splan;  // streamed plan picked for the current call
   std::vector<sgm::PendingEvent> pending_events;
   std::map<std::string, std::pair<double, long long>> kernel_stats;
   std::map<int, int> tuned_waves;   // lane group R -> waves per workgroup in path sweeps

@@ -291,6 +291,21 @@ __device__ __forceinline__ void store_cost4(uint16_t *cost, size_t p, int Dp, in
 __device__ __forceinline__ int zncc_cost(const CostArgs &A, int sab, long long sa, long long saa,
                                          long long sb, long long sbb);
 
+// FP32 ZNCC pieces: the inverse deviation 1 / sqrt(cnt*saa - sa^2) of a
+// window (-1 when the window has no variance), and the cost of a cell from
+// the exact covariance numerator and both inverse deviations (invalid code
+// when either is -1).
+__device__ __forceinline__ float zncc_inv_sd(long long v) {
+  return v > 0 ? 1.0f / sqrtf((float)v) : -1.0f;
+}
+__device__ __forceinline__ int zncc1024(int num, float ia, float ib, int inv_code) {
+  // evaluated for every cell, then selected: no branch in the cell loop
+  float ncc = ((float)num * ia) * ib;
+  ncc = fminf(1.0f, fmaxf(-1.0f, ncc));
+  const int v = round_half_up_f((1.0f - ncc) * 1024.0f);
+  return (ia > 0.0f && ib > 0.0f) ? v : inv_code;
+}
+
 // ZNCC = false compiles only the census-family modes 0..2 (register budget of
 // the batched kernel instance that never sees ZNCC).
 template <bool ZNCC = true>
@@ -344,90 +359,216 @@ __device__ __forceinline__ int zncc_cost(const CostArgs &A, int sab, long long s
     c = c > -1.0 ? c : -1.0;
     return (int)round_half_up_d((1.0 - c) * .5 * bits);
   }
-  // FP32 ZNCC: per-window mean / variance and covariance in FP32 from the
-  // exact window sums; cost = round((1 - ncc) * 1024).
-  if (va <= 0 || vb <= 0) return A.inv_code;
-  const float inv_n = 1.0f / (float)cnt;
-  const float ma = (float)sa * inv_n, mb = (float)sb * inv_n;
-  const float vara = (float)saa * inv_n - ma * ma;
-  const float varb = (float)sbb * inv_n - mb * mb;
-  const float cov = (float)sab * inv_n - ma * mb;
-  const float den = sqrtf(vara * varb);
-  float ncc = den > 0.0f ? cov / den : 0.0f;
-  ncc = fminf(1.0f, fmaxf(-1.0f, ncc));
-  return round_half_up_f((1.0f - ncc) * 1024.0f);
+  // FP32 ZNCC: the covariance numerator cnt*sab - sa*sb is exact (int32),
+  // normalised in FP32 by the inverse window deviations; cost =
+  // round((1 - ncc) * 1024).
+  return zncc1024((int)(cnt * sab - sa * sb), zncc_inv_sd(va), zncc_inv_sd(vb), A.inv_code);
 }
 
 struct CostJobs {
   CostJob job[2];
 };
 
-// ZNCC cost volume rows (modes 3 and 4): one image row, 64 columns x 64
-// disparities per workgroup.  The gray rows under the window are staged in
-// LDS; the cross sums are separable: for every window column x' and disparity
-// d the 7-row (wh-row) column sum of a(x') b(x' -/+ d) is formed once in LDS,
-// then each cell adds ww of them.  All sums are exact integers, so the result
-// equals the per-cell window loop.
-constexpr int kZnccTX = 64, kZnccTD = 64, kZnccMaxR = 5;
+// ZNCC cost volume (modes 3 and 4) over bands of image rows.  One wave owns
+// 64 consecutive disparities k of 64 output columns x0 .. x0+63 and sweeps a
+// band of rows top-down.  For each window column c (image column
+// x0 - RX + c) it keeps in registers the wh-row cross sum
+//   col[c] = sum over the window rows r of a(r, c) * b(r, target(c, k)),
+// exact integers.  A row step adds the row entering the window and removes the
+// row leaving it with one v_dot2 per column: (a_new, a_old) . (b_new, -b_old).
+// The target of disparity k at column c + 1 is the target of disparity k -/+ 1
+// at column c, so the b values of the next column are this column's shifted
+// by one lane (DPP) with one new value entering at the end; the targets' window
+// sums slide along x the same way.  Each cell sums 2*RX + 1 columns
+// incrementally and normalises with the window sums: the same integers as the
+// per-cell window loop (StereoCost.value mode 3, StereoCost.java:75-89, or the
+// FP32 variant).
+constexpr int kZnccTX = 64, kZnccBand = 32;
 
-static __global__ __launch_bounds__(256) void zncc_cost_kernel(CostJobs jobs, CostArgs A) {
-  __shared__ uint8_t arow[2 * kZnccMaxR + 1][kZnccTX + 2 * kZnccMaxR];
-  __shared__ uint8_t brow[2 * kZnccMaxR + 1][kZnccTX + 2 * kZnccMaxR + kZnccTD];
-  __shared__ int col[kZnccTX + 2 * kZnccMaxR][kZnccTD + 1];
-  const CostJob J = jobs.job[blockIdx.z];
-  const int y = blockIdx.y + J.y0;
+// Lane l <- lane l-1 (REV = false) or l+1 (REV = true); the vacated end lane
+// takes `fill` (uniform).
+template <bool REV>
+__device__ __forceinline__ int zncc_shift(int v, int fill) {
+  return REV ? dpp_shl1(v, fill) : dpp_shr1(v, fill);
+}
+template <bool REV>
+__device__ __forceinline__ float zncc_shift_f(float v, float fill) {
+  return __builtin_bit_cast(float, zncc_shift<REV>(__builtin_bit_cast(int, v),
+                                                   __builtin_bit_cast(int, fill)));
+}
+template <bool REV>
+__device__ __forceinline__ double zncc_shift_d(double v, double fill) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned long long f = __builtin_bit_cast(unsigned long long, fill);
+  const unsigned lo = (unsigned)zncc_shift<REV>((int)(unsigned)u, (int)(unsigned)f);
+  const unsigned hi = (unsigned)zncc_shift<REV>((int)(unsigned)(u >> 32), (int)(unsigned)(f >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ int zncc_lane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float zncc_lane_f(float v, int l) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double zncc_lane_d(double v, int l) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+typedef short zncc_s16x2 __attribute__((ext_vector_type(2)));
+
+// Gray value of `row` at column x, 0 outside the image or for row < 0.
+__device__ __forceinline__ int zncc_px(const uint8_t *img, int w, int row, int x) {
+  return (row >= 0 && x >= 0 && x < w) ? (int)img[(size_t)row * w + x] : 0;
+}
+
+template <int RX, int MODE, bool REV, bool OUT8>
+__device__ __forceinline__ void zncc_band(const CostJob &J, const CostArgs &A, int rows) {
+  constexpr int XW = kZnccTX + 2 * RX;  // window columns of one tile
   const int ntx = (J.w + kZnccTX - 1) / kZnccTX;
-  const int x0 = (int)(blockIdx.x % ntx) * kZnccTX, d0 = (int)(blockIdx.x / ntx) * kZnccTD;
-  if (y >= J.h || d0 >= J.Dp) return;  // whole workgroup
-  const int rx = A.ww >> 1, ry = A.wh >> 1;
-  const int XW = kZnccTX + 2 * rx, BW = XW + kZnccTD - 1;
-  // b column j holds target t = tb0 + j
-  const int tb0 = J.reverse ? x0 - rx + J.minD + d0 : x0 - rx - J.minD - d0 - (kZnccTD - 1);
-  const bool rows_ok = y >= ry && y < J.h - ry;
-  for (int i = threadIdx.x; i < A.wh * XW; i += 256) {
-    const int r = i / XW, c = i - r * XW, xx = x0 - rx + c;
-    arow[r][c] = (rows_ok && xx >= 0 && xx < J.w) ? J.gray_a[(size_t)(y - ry + r) * J.w + xx] : 0;
-  }
-  for (int i = threadIdx.x; i < A.wh * BW; i += 256) {
-    const int r = i / BW, c = i - r * BW, t = tb0 + c;
-    brow[r][c] = (rows_ok && t >= 0 && t < J.w) ? J.gray_b[(size_t)(y - ry + r) * J.w + t] : 0;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < XW * kZnccTD; i += 256) {
-    const int c = i / kZnccTD, dd = i - c * kZnccTD;
-    const int j = J.reverse ? c + dd : c - dd + kZnccTD - 1;
-    int s = 0;
-    for (int r = 0; r < A.wh; r++) s += (int)arow[r][c] * (int)brow[r][j];
-    col[c][dd] = s;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < kZnccTX * (kZnccTD / 4); i += 256) {
-    const int xo = i / (kZnccTD / 4), q4 = i - xo * (kZnccTD / 4);
-    const int x = x0 + xo, k0 = d0 + 4 * q4;
-    if (x >= J.w || k0 >= J.Dp) continue;
-    const size_t p = (size_t)y * J.w + x;
-    const bool pix_ok = rows_ok && x >= rx && x < J.w - rx && J.mask_a[p];
-    unsigned v[4];
+  const int x0 = (int)(blockIdx.x % ntx) * kZnccTX;
+  const int kw0 = (int)(blockIdx.x / ntx) * 64;
+  const int y_end = min(J.h, J.y0 + rows);
+  const int yb = J.y0 + (int)blockIdx.y * kZnccBand, ye = min(y_end, yb + kZnccBand);
+  if (kw0 >= J.Dp || yb >= ye) return;  // whole wave
+  const int lane = (int)threadIdx.x, k = kw0 + lane;
+  const int ry = A.wh >> 1, w = J.w;
+  const int cnt = A.ww * A.wh, bits = cnt - 1;
+  // target of window column c and lane l: tb + j, j = c + (REV ? l : 63 - l)
+  const int tb = REV ? x0 - RX + J.minD + kw0 : x0 - RX - J.minD - kw0 - 63;
+  const int j0 = REV ? lane : 63 - lane;
+  int col[XW];
 #pragma unroll
-    for (int jj = 0; jj < 4; jj++) {
-      const int k = k0 + jj, dd = 4 * q4 + jj;
-      int value = A.inv_code;
-      if (pix_ok && k < J.D) {
-        const int disp = J.minD + k;
-        const long long t = (long long)x + (J.reverse ? (long long)disp : -(long long)disp);
-        if (t >= rx && t < J.w - rx) {
-          const size_t q = (size_t)y * J.w + (size_t)t;
-          if (J.mask_b[q]) {
-            int sab = 0;
-            for (int c = xo; c <= xo + 2 * rx; c++) sab += col[c][dd];
-            value = zncc_cost(A, sab, J.sum_a[p], J.sum2_a[p], J.sum_b[q], J.sum2_b[q]);
+  for (int c = 0; c < XW; c++) col[c] = 0;
+  bool have = false;
+  for (int y = yb; y < ye; y++) {
+    const bool rows_ok = y >= ry && y < J.h - ry;
+    if (rows_ok) {
+      // rows entering / leaving the window: a full window after a gap, else one slide
+      const int first = have ? y + ry : y - ry, count = have ? 1 : 2 * ry + 1;
+      if (!have) {
+#pragma unroll
+        for (int c = 0; c < XW; c++) col[c] = 0;
+      }
+      for (int r = 0; r < count; r += 2) {
+        // rows P (added) and Q (added when filling, removed when sliding; -1: none)
+        const int P = first + r;
+        const int Q = have ? y - ry - 1 : (r + 1 < count ? P + 1 : -1);
+        const int sq = have ? -1 : 1;
+        const int ax0 = x0 - RX + lane, ax1 = ax0 + 64;
+        const int av0 = zncc_px(J.gray_a, w, P, ax0) | (sq * zncc_px(J.gray_a, w, Q, ax0)) << 16;
+        const int av1 = zncc_px(J.gray_a, w, P, ax1) | (sq * zncc_px(J.gray_a, w, Q, ax1)) << 16;
+        // b pairs of j = j0 (the first column) and of j = 64 + lane, 128 + lane
+        // (the values entering at the lane end, column after column)
+        auto bpair = [&](int j) {
+          return (zncc_px(J.gray_b, w, P, tb + j) & 0xFFFF) |
+                 (zncc_px(J.gray_b, w, Q, tb + j) << 16);
+        };
+        int B = bpair(j0);
+        const int bv1 = bpair(64 + lane), bv2 = bpair(128 + lane);
+#pragma unroll
+        for (int c = 0; c < XW; c++) {
+          const int apk = c < 64 ? zncc_lane(av0, c) : zncc_lane(av1, c - 64);
+          col[c] = __builtin_amdgcn_sdot2(__builtin_bit_cast(zncc_s16x2, apk),
+                                          __builtin_bit_cast(zncc_s16x2, B), col[c], false);
+          if (c + 1 < XW) {
+            const int ins = c < 64 ? zncc_lane(bv1, c) : zncc_lane(bv2, c - 64);
+            B = zncc_shift<REV>(B, ins);
           }
         }
       }
-      v[jj] = (unsigned)value & 0xFFFFu;
+      have = true;
+    } else {
+      have = false;
     }
-    store_cost4(J.cost, p - (size_t)J.y0 * J.w, J.Dp, k0, v, A.u8 != 0);
+    // window sums of the targets of columns c = RX + xo (x = x0 + xo): the lane's
+    // first target, and the targets entering at the lane end
+    auto target_ok = [&](int t) {
+      return rows_ok && t >= (A.ww >> 1) && t < w - (A.ww >> 1) && J.mask_b[(size_t)y * w + t];
+    };
+    // first targets (column RX): j = RX + j0; entering one per column:
+    // j = RX + 64 + xo, held by lane xo
+    const int jt0 = RX + j0, jt1 = RX + 64 + lane;
+    int tsb, sv1;
+    float tib = -1.0f, iv1 = -1.0f;
+    double tvb = -1.0, vv1 = -1.0;
+    auto tdata = [&](int j, int &sb, float &ib, double &vb) {
+      const int t = tb + j;
+      sb = 0;
+      if (target_ok(t)) {
+        const size_t q = (size_t)y * w + t;
+        sb = J.sum_b[q];
+        const long long v = (long long)cnt * J.sum2_b[q] - (long long)sb * sb;
+        if (MODE == 4) ib = zncc_inv_sd(v);
+        else vb = v > 0 ? (double)v : -1.0;
+      }
+    };
+    tdata(jt0, tsb, tib, tvb);
+    tdata(jt1, sv1, iv1, vv1);
+    // reference pixel x0 + lane: window sum and deviation, -1 when not valid
+    int psa = 0;
+    float pia = -1.0f;
+    double pva = -1.0;
+    {
+      const int x = x0 + lane;
+      const int rxw = A.ww >> 1;
+      if (rows_ok && x >= rxw && x < w - rxw && J.mask_a[(size_t)y * w + x]) {
+        const size_t p = (size_t)y * w + x;
+        psa = J.sum_a[p];
+        const long long v = (long long)cnt * J.sum2_a[p] - (long long)psa * psa;
+        if (MODE == 4) pia = zncc_inv_sd(v);
+        else pva = v > 0 ? (double)v : -1.0;
+      }
+    }
+    const bool kin = k < J.D, kst = k < J.Dp;
+    uint8_t *out = (uint8_t *)J.cost + (((size_t)(y - J.y0) * w + x0) * J.Dp + k) * (OUT8 ? 1 : 2);
+    const size_t ostep = (size_t)J.Dp * (OUT8 ? 1 : 2);
+    int sab = 0;
+#pragma unroll
+    for (int c = 0; c <= 2 * RX; c++) sab += col[c];
+#pragma unroll
+    for (int xo = 0; xo < kZnccTX; xo++) {
+      if (xo > 0) sab += col[xo + 2 * RX] - col[xo - 1];
+      if (x0 + xo >= w) break;  // uniform
+      const int sa = zncc_lane(psa, xo);
+      // exact: every operand fits 24 bits (sab < 121 * 255^2 < 2^23)
+      const int num = __mul24(cnt, sab) - __mul24(sa, tsb);
+      int value;
+      if (MODE == 4) {
+        value = zncc1024(num, zncc_lane_f(pia, xo), tib, A.inv_code);
+      } else {
+        // the reference's FP64 normalisation, evaluated for every cell and
+        // selected (no branch): StereoCost.java:82-88
+        const double va = zncc_lane_d(pva, xo);
+        const bool ok = va > 0.0 && tvb > 0.0;
+        const double corr = (double)num / sqrt(ok ? va * tvb : 1.0);
+        double c = corr < 1.0 ? corr : 1.0;
+        c = c > -1.0 ? c : -1.0;
+        value = ok ? (int)round_half_up_d((1.0 - c) * .5 * bits) : 512;
+      }
+      value = kin ? value : (MODE == 4 ? A.inv_code : 512);
+      if (kst) {
+        if (OUT8) out[(size_t)xo * ostep] = (uint8_t)min(value, 255);
+        else *(uint16_t *)(out + (size_t)xo * ostep) = (uint16_t)value;
+      }
+      if (xo + 1 < kZnccTX) {
+        // slide the targets by one column: j = RX + 64 + xo enters at the lane end
+        tsb = zncc_shift<REV>(tsb, zncc_lane(sv1, xo));
+        if (MODE == 4) tib = zncc_shift_f<REV>(tib, zncc_lane_f(iv1, xo));
+        else tvb = zncc_shift_d<REV>(tvb, zncc_lane_d(vv1, xo));
+      }
+    }
   }
+}
+
+// Both matching directions in one launch (blockIdx.z picks the job); the
+// direction of the target side is fixed per job, so each block takes one
+// branch at entry and the cell loop stays branch-free.
+template <int RX, int MODE, bool OUT8>
+static __global__ __launch_bounds__(64) void zncc_band_kernel(CostJobs jobs, CostArgs A, int rows) {
+  const CostJob &J = jobs.job[blockIdx.z];
+  if (J.reverse) zncc_band<RX, MODE, true, OUT8>(J, A, rows);
+  else zncc_band<RX, MODE, false, OUT8>(J, A, rows);
 }
 
 static __global__ __launch_bounds__(256) void cost_kernel(CostJobs jobs, CostArgs A) {

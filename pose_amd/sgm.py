@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(HERE, "libsgm_hip.so")
 SGM_OK = 0
 ERRORS = {
     -1: "invalid argument", -2: "no device", -3: "out of memory", -4: "no plan",
-    -5: "HIP error", -6: "rejected", -7: "timeout", -8: "not found",
+    -5: "HIP error", -6: "rejected", -7: "timeout", -8: "not found", -9: "interrupted",
 }
 
 VOLUME_AUTO, VOLUME_RESIDENT, VOLUME_RECOMPUTE = 0, 1, 2
@@ -33,6 +33,10 @@ PLAN_AUTO, PLAN_RESIDENT, PLAN_STREAMED, PLAN_ESGM = 0, 1, 2, 3
 NOALLOC_WORKSPACE_BYTES = 128 << 20
 PLAN_MODES = {"auto": PLAN_AUTO, "resident": PLAN_RESIDENT, "streamed": PLAN_STREAMED,
               "esgm": PLAN_ESGM}
+# streamed aggregate storage: --gpu-aggregate auto|ram|disk|recompute (GpuStereo.java:56-83)
+AGGREGATE_AUTO, AGGREGATE_RAM, AGGREGATE_DISK, AGGREGATE_RECOMPUTE = 0, 1, 2, 3
+AGGREGATE_STORAGES = {"auto": AGGREGATE_AUTO, "ram": AGGREGATE_RAM, "disk": AGGREGATE_DISK,
+                      "recompute": AGGREGATE_RECOMPUTE}
 COSTS = {"census": 0, "ad-census": 1, "census-gradient": 2, "zncc": 3, "zncc-fp32": 4}
 
 
@@ -90,7 +94,9 @@ class SgmPlan(ctypes.Structure):
                 ("streamed", ctypes.c_int32), ("strip_rows", ctypes.c_int32),
                 ("streamed_bytes", ctypes.c_int64), ("host_bytes", ctypes.c_int64),
                 ("esgm_candidates", ctypes.c_int32), ("esgm_rows", ctypes.c_int32),
-                ("esgm_bytes", ctypes.c_int64)]
+                ("esgm_bytes", ctypes.c_int64), ("aggregate_storage", ctypes.c_int32),
+                ("block_rows", ctypes.c_int32), ("disk_bytes", ctypes.c_int64),
+                ("compact", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class SgmPair(ctypes.Structure):
@@ -117,6 +123,7 @@ EXPORTS = [
     "sgm_match_pair_filtered", "sgm_match_pair_color",
     "sgm_set_keep_volumes", "sgm_read_volumes", "sgm_set_cost_volume", "sgm_set_plan_mode",
     "sgm_set_esgm_candidates", "sgm_esgm_counts", "sgm_synchronize",
+    "sgm_set_aggregate_storage", "sgm_interrupt", "sgm_debug_fail_host_allocations",
     "sgm_release_buffers", "sgm_set_profiling",
     "sgm_profile_json", "sgm_stream", "sgm_debug_fail_allocations", "sgm_last_error",
     "sgm_device_alloc", "sgm_device_free", "sgm_memcpy", "sgm_device_copy",
@@ -168,6 +175,9 @@ def lib():
             L.sgm_set_cost_volume.argtypes = [P, I]
             L.sgm_set_plan_mode.argtypes = [P, I, I]
             L.sgm_set_esgm_candidates.argtypes = [P, I]
+            L.sgm_set_aggregate_storage.argtypes = [P, I, ctypes.c_char_p, I64]
+            L.sgm_interrupt.argtypes = [P]
+            L.sgm_debug_fail_host_allocations.argtypes = [P, I]
             L.sgm_esgm_counts.argtypes = [P, ctypes.POINTER(I64)]
             L.sgm_read_volumes.argtypes = [P, I, P, P]
             L.sgm_synchronize.argtypes = [P]
@@ -259,6 +269,9 @@ class Options:
         self.esgmCandidates = 2     # Options.java:35
         self.censusMode = "gray"    # gray | color-distance (Options.java:31)
         self.gpuStripRows = 64      # Options.java:39
+        self.gpuAggregate = "auto"  # auto | ram | disk | recompute (streamed aggregate storage)
+        self.memoryMb = 0           # host memory of the streamed plan (0: 65% of RAM)
+        self.output = None          # directory of the disk aggregate storage
         self.rawPenalties = False  # p1/p2 already in raw scaled units
         # post-L-R filters (Options.java:19-23,67,76)
         self.median = 3
@@ -541,6 +554,23 @@ class Context:
         mode = PLAN_MODES[mode] if isinstance(mode, str) else int(mode)
         check(lib().sgm_set_plan_mode(self.h, mode, int(strip_rows)))
 
+    def aggregate_storage(self, storage, directory=None, host_budget_bytes=0):
+        """Host storage of the streamed aggregate (--gpu-aggregate): AGGREGATE_*
+        or its name, the directory of the disk storage, the host budget
+        (Options.memoryMb; 0: 65% of physical memory)."""
+        storage = AGGREGATE_STORAGES.get(storage, storage)
+        check(lib().sgm_set_aggregate_storage(
+            self.h, int(storage), None if directory is None else os.fsencode(directory),
+            int(host_budget_bytes)))
+
+    def interrupt(self):
+        """Stop the running (or next) matching call of this context: it fails
+        with SGM_ERR_INTERRUPTED (GpuStereo.java:350-351)."""
+        check(lib().sgm_interrupt(self.h))
+
+    def fail_host_allocations(self, count):
+        check(lib().sgm_debug_fail_host_allocations(self.h, count))
+
     def esgm_candidates(self, count):
         """--esgm-candidates: proposals per path of the compact plan (1..4)."""
         check(lib().sgm_set_esgm_candidates(self.h, int(count)))
@@ -671,9 +701,12 @@ class GpuSgm:
             raise ValueError("--gpu-sgm must be auto, resident, streamed or esgm")
         if not 1 <= o.esgmCandidates <= 4:
             raise ValueError("--esgm-candidates must be 1..4")
+        if o.gpuAggregate not in AGGREGATE_STORAGES:
+            raise ValueError("--gpu-aggregate must be auto, ram, disk or recompute")
         s = GpuSgm.session(o)
         s.plan_mode(o.gpuSgm, o.gpuStripRows)
         s.esgm_candidates(o.esgmCandidates)
+        s.aggregate_storage(o.gpuAggregate, o.output, o.memoryMb * 1048576)
         return s
 
     @staticmethod
@@ -777,7 +810,7 @@ class GpuStereo:
             return None
         try:
             p = GpuStereo.plan(w, h, o)
-            compact = o.gpuSgm == "esgm"
+            compact = bool(p["compact"])
             if compact and not p["esgm_candidates"]:
                 raise SgmError(-4, "No compact Census/16-bit GPU plan fits")
             streamed = not compact and bool(p["streamed"])
@@ -785,10 +818,13 @@ class GpuStereo:
                 raise SgmError(-4, "No GPU SGM memory plan fits the requested range")
             sb = p["aggregate_bytes"]
             computed = compact and p["esgm_rows"] == h
+            storage = {v: k for k, v in AGGREGATE_STORAGES.items()}.get(p["aggregate_storage"], "ram")
+            disk = streamed and storage == "disk"
+            blocks = -(-h // p["block_rows"]) if streamed and p["block_rows"] else 1
             strips = 1 if not (compact or streamed) else -(-h // (p["esgm_rows"] if compact else p["strip_rows"]))
             return {
                 "storage": ("compact_with_sparse_exact_verification" if compact
-                            else "streamed_ram" if streamed else "resident"),
+                            else "streamed_" + storage if streamed else "resident"),
                 "rows": p["esgm_rows"] if compact else p["strip_rows"] if streamed else h,
                 "bank_bytes": 0,
                 "estimated_device_bytes": (p["esgm_bytes"] if compact else p["streamed_bytes"]
@@ -801,14 +837,14 @@ class GpuStereo:
                 # once per strip pass (streamed), none when the compact sweeps
                 # compute their costs (3 passes x paths otherwise)
                 "cost_generation_passes": (0 if computed else 3 * 2 if compact
-                                           else 2 if streamed else 1),
-                "cost_generation_passes_are_upper_bound": False,
+                                           else 2 * blocks if streamed else 1),
+                "cost_generation_passes_are_upper_bound": streamed and storage == "recompute",
                 "aggregate_volume_reads": 1 if streamed else 0,
                 "aggregate_volume_writes": 1 if streamed else 0,
-                "aggregate_disk_bytes": 0,
-                "aggregate_ram_bytes": p["host_bytes"] if streamed else 0,
-                "verification_block_rows": h if compact else None,
-                "recomputation_blocks": strips if compact else 1,
+                "aggregate_disk_bytes": p["disk_bytes"] if disk else 0,
+                "aggregate_ram_bytes": p["host_bytes"] if streamed and not disk else 0,
+                "verification_block_rows": h if compact else p["block_rows"] if streamed else None,
+                "recomputation_blocks": strips if compact else blocks,
                 "candidates_per_path": p["esgm_candidates"] if compact else None,
                 "candidate_device_bytes": p["esgm_bytes"] if compact else 0,
             }

@@ -91,3 +91,24 @@ def test_census_sgm_mirror_runs_the_filters(gpu_context):
     want = O.compute_filtered(op, O.filters(speckle_size=30, median_classic=1), left, right, 200, 120)
     assert np.array_equal(bits(r.disparity), bits(want["disparity"]))
     assert r.stats.valid == want["stats"]["valid"]
+
+
+def test_subpixel_refinement_on_the_analytic_plane(gpu_context):
+    """SubpixelRegression (SubpixelRegression.java:11-37) on the device: the
+    refined map equals the oracle bit for bit, and against the known plane
+    disparity 13.37 refinement lowers the RMS error below 0.8x the parabola
+    fit's with coverage above 0.8x."""
+    from tests.cases import subpixel_texture
+    from tests.test_oracle_filters import _plane_metrics
+    w, h, d = 384, 192, 13.37
+    a, b = subpixel_texture(w, h, d)
+    op = O.params(disparities=32)
+    metrics = []
+    for refine in (0, 1):
+        f = O.filters(median=1, speckle_size=0, correlation=0.5, subpixel_refinement=refine)
+        got = gpu_context.match_pair_filtered(to_sgm(op), to_filters(f), a, b)
+        want = O.compute_filtered(op, f, a, b, w, h)
+        assert np.array_equal(bits(got["disparity"]), bits(want["disparity"])), refine
+        metrics.append(_plane_metrics(got["disparity"], w, h, d))
+    old, new = metrics
+    assert new[0] >= 0.8 * old[0] and new[1] < 0.8 * old[1], (old, new)
