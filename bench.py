@@ -127,12 +127,13 @@ def batch_workload(count=48, seed=4):
     return synth.batch_pairs(count=count, seed=seed)
 
 
-def run_batched(sgm, params_for, steps=2):
-    """Same 48 pairs through (a) the plain path, one pair per launch sequence, and
-    (b) one batched persistent launch replayed from a hipGraph; inputs in HBM."""
+def run_batched(sgm, params_for, steps=2, spec=None, label=None):
+    """Same pairs through (a) the plain path, one pair per launch sequence, and
+    (b) one batched persistent launch replayed from a hipGraph; inputs in HBM.
+    Default: the 48 uneven pairs of the batched workload."""
     import numpy as np
     from pose_amd import synth
-    spec = batch_workload()
+    spec = batch_workload() if spec is None else spec
     # the largest budget the library allows (95% of device memory): the batch
     # keeps its cost volumes resident when they fit, else recomputes census costs
     ctx = sgm.Context(0, budget_bytes=1 << 62)
@@ -179,8 +180,9 @@ def run_batched(sgm, params_for, steps=2):
             v.free()
     ctx.close()
     return {
-        "workload": "48 uneven pairs, long side U[800,3000], aspect 4:3/3:2, D U[64,384], "
-                    "min disparity U[-32,32], census 9x7, P1=10 P2=120, 8 paths, L+R + L-R",
+        "workload": label or ("48 uneven pairs, long side U[800,3000], aspect 4:3/3:2, "
+                              "D U[64,384], min disparity U[-32,32], census 9x7, P1=10 P2=120, "
+                              "8 paths, L+R + L-R"),
         "pairs": len(spec), "cell_updates": updates,
         "plain_ms": round(t_plain * 1e3, 3), "batched_ms": round(t_batch * 1e3, 3),
         "plain_gupdates_per_s": round(updates / t_plain / 1e9, 3),
@@ -348,7 +350,21 @@ def serve_trace(count, rate, seed=7):
     return trace
 
 
-def run_served(sgm, params_for, rate=20.0, count=60, budget=250e9, max_batch=8, streams=3):
+def run_served_sweep(sgm, params_for, rates=(2.0, 5.0, 10.0), budget=250e9, streams=3):
+    """The served stream at lower Poisson rates: the plain path (FIFO, one pair
+    at a time) and the lane server, p50/p99 latency and sustained pairs/s; about
+    three seconds of arrivals per rate (at least 12 requests)."""
+    out = {}
+    for rate in rates:
+        count = max(12, int(rate * 3))
+        r = run_served(sgm, params_for, rate=rate, count=count, budget=budget, streams=streams,
+                       burst=False, batching=False)
+        out["%g" % rate] = r["poisson"]
+    return out
+
+
+def run_served(sgm, params_for, rate=20.0, count=60, budget=250e9, max_batch=8, streams=3,
+               burst=True, batching=True):
     """The same request trace through (a) the plain path, one pair at a time in
     arrival order, and (b) the continuous-batching server (admission by HBM
     footprint against `budget`); host buffers in and out for both.  Latency =
@@ -363,7 +379,9 @@ def run_served(sgm, params_for, rate=20.0, count=60, budget=250e9, max_batch=8, 
 
     images = {}
     t_gen = time.perf_counter()
-    traces = {"poisson": serve_trace(count, rate), "saturated": serve_trace(count, None, seed=8)}
+    traces = {"poisson": serve_trace(count, rate)}
+    if burst:
+        traces["saturated"] = serve_trace(count, None, seed=8)
     for tr in traces.values():
         for _, _, (w, h, D, md, s) in tr:
             if s not in images:
@@ -445,7 +463,7 @@ def run_served(sgm, params_for, rate=20.0, count=60, budget=250e9, max_batch=8, 
                           "pairs_per_s": round(len(trace) / span, 3),
                           "gupdates_per_s": round(updates / span / 1e9, 3),
                           "batches": len(batches), "bit_exact_vs_plain": same}
-        span, served = replay(max_batch, streams)
+        span, served = replay(max_batch, streams) if batching else (None, None)
         span_l, lanes = replay(1, streams)
         out[name] = {
             "requests": len(trace), "classes": classes, "cell_updates": updates,
@@ -458,9 +476,10 @@ def run_served(sgm, params_for, rate=20.0, count=60, budget=250e9, max_batch=8, 
                       "mean_service_ms": {c: round(sum(v) / len(v) * 1e3, 2)
                                           for c, v in service.items()},
                       "max_service_ms": {c: round(max(v) * 1e3, 2) for c, v in service.items()}},
-            "speedup_vs_plain": round(plain_span / span, 3),
+            "speedup_vs_plain": round(plain_span / span, 3) if span else None,
             "lanes_speedup_vs_plain": round(plain_span / span_l, 3),
-            "bit_exact_vs_plain": served["bit_exact_vs_plain"] and lanes["bit_exact_vs_plain"],
+            "bit_exact_vs_plain": ((served is None or served["bit_exact_vs_plain"])
+                                   and lanes["bit_exact_vs_plain"]),
         }
         print("served leg %s done" % name, file=sys.stderr, flush=True)
     return out
@@ -646,6 +665,15 @@ def main():
     zncc = None if args.no_zncc else run_zncc(sgm)
     filtered = None if args.no_filtered else run_filtered(sgm, left, right, W, H)
     batched = None if args.no_batched else run_batched(sgm, params_for)
+    batched_small = None
+    if not args.no_batched:
+        # many small pairs: the per-launch overhead the persistent launch removes
+        batched_small = run_batched(sgm, params_for, steps=5,
+                                    spec=[(640, 480, 64, 0, 9000 + i) for i in range(48)],
+                                    label="48 pairs 640x480 D=64, census 9x7, P1=10 P2=120, "
+                                          "8 paths, L+R + L-R")
+    served_sweep = None if args.no_served else run_served_sweep(sgm, params_for,
+                                                                streams=args.serve_streams)
     served = None if args.no_served else run_served(sgm, params_for, rate=args.serve_rate,
                                                     count=args.serve_requests,
                                                     streams=args.serve_streams)
@@ -701,7 +729,9 @@ def main():
         "esgm": esgm,
         "zncc": zncc,
         "batched": batched,
+        "batched_small": batched_small,
         "served": served,
+        "served_rate_sweep": served_sweep,
     }
     print(json.dumps(line))
 
