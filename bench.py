@@ -56,7 +56,7 @@ def algorithmic_bytes_per_step(w, h, D, paths):
 # Sources whose change invalidates the committed HBM traffic measurement of the
 # path sweeps (profiles/path_traffic.json carries the hash it was measured at).
 TRAFFIC_SOURCES = ["pose_amd/csrc/sgm_paths.h", "pose_amd/csrc/sgm_kernels.h",
-                   "pose_amd/csrc/sgm_internal.h", "pose_amd/csrc/sgm_api.hip"]
+                   "pose_amd/csrc/sgm_path_launch.h"]
 
 
 def kernel_source_hash():
@@ -128,9 +128,10 @@ def batch_workload(count=48, seed=4):
 
 
 def run_batched(sgm, params_for, steps=2, spec=None, label=None):
-    """Same pairs through (a) the plain path, one pair per launch sequence, and
-    (b) one batched persistent launch replayed from a hipGraph; inputs in HBM.
-    Default: the 48 uneven pairs of the batched workload."""
+    """Same pairs through (a) the plain path, one pair per launch sequence,
+    (b) the phase-major batch and (c) the single persistent work-queue launch,
+    both replayed from a hipGraph; inputs in HBM.  Default: the 48 uneven pairs
+    of the batched workload."""
     import numpy as np
     from pose_amd import synth
     spec = batch_workload() if spec is None else spec
@@ -163,18 +164,29 @@ def run_batched(sgm, params_for, steps=2, spec=None, label=None):
         plain_all()
     t_plain = (time.perf_counter() - t0) / steps
     ctx.release_buffers()
-    batch = sgm.DeviceBatch(ctx, base, recs, graph=True)
-    batch.run()  # warm-up
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        batch.run()
-    t_batch = (time.perf_counter() - t0) / steps
-    same = all(np.array_equal(b["checked"].download(np.uint32, (w * h)),
-                              b["plain"].download(np.uint32, (w * h)))
-               for (w, h, _, _, _), b in zip(spec, bufs))
-    dev_bytes = batch.device_bytes()
-    shape = batch.describe()
-    batch.close()
+
+    def timed_batch(phased):
+        for b in bufs:
+            b["checked"].upload(np.zeros(b["checked"].nbytes // 4, np.uint32))
+        batch = sgm.DeviceBatch(ctx, base, recs, graph=True, phased=phased)
+        batch.run()  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            batch.run()
+        t = (time.perf_counter() - t0) / steps
+        same = all(np.array_equal(b["checked"].download(np.uint32, (w * h)),
+                                  b["plain"].download(np.uint32, (w * h)))
+                   for (w, h, _, _, _), b in zip(spec, bufs))
+        out = (t, same, batch.device_bytes(), batch.describe())
+        batch.close()
+        return out
+
+    # (b) phase-major batch: the work-queue kernel takes the census and L-R
+    # items, each direction runs one specialised sweep launch per lane group
+    # over every pair's lines; one graph
+    t_batch, same, dev_bytes, shape = timed_batch(True)
+    # (c) the whole batch as one persistent work-queue launch
+    t_queue, same_queue, _, queue_shape = timed_batch(False)
     for b in bufs:
         for v in b.values():
             v.free()
@@ -192,6 +204,10 @@ def run_batched(sgm, params_for, steps=2, spec=None, label=None):
         "speedup_vs_plain": round(t_plain / t_batch, 3),
         "bit_exact_vs_plain": bool(same), "batch_device_bytes": dev_bytes,
         "batch": shape,
+        "single_kernel_ms": round(t_queue * 1e3, 3),
+        "single_kernel_speedup_vs_plain": round(t_plain / t_queue, 3),
+        "single_kernel_bit_exact_vs_plain": bool(same_queue),
+        "single_kernel_batch": queue_shape,
         "input_generation_s": round(t_gen, 2),
     }
 

@@ -694,7 +694,7 @@ extern "C" int sgm_profile_json(sgm_context *ctx, char *buffer, size_t length) {
 // Launchers
 namespace sgm {
 
-static int launch_check(const char *what) {
+int launch_check(const char *what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, what);
   return SGM_OK;
@@ -794,65 +794,14 @@ PathArgs path_args(const sgm_params *p, int dir, int njobs, int total_lines, int
 
 int lines_of(int cls, int w, int h) { return cls == 0 ? h : cls == 1 ? w : w + h - 1; }
 
-template <int R, typename SumT, int MODE>
-static void launch_path_t(sgm_context *ctx, const PathJobs &jobs, const PathArgs &A, int phase,
-                          int waves) {
-  const int threads = waves * 64;
-  const unsigned blocks = (unsigned)((A.total_lines + waves - 1) / waves);
-  if (phase == PHASE_FIRST)
-    path_kernel<R, SumT, PHASE_FIRST, MODE><<<blocks, threads, 0, ctx->stream>>>(jobs, A);
-  else if (phase == PHASE_MID)
-    path_kernel<R, SumT, PHASE_MID, MODE><<<blocks, threads, 0, ctx->stream>>>(jobs, A);
-  else
-    path_kernel<R, SumT, PHASE_LAST, MODE><<<blocks, threads, 0, ctx->stream>>>(jobs, A);
-}
-
-// MODE 0: generic 32-bit sweep (32-bit aggregates); 1: packed 16-bit; 2: packed
-// 16-bit first sweep with the census cost fused in.
-static int launch_path_any(sgm_context *ctx, int R, int sum_bytes, int mode, const PathJobs &jobs,
-                           const PathArgs &A, int phase, int waves) {
-#define CASE_WIDE(RR)                                                       \
-  case RR:                                                                  \
-    launch_path_t<RR, int32_t, 0>(ctx, jobs, A, phase, waves);              \
-    break;
-#define CASE_PK(RR)                                                         \
-  case RR:                                                                  \
-    if (mode == 2) launch_path_t<RR, uint16_t, 2>(ctx, jobs, A, phase, waves); \
-    else if (mode == (2 | MODE_NT)) launch_path_t<RR, uint16_t, 2 | MODE_NT>(ctx, jobs, A, phase, waves); \
-    else if (mode == 3) launch_path_t<RR, uint16_t, 3>(ctx, jobs, A, phase, waves); \
-    else if (mode == 4) launch_path_t<RR, uint16_t, 4>(ctx, jobs, A, phase, waves); \
-    else if (mode == (4 | MODE_NT)) launch_path_t<RR, uint16_t, 4 | MODE_NT>(ctx, jobs, A, phase, waves); \
-    else if (mode == (1 | MODE_NT)) launch_path_t<RR, uint16_t, 1 | MODE_NT>(ctx, jobs, A, phase, waves); \
-    else launch_path_t<RR, uint16_t, 1>(ctx, jobs, A, phase, waves);        \
-    break;
-  if (sum_bytes == 4) {
-    switch (R) {
-      CASE_WIDE(1) CASE_WIDE(2) CASE_WIDE(4) CASE_WIDE(8) CASE_WIDE(16) CASE_WIDE(32) CASE_WIDE(64)
-      default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
-    }
-  } else {
-    switch (R) {
-      CASE_PK(2) CASE_PK(4) CASE_PK(8)
-#define CASE_PK_LOAD(RR)                                                    \
-  case RR:                                                                  \
-    if ((mode & 7) == 4) launch_path_t<RR, uint16_t, 4>(ctx, jobs, A, phase, waves); \
-    else launch_path_t<RR, uint16_t, 1>(ctx, jobs, A, phase, waves);        \
-    break;
-      CASE_PK_LOAD(16) CASE_PK_LOAD(32) CASE_PK_LOAD(64)
-#undef CASE_PK_LOAD
-      default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
-    }
-  }
-#undef CASE_WIDE
-#undef CASE_PK
-  return launch_check("path kernel");
-}
-
 int launch_path(sgm_context *ctx, int R, int sum_bytes, int mode, const PathJobs &jobs,
-                const PathArgs &A, int phase, int waves) {
+                const PathArgs &A, int phase, int waves, hipStream_t stream) {
   ProfileScope ps(ctx, "path");
   if (A.total_lines <= 0) return SGM_OK;
-  return launch_path_any(ctx, R, sum_bytes, mode, jobs, A, phase, waves);
+  hipStream_t st = stream ? stream : ctx->stream;
+  if (phase == PHASE_FIRST) return launch_path_first(st, R, sum_bytes, mode, jobs, A, waves);
+  if (phase == PHASE_MID) return launch_path_mid(st, R, sum_bytes, mode, jobs, A, waves);
+  return launch_path_last(st, R, sum_bytes, mode, jobs, A, waves);
 }
 
 int launch_lr(sgm_context *ctx, const LrArgs &A) {
@@ -939,36 +888,79 @@ struct PairBuffers {
   bool recompute;
 };
 
+// Whole-image device buffers of a pair, by name and size: inputs, descriptors
+// and disparities of both matching directions, the outputs, and (`volumes`)
+// the aggregate volumes with (`cost_volume`) the cost volumes.  Their sum is
+// pair_bytes(p, w, h, cost_volume) when `volumes` is set.
+static std::vector<std::pair<std::string, size_t>> pair_buffer_list(const sgm_params *p, int w,
+                                                                    int h, bool volumes,
+                                                                    bool cost_volume) {
+  const size_t n = (size_t)w * h, Dp = padded_disparities(p->disparities);
+  const int sb = aggregate_bytes(p), words = words_of(p);
+  const bool zncc = p->cost == SGM_COST_ZNCC || p->cost == SGM_COST_ZNCC_FP32;
+  std::vector<std::pair<std::string, size_t>> L;
+  for (int d = 0; d < 2; d++) {
+    const std::string s = std::to_string(d);
+    L.emplace_back("gray" + s, n);
+    L.emplace_back("valid" + s, n);
+    L.emplace_back("mask" + s, n);
+    L.emplace_back("gm" + s, n * 2);
+    L.emplace_back("desc" + s, n * 8 * words);
+    if (zncc) {
+      L.emplace_back("wsum" + s, n * 4);
+      L.emplace_back("wsum2" + s, n * 4);
+    }
+    if (volumes && cost_volume) L.emplace_back("cost" + s, n * Dp * (cost_u8(p) ? 1 : 2));
+    if (volumes) L.emplace_back("sum" + s, n * Dp * sb);
+    L.emplace_back("disp" + s, n * 4);
+  }
+  L.emplace_back("checked", n * 4);
+  L.emplace_back("reasons", n);
+  L.emplace_back("boundary", n);
+  L.emplace_back("reasons_out", n);
+  L.emplace_back("boundary_out", n);
+  L.emplace_back("stats", 16 * 8);
+  L.emplace_back("code", n);
+  return L;
+}
+
+}  // namespace
+
+// Device bytes the context holds once ensure_pair_buffers has served the pair
+// on the resident plan: each of the pair's buffers grows to its size when its
+// cached capacity is smaller, every other cached buffer stays as it is.
+int64_t sgm::resident_footprint(sgm_context *ctx, const sgm_params *p, int w, int h,
+                           bool cost_volume) {
+  int64_t total = ctx->allocated;
+  for (const auto &b : pair_buffer_list(p, w, h, true, cost_volume)) {
+    auto it = ctx->buffers.find(b.first);
+    const size_t cap = it != ctx->buffers.end() && it->second.ptr ? it->second.cap : 0;
+    if (b.second > cap) total += (int64_t)(b.second - cap);
+  }
+  return total;
+}
+
+namespace {
+
 // Resident (rows = 0), strip-streamed (rows > 0) or compact (pool > 0, strips
 // of `rows` rows) buffers of a pair.
 int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuffers &B,
                         int rows = 0, int pool = 0) {
   const size_t n = (size_t)w * h, Dp = padded_disparities(p->disparities);
-  const int sb = aggregate_bytes(p), words = words_of(p);
+  const int sb = aggregate_bytes(p);
   const bool zncc = p->cost == SGM_COST_ZNCC || p->cost == SGM_COST_ZNCC_FP32;
+  const bool recompute = !rows && recompute_cost(ctx, p, w, h);
   int rc;
 #define ENSURE(name, bytes)                                                   \
   if ((rc = ctx->buffers[name].ensure(ctx, (bytes), name)) != SGM_OK) return rc;
+  if (rows) {
+    // the streamed and compact plans hold no whole-image volume
+    for (const char *v : {"cost0", "cost1", "sum0", "sum1"}) ctx->buffers[v].release(ctx);
+  }
+  for (const auto &b : pair_buffer_list(p, w, h, rows == 0, !recompute))
+    ENSURE(b.first.c_str(), b.second);
   for (int d = 0; d < 2; d++) {
-    std::string s = std::to_string(d);
-    ENSURE(("gray" + s).c_str(), n);
-    ENSURE(("valid" + s).c_str(), n);
-    ENSURE(("mask" + s).c_str(), n);
-    ENSURE(("gm" + s).c_str(), n * 2);
-    ENSURE(("desc" + s).c_str(), n * 8 * words);
-    if (zncc) {
-      ENSURE(("wsum" + s).c_str(), n * 4);
-      ENSURE(("wsum2" + s).c_str(), n * 4);
-    }
-    if (rows) {
-      // the streamed plan's footprint holds no whole-image volume
-      ctx->buffers["cost" + s].release(ctx);
-      ctx->buffers["sum" + s].release(ctx);
-    } else {
-      if (!recompute_cost(ctx, p, w, h)) ENSURE(("cost" + s).c_str(), n * Dp * (cost_u8(p) ? 1 : 2));
-      ENSURE(("sum" + s).c_str(), n * Dp * sb);
-    }
-    ENSURE(("disp" + s).c_str(), n * 4);
+    const std::string s = std::to_string(d);
     B.gray[d] = (uint8_t *)ctx->buffers["gray" + s].ptr;
     B.valid[d] = (uint8_t *)ctx->buffers["valid" + s].ptr;
     B.mask[d] = (uint8_t *)ctx->buffers["mask" + s].ptr;
@@ -976,19 +968,11 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     B.desc[d] = (uint64_t *)ctx->buffers["desc" + s].ptr;
     B.wsum[d] = zncc ? (int32_t *)ctx->buffers["wsum" + s].ptr : nullptr;
     B.wsum2[d] = zncc ? (int32_t *)ctx->buffers["wsum2" + s].ptr : nullptr;
-    B.cost[d] = rows || recompute_cost(ctx, p, w, h) ? nullptr
-                                                     : (uint16_t *)ctx->buffers["cost" + s].ptr;
-    B.recompute = !rows && recompute_cost(ctx, p, w, h);
+    B.cost[d] = rows || recompute ? nullptr : (uint16_t *)ctx->buffers["cost" + s].ptr;
+    B.recompute = recompute;
     B.sum[d] = rows ? nullptr : ctx->buffers["sum" + s].ptr;
     B.disp[d] = (float *)ctx->buffers["disp" + s].ptr;
   }
-  ENSURE("checked", n * 4);
-  ENSURE("reasons", n);
-  ENSURE("boundary", n);
-  ENSURE("reasons_out", n);
-  ENSURE("boundary_out", n);
-  ENSURE("stats", 16 * 8);
-  ENSURE("code", n);
   B.bgr[0] = B.bgr[1] = nullptr;
   B.refine = nullptr;
   B.strip_rows = pool ? 0 : rows;

@@ -69,6 +69,132 @@ static int validate_pair(const sgm_params *p, const sgm_pair &pr, int flags) {
   return SGM_OK;
 }
 
+// Upper bound of the phase-major group tables (build_groups): the 2 * count
+// path jobs once, three line-prefix arrays per group, 256-byte alignment, at
+// most kMaxGroups lane groups.
+static constexpr int kMaxGroups = 8;
+static size_t group_table_bound(int count) {
+  return align_up(2 * (size_t)count * sizeof(PathJob) + 3 * 4 * (2 * (size_t)count + kMaxGroups) +
+                      (size_t)kMaxGroups * 4 * 256,
+                  256);
+}
+
+// Phase-major tables: the pairs' path jobs grouped by lane group R, and per
+// group and line class the first line of each job (prefix sums of the lines),
+// in one device allocation.
+static int build_groups(sgm_batch *b, const std::vector<PathJob> &jobs, const sgm_pair *pairs,
+                        int count) {
+  std::vector<int> rs;
+  for (const PathJob &J : jobs)
+    if (std::find(rs.begin(), rs.end(), J.R) == rs.end()) rs.push_back(J.R);
+  std::sort(rs.begin(), rs.end());
+  const int sb = b->sum_bytes;
+  std::vector<uint8_t> host;
+  struct Pending {
+    size_t jobs_off, lb_off[3];
+  };
+  std::vector<Pending> offs;
+  b->groups.clear();
+  for (int R : rs) {
+    sgm_batch::Group g;
+    g.R = R;
+    std::vector<PathJob> gj;
+    int64_t volume = 0;
+    bool aligned = true;
+    for (const PathJob &J : jobs)
+      if (J.R == R) {
+        gj.push_back(J);
+        volume += (int64_t)J.w * J.h * J.Dp * (b->cost8 ? 3 : 4);
+        aligned &= J.Dp % 64 == 0;
+      }
+    g.njobs = (int)gj.size();
+    // stream the volumes past the caches as the plain path does: whole 64-byte
+    // lines only, volumes far beyond the 256 MB MALL
+    g.nt = sb == 2 && !b->recompute && aligned && R <= 8 && volume > ((int64_t)768 << 20);
+    Pending pd;
+    pd.jobs_off = align_up(host.size(), 256);
+    host.resize(pd.jobs_off + gj.size() * sizeof(PathJob));
+    memcpy(host.data() + pd.jobs_off, gj.data(), gj.size() * sizeof(PathJob));
+    for (int cls = 0; cls < 3; cls++) {
+      std::vector<int> lb(gj.size() + 1, 0);
+      for (size_t j = 0; j < gj.size(); j++) lb[j + 1] = lb[j] + lines_of(cls, gj[j].w, gj[j].h);
+      g.total_lines[cls] = lb.back();
+      pd.lb_off[cls] = align_up(host.size(), 256);
+      host.resize(pd.lb_off[cls] + lb.size() * sizeof(int));
+      memcpy(host.data() + pd.lb_off[cls], lb.data(), lb.size() * sizeof(int));
+    }
+    offs.push_back(pd);
+    b->groups.push_back(g);
+  }
+  (void)pairs;
+  uint8_t *base;
+  if (b->external) {
+    // carved batches: the room batch_table_bytes reserves after the job tables
+    if (b->groups.size() > (size_t)kMaxGroups || host.size() > group_table_bound(count))
+      return fail(SGM_ERR_NO_PLAN, "batch group tables exceed their reserved room");
+    base = (uint8_t *)b->tables + align_up(b->tables_bytes, 256);
+  } else {
+    int rc = context_alloc(b->ctx, &b->group_tables, host.size(), "batch group tables");
+    if (rc) return rc;
+    b->group_tables_bytes = host.size();
+    base = (uint8_t *)b->group_tables;
+  }
+  hipError_t e = hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) return hip_fail(e, "batch group table upload");
+  for (size_t i = 0; i < b->groups.size(); i++) {
+    b->groups[i].jobs = (const PathJob *)(base + offs[i].jobs_off);
+    for (int cls = 0; cls < 3; cls++) b->groups[i].line_begin[cls] = (const int *)(base + offs[i].lb_off[cls]);
+  }
+  b->phased = true;
+  return SGM_OK;
+}
+
+// One launch of the work-queue kernel over items [lo, hi) with its own head.
+static int queue_launch(sgm_batch *b, int lo, int hi, unsigned *head, const char *what) {
+  if (hi <= lo) return SGM_OK;
+  BatchArgs A = b->args;
+  A.item_lo = lo;
+  A.item_hi = hi;
+  A.head = head;
+  A.phased = 1;
+  ProfileScope ps(b->ctx, what);
+  void *kargs[] = {(void *)&A};
+  hipError_t e = hipLaunchKernel(batch_kernel_of(b), dim3((unsigned)b->blocks),
+                                 dim3(b->waves_per_block * 64), kargs, 0, b->stream);
+  if (e == hipSuccess) e = hipGetLastError();
+  return e == hipSuccess ? SGM_OK : hip_fail(e, "batch kernel launch");
+}
+
+// Phase-major run: census (and cost) items, the sweeps direction by direction
+// and lane group by lane group with the specialised sweep kernel, the L-R items.
+static int phased_enqueue(sgm_batch *b) {
+  sgm_context *ctx = b->ctx;
+  const sgm_params *p = &b->params;
+  const std::vector<int> &pb = b->phase_begin;
+  const int last_path = BPH_PATH0 + p->directions - 1;
+  int rc;
+  if ((rc = queue_launch(b, 0, pb[BPH_PATH0], b->args.head, "batch census"))) return rc;
+  const int sb = b->sum_bytes;
+  const int base_mode = b->recompute ? 3 : sb == 2 ? (b->cost8 ? 4 : 1) : 0;
+  for (int dir = 0; dir < p->directions; dir++) {
+    const int phase = dir == 0 ? PHASE_FIRST : dir == p->directions - 1 ? PHASE_LAST : PHASE_MID;
+    for (const sgm_batch::Group &g : b->groups) {
+      PathArgs A = path_args(p, dir, g.njobs, 0, b->ctx->keep_volumes ? 1 : 0);
+      A.total_lines = g.total_lines[A.cls];
+      PathJobs pj;
+      memset(&pj, 0, sizeof pj);
+      pj.table = g.jobs;
+      pj.line_begin = g.line_begin[A.cls];
+      int mode = (dir == 0 && b->fused) ? 2 : base_mode;
+      if (g.nt) mode |= MODE_NT;
+      if ((rc = launch_path(ctx, g.R, sb, mode, pj, A, phase, path_waves(ctx, g.R), b->stream)))
+        return rc;
+    }
+  }
+  return queue_launch(b, pb[last_path + 1], pb[b->args.nph], b->head2, "batch lr");
+}
+
 static int batch_enqueue(sgm_batch *b) {
   sgm_context *ctx = b->ctx;
   hipStream_t st = b->stream;
@@ -94,7 +220,9 @@ static int batch_enqueue(sgm_batch *b) {
   }
   const int threads = b->waves_per_block * 64;
   const unsigned blocks = (unsigned)b->blocks;
-  {
+  if (b->phased) {
+    if ((rc = phased_enqueue(b))) return rc;
+  } else {
     ProfileScope ps(ctx, "batch");
     void *kargs[] = {(void *)&b->args};
     hipError_t e = hipLaunchKernel(batch_kernel_of(b), dim3(blocks), dim3(threads), kargs, 0, st);
@@ -155,6 +283,7 @@ size_t batch_arena_bytes(const sgm_params *params, const sgm_pair *pairs, int co
   const int nph = BPH_PATH0 + params->directions + 1, stride = 2 * count + 1;
   cv.take(4);
   cv.take(4);
+  cv.take(4);
   cv.take((size_t)nph * stride * 4);
   cv.take((size_t)count * 16 * 8);
   for (int i = 0; i < count; i++) {
@@ -185,7 +314,7 @@ size_t batch_arena_bytes(const sgm_params *params, const sgm_pair *pairs, int co
   return cv.off;
 }
 
-size_t batch_table_bytes(int count, const sgm_params *params) {
+static size_t base_table_bytes(int count, const sgm_params *params) {
   const int nph = BPH_PATH0 + params->directions + 1, stride = 2 * count + 1;
   Carver tc;
   tc.take(2 * count * sizeof(BatchImage));
@@ -195,6 +324,11 @@ size_t batch_table_bytes(int count, const sgm_params *params) {
   tc.take((nph + 1) * sizeof(int));
   tc.take((size_t)nph * stride * sizeof(int));
   return tc.off;
+}
+
+size_t batch_table_bytes(int count, const sgm_params *params) {
+  // job tables, then (carved batches) room for the phase-major group tables
+  return align_up(base_table_bytes(count, params), 256) + group_table_bound(count);
 }
 
 // Creates a batch.  With `region` non-null the arena and tables are carved from
@@ -263,7 +397,7 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
   const int nph = BPH_PATH0 + params->directions + 1;
   const int stride = 2 * count + 1;
   const size_t done_words = (size_t)nph * stride;
-  const size_t ctrl_head = cv.take(4), ctrl_error = cv.take(4);
+  const size_t ctrl_head = cv.take(4), ctrl_error = cv.take(4), ctrl_head2 = cv.take(4);
   const size_t ctrl_done = cv.take(done_words * 4);
   const size_t ctrl_stats = cv.take((size_t)count * 16 * 8);
   const size_t ctrl_end = cv.off;
@@ -317,6 +451,7 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
   }
   uint8_t *base = (uint8_t *)b->arena;
   b->d_control = (unsigned *)(base + ctrl_head);
+  b->head2 = (unsigned *)(base + ctrl_head2);
   b->control_words = (ctrl_end - ctrl_head) / 4;
   b->d_stats = (unsigned long long *)(base + ctrl_stats);
 
@@ -444,6 +579,7 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
   }
   phase_begin[nph] = total;
   b->total_items = total;
+  b->phase_begin = phase_begin;
 
   // ---- upload tables (one small allocation)
   Carver tc;
@@ -499,6 +635,15 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
   A.ww = params->window_w;
   A.wh = params->window_h;
   A.lr_threshold = params->lr_threshold;
+  A.item_lo = 0;
+  A.item_hi = total;
+  A.phased = 0;
+  if (flags & SGM_BATCH_PHASED) {
+    if ((rc = build_groups(b, jobs, pairs, count))) {
+      sgm_batch_destroy(b);
+      return rc;
+    }
+  }
   {
     // dependency waits give up after 20 s of wall clock (error flag, grid drains)
     int khz = 0;
@@ -619,6 +764,10 @@ extern "C" int sgm_batch_destroy(sgm_batch *b) {
     (void)hipFree(b->tables);
     ctx->allocated -= b->tables_bytes;
   }
+  if (b->group_tables) {
+    (void)hipFree(b->group_tables);
+    ctx->allocated -= b->group_tables_bytes;
+  }
   delete b;
   return SGM_OK;
 }
@@ -632,7 +781,8 @@ extern "C" int sgm_batch_describe(sgm_batch *b, char *buffer, size_t length) {
   const int n = snprintf(buffer, length,
                          "{\"pairs\":%zu,\"work_items\":%d,\"workgroups\":%d,"
                          "\"waves_per_workgroup\":%d,\"lines_per_item\":%d,"
-                         "\"cost_volume\":\"%s\",\"device_bytes\":%lld,\"graph\":%s}",
+                         "\"cost_volume\":\"%s\",\"device_bytes\":%lld,\"graph\":%s,"
+                         "\"phased\":%s,\"lane_groups\":%zu}",
                          b->pairs.size(), b->total_items, b->blocks, b->waves_per_block,
                          b->lines_per_item,
                          b->recompute ? "recomputed"
@@ -640,7 +790,8 @@ extern "C" int sgm_batch_describe(sgm_batch *b, char *buffer, size_t length) {
                          : b->cost8   ? "resident-u8"
                                       : "resident-u16",
                          (long long)(b->arena_bytes + b->tables_bytes),
-                         b->graph_exec ? "true" : "false");
+                         b->graph_exec ? "true" : "false", b->phased ? "true" : "false",
+                         b->groups.size());
   if (n < 0 || (size_t)n >= length) return fail(SGM_ERR_INVALID_ARGUMENT, "buffer too small");
   return SGM_OK;
 }

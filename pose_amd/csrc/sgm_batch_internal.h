@@ -55,4 +55,20 @@ struct sgm_batch {
   int total_items = 0;
   sgm::BatchArgs args{};
   hipGraphExec_t graph_exec = nullptr;
+  // phase-major runs (SGM_BATCH_PHASED): per lane group R the pairs' path jobs
+  // and, per line class (horizontal, vertical, diagonal), the first line of
+  // each job in the group's launch
+  struct Group {
+    int R = 0, njobs = 0;
+    bool nt = false;
+    const sgm::PathJob *jobs = nullptr;
+    const int *line_begin[3] = {nullptr, nullptr, nullptr};
+    int total_lines[3] = {0, 0, 0};
+  };
+  bool phased = false;
+  std::vector<Group> groups;
+  std::vector<int> phase_begin;  // host copy of the queue's phase boundaries
+  unsigned *head2 = nullptr;     // queue head of the L-R launch
+  void *group_tables = nullptr;
+  size_t group_tables_bytes = 0;
 };

@@ -57,6 +57,11 @@ struct BatchArgs {
   int ww, wh;
   double lr_threshold;
   long long spin_limit;    // wall-clock ticks a dependency wait may take before the error flag
+  // the queue range of this launch; a phase-major run launches the census /
+  // cost phases and the L-R phase separately (the sweeps run between them),
+  // and phases that end before item_lo were completed by earlier launches
+  int item_lo, item_hi;
+  int phased;
 };
 
 __device__ __forceinline__ int batch_phase_of(const BatchArgs &B, int item) {
@@ -90,6 +95,7 @@ __device__ __forceinline__ int batch_njobs(const BatchArgs &B, int ph) {
 // invalidation per iteration); the caller issues one acquire fence after the
 // wait succeeds.
 __device__ __forceinline__ bool batch_wait(const BatchArgs &B, int ph, int job) {
+  if (B.phased && B.phase_begin[ph] < B.item_lo) return true;  // an earlier launch ran it
   const unsigned need = (unsigned)batch_items(B, ph, job);
   unsigned *ctr = B.done + (size_t)ph * B.stride + job;
   if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need) return true;
@@ -320,7 +326,7 @@ void batch_kernel(BatchArgs B) {
   __syncthreads();
   const int lane = lane_id(), wave = threadIdx.x >> 6;
   uint8_t *tile = tiles[wave];
-  const int total = B.phase_begin[B.nph];
+  const int total = B.item_hi;
   static const int dirs[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
   // Completions of the current (phase, job) run are published together, before
   // the wave waits on anything else, so no wave waits while holding them.
@@ -330,7 +336,7 @@ void batch_kernel(BatchArgs B) {
   for (;;) {
     int got = 0;
     if (lane == 0) got = (int)atomicAdd(B.head, 1u);
-    const int item = __builtin_amdgcn_readlane(got, 0);
+    const int item = B.item_lo + __builtin_amdgcn_readlane(got, 0);
     if (item >= total) break;
     const int ph = batch_phase_of(B, item);
     const int nj = batch_njobs(B, ph);

@@ -63,6 +63,7 @@ bool census_recompute_possible(const sgm_params *p);
 bool recompute_cost(sgm_context *ctx, const sgm_params *p, int w, int h);
 int64_t pair_bytes(const sgm_params *p, int w, int h);
 int64_t pair_bytes(const sgm_params *p, int w, int h, bool cost_volume);
+int64_t resident_footprint(sgm_context *ctx, const sgm_params *p, int w, int h, bool cost_volume);
 int lines_of(int cls, int w, int h);
 void collect_profile(sgm_context *ctx);
 // plain pair on the context's stream without waiting (serving lanes)
@@ -93,8 +94,17 @@ const char *esgm_unsupported(const sgm_params *p, int w, int h);
 int esgm_plan(sgm_context *ctx, const sgm_params *p, int w, int h, int *rows, int *pool);
 CostArgs cost_args(const sgm_params *p);
 PathArgs path_args(const sgm_params *p, int dir, int njobs, int total_lines, int keep_sum);
+// on `stream` (the context's stream when null)
 int launch_path(sgm_context *ctx, int R, int sum_bytes, int mode, const PathJobs &jobs,
-                const PathArgs &A, int phase, int waves);
+                const PathArgs &A, int phase, int waves, hipStream_t stream = nullptr);
+int launch_check(const char *what);
+// one sweep phase's kernel instances each (sgm_path_first/mid/last.hip)
+int launch_path_first(hipStream_t st, int R, int sum_bytes, int mode, const PathJobs &jobs,
+                      const PathArgs &A, int waves);
+int launch_path_mid(hipStream_t st, int R, int sum_bytes, int mode, const PathJobs &jobs,
+                    const PathArgs &A, int waves);
+int launch_path_last(hipStream_t st, int R, int sum_bytes, int mode, const PathJobs &jobs,
+                     const PathArgs &A, int waves);
 int path_lanes(const sgm_params *p);
 bool fused_cost(const sgm_params *p);
 int launch_lr(sgm_context *ctx, const LrArgs &A);

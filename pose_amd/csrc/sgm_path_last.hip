@@ -1,0 +1,11 @@
+// Instances of the path-sweep kernel for the last sweep phase.
+#include "sgm_path_launch.h"
+
+namespace sgm {
+
+int launch_path_last(hipStream_t st, int R, int sum_bytes, int mode, const PathJobs &jobs,
+                     const PathArgs &A, int waves) {
+  return launch_path_phase<PHASE_LAST>(st, R, sum_bytes, mode, jobs, A, waves);
+}
+
+}  // namespace sgm

@@ -1,0 +1,61 @@
+// Path-sweep launch dispatch, shared by the three translation units that hold
+// the sweep kernel's instances (first, middle and last sweep phases; each unit
+// compiles one phase, so the instances build in parallel).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sgm_internal.h"
+
+namespace sgm {
+
+template <int R, typename SumT, int MODE, int PH>
+static void launch_path_t(hipStream_t st, const PathJobs &jobs, const PathArgs &A, int waves) {
+  const int threads = waves * 64;
+  const unsigned blocks = (unsigned)((A.total_lines + waves - 1) / waves);
+  path_kernel<R, SumT, PH, MODE><<<blocks, threads, 0, st>>>(jobs, A);
+}
+
+// MODE 0: generic 32-bit sweep (32-bit aggregates); 1: packed 16-bit; 2: packed
+// 16-bit first sweep with the census cost fused in.
+template <int PH>
+static int launch_path_phase(hipStream_t st, int R, int sum_bytes, int mode, const PathJobs &jobs,
+                             const PathArgs &A, int waves) {
+#define CASE_WIDE(RR)                                                       \
+  case RR:                                                                  \
+    launch_path_t<RR, int32_t, 0, PH>(st, jobs, A, waves);              \
+    break;
+#define CASE_PK(RR)                                                         \
+  case RR:                                                                  \
+    if (mode == 2) launch_path_t<RR, uint16_t, 2, PH>(st, jobs, A, waves); \
+    else if (mode == (2 | MODE_NT)) launch_path_t<RR, uint16_t, 2 | MODE_NT, PH>(st, jobs, A, waves); \
+    else if (mode == 3) launch_path_t<RR, uint16_t, 3, PH>(st, jobs, A, waves); \
+    else if (mode == 4) launch_path_t<RR, uint16_t, 4, PH>(st, jobs, A, waves); \
+    else if (mode == (4 | MODE_NT)) launch_path_t<RR, uint16_t, 4 | MODE_NT, PH>(st, jobs, A, waves); \
+    else if (mode == (1 | MODE_NT)) launch_path_t<RR, uint16_t, 1 | MODE_NT, PH>(st, jobs, A, waves); \
+    else launch_path_t<RR, uint16_t, 1, PH>(st, jobs, A, waves);        \
+    break;
+  if (sum_bytes == 4) {
+    switch (R) {
+      CASE_WIDE(1) CASE_WIDE(2) CASE_WIDE(4) CASE_WIDE(8) CASE_WIDE(16) CASE_WIDE(32) CASE_WIDE(64)
+      default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
+    }
+  } else {
+    switch (R) {
+      CASE_PK(2) CASE_PK(4) CASE_PK(8)
+#define CASE_PK_LOAD(RR)                                                    \
+  case RR:                                                                  \
+    if ((mode & 7) == 4) launch_path_t<RR, uint16_t, 4, PH>(st, jobs, A, waves); \
+    else launch_path_t<RR, uint16_t, 1, PH>(st, jobs, A, waves);        \
+    break;
+      CASE_PK_LOAD(16) CASE_PK_LOAD(32) CASE_PK_LOAD(64)
+#undef CASE_PK_LOAD
+      default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
+    }
+  }
+#undef CASE_WIDE
+#undef CASE_PK
+  return launch_check("path kernel");
+}
+
+}  // namespace sgm

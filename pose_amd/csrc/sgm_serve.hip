@@ -3,7 +3,8 @@
 // A server thread admits submitted pairs in arrival order as soon as their
 // device footprint fits the free part of a budgeted memory pool and a stream
 // slot is free, launches every admitted pair waiting at that moment as one
-// batch (persistent work-queue kernel) on that stream, and keeps up to
+// phase-major batch on that stream (work-queue kernel for the census and L-R
+// items, one sweep launch per direction and lane group), and keeps up to
 // `streams` batches in flight, so requests that arrive while others run
 // overlap with them on the device.  Volumes stay device-resident for the life
 // of their batch.  With max_batch == 1 each slot is a lane instead: a context
@@ -106,6 +107,7 @@ struct sgm_server {
   sgm_params params{};
   int max_batch = 16;
   bool lanes = false;  // max_batch == 1: plain per-pair lanes
+  int volume_policy = 0;  // the context's SGM_VOLUME_* at creation
   void *region = nullptr;
   size_t region_bytes = 0;
   Pool pool;
@@ -233,7 +235,7 @@ void sgm_server::run() {
       (void)id;
       lk.unlock();
       sgm_batch *b = nullptr;
-      int rc = batch_create(ctx, &params, prs.data(), (int)prs.size(), 0,
+      int rc = batch_create(ctx, &params, prs.data(), (int)prs.size(), SGM_BATCH_PHASED,
                             (uint8_t *)region + off, bytes, s.stream, &b, recompute);
       if (rc == SGM_OK) rc = batch_launch(b);
       if (rc == SGM_OK) {
@@ -275,45 +277,63 @@ void sgm_server::run() {
 }
 
 // Lane mode: the pair at the head of the queue on free lane `s` when the
-// plain-path footprint it will allocate fits what the other lanes leave of
-// the budget; cached buffers of idle lanes are released first if that makes
-// it fit.  The lane's own budget is capped at that remainder, so its plan
-// (resident cost volume or recomputed census costs) is chosen against the
-// memory actually left, and its cached buffers are dropped first when they
-// and the new footprint together would pass it.  Called and returning with
-// `mu` held.
+// device bytes the lane will hold after serving it, plus what the other lanes
+// hold, fit the budget.  The lane's cached buffers are reused where they are
+// large enough, so that footprint counts each of the pair's buffers at the
+// larger of its cached and required size.  Choices, in order: the pair's
+// cost volume (when the volume policy allows one) before the recomputed
+// census cost; the lane's cached buffers as they are, then the lane's own
+// buffers released, then the cached buffers of idle lanes released too.  The
+// lane's budget becomes what the other lanes leave and its volume policy the
+// chosen plan, so it runs exactly the admitted footprint.  Called and
+// returning with `mu` held.
 bool sgm_server::admit_lane(Slot &s) {
   Request *r = pending.front();
   sgm_params q = params;
   q.disparities = r->pair.disparities;
   q.min_disparity = r->pair.min_disparity;
   const int w = r->pair.width, h = r->pair.height;
+  sgm_context *lane = s.lane;
   auto others = [&]() {
     int64_t c = 0;
     for (Slot &o : slots)
       if (&o != &s) c += o.lane->allocated;
     return c;
   };
-  // the footprint of sgm_match_pair on a lane whose budget is `avail`
-  // (recompute_cost's rule: the cost volume stays when the pair fits with it)
-  auto need_for = [&](int64_t avail) {
-    const bool possible = census_recompute_possible(&q) && !s.lane->keep_volumes;
-    const int64_t with = pair_bytes(&q, w, h, true);
-    const int64_t without = possible ? pair_bytes(&q, w, h, false) : with;
-    if (s.lane->volume_policy == SGM_VOLUME_RECOMPUTE) return without;
-    if (s.lane->volume_policy == SGM_VOLUME_RESIDENT) return with;
-    return with <= avail ? with : without;
+  // plans the volume policy allows, preferred first
+  const bool possible = census_recompute_possible(&q) && !lane->keep_volumes;
+  bool options[2];
+  int nopt = 0;
+  if (!possible || volume_policy != SGM_VOLUME_RECOMPUTE) options[nopt++] = true;
+  if (possible && volume_policy != SGM_VOLUME_RESIDENT) options[nopt++] = false;
+  int chosen = -1;
+  auto pick = [&](bool fresh) {
+    const int64_t avail = (int64_t)region_bytes - others();
+    for (int i = 0; i < nopt; i++) {
+      const int64_t f = fresh ? pair_bytes(&q, w, h, options[i])
+                              : resident_footprint(lane, &q, w, h, options[i]);
+      if (f <= avail) return i;
+    }
+    return -1;
   };
-  int64_t avail = (int64_t)region_bytes - others();
-  if (need_for(avail) > avail) {
+  if ((chosen = pick(false)) < 0 && (chosen = pick(true)) >= 0) {
+    (void)sgm_release_buffers(lane);
+  } else if (chosen < 0) {
+    bool released = false;
     for (Slot &o : slots)
-      if (!o.busy && &o != &s && o.lane->allocated) (void)sgm_release_buffers(o.lane);
-    avail = (int64_t)region_bytes - others();
-    if (need_for(avail) > avail) return false;
+      if (!o.busy && &o != &s && o.lane->allocated) {
+        (void)sgm_release_buffers(o.lane);
+        released = true;
+      }
+    if (!released) return false;  // wait for a busy lane to finish
+    if ((chosen = pick(false)) < 0) {
+      if ((chosen = pick(true)) < 0) return false;
+      (void)sgm_release_buffers(lane);
+    }
   }
-  const int64_t need = need_for(avail);
-  if (s.lane->allocated && s.lane->allocated + need > avail) (void)sgm_release_buffers(s.lane);
-  s.lane->budget = avail;
+  lane->budget = (int64_t)region_bytes - others();
+  if (possible)
+    lane->volume_policy = options[chosen] ? SGM_VOLUME_RESIDENT : SGM_VOLUME_RECOMPUTE;
   pending.pop_front();
   const double t = now();
   r->admitted = t;
@@ -360,6 +380,7 @@ extern "C" int sgm_server_create(sgm_context *ctx, const sgm_params *params, int
   s->params = *params;
   s->max_batch = max_batch;
   s->lanes = max_batch == 1;
+  s->volume_policy = ctx->volume_policy;
   s->region_bytes = (size_t)bytes;
   if (!s->lanes) {
     rc = context_alloc(ctx, &s->region, s->region_bytes, "serving pool");

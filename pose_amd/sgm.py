@@ -897,7 +897,7 @@ class CensusSgm:
 # ---------------------------------------------------------------------------
 # Batched and served modes.
 
-BATCH_DEVICE, BATCH_GRAPH, BATCH_NO_SYNC = 1, 2, 4
+BATCH_DEVICE, BATCH_GRAPH, BATCH_NO_SYNC, BATCH_PHASED = 1, 2, 4, 8
 
 
 class Batch:
@@ -907,7 +907,7 @@ class Batch:
     min_disparity and optional valid_left / valid_right.  Outputs (checked, dl, dr,
     reasons, boundary, stats) are allocated here and refreshed by ``run()``."""
 
-    def __init__(self, ctx, params, pairs, graph=False):
+    def __init__(self, ctx, params, pairs, graph=False, phased=False):
         self.ctx = ctx
         self.params = params
         self.keep = []
@@ -935,8 +935,9 @@ class Batch:
             self.outputs.append(o)
         self.arr = arr
         h = ctypes.c_void_p()
-        check(lib().sgm_batch_create(ctx.h, ctypes.byref(params), arr, len(pairs),
-                                     BATCH_GRAPH if graph else 0, ctypes.byref(h)))
+        flags = (BATCH_GRAPH if graph else 0) | (BATCH_PHASED if phased else 0)
+        check(lib().sgm_batch_create(ctx.h, ctypes.byref(params), arr, len(pairs), flags,
+                                     ctypes.byref(h)))
         self.h = h
 
     def run(self):
@@ -968,11 +969,12 @@ class DeviceBatch:
     """A batch over caller device buffers (inputs resident in HBM): build it from
     sgm_pair records whose pointers are device pointers."""
 
-    def __init__(self, ctx, params, records, graph=True, no_sync=False):
+    def __init__(self, ctx, params, records, graph=True, no_sync=False, phased=False):
         self.ctx = ctx
         arr = (SgmPair * len(records))(*records)
         self.arr = arr
-        flags = BATCH_DEVICE | (BATCH_GRAPH if graph else 0) | (BATCH_NO_SYNC if no_sync else 0)
+        flags = (BATCH_DEVICE | (BATCH_GRAPH if graph else 0) | (BATCH_NO_SYNC if no_sync else 0)
+                 | (BATCH_PHASED if phased else 0))
         h = ctypes.c_void_p()
         check(lib().sgm_batch_create(ctx.h, ctypes.byref(params), arr, len(records), flags,
                                      ctypes.byref(h)))

@@ -363,3 +363,48 @@ def test_server_lanes_stay_within_budget_mixed_sizes(gpu_context):
         assert 0 < peak <= budget, (peak, budget)
     finally:
         srv.close()
+
+
+@pytest.mark.parametrize("policy", [sgm.VOLUME_RESIDENT, sgm.VOLUME_RECOMPUTE],
+                         ids=["resident", "recompute"])
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_phased_bit_exact(gpu_context, graph, policy):
+    """Phase-major run (queue kernel for census and L-R, one launch of the
+    specialised sweep kernel per direction and lane group): every pair equals
+    the oracle, over lane groups 2, 4 and 8, on graph replays too."""
+    pairs = uneven_pairs(11, 7, wrange=(96, 420), drange=(3, 500))
+    # disparities per lane of each pair's sweeps (16-bit aggregates: at least 2)
+    groups = {max(2, 1 << max(0, (p["disparities"] - 1) // 64).bit_length()) for p in pairs}
+    assert len(groups) >= 2, groups
+    params = to_sgm(O.params(**dict(CENSUS97, disparities=64)))
+    gpu_context.cost_volume(policy)
+    try:
+        b = sgm.Batch(gpu_context, params, pairs, graph=graph, phased=True)
+    finally:
+        gpu_context.cost_volume(sgm.VOLUME_AUTO)
+    try:
+        d = b.describe()
+        assert d["phased"] and d["lane_groups"] >= 2
+        for rep in range(2 if graph else 1):
+            res = b.run()
+            for i, pr in enumerate(pairs):
+                same(res[i], oracle_of(CENSUS97, pr), "phased pair %d rep %d" % (i, rep))
+    finally:
+        b.close()
+
+
+@pytest.mark.parametrize("kw", [
+    dict(window_w=7, window_h=7, cost=3, p1=12, p2=100),
+    dict(window_w=5, window_h=5, cost=2, p1=40, p2=9000),          # 32-bit aggregates
+    dict(window_w=11, window_h=11, cost=0, p1=20, p2=120, directions=4),
+], ids=["zncc", "census-gradient-agg32", "census11x11-4paths"])
+def test_batch_phased_other_costs_bit_exact(gpu_context, kw):
+    pairs = uneven_pairs(12, 4, wrange=(96, 260), drange=(3, 200))
+    params = to_sgm(O.params(**dict(kw, disparities=64)))
+    b = sgm.Batch(gpu_context, params, pairs, phased=True)
+    try:
+        res = b.run()
+        for i, pr in enumerate(pairs):
+            same(res[i], oracle_of(kw, pr), "phased pair %d" % i)
+    finally:
+        b.close()
