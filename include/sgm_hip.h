@@ -115,7 +115,7 @@ typedef struct sgm_plan {
   int32_t block_rows;           /* rows finished per recomputation block (h unless recompute) */
   int64_t disk_bytes;           /* temporary file of the disk storage */
   int32_t compact;              /* 1 if a pair runs the compact plan under the plan mode */
-  int32_t reserved;
+  int32_t paired;               /* 1 if a resident pair runs paired sweeps (sgm_set_paired_sweeps) */
 } sgm_plan;
 
 typedef struct sgm_context sgm_context;
@@ -337,6 +337,16 @@ int sgm_esgm_counts(sgm_context *ctx, int64_t counts[4]);
 #define SGM_VOLUME_RESIDENT 1
 #define SGM_VOLUME_RECOMPUTE 2
 int sgm_set_cost_volume(sgm_context *ctx, int policy);
+
+/* Paired path sweeps (default on; no counterpart in the reference, whose
+ * `paths` kernel runs one direction per launch, GpuStereo.java:732-757).  When
+ * a resident pair with 16-bit aggregates fits the budget with one more
+ * aggregate volume per matching direction, the middle directions run two per
+ * launch, each pair member accumulating into its own partial volume, and the
+ * last sweep adds both before the winner search.  The partial sums add up to
+ * the sequential aggregate modulo 2^16, so results are identical; off, or when
+ * the extra volume does not fit, each direction runs in its own launch. */
+int sgm_set_paired_sweeps(sgm_context *ctx, int enable);
 
 int sgm_synchronize(sgm_context *ctx);
 /* Free the per-pair buffers the plain entry points cache in the context. */

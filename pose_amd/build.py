@@ -5,7 +5,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SOURCES = ["csrc/sgm_api.hip", "csrc/sgm_path_first.hip", "csrc/sgm_path_mid.hip",
-           "csrc/sgm_path_last.hip", "csrc/sgm_batch.hip", "csrc/sgm_serve.hip"]
+           "csrc/sgm_path_last.hip", "csrc/sgm_path_pair.hip", "csrc/sgm_batch.hip",
+           "csrc/sgm_serve.hip"]
 OUTPUT = os.path.join(HERE, "libsgm_hip.so")
 
 HIPCC_FLAGS = [

@@ -502,6 +502,8 @@ extern "C" int sgm_plan_get(sgm_context *ctx, const sgm_params *params, int w, i
                   out->streamed && sp.storage == SGM_AGGREGATE_RECOMPUTE && pool > 0 &&
                   !esgm_unsupported(params, w, h));
   if (out->compact && ctx->plan_mode == SGM_PLAN_AUTO) out->streamed = 0;
+  out->paired = !out->compact && !out->streamed && out->resident &&
+                paired_sweeps(ctx, params, w, h, !recompute_cost(ctx, params, w, h), ctx->budget);
   return SGM_OK;
 }
 
@@ -596,6 +598,13 @@ extern "C" int sgm_set_cost_volume(sgm_context *ctx, int policy) {
   return SGM_OK;
 }
 
+extern "C" int sgm_set_paired_sweeps(sgm_context *ctx, int enable) {
+  if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> g(ctx->lock);
+  ctx->paired_sweeps = enable != 0;
+  return SGM_OK;
+}
+
 extern "C" int sgm_set_profiling(sgm_context *ctx, int enable) {
   if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
   ctx->profiling = enable != 0;
@@ -678,11 +687,12 @@ extern "C" int sgm_profile_json(sgm_context *ctx, char *buffer, size_t length) {
   static const char *storages[4] = {"auto", "ram", "disk", "recompute"};
   snprintf(b, sizeof b,
            "},\"allocated_bytes\":%lld,\"budget_bytes\":%lld,\"last_plan\":{\"plan\":\"%s\","
-           "\"storage\":\"%s\",\"block_rows\":%d}}",
+           "\"storage\":\"%s\",\"block_rows\":%d,\"paired\":%s}}",
            (long long)ctx->allocated, (long long)ctx->budget,
            ctx->last_compact ? "compact" : ctx->last_streamed ? "streamed" : "resident",
            ctx->last_streamed ? storages[ctx->splan.storage & 3] : "none",
-           ctx->last_streamed ? ctx->splan.block_rows : 0);
+           ctx->last_streamed ? ctx->splan.block_rows : 0,
+           !ctx->last_compact && !ctx->last_streamed && ctx->last_paired ? "true" : "false");
   s += b;
   if (s.size() + 1 > length) return fail(SGM_ERR_INVALID_ARGUMENT, "buffer too small (%zu)", s.size() + 1);
   memcpy(buffer, s.c_str(), s.size() + 1);
@@ -861,6 +871,9 @@ struct PairBuffers {
   int32_t *wsum[2], *wsum2[2];
   uint16_t *cost[2];
   void *sum[2];
+  // paired sweeps: the second partial aggregate volume of each matching
+  // direction (null: one aggregate, directions swept one per launch)
+  void *sum_b[2];
   float *disp[2], *checked;
   uint8_t *reasons, *boundary, *reasons_out, *boundary_out, *code;
   unsigned long long *stats;
@@ -888,13 +901,35 @@ struct PairBuffers {
   bool recompute;
 };
 
+}  // namespace
+
+// Bytes of the second partial aggregate volumes of paired sweeps (both
+// matching directions).
+int64_t sgm::paired_bytes(const sgm_params *p, int w, int h) {
+  return 2 * (int64_t)w * h * padded_disparities(p->disparities) * aggregate_bytes(p);
+}
+
+// Paired sweeps (two directions per launch over two partial aggregates): packed
+// 16-bit sweeps of lane groups 2..8 reading a cost volume, when the context
+// allows them and the pair with the second aggregate volumes fits `budget`.
+bool sgm::paired_sweeps(const sgm_context *ctx, const sgm_params *p, int w, int h,
+                        bool cost_volume, int64_t budget) {
+  if (!ctx->paired_sweeps || !cost_volume || aggregate_bytes(p) != 2 || path_lanes(p) > 8)
+    return false;
+  return pair_bytes(p, w, h, true) + paired_bytes(p, w, h) <= budget;
+}
+
+namespace {
+
 // Whole-image device buffers of a pair, by name and size: inputs, descriptors
 // and disparities of both matching directions, the outputs, and (`volumes`)
-// the aggregate volumes with (`cost_volume`) the cost volumes.  Their sum is
-// pair_bytes(p, w, h, cost_volume) when `volumes` is set.
+// the aggregate volumes with (`cost_volume`) the cost volumes and (`paired`)
+// the second partial aggregates of paired sweeps.  Their sum is
+// pair_bytes(p, w, h, cost_volume) (+ paired_bytes) when `volumes` is set.
 static std::vector<std::pair<std::string, size_t>> pair_buffer_list(const sgm_params *p, int w,
                                                                     int h, bool volumes,
-                                                                    bool cost_volume) {
+                                                                    bool cost_volume,
+                                                                    bool paired) {
   const size_t n = (size_t)w * h, Dp = padded_disparities(p->disparities);
   const int sb = aggregate_bytes(p), words = words_of(p);
   const bool zncc = p->cost == SGM_COST_ZNCC || p->cost == SGM_COST_ZNCC_FP32;
@@ -912,6 +947,7 @@ static std::vector<std::pair<std::string, size_t>> pair_buffer_list(const sgm_pa
     }
     if (volumes && cost_volume) L.emplace_back("cost" + s, n * Dp * (cost_u8(p) ? 1 : 2));
     if (volumes) L.emplace_back("sum" + s, n * Dp * sb);
+    if (volumes && paired) L.emplace_back("sumb" + s, n * Dp * sb);
     L.emplace_back("disp" + s, n * 4);
   }
   L.emplace_back("checked", n * 4);
@@ -927,12 +963,18 @@ static std::vector<std::pair<std::string, size_t>> pair_buffer_list(const sgm_pa
 }  // namespace
 
 // Device bytes the context holds once ensure_pair_buffers has served the pair
-// on the resident plan: each of the pair's buffers grows to its size when its
-// cached capacity is smaller, every other cached buffer stays as it is.
+// on the resident plan under `budget`: each of the pair's buffers grows to its
+// size when its cached capacity is smaller, every other cached buffer stays as
+// it is (`fresh`: no cached buffers).
 int64_t sgm::resident_footprint(sgm_context *ctx, const sgm_params *p, int w, int h,
-                           bool cost_volume) {
-  int64_t total = ctx->allocated;
-  for (const auto &b : pair_buffer_list(p, w, h, true, cost_volume)) {
+                                bool cost_volume, int64_t budget, bool fresh) {
+  const bool paired = paired_sweeps(ctx, p, w, h, cost_volume, budget);
+  int64_t total = fresh ? 0 : ctx->allocated;
+  for (const auto &b : pair_buffer_list(p, w, h, true, cost_volume, paired)) {
+    if (fresh) {
+      total += (int64_t)b.second;
+      continue;
+    }
     auto it = ctx->buffers.find(b.first);
     const size_t cap = it != ctx->buffers.end() && it->second.ptr ? it->second.cap : 0;
     if (b.second > cap) total += (int64_t)(b.second - cap);
@@ -950,6 +992,7 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
   const int sb = aggregate_bytes(p);
   const bool zncc = p->cost == SGM_COST_ZNCC || p->cost == SGM_COST_ZNCC_FP32;
   const bool recompute = !rows && recompute_cost(ctx, p, w, h);
+  const bool paired = !rows && paired_sweeps(ctx, p, w, h, !recompute, ctx->budget);
   int rc;
 #define ENSURE(name, bytes)                                                   \
   if ((rc = ctx->buffers[name].ensure(ctx, (bytes), name)) != SGM_OK) return rc;
@@ -957,7 +1000,9 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     // the streamed and compact plans hold no whole-image volume
     for (const char *v : {"cost0", "cost1", "sum0", "sum1"}) ctx->buffers[v].release(ctx);
   }
-  for (const auto &b : pair_buffer_list(p, w, h, rows == 0, !recompute))
+  if (!paired)
+    for (const char *v : {"sumb0", "sumb1"}) ctx->buffers[v].release(ctx);
+  for (const auto &b : pair_buffer_list(p, w, h, rows == 0, !recompute, paired))
     ENSURE(b.first.c_str(), b.second);
   for (int d = 0; d < 2; d++) {
     const std::string s = std::to_string(d);
@@ -971,6 +1016,7 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     B.cost[d] = rows || recompute ? nullptr : (uint16_t *)ctx->buffers["cost" + s].ptr;
     B.recompute = recompute;
     B.sum[d] = rows ? nullptr : ctx->buffers["sum" + s].ptr;
+    B.sum_b[d] = paired ? ctx->buffers["sumb" + s].ptr : nullptr;
     B.disp[d] = (float *)ctx->buffers["disp" + s].ptr;
   }
   B.bgr[0] = B.bgr[1] = nullptr;
@@ -1715,17 +1761,46 @@ int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuff
     J.R = R;
   }
   const int waves = path_waves(ctx, R);
-  for (int dir = 0; dir < p->directions; dir++) {
-    if (ctx->interrupt.exchange(0)) {
-      (void)hipStreamSynchronize(ctx->stream);
-      return fail(SGM_ERR_INTERRUPTED, "GPU SGM interrupted");
-    }
-    const int phase = dir == 0 ? PHASE_FIRST : dir == p->directions - 1 ? PHASE_LAST : PHASE_MID;
+  auto dir_args = [&](int dir) {
     PathArgs A = path_args(p, dir, nd, 0, ctx->keep_volumes ? 1 : 0);
     A.total_lines = nd * lines_of(A.cls, w, h);
     A.split = lines_of(A.cls, w, h);
-    const int m = (dir == 0 && fused) ? (2 | (mode & MODE_NT)) : mode;
-    if ((rc = launch_path(ctx, R, sb, m, pj, A, phase, waves))) return rc;
+    return A;
+  };
+  auto interrupted = [&]() {
+    if (!ctx->interrupt.exchange(0)) return false;
+    (void)hipStreamSynchronize(ctx->stream);
+    return true;
+  };
+  // Paired sweeps: after the first direction, the middle directions run two
+  // per launch, the first of each pair adding into the aggregate volume and
+  // the second into a second partial volume; the last sweep adds both.
+  const bool paired = B.sum_b[slots[0]] != nullptr && (mode & 7) != 3;
+  ctx->last_paired = paired;
+  PathJobs pjb = pj;
+  if (paired)
+    for (int i = 0; i < nd; i++) pjb.local[i].sum = B.sum_b[slots[i]];
+  const int last = p->directions - 1;
+  for (int dir = 0; dir <= last;) {
+    if (interrupted()) return fail(SGM_ERR_INTERRUPTED, "GPU SGM interrupted");
+    if (paired && dir > 0 && dir + 1 < last) {
+      // dir into the aggregate, dir + 1 into the second partial aggregate
+      // (its first direction when dir == 1)
+      if ((rc = launch_path_pair(ctx, R, mode, dir == 1 ? PHASE_FIRST : PHASE_MID, pj,
+                                 dir_args(dir), pjb, dir_args(dir + 1), waves)))
+        return rc;
+      dir += 2;
+      continue;
+    }
+    const int phase = dir == 0 ? PHASE_FIRST : dir == last ? PHASE_LAST : PHASE_MID;
+    int m = (dir == 0 && fused) ? (2 | (mode & MODE_NT)) : mode;
+    PathJobs jobs = pj;
+    if (paired && dir == last) {
+      for (int i = 0; i < nd; i++) jobs.local[i].sum2 = B.sum_b[slots[i]];
+      m |= MODE_SUM2;
+    }
+    if ((rc = launch_path(ctx, R, sb, m, jobs, dir_args(dir), phase, waves))) return rc;
+    dir++;
   }
   return SGM_OK;
 }

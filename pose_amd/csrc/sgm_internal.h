@@ -63,7 +63,11 @@ bool census_recompute_possible(const sgm_params *p);
 bool recompute_cost(sgm_context *ctx, const sgm_params *p, int w, int h);
 int64_t pair_bytes(const sgm_params *p, int w, int h);
 int64_t pair_bytes(const sgm_params *p, int w, int h, bool cost_volume);
-int64_t resident_footprint(sgm_context *ctx, const sgm_params *p, int w, int h, bool cost_volume);
+int64_t resident_footprint(sgm_context *ctx, const sgm_params *p, int w, int h, bool cost_volume,
+                           int64_t budget, bool fresh);
+int64_t paired_bytes(const sgm_params *p, int w, int h);
+bool paired_sweeps(const sgm_context *ctx, const sgm_params *p, int w, int h, bool cost_volume,
+                   int64_t budget);
 int lines_of(int cls, int w, int h);
 void collect_profile(sgm_context *ctx);
 // plain pair on the context's stream without waiting (serving lanes)
@@ -98,6 +102,11 @@ PathArgs path_args(const sgm_params *p, int dir, int njobs, int total_lines, int
 int launch_path(sgm_context *ctx, int R, int sum_bytes, int mode, const PathJobs &jobs,
                 const PathArgs &A, int phase, int waves, hipStream_t stream = nullptr);
 int launch_check(const char *what);
+// two directions in one launch: `ja` (phase MID) and `jb` (phase_b: FIRST or
+// MID) accumulate into different partial aggregate volumes
+int launch_path_pair(sgm_context *ctx, int R, int mode, int phase_b, const PathJobs &ja,
+                     const PathArgs &A, const PathJobs &jb, const PathArgs &B, int waves,
+                     hipStream_t stream = nullptr);
 // one sweep phase's kernel instances each (sgm_path_first/mid/last.hip)
 int launch_path_first(hipStream_t st, int R, int sum_bytes, int mode, const PathJobs &jobs,
                       const PathArgs &A, int waves);
@@ -142,6 +151,8 @@ struct sgm_context {
   // sgm_interrupt: set by any thread, consumed by the matching call that sees it
   std::atomic<int> interrupt{0};
   bool last_compact = false;
+  bool paired_sweeps = true;  // sgm_set_paired_sweeps
+  bool last_paired = false;
   sgm::StreamedPlan splan;  // streamed plan picked for the current call
   std::vector<sgm::PendingEvent> pending_events;
   std::map<std::string, std::pair<double, long long>> kernel_stats;

@@ -35,6 +35,29 @@ static int launch_path_phase(hipStream_t st, int R, int sum_bytes, int mode, con
     else if (mode == (1 | MODE_NT)) launch_path_t<RR, uint16_t, 1 | MODE_NT, PH>(st, jobs, A, waves); \
     else launch_path_t<RR, uint16_t, 1, PH>(st, jobs, A, waves);        \
     break;
+  if (mode & MODE_SUM2) {
+    // last sweep over paired partial aggregates (packed lane groups 2, 4, 8)
+    if constexpr (PH == PHASE_LAST) {
+      const int m = mode & ~MODE_SUM2;
+#define CASE_SUM2(RR)                                                                     \
+  case RR:                                                                                \
+    if (m == 1) launch_path_t<RR, uint16_t, 1 | MODE_SUM2, PH>(st, jobs, A, waves);       \
+    else if (m == (1 | MODE_NT)) launch_path_t<RR, uint16_t, 1 | MODE_NT | MODE_SUM2, PH>(st, jobs, A, waves); \
+    else if (m == 4) launch_path_t<RR, uint16_t, 4 | MODE_SUM2, PH>(st, jobs, A, waves);  \
+    else if (m == (4 | MODE_NT)) launch_path_t<RR, uint16_t, 4 | MODE_NT | MODE_SUM2, PH>(st, jobs, A, waves); \
+    else return fail(SGM_ERR_INVALID_ARGUMENT, "no paired-aggregate sweep for mode %d", m); \
+    break;
+      if (sum_bytes != 2) return fail(SGM_ERR_INVALID_ARGUMENT, "paired aggregates are 16-bit");
+      switch (R) {
+        CASE_SUM2(2) CASE_SUM2(4) CASE_SUM2(8)
+        default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
+      }
+#undef CASE_SUM2
+      return launch_check("path kernel");
+    } else {
+      return fail(SGM_ERR_INVALID_ARGUMENT, "a second partial aggregate is read by the last sweep");
+    }
+  }
   if (sum_bytes == 4) {
     switch (R) {
       CASE_WIDE(1) CASE_WIDE(2) CASE_WIDE(4) CASE_WIDE(8) CASE_WIDE(16) CASE_WIDE(32) CASE_WIDE(64)

@@ -48,6 +48,9 @@ struct PathJob {
   uint8_t *code;            // optional: per-pixel selection code for the counters
   int w, h, D, Dp, minD, rx, ry, reverse;
   int R;                    // disparities per lane of this job's sweeps
+  // last sweep over paired partial aggregates: the second partial volume,
+  // added to `sum` before the winner search (null: a single aggregate)
+  const void *sum2;
 };
 
 struct PathArgs {
@@ -817,7 +820,8 @@ __device__ __forceinline__ void census_costs(const uint64_t *tw, uint32_t alo, u
 // line, fixed at compile time (1: towards higher x indices of the window, 0:
 // lower) so the window rotates by register renaming in the unrolled steps; -1
 // where the line has no sliding window.
-template <int R, int PHASE, int CM, int PFCAP = 64, int SLIDE_UP = -1, bool NT = false>
+template <int R, int PHASE, int CM, int PFCAP = 64, int SLIDE_UP = -1, bool NT = false,
+          bool SUM2 = false>
 struct PkSweep {
   static constexpr int NP = R / 2;
   static constexpr int POL = NT ? POL_NT : POL_DEFAULT;
@@ -839,7 +843,8 @@ struct PkSweep {
   ptrdiff_t cstep, sstep;      // bytes per pixel step in the cost / aggregate volume
   uint32_t keep[NP], infm[NP];
   uint32_t vmask, p1pk;
-  uint32_t cb[kLoaded ? PF : 1][NC], sb[PF][NP];
+  uint32_t cb[kLoaded ? PF : 1][NC], sb[PF][NP], sb2[SUM2 ? PF : 1][NP];
+  const uint8_t *ld_s2;  // second partial aggregate (SUM2): rows of the next prefetched step
   uint64_t win[kComputed ? R : 1];
   uint64_t dsc[kDirect][CM == CM_RECOMPUTE ? R : 1];
   uint32_t prev[NP];
@@ -873,6 +878,7 @@ struct PkSweep {
       st_c = (uint8_t *)J.cost_out + p0 * J.Dp * cb_;
       ld_s = (const uint8_t *)J.sum + p0 * J.Dp * 2;
       st_s = (uint8_t *)J.sum + p0 * J.Dp * 2;
+      ld_s2 = SUM2 ? (const uint8_t *)J.sum2 + p0 * J.Dp * 2 : nullptr;
     }
     sum_store_bytes = (PHASE != PHASE_LAST || A.keep_sum) ? (int)row_bytes : 0;
     slide = CM == CM_FUSED || (CM == CM_RECOMPUTE && A.dy == 0);
@@ -943,6 +949,10 @@ struct PkSweep {
       if (!slide) load_targets(dsc[slot], s);
     if constexpr (PHASE != PHASE_FIRST)
       bload_pairs<NP, POL>(sb[slot], make_rsrc(ld_s, in ? row_bytes : 0u), loff);
+    if constexpr (SUM2) {
+      bload_pairs<NP, POL>(sb2[slot], make_rsrc(ld_s2, in ? row_bytes : 0u), loff);
+      ld_s2 += sstep;
+    }
     ld_c += cstep;
     ld_s += sstep;
   }
@@ -1041,7 +1051,10 @@ struct PkSweep {
     int lmin = (int)min(lm & 0xFFFFu, lm >> 16);
     uint32_t S[NP];
 #pragma unroll
-    for (int i = 0; i < NP; i++) S[i] = PHASE == PHASE_FIRST ? L[i] : pk_add(sb[slot][i], L[i]);
+    for (int i = 0; i < NP; i++) {
+      if constexpr (SUM2) S[i] = pk_add(pk_add(sb[slot][i], sb2[slot][i]), L[i]);
+      else S[i] = PHASE == PHASE_FIRST ? L[i] : pk_add(sb[slot][i], L[i]);
+    }
     bstore_pairs<NP, POL>(S, make_rsrc(st_s, (uint32_t)sum_store_bytes), loff);
     st_s += sstep;
     if constexpr (PHASE == PHASE_LAST) {
@@ -1197,7 +1210,7 @@ struct PkSweep {
   }
 };
 
-template <int R, int PHASE, int CM, int PFCAP = 64, bool NT = false>
+template <int R, int PHASE, int CM, int PFCAP = 64, bool NT = false, bool SUM2 = false>
 __device__ __forceinline__ void sweep_line_pk(const PathJob &J, const PathArgs &A, int line,
                                               const int *p2_table) {
   static_assert(R % 2 == 0, "packed sweep needs an even lane group");
@@ -1212,7 +1225,7 @@ __device__ __forceinline__ void sweep_line_pk(const PathJob &J, const PathArgs &
     }
     return;
   }
-  PkSweep<R, PHASE, CM, PFCAP, -1, NT> sw(J, A, line, p2_table);
+  PkSweep<R, PHASE, CM, PFCAP, -1, NT, SUM2> sw(J, A, line, p2_table);
   sw.run();
 }
 
@@ -1250,19 +1263,60 @@ struct PathJobs {
 // packed u16 first sweep with the census cost fused in (writes the uint8 cost
 // volume); 3 packed u16 with the census cost recomputed in the sweep (no cost
 // volume); 4 packed u16 reading the uint8 cost volume.  MODE_NT (added to 1, 2
-// or 4): the volume accesses stream (nontemporal policy).
+// or 4): the volume accesses stream (nontemporal policy).  MODE_SUM2 (added
+// to 1 or 4 of a last sweep): the job's second partial aggregate is added in.
 constexpr int MODE_NT = 8;
+constexpr int MODE_SUM2 = 16;
 
 template <int R, typename SumT, int PHASE, int MODE>
 __device__ __forceinline__ void sweep_dispatch(const PathJob &J, const PathArgs &A, int line,
                                                const int *p2_table) {
   constexpr int M = MODE & 7;
   constexpr bool NT = (MODE & MODE_NT) != 0;
+  constexpr bool SUM2 = (MODE & MODE_SUM2) != 0;
+  static_assert(!SUM2 || ((M == 1 || M == 4) && PHASE == PHASE_LAST),
+                "a second partial aggregate is read by last sweeps over a cost volume");
   if constexpr (M == 0) sweep_line<R, SumT, PHASE>(J, A, line, p2_table);
-  else if constexpr (M == 1) sweep_line_pk<R, PHASE, CM_LOAD, 64, NT>(J, A, line, p2_table);
+  else if constexpr (M == 1) sweep_line_pk<R, PHASE, CM_LOAD, 64, NT, SUM2>(J, A, line, p2_table);
   else if constexpr (M == 2) sweep_line_pk<R, PHASE, CM_FUSED, 64, NT>(J, A, line, p2_table);
   else if constexpr (M == 3) sweep_line_pk<R, PHASE, CM_RECOMPUTE>(J, A, line, p2_table);
-  else sweep_line_pk<R, PHASE, CM_LOAD8, 64, NT>(J, A, line, p2_table);
+  else sweep_line_pk<R, PHASE, CM_LOAD8, 64, NT, SUM2>(J, A, line, p2_table);
+}
+
+// Two path directions in one launch, each accumulating into its own partial
+// aggregate volume (the jobs of `ja` and `jb` point at different volumes):
+// lines [0, A.total_lines) sweep direction A in phase PHA, the next
+// B.total_lines lines direction B in phase PHB.  Uint16 sums wrap modulo 2^16
+// in any order, so the partial volumes add up to the sequential aggregate
+// exactly; one launch carries the lines of both directions, twice the waves
+// of one, which keeps more volume requests in flight per CU.
+template <int R, int PHA, int PHB, int MODE>
+__global__ __launch_bounds__(256) void path_pair_kernel(PathJobs ja, PathArgs A, PathJobs jb,
+                                                        PathArgs B) {
+  __shared__ int p2_table[256];
+  build_p2_table(p2_table, A);  // P1, P2 and the edge scale are shared by both directions
+  __syncthreads();
+  const int waves = blockDim.x >> 6;
+  int gline = __builtin_amdgcn_readfirstlane(blockIdx.x * waves + (int)(threadIdx.x >> 6));
+  if (gline < A.total_lines) {
+    if (A.njobs > 1 && gline >= A.split) {
+      const PathJob J = ja.local[1];
+      sweep_dispatch<R, uint16_t, PHA, MODE>(J, A, gline - A.split, p2_table);
+    } else {
+      const PathJob J = ja.local[0];
+      sweep_dispatch<R, uint16_t, PHA, MODE>(J, A, gline, p2_table);
+    }
+    return;
+  }
+  gline -= A.total_lines;
+  if (gline >= B.total_lines) return;
+  if (B.njobs > 1 && gline >= B.split) {
+    const PathJob J = jb.local[1];
+    sweep_dispatch<R, uint16_t, PHB, MODE>(J, B, gline - B.split, p2_table);
+  } else {
+    const PathJob J = jb.local[0];
+    sweep_dispatch<R, uint16_t, PHB, MODE>(J, B, gline, p2_table);
+  }
 }
 
 template <int R, typename SumT, int PHASE, int MODE>

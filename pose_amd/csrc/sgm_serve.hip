@@ -310,8 +310,7 @@ bool sgm_server::admit_lane(Slot &s) {
   auto pick = [&](bool fresh) {
     const int64_t avail = (int64_t)region_bytes - others();
     for (int i = 0; i < nopt; i++) {
-      const int64_t f = fresh ? pair_bytes(&q, w, h, options[i])
-                              : resident_footprint(lane, &q, w, h, options[i]);
+      const int64_t f = resident_footprint(lane, &q, w, h, options[i], avail, fresh);
       if (f <= avail) return i;
     }
     return -1;

@@ -96,7 +96,7 @@ class SgmPlan(ctypes.Structure):
                 ("esgm_candidates", ctypes.c_int32), ("esgm_rows", ctypes.c_int32),
                 ("esgm_bytes", ctypes.c_int64), ("aggregate_storage", ctypes.c_int32),
                 ("block_rows", ctypes.c_int32), ("disk_bytes", ctypes.c_int64),
-                ("compact", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("compact", ctypes.c_int32), ("paired", ctypes.c_int32)]
 
 
 class SgmPair(ctypes.Structure):
@@ -121,7 +121,8 @@ EXPORTS = [
     "sgm_disparity_limit", "sgm_match", "sgm_match_desc", "sgm_match_pair", "sgm_match_pair_device",
     "sgm_match_pair_device_noalloc",
     "sgm_match_pair_filtered", "sgm_match_pair_color",
-    "sgm_set_keep_volumes", "sgm_read_volumes", "sgm_set_cost_volume", "sgm_set_plan_mode",
+    "sgm_set_keep_volumes", "sgm_read_volumes", "sgm_set_cost_volume", "sgm_set_paired_sweeps",
+    "sgm_set_plan_mode",
     "sgm_set_esgm_candidates", "sgm_esgm_counts", "sgm_synchronize",
     "sgm_set_aggregate_storage", "sgm_interrupt", "sgm_debug_fail_host_allocations",
     "sgm_release_buffers", "sgm_set_profiling",
@@ -173,6 +174,7 @@ def lib():
                                                P, P, P, I, I, P, P, P, ps, P, P]
             L.sgm_set_keep_volumes.argtypes = [P, I]
             L.sgm_set_cost_volume.argtypes = [P, I]
+            L.sgm_set_paired_sweeps.argtypes = [P, I]
             L.sgm_set_plan_mode.argtypes = [P, I, I]
             L.sgm_set_esgm_candidates.argtypes = [P, I]
             L.sgm_set_aggregate_storage.argtypes = [P, I, ctypes.c_char_p, I64]
@@ -546,6 +548,11 @@ class Context:
     def cost_volume(self, policy):
         """Cost-volume policy: VOLUME_AUTO, VOLUME_RESIDENT or VOLUME_RECOMPUTE."""
         check(lib().sgm_set_cost_volume(self.h, int(policy)))
+
+    def paired_sweeps(self, on=True):
+        """Two middle path directions per launch over two partial aggregate
+        volumes when the extra volume fits the budget (default on)."""
+        check(lib().sgm_set_paired_sweeps(self.h, int(bool(on))))
 
     def plan_mode(self, mode, strip_rows=64):
         """Aggregation plan: PLAN_AUTO, PLAN_RESIDENT, PLAN_STREAMED or PLAN_ESGM
