@@ -31,17 +31,22 @@ def workload_params(sgm):
                          lr_threshold=1.0)
 
 
-def path_bytes_per_step(w, h, D, paths, cost_bytes=1):
-    """HBM bytes this build's path sweeps move for one pair (both directions),
-    uint16 aggregates S and a cost volume C of `cost_bytes` per cell: the first
-    sweep computes the census cost and writes C and S (1 + 2 B/cell), middle
-    sweeps read C, read S and write S (1 + 2 + 2 B/cell), the last sweep reads C
-    and S and selects (1 + 2 B/cell): 36 B per cell at 8 paths."""
+def path_bytes_per_step(w, h, D, paths, schedule, cost_bytes=1):
+    """HBM bytes this build's path sweeps move for one pair (both directions)
+    with a uint8 cost volume C, by sweep schedule (the plan's `sweeps`).
+    Path-cost planes (3): the first sweep computes the census cost and writes C
+    and its plane (1 + 1 B/cell), the middle directions read C and write their
+    planes (1 + 1 B/cell each), the last sweep reads C and the P-1 planes and
+    selects (1 + 7 B/cell): 22 B per cell at 8 paths.  One uint16 aggregate S
+    (1, 2): first 1 + 2, middle 1 + 2 + 2 (a paired launch's second member 1 +
+    2, and the last sweep then reads both partial sums), last 1 + 2: 36 B per
+    cell at 8 paths either way."""
     cells = 2 * w * h * D
-    first = cost_bytes + 2
-    mid = cost_bytes + 4
-    last = cost_bytes + 2
-    return cells * (first + mid * (paths - 2) + last)
+    if schedule == 3:
+        per_cell = (cost_bytes + 1) * (paths - 1) + cost_bytes + (paths - 1)
+    else:
+        per_cell = (cost_bytes + 2) + (cost_bytes + 4) * (paths - 2) + (cost_bytes + 2)
+    return cells * per_cell
 
 
 def algorithmic_bytes_per_step(w, h, D, paths):
@@ -522,8 +527,10 @@ def write_traffic(fetch_dir, write_dir, copy_bytes=2 << 30, paths=8):
     `bench.py --steps 1 --warmup 1 --profile-steps 1 --no-...` (FETCH_SIZE,
     then WRITE_SIZE, KiB per dispatch).  Each counter is scaled by what it
     reads for the library's copy kernel (copy_bytes read + copy_bytes written
-    per launch); path launches are grouped by sweep phase (first / middle /
-    last), and the per-launch figure weights them as one pair does."""
+    per launch); path launches are grouped by kind (first sweep, middle
+    directions, last sweep), and the per-launch figure weights them as one pair
+    does under the schedule the passes ran (path-cost planes: one launch of
+    each kind; one aggregate: one first, P - 2 middle, one last)."""
     W, H, D = 1600, 1067, 256
     fetch, write = read_counter_csv(fetch_dir, "FETCH_SIZE"), read_counter_csv(write_dir, "WRITE_SIZE")
 
@@ -537,8 +544,17 @@ def write_traffic(fetch_dir, write_dir, copy_bytes=2 << 30, paths=8):
     fetch_scale = copy_bytes / 1024.0 / mean_of(fetch, is_copy)
     write_scale = copy_bytes / 1024.0 / mean_of(write, is_copy)
     cells = 2 * W * H * D
-    alg = {"first": 3 * cells, "mid": 5 * cells, "last": 3 * cells}
+    planes = any("path_multi_kernel" in k for k, _ in fetch)
+    if planes:
+        kinds = (("first", lambda k: "path_kernel" in k and ", 0, " in k, 2 * cells, 1),
+                 ("multi", lambda k: "path_multi_kernel" in k, 2 * (paths - 2) * cells, 1),
+                 ("last", lambda k: "path_kernel" in k and ", 2, " in k, paths * cells, 1))
+    else:
+        kinds = (("first", lambda k: "path_kernel" in k and ", 0, " in k, 3 * cells, 1),
+                 ("mid", lambda k: "path_kernel" in k and ", 1, " in k, 5 * cells, paths - 2),
+                 ("last", lambda k: "path_kernel" in k and ", 2, " in k, 3 * cells, 1))
     out = {"workload": "1600x1067, census 9x7, D=256, 8 paths, both matching directions per launch",
+           "sweeps": "planes" if planes else "sequential",
            "source_hash": kernel_source_hash(), "fetch_scale": round(fetch_scale, 4),
            "write_scale": round(write_scale, 4),
            "calibration": "copy_kernel: %d B read + %d B written per launch; FETCH_SIZE read %.0f "
@@ -546,18 +562,19 @@ def write_traffic(fetch_dir, write_dir, copy_bytes=2 << 30, paths=8):
                                                         mean_of(fetch, is_copy),
                                                         mean_of(write, is_copy)),
            "per_launch": {}}
-    for phase, tag in (("first", ", 0, "), ("mid", ", 1, "), ("last", ", 2, ")):
-        pred = lambda k, tag=tag: "path_kernel" in k and tag in k
+    total_hbm = total_build = launches = 0
+    for name, pred, build, count in kinds:
         names = sorted({k for k, _ in fetch if pred(k)})
         rd = mean_of(fetch, pred) * 1024 * fetch_scale
         wr = mean_of(write, pred) * 1024 * write_scale
-        out["per_launch"][phase] = {"kernel": names, "read_bytes": int(rd), "write_bytes": int(wr),
-                                    "hbm_bytes": int(rd + wr), "algorithmic_bytes": alg[phase],
-                                    "ratio": round((rd + wr) / alg[phase], 4)}
-    pl = out["per_launch"]
-    out["bytes_per_launch"] = int((pl["first"]["hbm_bytes"] + (paths - 2) * pl["mid"]["hbm_bytes"]
-                                   + pl["last"]["hbm_bytes"]) / paths)
-    out["build_bytes_per_launch"] = int((alg["first"] + (paths - 2) * alg["mid"] + alg["last"]) / paths)
+        out["per_launch"][name] = {"kernel": names, "read_bytes": int(rd), "write_bytes": int(wr),
+                                   "hbm_bytes": int(rd + wr), "build_bytes": build,
+                                   "ratio": round((rd + wr) / build, 4)}
+        total_hbm += (rd + wr) * count
+        total_build += build * count
+        launches += count
+    out["bytes_per_launch"] = int(total_hbm / launches)
+    out["build_bytes_per_launch"] = int(total_build / launches)
     with open(os.path.join(ROOT, "profiles", "path_traffic.json"), "w") as fh:
         json.dump(out, fh, indent=1)
     print(json.dumps(out))
@@ -630,13 +647,28 @@ def main():
     ctx.synchronize()
     prof = ctx.profile()["device_kernel_milliseconds"]
     ctx.set_profiling(False)
-    path_ms = prof["path"]["total_ms"] / args.profile_steps
-    path_launches = prof["path"]["launches"] / args.profile_steps
+    schedule = ctx.plan(params, W, H)["sweeps"]
+    # every path-sweep launch of the pair (first, middle / paired / multi-direction, last)
+    path_keys = [k for k in prof if k.startswith("path")]
+    path_ms = sum(prof[k]["total_ms"] for k in path_keys) / args.profile_steps
+    path_launches = sum(prof[k]["launches"] for k in path_keys) / args.profile_steps
     launch_s = path_ms / 1e3 / path_launches
     alg_launch = algorithmic_bytes_per_step(W, H, D, P) / path_launches
-    build_launch = path_bytes_per_step(W, H, D, P) / path_launches
+    build_launch = path_bytes_per_step(W, H, D, P, schedule) / path_launches
     achieved = alg_launch / launch_s / 1e9
     kernels = {k: round(v["total_ms"] / args.profile_steps, 4) for k, v in prof.items()}
+    # each path launch of the schedule on its own: duration and the build's bytes
+    cells = 2 * W * H * D
+    own_bytes = ({"path_first": 2, "path_multi": 2 * (P - 2), "path_last": 1 + (P - 1)}
+                 if schedule == 3 else {"path_first": 3, "path_mid": 5, "path_pair": 8, "path_last": 3})
+    path_kernels = {}
+    for k in path_keys:
+        ms = prof[k]["total_ms"] / prof[k]["launches"]
+        b = own_bytes.get(k, 0) * cells
+        path_kernels[k] = {"launches_per_step": prof[k]["launches"] / args.profile_steps,
+                           "ms": round(ms, 4), "build_bytes": b,
+                           "build_achieved": round(b / (ms / 1e3) / 1e9, 1),
+                           "build_frac": round(b / (ms / 1e3) / 1e9 / PEAK_HBM_GBPS, 4)}
 
     # Achievable HBM bandwidth: the library's plain streaming copy kernel over
     # 2 GiB (reads 2 GiB, writes 2 GiB); the same launch calibrates the HBM
@@ -713,7 +745,8 @@ def main():
         "dtype": "u16",
         "data": "synthetic",
         "config": {"workload": "plain SGM pair 1600x1067, census 9x7, D=256, P1=8 P2=96, "
-                               "8 paths, uint8 cost / uint16 aggregate volumes, left+right + L-R check",
+                               "8 paths, left+right + L-R check; uint8 cost volume, uint8 "
+                               "path-cost planes summed in packed uint16 by the last sweep",
                    "width": W, "height": H, "disparities": D, "paths": P, "pairs_per_step": 1},
         "device": {k: dev[k] for k in ("name", "arch", "memory", "compute_units")},
         "pairs_per_s": round(1e3 / ms_per_step, 3),
@@ -723,12 +756,19 @@ def main():
         "kernel_ms_per_step": kernels,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
-                     "traffic": traffic, "kernel": "sgm::path_kernel",
+                     "traffic": traffic,
+                     "kernel": "path sweeps of one pair (sgm::path_kernel first and last, "
+                               "sgm::path_multi_kernel for the middle directions)",
+                     "sweeps": ["auto", "sequential", "paired", "planes"][schedule],
                      "launches_per_step": path_launches,
                      "launch_ms": round(launch_s * 1e3, 4),
                      "algorithmic_bytes_per_launch": alg_launch,
                      "algorithmic_model": "6P-2 B per cell (uint16 C and S, selection fused "
-                                          "into the last sweep), 2*W*H*D cells per pair",
+                                          "into the last sweep), 2*W*H*D cells per pair; the "
+                                          "plane schedule moves 22 B per cell, so the model's "
+                                          "rate can pass the HBM peak (build_frac is the "
+                                          "fraction of peak used)",
+                     "path_kernels": path_kernels,
                      "build_bytes_per_launch": build_launch,
                      "build_achieved": round(build_launch / launch_s / 1e9, 1),
                      "build_frac": round(build_launch / launch_s / 1e9 / PEAK_HBM_GBPS, 4),

@@ -115,7 +115,7 @@ typedef struct sgm_plan {
   int32_t block_rows;           /* rows finished per recomputation block (h unless recompute) */
   int64_t disk_bytes;           /* temporary file of the disk storage */
   int32_t compact;              /* 1 if a pair runs the compact plan under the plan mode */
-  int32_t paired;               /* 1 if a resident pair runs paired sweeps (sgm_set_paired_sweeps) */
+  int32_t sweeps;               /* SGM_SWEEPS_* schedule a resident pair runs (sgm_set_sweeps) */
 } sgm_plan;
 
 typedef struct sgm_context sgm_context;
@@ -338,15 +338,29 @@ int sgm_esgm_counts(sgm_context *ctx, int64_t counts[4]);
 #define SGM_VOLUME_RECOMPUTE 2
 int sgm_set_cost_volume(sgm_context *ctx, int policy);
 
-/* Paired path sweeps (default on; no counterpart in the reference, whose
- * `paths` kernel runs one direction per launch, GpuStereo.java:732-757).  When
- * a resident pair with 16-bit aggregates fits the budget with one more
- * aggregate volume per matching direction, the middle directions run two per
- * launch, each pair member accumulating into its own partial volume, and the
- * last sweep adds both before the winner search.  The partial sums add up to
- * the sequential aggregate modulo 2^16, so results are identical; off, or when
- * the extra volume does not fit, each direction runs in its own launch. */
-int sgm_set_paired_sweeps(sgm_context *ctx, int enable);
+/* Sweep schedule of the resident plan (no counterpart in the reference, whose
+ * `paths` kernel runs one direction per launch into one aggregate volume,
+ * GpuStereo.java:732-757).  Results are identical under every schedule.
+ *   SEQUENTIAL: one direction per launch, each adding into the aggregate.
+ *   PAIRED: the middle directions two per launch, each pair member adding into
+ *     its own partial aggregate volume; the last sweep adds both (uint16 sums
+ *     wrap modulo 2^16 in any order).  One more aggregate volume per matching
+ *     direction.
+ *   PLANES: every direction but the last stores its path costs as bytes into a
+ *     plane of its own (the middle directions all in one launch, as they only
+ *     read the cost volume); the last sweep adds the 7 planes to its own path
+ *     costs before the winner search.  A valid cell's path cost is at most its
+ *     cost plus P2, so this needs window bits - 1 + P2 <= 255, uint8 costs, 8
+ *     paths and no kept volumes; 7 bytes per cell in place of the 2-byte
+ *     aggregate, and 22 instead of 36 bytes of HBM traffic per cell.
+ *   AUTO (default): PLANES where it applies and fits the budget, else PAIRED
+ *     where that fits, else SEQUENTIAL.  A requested schedule that does not
+ *     apply falls back the same way (PLANES -> PAIRED -> SEQUENTIAL). */
+#define SGM_SWEEPS_AUTO 0
+#define SGM_SWEEPS_SEQUENTIAL 1
+#define SGM_SWEEPS_PAIRED 2
+#define SGM_SWEEPS_PLANES 3
+int sgm_set_sweeps(sgm_context *ctx, int schedule);
 
 int sgm_synchronize(sgm_context *ctx);
 /* Free the per-pair buffers the plain entry points cache in the context. */

@@ -502,8 +502,10 @@ extern "C" int sgm_plan_get(sgm_context *ctx, const sgm_params *params, int w, i
                   out->streamed && sp.storage == SGM_AGGREGATE_RECOMPUTE && pool > 0 &&
                   !esgm_unsupported(params, w, h));
   if (out->compact && ctx->plan_mode == SGM_PLAN_AUTO) out->streamed = 0;
-  out->paired = !out->compact && !out->streamed && out->resident &&
-                paired_sweeps(ctx, params, w, h, !recompute_cost(ctx, params, w, h), ctx->budget);
+  out->sweeps = !out->compact && !out->streamed && out->resident
+                    ? sweep_schedule(ctx, params, w, h, !recompute_cost(ctx, params, w, h),
+                                     ctx->budget)
+                    : SGM_SWEEPS_SEQUENTIAL;
   return SGM_OK;
 }
 
@@ -598,10 +600,12 @@ extern "C" int sgm_set_cost_volume(sgm_context *ctx, int policy) {
   return SGM_OK;
 }
 
-extern "C" int sgm_set_paired_sweeps(sgm_context *ctx, int enable) {
+extern "C" int sgm_set_sweeps(sgm_context *ctx, int schedule) {
   if (!ctx) return fail(SGM_ERR_INVALID_ARGUMENT, "null context");
+  if (schedule < SGM_SWEEPS_AUTO || schedule > SGM_SWEEPS_PLANES)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "sweep schedule %d not in 0..3", schedule);
   std::lock_guard<std::mutex> g(ctx->lock);
-  ctx->paired_sweeps = enable != 0;
+  ctx->sweeps = schedule;
   return SGM_OK;
 }
 
@@ -685,14 +689,16 @@ extern "C" int sgm_profile_json(sgm_context *ctx, char *buffer, size_t length) {
   }
   char b[256];
   static const char *storages[4] = {"auto", "ram", "disk", "recompute"};
+  static const char *sweep_names[4] = {"auto", "sequential", "paired", "planes"};
   snprintf(b, sizeof b,
            "},\"allocated_bytes\":%lld,\"budget_bytes\":%lld,\"last_plan\":{\"plan\":\"%s\","
-           "\"storage\":\"%s\",\"block_rows\":%d,\"paired\":%s}}",
+           "\"storage\":\"%s\",\"block_rows\":%d,\"sweeps\":\"%s\"}}",
            (long long)ctx->allocated, (long long)ctx->budget,
            ctx->last_compact ? "compact" : ctx->last_streamed ? "streamed" : "resident",
            ctx->last_streamed ? storages[ctx->splan.storage & 3] : "none",
            ctx->last_streamed ? ctx->splan.block_rows : 0,
-           !ctx->last_compact && !ctx->last_streamed && ctx->last_paired ? "true" : "false");
+           ctx->last_compact || ctx->last_streamed ? "sequential"
+                                                   : sweep_names[ctx->last_sweeps & 3]);
   s += b;
   if (s.size() + 1 > length) return fail(SGM_ERR_INVALID_ARGUMENT, "buffer too small (%zu)", s.size() + 1);
   memcpy(buffer, s.c_str(), s.size() + 1);
@@ -806,7 +812,9 @@ int lines_of(int cls, int w, int h) { return cls == 0 ? h : cls == 1 ? w : w + h
 
 int launch_path(sgm_context *ctx, int R, int sum_bytes, int mode, const PathJobs &jobs,
                 const PathArgs &A, int phase, int waves, hipStream_t stream) {
-  ProfileScope ps(ctx, "path");
+  ProfileScope ps(ctx, phase == PHASE_FIRST ? "path_first"
+                       : phase == PHASE_MID ? "path_mid"
+                                            : "path_last");
   if (A.total_lines <= 0) return SGM_OK;
   hipStream_t st = stream ? stream : ctx->stream;
   if (phase == PHASE_FIRST) return launch_path_first(st, R, sum_bytes, mode, jobs, A, waves);
@@ -874,6 +882,10 @@ struct PairBuffers {
   // paired sweeps: the second partial aggregate volume of each matching
   // direction (null: one aggregate, directions swept one per launch)
   void *sum_b[2];
+  // path-cost plane schedule: kPlanes planes of Dp bytes per pixel (interleaved
+  // per pixel) for each matching direction in place of the aggregate (null: an
+  // aggregate volume)
+  uint8_t *planes[2];
   float *disp[2], *checked;
   uint8_t *reasons, *boundary, *reasons_out, *boundary_out, *code;
   unsigned long long *stats;
@@ -903,33 +915,54 @@ struct PairBuffers {
 
 }  // namespace
 
-// Bytes of the second partial aggregate volumes of paired sweeps (both
-// matching directions).
-int64_t sgm::paired_bytes(const sgm_params *p, int w, int h) {
-  return 2 * (int64_t)w * h * padded_disparities(p->disparities) * aggregate_bytes(p);
+// Device bytes a sweep schedule adds to pair_bytes(p, w, h, true): the second
+// partial aggregate volumes of paired sweeps, or the path-cost planes of both
+// matching directions (kPlanes bytes per cell) in place of the aggregates.
+int64_t sgm::schedule_bytes(const sgm_params *p, int w, int h, int schedule) {
+  const int64_t cells = 2 * (int64_t)w * h * padded_disparities(p->disparities);
+  if (schedule == SGM_SWEEPS_PAIRED) return cells * aggregate_bytes(p);
+  if (schedule == SGM_SWEEPS_PLANES) return cells * (kPlanes - aggregate_bytes(p));
+  return 0;
 }
 
-// Paired sweeps (two directions per launch over two partial aggregates): packed
-// 16-bit sweeps of lane groups 2..8 reading a cost volume, when the context
-// allows them and the pair with the second aggregate volumes fits `budget`.
-bool sgm::paired_sweeps(const sgm_context *ctx, const sgm_params *p, int w, int h,
+// Sweep schedule of a resident pair under `budget` (SGM_SWEEPS_SEQUENTIAL,
+// _PAIRED or _PLANES).  Both multi-direction schedules need packed 16-bit
+// sweeps of lane groups 2..8 over a cost volume.  Path-cost planes also need
+// 8 paths, uint8 costs, no kept volumes (the planes replace the aggregate), and
+// path costs that fit a byte: every valid cell's path cost is at most its cost
+// (<= window bits - 1 for the uint8 costs) plus P2.  AUTO and PLANES take the
+// planes, then the pairs, where they apply and fit the budget.
+int sgm::sweep_schedule(const sgm_context *ctx, const sgm_params *p, int w, int h,
                         bool cost_volume, int64_t budget) {
-  if (!ctx->paired_sweeps || !cost_volume || aggregate_bytes(p) != 2 || path_lanes(p) > 8)
-    return false;
-  return pair_bytes(p, w, h, true) + paired_bytes(p, w, h) <= budget;
+  const int want = ctx->sweeps;
+  if (want == SGM_SWEEPS_SEQUENTIAL || !cost_volume || aggregate_bytes(p) != 2 ||
+      path_lanes(p) > 8)
+    return SGM_SWEEPS_SEQUENTIAL;
+  const int64_t base = pair_bytes(p, w, h, true);
+  if (want == SGM_SWEEPS_AUTO || want == SGM_SWEEPS_PLANES) {
+    const bool planes = !ctx->keep_volumes && cost_u8(p) && p->directions == kPlanes + 1 &&
+                        p->window_w * p->window_h - 1 + effective_p2(p) <= 255 &&
+                        base + schedule_bytes(p, w, h, SGM_SWEEPS_PLANES) <= budget;
+    if (planes) return SGM_SWEEPS_PLANES;
+  }
+  // PAIRED, or the fallback of AUTO and PLANES
+  if (base + schedule_bytes(p, w, h, SGM_SWEEPS_PAIRED) <= budget)
+    return SGM_SWEEPS_PAIRED;
+  return SGM_SWEEPS_SEQUENTIAL;
 }
 
 namespace {
 
 // Whole-image device buffers of a pair, by name and size: inputs, descriptors
 // and disparities of both matching directions, the outputs, and (`volumes`)
-// the aggregate volumes with (`cost_volume`) the cost volumes and (`paired`)
-// the second partial aggregates of paired sweeps.  Their sum is
-// pair_bytes(p, w, h, cost_volume) (+ paired_bytes) when `volumes` is set.
+// the aggregate volumes (path-cost planes under SGM_SWEEPS_PLANES, a second
+// partial aggregate under SGM_SWEEPS_PAIRED) with (`cost_volume`) the cost
+// volumes.  Their sum is pair_bytes(p, w, h, cost_volume) (+ schedule_bytes)
+// when `volumes` is set.
 static std::vector<std::pair<std::string, size_t>> pair_buffer_list(const sgm_params *p, int w,
                                                                     int h, bool volumes,
                                                                     bool cost_volume,
-                                                                    bool paired) {
+                                                                    int schedule) {
   const size_t n = (size_t)w * h, Dp = padded_disparities(p->disparities);
   const int sb = aggregate_bytes(p), words = words_of(p);
   const bool zncc = p->cost == SGM_COST_ZNCC || p->cost == SGM_COST_ZNCC_FP32;
@@ -946,8 +979,9 @@ static std::vector<std::pair<std::string, size_t>> pair_buffer_list(const sgm_pa
       L.emplace_back("wsum2" + s, n * 4);
     }
     if (volumes && cost_volume) L.emplace_back("cost" + s, n * Dp * (cost_u8(p) ? 1 : 2));
-    if (volumes) L.emplace_back("sum" + s, n * Dp * sb);
-    if (volumes && paired) L.emplace_back("sumb" + s, n * Dp * sb);
+    if (volumes && schedule == SGM_SWEEPS_PLANES) L.emplace_back("planes" + s, n * Dp * kPlanes);
+    else if (volumes) L.emplace_back("sum" + s, n * Dp * sb);
+    if (volumes && schedule == SGM_SWEEPS_PAIRED) L.emplace_back("sumb" + s, n * Dp * sb);
     L.emplace_back("disp" + s, n * 4);
   }
   L.emplace_back("checked", n * 4);
@@ -968,9 +1002,9 @@ static std::vector<std::pair<std::string, size_t>> pair_buffer_list(const sgm_pa
 // it is (`fresh`: no cached buffers).
 int64_t sgm::resident_footprint(sgm_context *ctx, const sgm_params *p, int w, int h,
                                 bool cost_volume, int64_t budget, bool fresh) {
-  const bool paired = paired_sweeps(ctx, p, w, h, cost_volume, budget);
+  const int schedule = sweep_schedule(ctx, p, w, h, cost_volume, budget);
   int64_t total = fresh ? 0 : ctx->allocated;
-  for (const auto &b : pair_buffer_list(p, w, h, true, cost_volume, paired)) {
+  for (const auto &b : pair_buffer_list(p, w, h, true, cost_volume, schedule)) {
     if (fresh) {
       total += (int64_t)b.second;
       continue;
@@ -992,17 +1026,23 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
   const int sb = aggregate_bytes(p);
   const bool zncc = p->cost == SGM_COST_ZNCC || p->cost == SGM_COST_ZNCC_FP32;
   const bool recompute = !rows && recompute_cost(ctx, p, w, h);
-  const bool paired = !rows && paired_sweeps(ctx, p, w, h, !recompute, ctx->budget);
+  const int schedule =
+      rows ? SGM_SWEEPS_SEQUENTIAL : sweep_schedule(ctx, p, w, h, !recompute, ctx->budget);
   int rc;
 #define ENSURE(name, bytes)                                                   \
   if ((rc = ctx->buffers[name].ensure(ctx, (bytes), name)) != SGM_OK) return rc;
+  // volumes the plan and schedule do not use are released before allocating
   if (rows) {
     // the streamed and compact plans hold no whole-image volume
     for (const char *v : {"cost0", "cost1", "sum0", "sum1"}) ctx->buffers[v].release(ctx);
   }
-  if (!paired)
+  if (schedule != SGM_SWEEPS_PAIRED)
     for (const char *v : {"sumb0", "sumb1"}) ctx->buffers[v].release(ctx);
-  for (const auto &b : pair_buffer_list(p, w, h, rows == 0, !recompute, paired))
+  if (schedule != SGM_SWEEPS_PLANES)
+    for (const char *v : {"planes0", "planes1"}) ctx->buffers[v].release(ctx);
+  else
+    for (const char *v : {"sum0", "sum1"}) ctx->buffers[v].release(ctx);
+  for (const auto &b : pair_buffer_list(p, w, h, rows == 0, !recompute, schedule))
     ENSURE(b.first.c_str(), b.second);
   for (int d = 0; d < 2; d++) {
     const std::string s = std::to_string(d);
@@ -1015,8 +1055,10 @@ int ensure_pair_buffers(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     B.wsum2[d] = zncc ? (int32_t *)ctx->buffers["wsum2" + s].ptr : nullptr;
     B.cost[d] = rows || recompute ? nullptr : (uint16_t *)ctx->buffers["cost" + s].ptr;
     B.recompute = recompute;
-    B.sum[d] = rows ? nullptr : ctx->buffers["sum" + s].ptr;
-    B.sum_b[d] = paired ? ctx->buffers["sumb" + s].ptr : nullptr;
+    B.sum[d] = rows || schedule == SGM_SWEEPS_PLANES ? nullptr : ctx->buffers["sum" + s].ptr;
+    B.sum_b[d] = schedule == SGM_SWEEPS_PAIRED ? ctx->buffers["sumb" + s].ptr : nullptr;
+    B.planes[d] =
+        schedule == SGM_SWEEPS_PLANES ? (uint8_t *)ctx->buffers["planes" + s].ptr : nullptr;
     B.disp[d] = (float *)ctx->buffers["disp" + s].ptr;
   }
   B.bgr[0] = B.bgr[1] = nullptr;
@@ -1772,11 +1814,43 @@ int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuff
     (void)hipStreamSynchronize(ctx->stream);
     return true;
   };
+  if (B.planes[slots[0]] && (mode & 7) == 4) {
+    // Path-cost planes: the first direction (fused census cost or the uint8
+    // cost volume) stores its path costs into plane 0, the middle directions
+    // run in one launch into planes 1..6, the last sweep adds the seven planes
+    // to its own path costs before the winner search.
+    ctx->last_sweeps = SGM_SWEEPS_PLANES;
+    PathJobs jf = pj;
+    for (int i = 0; i < nd; i++) jf.local[i].sum = B.planes[slots[i]];
+    if (interrupted()) return fail(SGM_ERR_INTERRUPTED, "GPU SGM interrupted");
+    const int mf = (fused ? (2 | (mode & MODE_NT)) : mode) | MODE_PLANE;
+    if ((rc = launch_path(ctx, R, sb, mf, jf, dir_args(0), PHASE_FIRST, waves))) return rc;
+    MultiDir M;
+    memset(&M, 0, sizeof M);
+    M.ndirs = p->directions - 2;
+    for (int k = 0; k < M.ndirs; k++) {
+      const PathArgs Ak = dir_args(1 + k);
+      M.dx[k] = Ak.dx;
+      M.dy[k] = Ak.dy;
+      M.cls[k] = Ak.cls;
+      M.split[k] = Ak.split;
+      M.begin[k] = M.total_lines;
+      M.total_lines += Ak.total_lines;
+      for (int i = 0; i < nd; i++) M.plane[k][i] = B.planes[slots[i]] + (size_t)(1 + k) * Dp;
+    }
+    if (interrupted()) return fail(SGM_ERR_INTERRUPTED, "GPU SGM interrupted");
+    if ((rc = launch_path_multi(ctx, R, mode | MODE_PLANE, pj, dir_args(1), M, waves))) return rc;
+    PathJobs jl = pj;
+    for (int i = 0; i < nd; i++) jl.local[i].sum = B.planes[slots[i]];
+    if (interrupted()) return fail(SGM_ERR_INTERRUPTED, "GPU SGM interrupted");
+    return launch_path(ctx, R, sb, mode | MODE_PLANE, jl, dir_args(p->directions - 1),
+                       PHASE_LAST, waves);
+  }
   // Paired sweeps: after the first direction, the middle directions run two
   // per launch, the first of each pair adding into the aggregate volume and
   // the second into a second partial volume; the last sweep adds both.
   const bool paired = B.sum_b[slots[0]] != nullptr && (mode & 7) != 3;
-  ctx->last_paired = paired;
+  ctx->last_sweeps = paired ? SGM_SWEEPS_PAIRED : SGM_SWEEPS_SEQUENTIAL;
   PathJobs pjb = pj;
   if (paired)
     for (int i = 0; i < nd; i++) pjb.local[i].sum = B.sum_b[slots[i]];

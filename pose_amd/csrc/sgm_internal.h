@@ -65,8 +65,8 @@ int64_t pair_bytes(const sgm_params *p, int w, int h);
 int64_t pair_bytes(const sgm_params *p, int w, int h, bool cost_volume);
 int64_t resident_footprint(sgm_context *ctx, const sgm_params *p, int w, int h, bool cost_volume,
                            int64_t budget, bool fresh);
-int64_t paired_bytes(const sgm_params *p, int w, int h);
-bool paired_sweeps(const sgm_context *ctx, const sgm_params *p, int w, int h, bool cost_volume,
+int64_t schedule_bytes(const sgm_params *p, int w, int h, int schedule);
+int sweep_schedule(const sgm_context *ctx, const sgm_params *p, int w, int h, bool cost_volume,
                    int64_t budget);
 int lines_of(int cls, int w, int h);
 void collect_profile(sgm_context *ctx);
@@ -107,6 +107,9 @@ int launch_check(const char *what);
 int launch_path_pair(sgm_context *ctx, int R, int mode, int phase_b, const PathJobs &ja,
                      const PathArgs &A, const PathJobs &jb, const PathArgs &B, int waves,
                      hipStream_t stream = nullptr);
+// the directions of `M` in one launch, each storing its path-cost plane
+int launch_path_multi(sgm_context *ctx, int R, int mode, const PathJobs &jobs, const PathArgs &A,
+                      const MultiDir &M, int waves, hipStream_t stream = nullptr);
 // one sweep phase's kernel instances each (sgm_path_first/mid/last.hip)
 int launch_path_first(hipStream_t st, int R, int sum_bytes, int mode, const PathJobs &jobs,
                       const PathArgs &A, int waves);
@@ -151,8 +154,8 @@ struct sgm_context {
   // sgm_interrupt: set by any thread, consumed by the matching call that sees it
   std::atomic<int> interrupt{0};
   bool last_compact = false;
-  bool paired_sweeps = true;  // sgm_set_paired_sweeps
-  bool last_paired = false;
+  int sweeps = SGM_SWEEPS_AUTO;  // sgm_set_sweeps
+  int last_sweeps = SGM_SWEEPS_SEQUENTIAL;
   sgm::StreamedPlan splan;  // streamed plan picked for the current call
   std::vector<sgm::PendingEvent> pending_events;
   std::map<std::string, std::pair<double, long long>> kernel_stats;

@@ -35,6 +35,33 @@ static int launch_path_phase(hipStream_t st, int R, int sum_bytes, int mode, con
     else if (mode == (1 | MODE_NT)) launch_path_t<RR, uint16_t, 1 | MODE_NT, PH>(st, jobs, A, waves); \
     else launch_path_t<RR, uint16_t, 1, PH>(st, jobs, A, waves);        \
     break;
+  if (mode & MODE_PLANE) {
+    // path-cost planes (packed lane groups 2, 4, 8): first-phase sweeps store
+    // their plane (fused census cost or uint8 cost volume), last sweeps add them
+    const int m = mode & ~MODE_PLANE;
+    if (sum_bytes != 2) return fail(SGM_ERR_INVALID_ARGUMENT, "path-cost planes need 16-bit sums");
+    if (R != 2 && R != 4 && R != 8) return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
+    if constexpr (PH == PHASE_MID) {
+      return fail(SGM_ERR_INVALID_ARGUMENT, "no middle sweep over path-cost planes");
+    } else {
+      constexpr int P = MODE_PLANE;
+#define PLANE_R(MM)                                                                   \
+  (R == 2 ? (launch_path_t<2, uint16_t, MM, PH>(st, jobs, A, waves), 0)               \
+   : R == 4 ? (launch_path_t<4, uint16_t, MM, PH>(st, jobs, A, waves), 0)             \
+            : (launch_path_t<8, uint16_t, MM, PH>(st, jobs, A, waves), 0))
+      if (m == 4) (void)PLANE_R(4 | P);
+      else if (m == (4 | MODE_NT)) (void)PLANE_R(4 | MODE_NT | P);
+      else if constexpr (PH == PHASE_FIRST) {
+        if (m == 2) (void)PLANE_R(2 | P);
+        else if (m == (2 | MODE_NT)) (void)PLANE_R(2 | MODE_NT | P);
+        else return fail(SGM_ERR_INVALID_ARGUMENT, "no path-cost plane sweep for mode %d", m);
+      } else {
+        return fail(SGM_ERR_INVALID_ARGUMENT, "no path-cost plane sweep for mode %d", m);
+      }
+#undef PLANE_R
+      return launch_check("path kernel");
+    }
+  }
   if (mode & MODE_SUM2) {
     // last sweep over paired partial aggregates (packed lane groups 2, 4, 8)
     if constexpr (PH == PHASE_LAST) {

@@ -820,8 +820,16 @@ __device__ __forceinline__ void census_costs(const uint64_t *tw, uint32_t alo, u
 // line, fixed at compile time (1: towards higher x indices of the window, 0:
 // lower) so the window rotates by register renaming in the unrolled steps; -1
 // where the line has no sliding window.
+// PLANE: path-cost planes instead of a running aggregate.  The planes of a
+// matching direction are interleaved per pixel: pixel p holds kPlanes rows of
+// Dp bytes at (p * kPlanes + k) * Dp, so the last sweep reads one contiguous
+// block per pixel.  A first-phase sweep stores its path costs L as bytes into
+// its own plane (`sum` points at plane k of pixel 0); the last sweep adds the
+// kPlanes planes of the other directions to its own L (`sum` at plane 0).
+constexpr int kPlanes = 7;
+
 template <int R, int PHASE, int CM, int PFCAP = 64, int SLIDE_UP = -1, bool NT = false,
-          bool SUM2 = false>
+          bool SUM2 = false, bool PLANE = false>
 struct PkSweep {
   static constexpr int NP = R / 2;
   static constexpr int POL = NT ? POL_NT : POL_DEFAULT;
@@ -830,6 +838,10 @@ struct PkSweep {
   static constexpr bool kLoaded = CM == CM_LOAD || CM == CM_LOAD8;
   static constexpr int NC = CM == CM_LOAD8 ? (R + 3) / 4 : NP;  // dwords of loaded cost per slot
   static constexpr int kDirect = CM == CM_RECOMPUTE ? PF : 1;  // direct target-row prefetch slots
+  static constexpr bool kPlaneIn = PLANE && PHASE == PHASE_LAST;    // reads the planes
+  static constexpr bool kPlaneOut = PLANE && PHASE == PHASE_FIRST;  // writes its plane
+  static constexpr int NB = (R + 3) / 4;                             // dwords of R bytes
+  static_assert(!PLANE || PHASE != PHASE_MID, "plane sweeps are first-phase or last");
 
   const PathJob &J;
   const PathArgs &A;
@@ -843,8 +855,10 @@ struct PkSweep {
   ptrdiff_t cstep, sstep;      // bytes per pixel step in the cost / aggregate volume
   uint32_t keep[NP], infm[NP];
   uint32_t vmask, p1pk;
-  uint32_t cb[kLoaded ? PF : 1][NC], sb[PF][NP], sb2[SUM2 ? PF : 1][NP];
+  uint32_t cb[kLoaded ? PF : 1][NC], sb[kPlaneIn ? 1 : PF][NP], sb2[SUM2 ? PF : 1][NP];
+  uint32_t pl[kPlaneIn ? PF : 1][kPlaneIn ? kPlanes : 1][NB];
   const uint8_t *ld_s2;  // second partial aggregate (SUM2): rows of the next prefetched step
+  int poff;              // plane reads: lane offset, dropped for lanes beyond D
   uint64_t win[kComputed ? R : 1];
   uint64_t dsc[kDirect][CM == CM_RECOMPUTE ? R : 1];
   uint32_t prev[NP];
@@ -873,14 +887,19 @@ struct PkSweep {
     {
       const ptrdiff_t cb_ = (CM == CM_LOAD8 || CM == CM_FUSED) ? 1 : 2;
       cstep = step_off * J.Dp * cb_;
-      sstep = step_off * J.Dp * 2;
       ld_c = (const uint8_t *)J.cost + p0 * J.Dp * cb_;
       st_c = (uint8_t *)J.cost_out + p0 * J.Dp * cb_;
-      ld_s = (const uint8_t *)J.sum + p0 * J.Dp * 2;
-      st_s = (uint8_t *)J.sum + p0 * J.Dp * 2;
+      const ptrdiff_t sb_ = PLANE ? kPlanes : 2;  // aggregate / plane-block bytes per cell
+      sstep = step_off * J.Dp * sb_;
+      ld_s = (const uint8_t *)J.sum + p0 * J.Dp * sb_;
+      st_s = (uint8_t *)J.sum + p0 * J.Dp * sb_;
       ld_s2 = SUM2 ? (const uint8_t *)J.sum2 + p0 * J.Dp * 2 : nullptr;
     }
-    sum_store_bytes = (PHASE != PHASE_LAST || A.keep_sum) ? (int)row_bytes : 0;
+    // beyond the range of every plane read (offsets stay below 2^31)
+    poff = kbase < J.D ? kbase : 0x40000000;
+    sum_store_bytes = kPlaneOut ? J.Dp
+                      : kPlaneIn ? 0
+                                 : (PHASE != PHASE_LAST || A.keep_sum) ? (int)row_bytes : 0;
     slide = CM == CM_FUSED || (CM == CM_RECOMPUTE && A.dy == 0);
 #pragma unroll
     for (int i = 0; i < NP; i++) {
@@ -890,7 +909,7 @@ struct PkSweep {
       prev[i] = kInfPk;
     }
 #pragma unroll
-    for (int j = 0; j < PF; j++)
+    for (int j = 0; j < (kPlaneIn ? 1 : PF); j++)
 #pragma unroll
       for (int i = 0; i < NP; i++) sb[j][i] = 0;
     vmask = A.val_mask == 0xFFFF ? 0xFFFFFFFFu : splat16((uint32_t)A.val_mask);
@@ -947,8 +966,13 @@ struct PkSweep {
       bload_bytes<R, POL>(cb[slot], make_rsrc(ld_c, in ? cost_row_bytes : 0u), boff);
     if constexpr (CM == CM_RECOMPUTE)
       if (!slide) load_targets(dsc[slot], s);
-    if constexpr (PHASE != PHASE_FIRST)
+    if constexpr (kPlaneIn) {
+      const __amdgpu_buffer_rsrc_t r = make_rsrc(ld_s, in ? (uint32_t)(kPlanes * J.Dp) : 0u);
+#pragma unroll
+      for (int k = 0; k < kPlanes; k++) bload_bytes<R, POL>(pl[slot][k], r, poff + k * J.Dp);
+    } else if constexpr (PHASE != PHASE_FIRST) {
       bload_pairs<NP, POL>(sb[slot], make_rsrc(ld_s, in ? row_bytes : 0u), loff);
+    }
     if constexpr (SUM2) {
       bload_pairs<NP, POL>(sb2[slot], make_rsrc(ld_s2, in ? row_bytes : 0u), loff);
       ld_s2 += sstep;
@@ -1052,10 +1076,29 @@ struct PkSweep {
     uint32_t S[NP];
 #pragma unroll
     for (int i = 0; i < NP; i++) {
-      if constexpr (SUM2) S[i] = pk_add(pk_add(sb[slot][i], sb2[slot][i]), L[i]);
-      else S[i] = PHASE == PHASE_FIRST ? L[i] : pk_add(sb[slot][i], L[i]);
+      if constexpr (SUM2) {
+        S[i] = pk_add(pk_add(sb[slot][i], sb2[slot][i]), L[i]);
+      } else if constexpr (kPlaneIn) {
+        // the other directions' path costs: bytes 2i, 2i+1 of each plane
+        uint32_t t = L[i];
+#pragma unroll
+        for (int k = 0; k < kPlanes; k++)
+          t = pk_add(t, __builtin_amdgcn_perm(0u, pl[slot][k][i >> 1],
+                                              (i & 1) ? 0x0C030C02u : 0x0C010C00u));
+        S[i] = t;
+      } else {
+        S[i] = PHASE == PHASE_FIRST ? L[i] : pk_add(sb[slot][i], L[i]);
+      }
     }
-    bstore_pairs<NP, POL>(S, make_rsrc(st_s, (uint32_t)sum_store_bytes), loff);
+    if constexpr (kPlaneOut) {
+      // path costs of valid cells are <= max cost + P2 <= 255 (checked by the
+      // host); other cells saturate at 255 and are never selected
+      uint32_t raw[NB];
+      narrow_costs<NP>(L, raw);
+      bstore_bytes<R, POL>(raw, make_rsrc(st_s, (uint32_t)sum_store_bytes), boff);
+    } else if constexpr (!kPlaneIn) {
+      bstore_pairs<NP, POL>(S, make_rsrc(st_s, (uint32_t)sum_store_bytes), loff);
+    }
     st_s += sstep;
     if constexpr (PHASE == PHASE_LAST) {
       // the winner key reduces beside the path minimum, interleaved
@@ -1210,22 +1253,23 @@ struct PkSweep {
   }
 };
 
-template <int R, int PHASE, int CM, int PFCAP = 64, bool NT = false, bool SUM2 = false>
+template <int R, int PHASE, int CM, int PFCAP = 64, bool NT = false, bool SUM2 = false,
+          bool PLANE = false>
 __device__ __forceinline__ void sweep_line_pk(const PathJob &J, const PathArgs &A, int line,
                                               const int *p2_table) {
   static_assert(R % 2 == 0, "packed sweep needs an even lane group");
   // the fused first sweep runs a copy of its loop per slide direction
   if constexpr (CM == CM_FUSED) {
     if (!J.reverse) {
-      PkSweep<R, PHASE, CM, PFCAP, 1, NT> sw(J, A, line, p2_table);
+      PkSweep<R, PHASE, CM, PFCAP, 1, NT, false, PLANE> sw(J, A, line, p2_table);
       sw.run();
     } else {
-      PkSweep<R, PHASE, CM, PFCAP, 0, NT> sw(J, A, line, p2_table);
+      PkSweep<R, PHASE, CM, PFCAP, 0, NT, false, PLANE> sw(J, A, line, p2_table);
       sw.run();
     }
     return;
   }
-  PkSweep<R, PHASE, CM, PFCAP, -1, NT, SUM2> sw(J, A, line, p2_table);
+  PkSweep<R, PHASE, CM, PFCAP, -1, NT, SUM2, PLANE> sw(J, A, line, p2_table);
   sw.run();
 }
 
@@ -1265,8 +1309,11 @@ struct PathJobs {
 // volume); 4 packed u16 reading the uint8 cost volume.  MODE_NT (added to 1, 2
 // or 4): the volume accesses stream (nontemporal policy).  MODE_SUM2 (added
 // to 1 or 4 of a last sweep): the job's second partial aggregate is added in.
+// MODE_PLANE (added to 2 or 4 of a first-phase sweep, or to 4 of a last
+// sweep): path-cost planes instead of the running aggregate.
 constexpr int MODE_NT = 8;
 constexpr int MODE_SUM2 = 16;
+constexpr int MODE_PLANE = 32;
 
 template <int R, typename SumT, int PHASE, int MODE>
 __device__ __forceinline__ void sweep_dispatch(const PathJob &J, const PathArgs &A, int line,
@@ -1274,13 +1321,70 @@ __device__ __forceinline__ void sweep_dispatch(const PathJob &J, const PathArgs 
   constexpr int M = MODE & 7;
   constexpr bool NT = (MODE & MODE_NT) != 0;
   constexpr bool SUM2 = (MODE & MODE_SUM2) != 0;
+  constexpr bool PLANE = (MODE & MODE_PLANE) != 0;
   static_assert(!SUM2 || ((M == 1 || M == 4) && PHASE == PHASE_LAST),
                 "a second partial aggregate is read by last sweeps over a cost volume");
+  static_assert(!PLANE || (M == 2 && PHASE == PHASE_FIRST) ||
+                    (M == 4 && PHASE != PHASE_MID),
+                "path-cost planes: first-phase sweeps (fused or uint8 costs) and last sweeps");
   if constexpr (M == 0) sweep_line<R, SumT, PHASE>(J, A, line, p2_table);
   else if constexpr (M == 1) sweep_line_pk<R, PHASE, CM_LOAD, 64, NT, SUM2>(J, A, line, p2_table);
-  else if constexpr (M == 2) sweep_line_pk<R, PHASE, CM_FUSED, 64, NT>(J, A, line, p2_table);
+  else if constexpr (M == 2)
+    sweep_line_pk<R, PHASE, CM_FUSED, 64, NT, false, PLANE>(J, A, line, p2_table);
   else if constexpr (M == 3) sweep_line_pk<R, PHASE, CM_RECOMPUTE>(J, A, line, p2_table);
-  else sweep_line_pk<R, PHASE, CM_LOAD8, 64, NT, SUM2>(J, A, line, p2_table);
+  else sweep_line_pk<R, PHASE, CM_LOAD8, 64, NT, SUM2, PLANE>(J, A, line, p2_table);
+}
+
+// Up to kMultiDirs path directions in one launch, each a first-phase sweep
+// storing its path costs into its own plane (PLANE modes).  Lines
+// [begin[i], begin[i + 1]) sweep direction i; the first split[i] of them are
+// matching direction 0's.  The directions are independent (each reads only
+// the cost volume), so one launch carries all their lines.
+constexpr int kMultiDirs = 6;
+struct MultiDir {
+  int ndirs, total_lines;
+  int dx[kMultiDirs], dy[kMultiDirs], cls[kMultiDirs], split[kMultiDirs];
+  int begin[kMultiDirs];
+  void *plane[kMultiDirs][2];
+};
+
+template <int R, int MODE>
+__global__ __launch_bounds__(256) void path_multi_kernel(PathJobs jobs, PathArgs A, MultiDir M) {
+  __shared__ int p2_table[256];
+  build_p2_table(p2_table, A);  // P1, P2 and the edge scale are shared by every direction
+  __syncthreads();
+  const int waves = blockDim.x >> 6;
+  const int gline = __builtin_amdgcn_readfirstlane(blockIdx.x * waves + (int)(threadIdx.x >> 6));
+  if (gline >= M.total_lines) return;
+  // direction of this line: fields picked with constant indices (uniform)
+  PathArgs B = A;
+  B.dx = M.dx[0];
+  B.dy = M.dy[0];
+  B.cls = M.cls[0];
+  int begin = M.begin[0], split = M.split[0];
+  void *pl0 = M.plane[0][0], *pl1 = M.plane[0][1];
+#pragma unroll
+  for (int i = 1; i < kMultiDirs; i++) {
+    if (i < M.ndirs && gline >= M.begin[i]) {
+      B.dx = M.dx[i];
+      B.dy = M.dy[i];
+      B.cls = M.cls[i];
+      begin = M.begin[i];
+      split = M.split[i];
+      pl0 = M.plane[i][0];
+      pl1 = M.plane[i][1];
+    }
+  }
+  const int local = gline - begin;
+  if (A.njobs > 1 && local >= split) {
+    PathJob J = jobs.local[1];
+    J.sum = pl1;
+    sweep_dispatch<R, uint16_t, PHASE_FIRST, MODE>(J, B, local - split, p2_table);
+  } else {
+    PathJob J = jobs.local[0];
+    J.sum = pl0;
+    sweep_dispatch<R, uint16_t, PHASE_FIRST, MODE>(J, B, local, p2_table);
+  }
 }
 
 // Two path directions in one launch, each accumulating into its own partial

@@ -27,6 +27,9 @@ ERRORS = {
 }
 
 VOLUME_AUTO, VOLUME_RESIDENT, VOLUME_RECOMPUTE = 0, 1, 2
+SWEEPS_AUTO, SWEEPS_SEQUENTIAL, SWEEPS_PAIRED, SWEEPS_PLANES = 0, 1, 2, 3
+SWEEPS = {"auto": SWEEPS_AUTO, "sequential": SWEEPS_SEQUENTIAL, "paired": SWEEPS_PAIRED,
+          "planes": SWEEPS_PLANES}
 # aggregation plan: --gpu-sgm auto|resident|streamed|esgm (Options.java:32,780)
 PLAN_AUTO, PLAN_RESIDENT, PLAN_STREAMED, PLAN_ESGM = 0, 1, 2, 3
 # fixed per-device workspace of match_pair_device_noalloc (SGM_NOALLOC_WORKSPACE_BYTES)
@@ -96,7 +99,7 @@ class SgmPlan(ctypes.Structure):
                 ("esgm_candidates", ctypes.c_int32), ("esgm_rows", ctypes.c_int32),
                 ("esgm_bytes", ctypes.c_int64), ("aggregate_storage", ctypes.c_int32),
                 ("block_rows", ctypes.c_int32), ("disk_bytes", ctypes.c_int64),
-                ("compact", ctypes.c_int32), ("paired", ctypes.c_int32)]
+                ("compact", ctypes.c_int32), ("sweeps", ctypes.c_int32)]
 
 
 class SgmPair(ctypes.Structure):
@@ -121,7 +124,7 @@ EXPORTS = [
     "sgm_disparity_limit", "sgm_match", "sgm_match_desc", "sgm_match_pair", "sgm_match_pair_device",
     "sgm_match_pair_device_noalloc",
     "sgm_match_pair_filtered", "sgm_match_pair_color",
-    "sgm_set_keep_volumes", "sgm_read_volumes", "sgm_set_cost_volume", "sgm_set_paired_sweeps",
+    "sgm_set_keep_volumes", "sgm_read_volumes", "sgm_set_cost_volume", "sgm_set_sweeps",
     "sgm_set_plan_mode",
     "sgm_set_esgm_candidates", "sgm_esgm_counts", "sgm_synchronize",
     "sgm_set_aggregate_storage", "sgm_interrupt", "sgm_debug_fail_host_allocations",
@@ -174,7 +177,7 @@ def lib():
                                                P, P, P, I, I, P, P, P, ps, P, P]
             L.sgm_set_keep_volumes.argtypes = [P, I]
             L.sgm_set_cost_volume.argtypes = [P, I]
-            L.sgm_set_paired_sweeps.argtypes = [P, I]
+            L.sgm_set_sweeps.argtypes = [P, I]
             L.sgm_set_plan_mode.argtypes = [P, I, I]
             L.sgm_set_esgm_candidates.argtypes = [P, I]
             L.sgm_set_aggregate_storage.argtypes = [P, I, ctypes.c_char_p, I64]
@@ -549,10 +552,11 @@ class Context:
         """Cost-volume policy: VOLUME_AUTO, VOLUME_RESIDENT or VOLUME_RECOMPUTE."""
         check(lib().sgm_set_cost_volume(self.h, int(policy)))
 
-    def paired_sweeps(self, on=True):
-        """Two middle path directions per launch over two partial aggregate
-        volumes when the extra volume fits the budget (default on)."""
-        check(lib().sgm_set_paired_sweeps(self.h, int(bool(on))))
+    def sweeps(self, schedule):
+        """Sweep schedule of the resident plan: SWEEPS_AUTO (default),
+        SWEEPS_SEQUENTIAL, SWEEPS_PAIRED or SWEEPS_PLANES (or their names)."""
+        schedule = SWEEPS[schedule] if isinstance(schedule, str) else int(schedule)
+        check(lib().sgm_set_sweeps(self.h, schedule))
 
     def plan_mode(self, mode, strip_rows=64):
         """Aggregation plan: PLAN_AUTO, PLAN_RESIDENT, PLAN_STREAMED or PLAN_ESGM

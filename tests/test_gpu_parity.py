@@ -213,78 +213,76 @@ def test_single_direction_match_merges_like_reference(gpu_context):
         assert st.full_interval_pixels == st_o["full_interval_pixels"]
 
 
-PAIRED_CASES = [
-    # (w, h, D, minD, window, cost, dirs, masks)
-    (200, 90, 64, -7, (9, 7), 0, 8, True),     # uint8 cost volume, lane group 2
-    (240, 70, 100, 5, (7, 5), 1, 4, False),    # 4 paths: one pair, then the last sweep
-    (260, 60, 250, -20, (9, 7), 2, 8, True),   # lane group 4
-    (180, 50, 90, 0, (5, 5), 3, 8, False),     # integer ZNCC: uint16 cost volume
-    (640, 40, 500, 3, (9, 7), 0, 8, False),    # lane group 8
+SCHEDULE_CASES = [
+    # (w, h, D, minD, window, cost, dirs, p2, masks)
+    (200, 90, 64, -7, (9, 7), 0, 8, 96, True),     # uint8 costs, lane group 2
+    (240, 70, 100, 5, (7, 5), 1, 4, 96, False),    # 4 paths: pairs, no planes
+    (260, 60, 250, -20, (9, 7), 2, 8, 96, True),   # lane group 4
+    (180, 50, 90, 0, (5, 5), 3, 8, 96, False),     # integer ZNCC
+    (640, 40, 500, 3, (9, 7), 0, 8, 96, False),    # lane group 8
+    (220, 60, 80, 0, (11, 11), 0, 8, 135, True),   # 2-word census: 120 + 135 = 255, planes
+    (220, 60, 80, 0, (11, 11), 0, 8, 136, False),  # 120 + 136 > 255: no planes
+    (150, 80, 48, -3, (9, 7), 0, 8, 193, True),    # 62 + 193 = 255, planes
 ]
 
 
-@pytest.mark.parametrize("cfg", PAIRED_CASES, ids=[str(i) for i in range(len(PAIRED_CASES))])
-def test_paired_sweeps_equal_one_direction_per_launch(cfg):
-    """Middle directions two per launch over two partial aggregate volumes (the
-    default when the second volume fits) and one direction per launch: both
-    equal the oracle bit for bit, and so does the final aggregate volume."""
-    w, h, D, md, (ww, wh), cost, dirs, masks = cfg
+@pytest.mark.parametrize("cfg", SCHEDULE_CASES, ids=[str(i) for i in range(len(SCHEDULE_CASES))])
+def test_sweep_schedules_equal_the_oracle(cfg):
+    """Every sweep schedule (one direction per launch, paired partial
+    aggregates, path-cost planes) equals the oracle bit for bit; the plan and
+    the profile name the schedule that ran, and the planes run exactly where
+    every valid path cost fits a byte (window bits - 1 + P2 <= 255, 8 paths)."""
+    w, h, D, md, (ww, wh), cost, dirs, p2, masks = cfg
     left, right, _ = synth.stereo_pair(w, h, D, md, seed=300 + D)
     rng = np.random.default_rng(D)
     vl = (rng.random((h, w)) > 0.05).astype(np.uint8) if masks else None
     vr = (rng.random((h, w)) > 0.05).astype(np.uint8) if masks else None
     op = O.params(window_w=ww, window_h=wh, disparities=D, min_disparity=md, cost=cost,
-                  directions=dirs, p1=8, p2=96, lr_threshold=1.0)
+                  directions=dirs, p1=8, p2=p2, lr_threshold=1.0)
     want = O.compute(op, left, right, w, h, vl, vr)
-    for paired in (True, False):
+    planes_apply = dirs == 8 and ww * wh - 1 + p2 <= 255
+    for name in ("sequential", "paired", "planes", "auto"):
         ctx = sgm.Context(0)
         try:
-            ctx.paired_sweeps(paired)
+            ctx.sweeps(name)
+            ran = ctx.plan(to_sgm(op), w, h)["sweeps"]
+            expect = {"sequential": 1, "paired": 2, "planes": 3 if planes_apply else 2,
+                      "auto": 3 if planes_apply else 2}[name]
+            assert ran == expect, (name, ran)
             ctx.set_profiling(True)
-            print("paired", paired, cfg, flush=True)
+            print("schedule", name, cfg, flush=True)
             got = ctx.match_pair(to_sgm(op), left, right, vl, vr)
-            assert ctx.profile()["last_plan"]["paired"] == paired
-            assert_same(got, want, "paired %s %r" % (paired, cfg))
+            assert ctx.profile()["last_plan"]["sweeps"] == ["", "sequential", "paired",
+                                                           "planes"][expect]
+            assert_same(got, want, "%s %r" % (name, cfg))
         finally:
             ctx.close()
-    # the final aggregate of the paired run (stored by the last sweep)
-    ctx = sgm.Context(0)
-    try:
-        ctx.keep_volumes(True)
-        ctx.match_pair(to_sgm(op), left, right, vl, vr)
-        n = w * h
-        da, db = O.census(left, w, h, ww, wh), O.census(right, w, h, ww, wh)
-        ta, tb = O.texture(left, w, h, ww, wh, vl), O.texture(right, w, h, ww, wh, vr)
-        if cost == 0:
-            for d, (a_, b_, A, B, TA, TB) in enumerate([(da, db, left, right, ta, tb),
-                                                        (db, da, right, left, tb, ta)]):
-                S = O.match(op, a_, b_, A.reshape(-1), B.reshape(-1), TA, TB, w, h, d,
-                            volumes=True)[5]
-                assert np.array_equal(ctx.read_volumes(d, n, D)[1], S), "aggregate %d" % d
-    finally:
-        ctx.close()
 
 
-def test_paired_sweeps_fall_back_when_the_second_volume_does_not_fit():
+def test_schedules_fall_back_when_their_volumes_do_not_fit():
     w, h, D = 300, 120, 64
     left, right, _ = synth.stereo_pair(w, h, D, 0, seed=77)
     op = O.params(window_w=9, window_h=7, disparities=D, p1=10, p2=120, lr_threshold=1.0)
+    want = O.compute(op, left, right, w, h)
     probe = sgm.Context(0)
     plan = probe.plan(to_sgm(op), w, h)
     probe.close()
-    assert plan["paired"] == 1
-    # resident with the uint8 cost volume, no room for the second aggregate volume
-    cost_volume = 2 * w * h * plan["padded_disparities"]
-    ctx = sgm.Context(0, budget_bytes=plan["pair_bytes"] + cost_volume + (1 << 20))
-    try:
-        assert ctx.plan(to_sgm(op), w, h)["paired"] == 0
-        ctx.set_profiling(True)
-        got = ctx.match_pair(to_sgm(op), left, right)
-        info = ctx.profile()["last_plan"]
-        assert info["plan"] == "resident" and not info["paired"]
-        assert_same(got, O.compute(op, left, right, w, h), "unpaired under a tight budget")
-    finally:
-        ctx.close()
+    assert plan["sweeps"] == sgm.SWEEPS_PLANES
+    cells = 2 * w * h * plan["padded_disparities"]
+    base = plan["pair_bytes"] + cells  # resident with the uint8 cost volume
+    # room for the second partial aggregate, not for the planes; then for neither
+    for extra, expect in ((2 * cells, sgm.SWEEPS_PAIRED), (cells, sgm.SWEEPS_SEQUENTIAL)):
+        ctx = sgm.Context(0, budget_bytes=base + extra + (1 << 20))
+        try:
+            assert ctx.plan(to_sgm(op), w, h)["sweeps"] == expect
+            ctx.set_profiling(True)
+            got = ctx.match_pair(to_sgm(op), left, right)
+            info = ctx.profile()["last_plan"]
+            assert info["plan"] == "resident"
+            assert info["sweeps"] == ["", "sequential", "paired", "planes"][expect]
+            assert_same(got, want, "schedule %d under a tight budget" % expect)
+        finally:
+            ctx.close()
 
 
 def test_plain_workload_640x480(gpu_context):
