@@ -925,29 +925,34 @@ int64_t sgm::schedule_bytes(const sgm_params *p, int w, int h, int schedule) {
   return 0;
 }
 
+// Whether a multi-direction schedule applies to the pair (budget aside).  Both
+// need packed 16-bit sweeps of lane groups 2..8 over a cost volume.  Path-cost
+// planes also need 8 paths, uint8 costs, no kept volumes (the planes replace
+// the aggregate), and path costs that fit a byte: every valid cell's path cost
+// is at most its cost (<= window bits - 1 for the uint8 costs) plus P2.
+bool sgm::schedule_applies(const sgm_context *ctx, const sgm_params *p, int schedule,
+                           bool cost_volume) {
+  if (schedule == SGM_SWEEPS_SEQUENTIAL) return true;
+  if (!cost_volume || aggregate_bytes(p) != 2 || path_lanes(p) > 8) return false;
+  if (schedule == SGM_SWEEPS_PAIRED) return true;
+  return schedule == SGM_SWEEPS_PLANES && !ctx->keep_volumes && cost_u8(p) &&
+         p->directions == kPlanes + 1 && p->window_w * p->window_h - 1 + effective_p2(p) <= 255;
+}
+
 // Sweep schedule of a resident pair under `budget` (SGM_SWEEPS_SEQUENTIAL,
-// _PAIRED or _PLANES).  Both multi-direction schedules need packed 16-bit
-// sweeps of lane groups 2..8 over a cost volume.  Path-cost planes also need
-// 8 paths, uint8 costs, no kept volumes (the planes replace the aggregate), and
-// path costs that fit a byte: every valid cell's path cost is at most its cost
-// (<= window bits - 1 for the uint8 costs) plus P2.  AUTO and PLANES take the
-// planes, then the pairs, where they apply and fit the budget.
+// _PAIRED or _PLANES): AUTO and PLANES take the planes, then the pairs, where
+// they apply and fit the budget; PAIRED the pairs where they fit.
 int sgm::sweep_schedule(const sgm_context *ctx, const sgm_params *p, int w, int h,
                         bool cost_volume, int64_t budget) {
   const int want = ctx->sweeps;
-  if (want == SGM_SWEEPS_SEQUENTIAL || !cost_volume || aggregate_bytes(p) != 2 ||
-      path_lanes(p) > 8)
-    return SGM_SWEEPS_SEQUENTIAL;
+  if (want == SGM_SWEEPS_SEQUENTIAL) return SGM_SWEEPS_SEQUENTIAL;
   const int64_t base = pair_bytes(p, w, h, true);
-  if (want == SGM_SWEEPS_AUTO || want == SGM_SWEEPS_PLANES) {
-    const bool planes = !ctx->keep_volumes && cost_u8(p) && p->directions == kPlanes + 1 &&
-                        p->window_w * p->window_h - 1 + effective_p2(p) <= 255 &&
-                        base + schedule_bytes(p, w, h, SGM_SWEEPS_PLANES) <= budget;
-    if (planes) return SGM_SWEEPS_PLANES;
+  for (int sched : {SGM_SWEEPS_PLANES, SGM_SWEEPS_PAIRED}) {
+    if (sched == SGM_SWEEPS_PLANES && want == SGM_SWEEPS_PAIRED) continue;
+    if (schedule_applies(ctx, p, sched, cost_volume) &&
+        base + schedule_bytes(p, w, h, sched) <= budget)
+      return sched;
   }
-  // PAIRED, or the fallback of AUTO and PLANES
-  if (base + schedule_bytes(p, w, h, SGM_SWEEPS_PAIRED) <= budget)
-    return SGM_SWEEPS_PAIRED;
   return SGM_SWEEPS_SEQUENTIAL;
 }
 
@@ -997,12 +1002,11 @@ static std::vector<std::pair<std::string, size_t>> pair_buffer_list(const sgm_pa
 }  // namespace
 
 // Device bytes the context holds once ensure_pair_buffers has served the pair
-// on the resident plan under `budget`: each of the pair's buffers grows to its
-// size when its cached capacity is smaller, every other cached buffer stays as
-// it is (`fresh`: no cached buffers).
+// on the resident plan with sweep schedule `schedule`: each of the pair's
+// buffers grows to its size when its cached capacity is smaller, every other
+// cached buffer stays as it is (`fresh`: no cached buffers).
 int64_t sgm::resident_footprint(sgm_context *ctx, const sgm_params *p, int w, int h,
-                                bool cost_volume, int64_t budget, bool fresh) {
-  const int schedule = sweep_schedule(ctx, p, w, h, cost_volume, budget);
+                                bool cost_volume, int schedule, bool fresh) {
   int64_t total = fresh ? 0 : ctx->allocated;
   for (const auto &b : pair_buffer_list(p, w, h, true, cost_volume, schedule)) {
     if (fresh) {

@@ -64,7 +64,8 @@ bool recompute_cost(sgm_context *ctx, const sgm_params *p, int w, int h);
 int64_t pair_bytes(const sgm_params *p, int w, int h);
 int64_t pair_bytes(const sgm_params *p, int w, int h, bool cost_volume);
 int64_t resident_footprint(sgm_context *ctx, const sgm_params *p, int w, int h, bool cost_volume,
-                           int64_t budget, bool fresh);
+                           int schedule, bool fresh);
+bool schedule_applies(const sgm_context *ctx, const sgm_params *p, int schedule, bool cost_volume);
 int64_t schedule_bytes(const sgm_params *p, int w, int h, int schedule);
 int sweep_schedule(const sgm_context *ctx, const sgm_params *p, int w, int h, bool cost_volume,
                    int64_t budget);

@@ -108,6 +108,7 @@ struct sgm_server {
   int max_batch = 16;
   bool lanes = false;  // max_batch == 1: plain per-pair lanes
   int volume_policy = 0;  // the context's SGM_VOLUME_* at creation
+  int sweeps = 0;         // the context's SGM_SWEEPS_* at creation
   void *region = nullptr;
   size_t region_bytes = 0;
   Pool pool;
@@ -280,13 +281,14 @@ void sgm_server::run() {
 // device bytes the lane will hold after serving it, plus what the other lanes
 // hold, fit the budget.  The lane's cached buffers are reused where they are
 // large enough, so that footprint counts each of the pair's buffers at the
-// larger of its cached and required size.  Choices, in order: the pair's
-// cost volume (when the volume policy allows one) before the recomputed
-// census cost; the lane's cached buffers as they are, then the lane's own
-// buffers released, then the cached buffers of idle lanes released too.  The
-// lane's budget becomes what the other lanes leave and its volume policy the
-// chosen plan, so it runs exactly the admitted footprint.  Called and
-// returning with `mu` held.
+// larger of its cached and required size.  Plans, preferred first: path-cost
+// planes, paired sweeps, one aggregate with the cost volume, one aggregate
+// with recomputed census costs (as the sweep schedule and the volume policy
+// allow).  Tried in order: every plan on the lane's cached buffers as they
+// are, then on its buffers released, then with the cached buffers of idle
+// lanes released too.  The lane's budget becomes what the other lanes leave
+// and its schedule and volume policy the chosen plan, so it runs exactly the
+// admitted footprint.  Called and returning with `mu` held.
 bool sgm_server::admit_lane(Slot &s) {
   Request *r = pending.front();
   sgm_params q = params;
@@ -300,21 +302,35 @@ bool sgm_server::admit_lane(Slot &s) {
       if (&o != &s) c += o.lane->allocated;
     return c;
   };
-  // plans the volume policy allows, preferred first
+  // plans the volume policy and the sweep schedule allow, preferred first
+  struct Plan {
+    bool cost_volume;
+    int schedule;
+  };
   const bool possible = census_recompute_possible(&q) && !lane->keep_volumes;
-  bool options[2];
-  int nopt = 0;
-  if (!possible || volume_policy != SGM_VOLUME_RECOMPUTE) options[nopt++] = true;
-  if (possible && volume_policy != SGM_VOLUME_RESIDENT) options[nopt++] = false;
-  int chosen = -1;
+  const bool with_cv = !possible || volume_policy != SGM_VOLUME_RECOMPUTE;
+  Plan plans[4];
+  int nplan = 0;
+  if (with_cv) {
+    if (sweeps == SGM_SWEEPS_AUTO || sweeps == SGM_SWEEPS_PLANES) {
+      if (schedule_applies(lane, &q, SGM_SWEEPS_PLANES, true))
+        plans[nplan++] = {true, SGM_SWEEPS_PLANES};
+    }
+    if (sweeps != SGM_SWEEPS_SEQUENTIAL && schedule_applies(lane, &q, SGM_SWEEPS_PAIRED, true))
+      plans[nplan++] = {true, SGM_SWEEPS_PAIRED};
+    plans[nplan++] = {true, SGM_SWEEPS_SEQUENTIAL};
+  }
+  if (possible && volume_policy != SGM_VOLUME_RESIDENT)
+    plans[nplan++] = {false, SGM_SWEEPS_SEQUENTIAL};
   auto pick = [&](bool fresh) {
     const int64_t avail = (int64_t)region_bytes - others();
-    for (int i = 0; i < nopt; i++) {
-      const int64_t f = resident_footprint(lane, &q, w, h, options[i], avail, fresh);
-      if (f <= avail) return i;
-    }
+    for (int i = 0; i < nplan; i++)
+      if (resident_footprint(lane, &q, w, h, plans[i].cost_volume, plans[i].schedule, fresh) <=
+          avail)
+        return i;
     return -1;
   };
+  int chosen;
   if ((chosen = pick(false)) < 0 && (chosen = pick(true)) >= 0) {
     (void)sgm_release_buffers(lane);
   } else if (chosen < 0) {
@@ -332,7 +348,8 @@ bool sgm_server::admit_lane(Slot &s) {
   }
   lane->budget = (int64_t)region_bytes - others();
   if (possible)
-    lane->volume_policy = options[chosen] ? SGM_VOLUME_RESIDENT : SGM_VOLUME_RECOMPUTE;
+    lane->volume_policy = plans[chosen].cost_volume ? SGM_VOLUME_RESIDENT : SGM_VOLUME_RECOMPUTE;
+  lane->sweeps = plans[chosen].schedule;
   pending.pop_front();
   const double t = now();
   r->admitted = t;
@@ -380,6 +397,7 @@ extern "C" int sgm_server_create(sgm_context *ctx, const sgm_params *params, int
   s->max_batch = max_batch;
   s->lanes = max_batch == 1;
   s->volume_policy = ctx->volume_policy;
+  s->sweeps = ctx->sweeps;
   s->region_bytes = (size_t)bytes;
   if (!s->lanes) {
     rc = context_alloc(ctx, &s->region, s->region_bytes, "serving pool");
