@@ -70,12 +70,13 @@ static int validate_pair(const sgm_params *p, const sgm_pair &pr, int flags) {
 }
 
 // Upper bound of the phase-major group tables (build_groups): the 2 * count
-// path jobs once, three line-prefix arrays per group, 256-byte alignment, at
-// most kMaxGroups lane groups.
-static constexpr int kMaxGroups = 8;
+// path jobs once, three line-prefix arrays per group, 256-byte alignment.  A
+// group (one lane group R of one sub-batch) holds at least one pair, so there
+// are at most `count` groups.
 static size_t group_table_bound(int count) {
-  return align_up(2 * (size_t)count * sizeof(PathJob) + 3 * 4 * (2 * (size_t)count + kMaxGroups) +
-                      (size_t)kMaxGroups * 4 * 256,
+  const size_t G = (size_t)count;
+  return align_up(2 * (size_t)count * sizeof(PathJob) + 3 * 4 * (2 * (size_t)count + G) +
+                      G * 4 * 256,
                   256);
 }
 
@@ -95,18 +96,25 @@ static int build_groups(sgm_batch *b, const std::vector<PathJob> &jobs, const sg
   };
   std::vector<Pending> offs;
   b->groups.clear();
+  // sub-batch by sub-batch (path-cost planes reuse one scratch region), and
+  // within a sub-batch one group per lane group R
+  for (int sub = 0; sub < b->nsub; sub++)
   for (int R : rs) {
     sgm_batch::Group g;
     g.R = R;
+    g.sub = sub;
     std::vector<PathJob> gj;
     int64_t volume = 0;
     bool aligned = true;
-    for (const PathJob &J : jobs)
-      if (J.R == R) {
+    for (size_t j = 0; j < jobs.size(); j++) {
+      const PathJob &J = jobs[j];
+      if (J.R == R && b->pair_sub[j / 2] == sub) {
         gj.push_back(J);
         volume += (int64_t)J.w * J.h * J.Dp * (b->cost8 ? 3 : 4);
         aligned &= J.Dp % 64 == 0;
       }
+    }
+    if (gj.empty()) continue;
     g.njobs = (int)gj.size();
     // stream the volumes past the caches as the plain path does: whole 64-byte
     // lines only, volumes far beyond the 256 MB MALL
@@ -130,7 +138,7 @@ static int build_groups(sgm_batch *b, const std::vector<PathJob> &jobs, const sg
   uint8_t *base;
   if (b->external) {
     // carved batches: the room batch_table_bytes reserves after the job tables
-    if (b->groups.size() > (size_t)kMaxGroups || host.size() > group_table_bound(count))
+    if (b->groups.size() > (size_t)count || host.size() > group_table_bound(count))
       return fail(SGM_ERR_NO_PLAN, "batch group tables exceed their reserved room");
     base = (uint8_t *)b->tables + align_up(b->tables_bytes, 256);
   } else {
@@ -176,6 +184,49 @@ static int phased_enqueue(sgm_batch *b) {
   int rc;
   if ((rc = queue_launch(b, 0, pb[BPH_PATH0], b->args.head, "batch census"))) return rc;
   const int sb = b->sum_bytes;
+  if (b->planes) {
+    // path-cost planes, sub-batch after sub-batch (one scratch region): per
+    // lane group the first direction (fused census cost, plane 0), the middle
+    // directions in one launch (planes 1..6), the last sweep over the planes
+    for (const sgm_batch::Group &g : b->groups) {
+      const int nt = g.nt ? MODE_NT : 0;
+      auto args = [&](int dir) {
+        PathArgs A = path_args(p, dir, g.njobs, 0, 0);
+        A.total_lines = g.total_lines[A.cls];
+        return A;
+      };
+      PathJobs pj;
+      memset(&pj, 0, sizeof pj);
+      pj.table = g.jobs;
+      const PathArgs A0 = args(0);
+      pj.line_begin = g.line_begin[A0.cls];
+      if ((rc = launch_path(ctx, g.R, sb, 2 | nt | MODE_PLANE, pj, A0, PHASE_FIRST,
+                            path_waves(ctx, g.R), b->stream)))
+        return rc;
+      MultiDir M;
+      memset(&M, 0, sizeof M);
+      M.ndirs = p->directions - 2;
+      for (int k = 0; k < M.ndirs; k++) {
+        const PathArgs Ak = args(1 + k);
+        M.dx[k] = Ak.dx;
+        M.dy[k] = Ak.dy;
+        M.cls[k] = Ak.cls;
+        M.begin[k] = M.total_lines;
+        M.total_lines += Ak.total_lines;
+      }
+      for (int cls = 0; cls < 3; cls++) M.line_begin[cls] = g.line_begin[cls];
+      M.njobs = g.njobs;
+      if ((rc = launch_path_multi(ctx, g.R, 4 | nt | MODE_PLANE, pj, args(1), M,
+                                  path_waves(ctx, g.R), b->stream)))
+        return rc;
+      const PathArgs AL = args(p->directions - 1);
+      pj.line_begin = g.line_begin[AL.cls];
+      if ((rc = launch_path(ctx, g.R, sb, 4 | nt | MODE_PLANE, pj, AL, PHASE_LAST,
+                            path_waves(ctx, g.R), b->stream)))
+        return rc;
+    }
+    return queue_launch(b, pb[last_path + 1], pb[b->args.nph], b->head2, "batch lr");
+  }
   const int base_mode = b->recompute ? 3 : sb == 2 ? (b->cost8 ? 4 : 1) : 0;
   for (int dir = 0; dir < p->directions; dir++) {
     const int phase = dir == 0 ? PHASE_FIRST : dir == p->directions - 1 ? PHASE_LAST : PHASE_MID;
@@ -255,7 +306,8 @@ extern "C" int sgm_batch_create(sgm_context *ctx, const sgm_params *params, cons
   *out = nullptr;
   std::lock_guard<std::mutex> g(ctx->lock);
   (void)hipSetDevice(ctx->device);
-  return batch_create(ctx, params, pairs, count, flags, nullptr, 0, ctx->stream, out, -1);
+  return batch_create(ctx, params, pairs, count, flags, nullptr, 0, ctx->stream, out, -1,
+                      (flags & SGM_BATCH_PHASED) ? -1 : 0);
 }
 
 namespace sgm {
@@ -273,9 +325,28 @@ bool batch_recompute_possible(const sgm_params *params, bool keep_volumes) {
          aggregate_bytes(params) == 2;
 }
 
+// Path-cost planes in a phase-major batch: the plain plan's plane conditions
+// with the census cost fused into the first sweep (1-word Census, 16-bit sums).
+bool batch_planes_possible(const sgm_context *ctx, const sgm_params *params, int flags) {
+  return (flags & SGM_BATCH_PHASED) && params->cost == SGM_COST_CENSUS && words_of(params) == 1 &&
+         aggregate_bytes(params) == 2 && params->directions == kPlanes + 1 &&
+         !ctx->keep_volumes &&
+         params->window_w * params->window_h - 1 + effective_p2(params) <= 255 &&
+         (ctx->sweeps == SGM_SWEEPS_AUTO || ctx->sweeps == SGM_SWEEPS_PLANES);
+}
+
+// Scratch bytes of one pair under path-cost planes (carved as batch_create
+// carves them): the uint8 cost volume and the plane block of both matching
+// directions.
+size_t batch_plane_bytes(const sgm_pair &pr) {
+  const size_t n = (size_t)pr.width * pr.height, Dp = padded_disparities(pr.disparities);
+  return 2 * (align_up(n * Dp, 256) + align_up(n * Dp * kPlanes, 256));
+}
+
 size_t batch_arena_bytes(const sgm_params *params, const sgm_pair *pairs, int count, int flags,
-                         bool recompute) {
-  // must mirror the carving in batch_create
+                         bool recompute, bool planes) {
+  // must mirror the carving in batch_create; with planes every pair in one
+  // sub-batch (scratch for all of them)
   const bool host_io = !(flags & SGM_BATCH_DEVICE);
   const int words = words_of(params), sb = aggregate_bytes(params);
   const bool zncc = params->cost == SGM_COST_ZNCC || params->cost == SGM_COST_ZNCC_FP32;
@@ -299,8 +370,8 @@ size_t batch_arena_bytes(const sgm_params *params, const sgm_pair *pairs, int co
         cv.take(n * 4);
         cv.take(n * 4);
       }
-      if (!recompute) cv.take(n * Dp * (batch_cost8(params) ? 1 : 2));
-      cv.take(n * Dp * sb);
+      if (!recompute && !planes) cv.take(n * Dp * (batch_cost8(params) ? 1 : 2));
+      if (!planes) cv.take(n * Dp * sb);
       const bool own_disp = host_io || (d == 0 ? pr.dl == nullptr : pr.dr == nullptr);
       if (own_disp) cv.take(n * 4);
     }
@@ -311,6 +382,8 @@ size_t batch_arena_bytes(const sgm_params *params, const sgm_pair *pairs, int co
     if (host_io && pr.reasons) cv.take(n);
     if (host_io && pr.boundary) cv.take(n);
   }
+  if (planes)
+    for (int i = 0; i < count; i++) cv.off += batch_plane_bytes(pairs[i]);
   return cv.off;
 }
 
@@ -335,7 +408,7 @@ size_t batch_table_bytes(int count, const sgm_params *params) {
 // that caller-owned device region (serving pool) instead of being allocated.
 int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pairs, int count,
                  int flags, void *region, size_t region_bytes, hipStream_t stream,
-                 sgm_batch **out, int recompute) {
+                 sgm_batch **out, int recompute, int planes) {
   for (int i = 0; i < count; i++) {
     int rc = validate_pair(params, pairs[i], flags);
     if (rc) return rc;
@@ -361,7 +434,7 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
       recompute = 1;
     } else {
       const int64_t avail = region ? (int64_t)region_bytes : ctx->budget - ctx->allocated;
-      const int64_t need = (int64_t)(batch_arena_bytes(params, pairs, count, flags, false) +
+      const int64_t need = (int64_t)(batch_arena_bytes(params, pairs, count, flags, false, false) +
                                      batch_table_bytes(count, params));
       recompute = need > avail ? 1 : 0;
     }
@@ -370,6 +443,44 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
   b->cost8 = !b->recompute && batch_cost8(params);
   b->fused = !b->recompute && params->cost == SGM_COST_CENSUS && b->words == 1 &&
              b->sum_bytes == 2;
+  // Path-cost planes (phase-major runs of the fused census cost): no resident
+  // cost or aggregate volume per pair; the cost volumes and plane blocks of a
+  // sub-batch of pairs live in one scratch region that the sub-batches reuse
+  // in turn.  planes = 1: all pairs in one sub-batch; -1: sub-batches as large
+  // as the memory at hand allows (off when not even the largest pair fits).
+  b->nsub = 1;
+  b->pair_sub.assign(count, 0);
+  if (planes != 0 && b->fused && batch_planes_possible(ctx, params, flags)) {
+    size_t total = 0, largest = 0;
+    for (int i = 0; i < count; i++) {
+      total += batch_plane_bytes(pairs[i]);
+      largest = std::max(largest, batch_plane_bytes(pairs[i]));
+    }
+    size_t cap = total;
+    if (planes < 0) {
+      const int64_t avail = region ? (int64_t)region_bytes : ctx->budget - ctx->allocated;
+      const int64_t rest = avail - (int64_t)(batch_arena_bytes(params, pairs, count, flags, false,
+                                                               true) - total) -
+                           (int64_t)batch_table_bytes(count, params);
+      cap = rest > 0 ? std::min(total, (size_t)rest) : 0;
+    }
+    if (cap >= largest) {
+      b->planes = true;
+      size_t used = 0;
+      int sub = 0;
+      for (int i = 0; i < count; i++) {
+        const size_t need = batch_plane_bytes(pairs[i]);
+        if (used + need > cap) {
+          sub++;
+          used = 0;
+        }
+        b->pair_sub[i] = sub;
+        used += need;
+        b->scratch_bytes = std::max(b->scratch_bytes, used);
+      }
+      b->nsub = sub + 1;
+    }
+  }
   b->waves_per_block = ctx->batch_waves_per_block;
   b->lines_per_item = ctx->batch_lines_per_item;
   {
@@ -413,8 +524,8 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
       o.gm[d] = cv.take(n * 2);
       o.wsum[d] = zncc ? cv.take(n * 4) : 0;
       o.wsum2[d] = zncc ? cv.take(n * 4) : 0;
-      o.cost[d] = b->recompute ? 0 : cv.take(n * Dp * (b->cost8 ? 1 : 2));
-      o.sum[d] = cv.take(n * Dp * b->sum_bytes);
+      o.cost[d] = b->recompute || b->planes ? 0 : cv.take(n * Dp * (b->cost8 ? 1 : 2));
+      o.sum[d] = b->planes ? 0 : cv.take(n * Dp * b->sum_bytes);
       const bool own_disp = host_io || (d == 0 ? pr.dl == nullptr : pr.dr == nullptr);
       o.disp[d] = own_disp ? cv.take(n * 4) : 0;
     }
@@ -424,6 +535,24 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
     o.checked = host_io ? cv.take(n * 4) : 0;
     o.reasons = (host_io && pr.reasons) ? cv.take(n) : 0;
     o.boundary = (host_io && pr.boundary) ? cv.take(n) : 0;
+  }
+  if (b->planes) {
+    // the scratch region, carved again for each sub-batch's pairs
+    const size_t scratch = cv.off;
+    cv.off += b->scratch_bytes;
+    size_t at = scratch;
+    for (int i = 0; i < count; i++) {
+      if (i > 0 && b->pair_sub[i] != b->pair_sub[i - 1]) at = scratch;
+      const size_t n = (size_t)pairs[i].width * pairs[i].height;
+      const size_t Dp = padded_disparities(pairs[i].disparities);
+      Carver sc;
+      sc.off = at;
+      for (int d = 0; d < 2; d++) {
+        offs[i].cost[d] = sc.take(n * Dp);
+        offs[i].sum[d] = sc.take(n * Dp * kPlanes);
+      }
+      at = sc.off;
+    }
   }
   b->arena_bytes = cv.off;
   int rc = SGM_OK;
@@ -782,7 +911,8 @@ extern "C" int sgm_batch_describe(sgm_batch *b, char *buffer, size_t length) {
                          "{\"pairs\":%zu,\"work_items\":%d,\"workgroups\":%d,"
                          "\"waves_per_workgroup\":%d,\"lines_per_item\":%d,"
                          "\"cost_volume\":\"%s\",\"device_bytes\":%lld,\"graph\":%s,"
-                         "\"phased\":%s,\"lane_groups\":%zu}",
+                         "\"phased\":%s,\"lane_groups\":%zu,\"sweeps\":\"%s\","
+                         "\"sub_batches\":%d}",
                          b->pairs.size(), b->total_items, b->blocks, b->waves_per_block,
                          b->lines_per_item,
                          b->recompute ? "recomputed"
@@ -791,7 +921,7 @@ extern "C" int sgm_batch_describe(sgm_batch *b, char *buffer, size_t length) {
                                       : "resident-u16",
                          (long long)(b->arena_bytes + b->tables_bytes),
                          b->graph_exec ? "true" : "false", b->phased ? "true" : "false",
-                         b->groups.size());
+                         b->groups.size(), b->planes ? "planes" : "sequential", b->nsub);
   if (n < 0 || (size_t)n >= length) return fail(SGM_ERR_INVALID_ARGUMENT, "buffer too small");
   return SGM_OK;
 }

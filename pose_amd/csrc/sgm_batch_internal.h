@@ -19,13 +19,18 @@ struct PairSlots {
 
 bool batch_recompute_possible(const sgm_params *params, bool keep_volumes);
 size_t batch_arena_bytes(const sgm_params *params, const sgm_pair *pairs, int count, int flags,
-                         bool recompute);
+                         bool recompute, bool planes);
+bool batch_planes_possible(const sgm_context *ctx, const sgm_params *params, int flags);
+size_t batch_plane_bytes(const sgm_pair &pr);
 size_t batch_table_bytes(int count, const sgm_params *params);
 // recompute: 1 census cost recomputed in the sweeps (no cost volume), 0 resident
 // cost volume, -1 chosen by the context's policy and the memory at hand.
+// planes (phase-major runs): 1 path-cost planes with every pair in one
+// sub-batch, 0 one aggregate volume per pair, -1 planes in sub-batches as
+// large as the memory at hand allows, where they apply.
 int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pairs, int count,
                  int flags, void *region, size_t region_bytes, hipStream_t stream,
-                 sgm_batch **out, int recompute);
+                 sgm_batch **out, int recompute, int planes);
 const void *batch_kernel_of(const sgm_batch *b);
 int batch_launch(sgm_batch *b);
 int batch_finish(sgm_batch *b);
@@ -59,13 +64,19 @@ struct sgm_batch {
   // and, per line class (horizontal, vertical, diagonal), the first line of
   // each job in the group's launch
   struct Group {
-    int R = 0, njobs = 0;
+    int R = 0, njobs = 0, sub = 0;
     bool nt = false;
     const sgm::PathJob *jobs = nullptr;
     const int *line_begin[3] = {nullptr, nullptr, nullptr};
     int total_lines[3] = {0, 0, 0};
   };
   bool phased = false;
+  // path-cost planes: the scratch region of a sub-batch's cost volumes and
+  // plane blocks, each pair's sub-batch, and the sub-batch count
+  bool planes = false;
+  size_t scratch_bytes = 0;
+  std::vector<int> pair_sub;
+  int nsub = 1;
   std::vector<Group> groups;
   std::vector<int> phase_begin;  // host copy of the queue's phase boundaries
   unsigned *head2 = nullptr;     // queue head of the L-R launch

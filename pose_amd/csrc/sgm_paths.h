@@ -1345,7 +1345,12 @@ struct MultiDir {
   int ndirs, total_lines;
   int dx[kMultiDirs], dy[kMultiDirs], cls[kMultiDirs], split[kMultiDirs];
   int begin[kMultiDirs];
-  void *plane[kMultiDirs][2];
+  void *plane[kMultiDirs][2];  // jobs passed by value: each direction's plane per job
+  // jobs in a device table (batches): the first line of each job per line
+  // class, and the job count; a job's `sum` points at its plane block, plane
+  // 1 + i of direction i lies i + 1 rows of Dp bytes further
+  const int *line_begin[3];
+  int njobs;
 };
 
 template <int R, int MODE>
@@ -1361,7 +1366,7 @@ __global__ __launch_bounds__(256) void path_multi_kernel(PathJobs jobs, PathArgs
   B.dx = M.dx[0];
   B.dy = M.dy[0];
   B.cls = M.cls[0];
-  int begin = M.begin[0], split = M.split[0];
+  int dir = 0, begin = M.begin[0], split = M.split[0];
   void *pl0 = M.plane[0][0], *pl1 = M.plane[0][1];
 #pragma unroll
   for (int i = 1; i < kMultiDirs; i++) {
@@ -1369,6 +1374,7 @@ __global__ __launch_bounds__(256) void path_multi_kernel(PathJobs jobs, PathArgs
       B.dx = M.dx[i];
       B.dy = M.dy[i];
       B.cls = M.cls[i];
+      dir = i;
       begin = M.begin[i];
       split = M.split[i];
       pl0 = M.plane[i][0];
@@ -1376,7 +1382,13 @@ __global__ __launch_bounds__(256) void path_multi_kernel(PathJobs jobs, PathArgs
     }
   }
   const int local = gline - begin;
-  if (A.njobs > 1 && local >= split) {
+  if (jobs.table) {
+    const int *lb = B.cls == 0 ? M.line_begin[0] : B.cls == 1 ? M.line_begin[1] : M.line_begin[2];
+    const int ji = __builtin_amdgcn_readfirstlane(find_job(lb, M.njobs, local));
+    PathJob J = jobs.table[ji];
+    J.sum = (uint8_t *)J.sum + (size_t)(1 + dir) * J.Dp;
+    sweep_dispatch<R, uint16_t, PHASE_FIRST, MODE>(J, B, local - lb[ji], p2_table);
+  } else if (A.njobs > 1 && local >= split) {
     PathJob J = jobs.local[1];
     J.sum = pl1;
     sweep_dispatch<R, uint16_t, PHASE_FIRST, MODE>(J, B, local - split, p2_table);

@@ -133,7 +133,7 @@ struct sgm_server {
 static size_t request_bytes(const sgm_context *ctx, const sgm_params *p, const sgm_pair &pr) {
   const bool recompute = batch_recompute_possible(p, ctx->keep_volumes) &&
                          ctx->volume_policy != SGM_VOLUME_RESIDENT;
-  return batch_arena_bytes(p, &pr, 1, 0, recompute) + batch_table_bytes(1, p);
+  return batch_arena_bytes(p, &pr, 1, 0, recompute, false) + batch_table_bytes(1, p);
 }
 
 void sgm_server::run() {
@@ -194,23 +194,33 @@ void sgm_server::run() {
         }
       }
       // Shrink until the group fits the pool (FIFO: keep the oldest).  Each
-      // group first tries resident cost volumes, then recomputed census costs.
+      // group first tries path-cost planes, then resident cost and aggregate
+      // volumes, then recomputed census costs.
       size_t off = 0;
-      int recompute = 0;
+      int recompute = 0, planes = 0;
       const bool can_recompute = batch_recompute_possible(&params, ctx->keep_volumes);
+      const bool can_planes = ctx->volume_policy != SGM_VOLUME_RECOMPUTE &&
+                              batch_planes_possible(ctx, &params, SGM_BATCH_PHASED);
       while (!take.empty()) {
         std::vector<sgm_pair> prs;
         for (Request *r : take) prs.push_back(r->pair);
         const int n = (int)prs.size();
         const size_t tables = batch_table_bytes(n, &params);
+        if (can_planes) {
+          recompute = 0;
+          planes = 1;
+          bytes = batch_arena_bytes(&params, prs.data(), n, 0, false, true) + tables;
+          if (pool.alloc(bytes, &off)) break;
+        }
+        planes = 0;
         if (ctx->volume_policy != SGM_VOLUME_RECOMPUTE || !can_recompute) {
           recompute = 0;
-          bytes = batch_arena_bytes(&params, prs.data(), n, 0, false) + tables;
+          bytes = batch_arena_bytes(&params, prs.data(), n, 0, false, false) + tables;
           if (pool.alloc(bytes, &off)) break;
         }
         if (can_recompute && ctx->volume_policy != SGM_VOLUME_RESIDENT) {
           recompute = 1;
-          bytes = batch_arena_bytes(&params, prs.data(), n, 0, true) + tables;
+          bytes = batch_arena_bytes(&params, prs.data(), n, 0, true, false) + tables;
           if (pool.alloc(bytes, &off)) break;
         }
         take.pop_back();
@@ -237,7 +247,7 @@ void sgm_server::run() {
       lk.unlock();
       sgm_batch *b = nullptr;
       int rc = batch_create(ctx, &params, prs.data(), (int)prs.size(), SGM_BATCH_PHASED,
-                            (uint8_t *)region + off, bytes, s.stream, &b, recompute);
+                            (uint8_t *)region + off, bytes, s.stream, &b, recompute, planes);
       if (rc == SGM_OK) rc = batch_launch(b);
       if (rc == SGM_OK) {
         hipError_t e = hipEventRecord(s.done, s.stream);

@@ -393,6 +393,61 @@ def test_batch_phased_bit_exact(gpu_context, graph, policy):
         b.close()
 
 
+def plane_scratch_bytes(pr):
+    """Scratch of one pair under path-cost planes: uint8 cost volume and the 7
+    plane rows per pixel, both matching directions, 256-byte aligned."""
+    h, w = pr["left"].shape
+    D = pr["disparities"]
+    R = 1 << max(0, (D - 1) // 64).bit_length()  # lane group
+    Dp = -(-D // max(R, 4)) * max(R, 4)           # padded disparities
+    a = lambda v: -(-v // 256) * 256
+    return 2 * (a(w * h * Dp) + a(w * h * Dp * 7))
+
+
+def test_batch_planes_in_sub_batches():
+    """Phase-major batches run the path-cost plane schedule; when the planes of
+    every pair do not fit the budget together, the pairs run in sub-batches that
+    reuse one scratch region.  Both equal the oracle; the sequential schedule
+    (the context's sweep setting) too."""
+    pairs = uneven_pairs(13, 6, wrange=(200, 420), drange=(64, 300))
+    params = to_sgm(O.params(**dict(CENSUS97, disparities=64)))
+    want = [oracle_of(CENSUS97, pr) for pr in pairs]
+    ctx = sgm.Context(0)
+    try:
+        b = sgm.Batch(ctx, params, pairs, phased=True)
+        d = b.describe()
+        full = d["device_bytes"]
+        assert d["sweeps"] == "planes" and d["sub_batches"] == 1, d
+        for i, r in enumerate(b.run()):
+            same(r, want[i], "planes pair %d" % i)
+        b.close()
+        ctx.sweeps("sequential")
+        b = sgm.Batch(ctx, params, pairs, phased=True)
+        assert b.describe()["sweeps"] == "sequential"
+        for i, r in enumerate(b.run()):
+            same(r, want[i], "sequential pair %d" % i)
+        b.close()
+    finally:
+        ctx.close()
+    sizes = [plane_scratch_bytes(pr) for pr in pairs]
+    total = sum(sizes)
+    assert max(sizes) < total // 2
+    budget = full - total // 2 + (8 << 20)
+    ctx = sgm.Context(0, budget_bytes=budget)
+    try:
+        b = sgm.Batch(ctx, params, pairs, phased=True, graph=True)
+        d = b.describe()
+        print("sub-batches", d, flush=True)
+        assert d["sweeps"] == "planes" and d["sub_batches"] >= 2, d
+        assert d["device_bytes"] < full
+        for rep in range(2):
+            for i, r in enumerate(b.run()):
+                same(r, want[i], "sub-batched pair %d rep %d" % (i, rep))
+        b.close()
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("kw", [
     dict(window_w=7, window_h=7, cost=3, p1=12, p2=100),
     dict(window_w=5, window_h=5, cost=2, p1=40, p2=9000),          # 32-bit aggregates
