@@ -34,16 +34,16 @@ def workload_params(sgm):
 def path_bytes_per_step(w, h, D, paths, schedule, cost_bytes=1):
     """HBM bytes this build's path sweeps move for one pair (both directions)
     with a uint8 cost volume C, by sweep schedule (the plan's `sweeps`).
-    Path-cost planes (3): the first sweep computes the census cost and writes C
-    and its plane (1 + 1 B/cell), the middle directions read C and write their
-    planes (1 + 1 B/cell each), the last sweep reads C and the P-1 planes and
-    selects (1 + 7 B/cell): 22 B per cell at 8 paths.  One uint16 aggregate S
+    Path-cost planes (3): the census cost pass writes C (1 B/cell), every
+    direction but the last reads C and writes its plane (1 + 1 B/cell each), the
+    last sweep reads C and the P-1 planes and selects (1 + 7 B/cell): 23 B per
+    cell at 8 paths.  One uint16 aggregate S
     (1, 2): first 1 + 2, middle 1 + 2 + 2 (a paired launch's second member 1 +
     2, and the last sweep then reads both partial sums), last 1 + 2: 36 B per
     cell at 8 paths either way."""
     cells = 2 * w * h * D
     if schedule == 3:
-        per_cell = (cost_bytes + 1) * (paths - 1) + cost_bytes + (paths - 1)
+        per_cell = cost_bytes + (cost_bytes + 1) * (paths - 1) + cost_bytes + (paths - 1)
     else:
         per_cell = (cost_bytes + 2) + (cost_bytes + 4) * (paths - 2) + (cost_bytes + 2)
     return cells * per_cell
@@ -527,10 +527,10 @@ def write_traffic(fetch_dir, write_dir, copy_bytes=2 << 30, paths=8):
     `bench.py --steps 1 --warmup 1 --profile-steps 1 --no-...` (FETCH_SIZE,
     then WRITE_SIZE, KiB per dispatch).  Each counter is scaled by what it
     reads for the library's copy kernel (copy_bytes read + copy_bytes written
-    per launch); path launches are grouped by kind (first sweep, middle
-    directions, last sweep), and the per-launch figure weights them as one pair
-    does under the schedule the passes ran (path-cost planes: one launch of
-    each kind; one aggregate: one first, P - 2 middle, one last)."""
+    per launch); path launches are grouped by kind, and the per-launch figure
+    weights them as one pair does under the schedule the passes ran (path-cost
+    planes: the census cost pass, the P - 1 directions of the planes, the last
+    sweep, one launch each; one aggregate: one first, P - 2 middle, one last)."""
     W, H, D = 1600, 1067, 256
     fetch, write = read_counter_csv(fetch_dir, "FETCH_SIZE"), read_counter_csv(write_dir, "WRITE_SIZE")
 
@@ -546,8 +546,8 @@ def write_traffic(fetch_dir, write_dir, copy_bytes=2 << 30, paths=8):
     cells = 2 * W * H * D
     planes = any("path_multi_kernel" in k for k, _ in fetch)
     if planes:
-        kinds = (("first", lambda k: "path_kernel" in k and ", 0, " in k, 2 * cells, 1),
-                 ("multi", lambda k: "path_multi_kernel" in k, 2 * (paths - 2) * cells, 1),
+        kinds = (("cost", lambda k: "path_kernel" in k and ", 3, " in k, cells, 1),
+                 ("multi", lambda k: "path_multi_kernel" in k, 2 * (paths - 1) * cells, 1),
                  ("last", lambda k: "path_kernel" in k and ", 2, " in k, paths * cells, 1))
     else:
         kinds = (("first", lambda k: "path_kernel" in k and ", 0, " in k, 3 * cells, 1),
@@ -659,7 +659,7 @@ def main():
     kernels = {k: round(v["total_ms"] / args.profile_steps, 4) for k, v in prof.items()}
     # each path launch of the schedule on its own: duration and the build's bytes
     cells = 2 * W * H * D
-    own_bytes = ({"path_first": 2, "path_multi": 2 * (P - 2), "path_last": 1 + (P - 1)}
+    own_bytes = ({"path_cost": 1, "path_multi": 2 * (P - 1), "path_last": 1 + (P - 1)}
                  if schedule == 3 else {"path_first": 3, "path_mid": 5, "path_pair": 8, "path_last": 3})
     path_kernels = {}
     for k in path_keys:

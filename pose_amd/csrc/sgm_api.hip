@@ -804,6 +804,7 @@ PathArgs path_args(const sgm_params *p, int dir, int njobs, int total_lines, int
   A.keep_sum = keep_sum;
   A.njobs = njobs;
   A.total_lines = total_lines;
+  A.split = 0;
   A.uniq = p->uniqueness;
   return A;
 }
@@ -814,10 +815,12 @@ int launch_path(sgm_context *ctx, int R, int sum_bytes, int mode, const PathJobs
                 const PathArgs &A, int phase, int waves, hipStream_t stream) {
   ProfileScope ps(ctx, phase == PHASE_FIRST ? "path_first"
                        : phase == PHASE_MID ? "path_mid"
-                                            : "path_last");
+                       : phase == PHASE_COST ? "path_cost"
+                                             : "path_last");
   if (A.total_lines <= 0) return SGM_OK;
   hipStream_t st = stream ? stream : ctx->stream;
   if (phase == PHASE_FIRST) return launch_path_first(st, R, sum_bytes, mode, jobs, A, waves);
+  if (phase == PHASE_COST) return launch_path_cost(st, R, sum_bytes, mode, jobs, A, waves);
   if (phase == PHASE_MID) return launch_path_mid(st, R, sum_bytes, mode, jobs, A, waves);
   return launch_path_last(st, R, sum_bytes, mode, jobs, A, waves);
 }
@@ -1819,31 +1822,34 @@ int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuff
     return true;
   };
   if (B.planes[slots[0]] && (mode & 7) == 4) {
-    // Path-cost planes: the first direction (fused census cost or the uint8
-    // cost volume) stores its path costs into plane 0, the middle directions
-    // run in one launch into planes 1..6, the last sweep adds the seven planes
-    // to its own path costs before the winner search.
+    // Path-cost planes: the census cost pass (fused census cost of the
+    // left-to-right lines; other costs come from the cost kernel above), every
+    // direction but the last in one launch, each into its plane 0..6, and the
+    // last sweep, which adds the seven planes to its own path costs before the
+    // winner search.
     ctx->last_sweeps = SGM_SWEEPS_PLANES;
-    PathJobs jf = pj;
-    for (int i = 0; i < nd; i++) jf.local[i].sum = B.planes[slots[i]];
     if (interrupted()) return fail(SGM_ERR_INTERRUPTED, "GPU SGM interrupted");
-    const int mf = (fused ? (2 | (mode & MODE_NT)) : mode) | MODE_PLANE;
-    if ((rc = launch_path(ctx, R, sb, mf, jf, dir_args(0), PHASE_FIRST, waves))) return rc;
+    if (fused) {
+      PathArgs Ac = dir_args(0);
+      Ac.split = h * cost_segments(w);
+      Ac.total_lines = nd * Ac.split;
+      if ((rc = launch_path(ctx, R, sb, 2 | (mode & MODE_NT), pj, Ac, PHASE_COST, waves))) return rc;
+    }
     MultiDir M;
     memset(&M, 0, sizeof M);
-    M.ndirs = p->directions - 2;
+    M.ndirs = p->directions - 1;
     for (int k = 0; k < M.ndirs; k++) {
-      const PathArgs Ak = dir_args(1 + k);
+      const PathArgs Ak = dir_args(k);
       M.dx[k] = Ak.dx;
       M.dy[k] = Ak.dy;
       M.cls[k] = Ak.cls;
       M.split[k] = Ak.split;
       M.begin[k] = M.total_lines;
       M.total_lines += Ak.total_lines;
-      for (int i = 0; i < nd; i++) M.plane[k][i] = B.planes[slots[i]] + (size_t)(1 + k) * Dp;
+      for (int i = 0; i < nd; i++) M.plane[k][i] = B.planes[slots[i]] + (size_t)k * Dp;
     }
     if (interrupted()) return fail(SGM_ERR_INTERRUPTED, "GPU SGM interrupted");
-    if ((rc = launch_path_multi(ctx, R, mode | MODE_PLANE, pj, dir_args(1), M, waves))) return rc;
+    if ((rc = launch_path_multi(ctx, R, mode | MODE_PLANE, pj, dir_args(0), M, waves))) return rc;
     PathJobs jl = pj;
     for (int i = 0; i < nd; i++) jl.local[i].sum = B.planes[slots[i]];
     if (interrupted()) return fail(SGM_ERR_INTERRUPTED, "GPU SGM interrupted");

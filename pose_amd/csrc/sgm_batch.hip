@@ -70,13 +70,13 @@ static int validate_pair(const sgm_params *p, const sgm_pair &pr, int flags) {
 }
 
 // Upper bound of the phase-major group tables (build_groups): the 2 * count
-// path jobs once, three line-prefix arrays per group, 256-byte alignment.  A
+// path jobs once, four line-prefix arrays per group, 256-byte alignment.  A
 // group (one lane group R of one sub-batch) holds at least one pair, so there
 // are at most `count` groups.
 static size_t group_table_bound(int count) {
   const size_t G = (size_t)count;
-  return align_up(2 * (size_t)count * sizeof(PathJob) + 3 * 4 * (2 * (size_t)count + G) +
-                      G * 4 * 256,
+  return align_up(2 * (size_t)count * sizeof(PathJob) + 4 * 4 * (2 * (size_t)count + G) +
+                      G * 5 * 256,
                   256);
 }
 
@@ -92,7 +92,7 @@ static int build_groups(sgm_batch *b, const std::vector<PathJob> &jobs, const sg
   const int sb = b->sum_bytes;
   std::vector<uint8_t> host;
   struct Pending {
-    size_t jobs_off, lb_off[3];
+    size_t jobs_off, lb_off[4];
   };
   std::vector<Pending> offs;
   b->groups.clear();
@@ -123,9 +123,12 @@ static int build_groups(sgm_batch *b, const std::vector<PathJob> &jobs, const sg
     pd.jobs_off = align_up(host.size(), 256);
     host.resize(pd.jobs_off + gj.size() * sizeof(PathJob));
     memcpy(host.data() + pd.jobs_off, gj.data(), gj.size() * sizeof(PathJob));
-    for (int cls = 0; cls < 3; cls++) {
+    for (int cls = 0; cls < 4; cls++) {
+      // line classes 0..2, and 3: the row segments of the census cost pass
       std::vector<int> lb(gj.size() + 1, 0);
-      for (size_t j = 0; j < gj.size(); j++) lb[j + 1] = lb[j] + lines_of(cls, gj[j].w, gj[j].h);
+      for (size_t j = 0; j < gj.size(); j++)
+        lb[j + 1] = lb[j] + (cls == 3 ? gj[j].h * cost_segments(gj[j].w)
+                                      : lines_of(cls, gj[j].w, gj[j].h));
       g.total_lines[cls] = lb.back();
       pd.lb_off[cls] = align_up(host.size(), 256);
       host.resize(pd.lb_off[cls] + lb.size() * sizeof(int));
@@ -152,7 +155,7 @@ static int build_groups(sgm_batch *b, const std::vector<PathJob> &jobs, const sg
   if (e != hipSuccess) return hip_fail(e, "batch group table upload");
   for (size_t i = 0; i < b->groups.size(); i++) {
     b->groups[i].jobs = (const PathJob *)(base + offs[i].jobs_off);
-    for (int cls = 0; cls < 3; cls++) b->groups[i].line_begin[cls] = (const int *)(base + offs[i].lb_off[cls]);
+    for (int cls = 0; cls < 4; cls++) b->groups[i].line_begin[cls] = (const int *)(base + offs[i].lb_off[cls]);
   }
   b->phased = true;
   return SGM_OK;
@@ -186,8 +189,8 @@ static int phased_enqueue(sgm_batch *b) {
   const int sb = b->sum_bytes;
   if (b->planes) {
     // path-cost planes, sub-batch after sub-batch (one scratch region): per
-    // lane group the first direction (fused census cost, plane 0), the middle
-    // directions in one launch (planes 1..6), the last sweep over the planes
+    // lane group the census cost pass, every direction but the last in one
+    // launch (planes 0..6), the last sweep over the planes
     for (const sgm_batch::Group &g : b->groups) {
       const int nt = g.nt ? MODE_NT : 0;
       auto args = [&](int dir) {
@@ -198,16 +201,17 @@ static int phased_enqueue(sgm_batch *b) {
       PathJobs pj;
       memset(&pj, 0, sizeof pj);
       pj.table = g.jobs;
-      const PathArgs A0 = args(0);
-      pj.line_begin = g.line_begin[A0.cls];
-      if ((rc = launch_path(ctx, g.R, sb, 2 | nt | MODE_PLANE, pj, A0, PHASE_FIRST,
-                            path_waves(ctx, g.R), b->stream)))
+      PathArgs Ac = args(0);
+      Ac.total_lines = g.total_lines[3];
+      pj.line_begin = g.line_begin[3];  // row segments of the cost pass
+      if ((rc = launch_path(ctx, g.R, sb, 2 | nt, pj, Ac, PHASE_COST, path_waves(ctx, g.R),
+                            b->stream)))
         return rc;
       MultiDir M;
       memset(&M, 0, sizeof M);
-      M.ndirs = p->directions - 2;
+      M.ndirs = p->directions - 1;
       for (int k = 0; k < M.ndirs; k++) {
-        const PathArgs Ak = args(1 + k);
+        const PathArgs Ak = args(k);
         M.dx[k] = Ak.dx;
         M.dy[k] = Ak.dy;
         M.cls[k] = Ak.cls;
@@ -216,7 +220,8 @@ static int phased_enqueue(sgm_batch *b) {
       }
       for (int cls = 0; cls < 3; cls++) M.line_begin[cls] = g.line_begin[cls];
       M.njobs = g.njobs;
-      if ((rc = launch_path_multi(ctx, g.R, 4 | nt | MODE_PLANE, pj, args(1), M,
+      M.plane0 = 0;
+      if ((rc = launch_path_multi(ctx, g.R, 4 | nt | MODE_PLANE, pj, args(0), M,
                                   path_waves(ctx, g.R), b->stream)))
         return rc;
       const PathArgs AL = args(p->directions - 1);

@@ -67,8 +67,9 @@ struct sgm_batch {
     int R = 0, njobs = 0, sub = 0;
     bool nt = false;
     const sgm::PathJob *jobs = nullptr;
-    const int *line_begin[3] = {nullptr, nullptr, nullptr};
-    int total_lines[3] = {0, 0, 0};
+    // per line class (horizontal, vertical, diagonal, cost-pass row segments)
+    const int *line_begin[4] = {nullptr, nullptr, nullptr, nullptr};
+    int total_lines[4] = {0, 0, 0, 0};
   };
   bool phased = false;
   // path-cost planes: the scratch region of a sub-batch's cost volumes and

@@ -118,6 +118,8 @@ int launch_path_mid(hipStream_t st, int R, int sum_bytes, int mode, const PathJo
                     const PathArgs &A, int waves);
 int launch_path_last(hipStream_t st, int R, int sum_bytes, int mode, const PathJobs &jobs,
                      const PathArgs &A, int waves);
+int launch_path_cost(hipStream_t st, int R, int sum_bytes, int mode, const PathJobs &jobs,
+                     const PathArgs &A, int waves);
 int path_lanes(const sgm_params *p);
 bool fused_cost(const sgm_params *p);
 int launch_lr(sgm_context *ctx, const LrArgs &A);

@@ -35,6 +35,22 @@ static int launch_path_phase(hipStream_t st, int R, int sum_bytes, int mode, con
     else if (mode == (1 | MODE_NT)) launch_path_t<RR, uint16_t, 1 | MODE_NT, PH>(st, jobs, A, waves); \
     else launch_path_t<RR, uint16_t, 1, PH>(st, jobs, A, waves);        \
     break;
+  if constexpr (PH == PHASE_COST) {
+    // the census cost pass (fused census cost, packed lane groups 2, 4, 8)
+    if (sum_bytes != 2 || (mode & ~MODE_NT) != 2)
+      return fail(SGM_ERR_INVALID_ARGUMENT, "the cost pass takes the fused census cost");
+    const bool nt = (mode & MODE_NT) != 0;
+    switch (R) {
+      case 2: nt ? launch_path_t<2, uint16_t, 2 | MODE_NT, PH>(st, jobs, A, waves)
+                 : launch_path_t<2, uint16_t, 2, PH>(st, jobs, A, waves); break;
+      case 4: nt ? launch_path_t<4, uint16_t, 2 | MODE_NT, PH>(st, jobs, A, waves)
+                 : launch_path_t<4, uint16_t, 2, PH>(st, jobs, A, waves); break;
+      case 8: nt ? launch_path_t<8, uint16_t, 2 | MODE_NT, PH>(st, jobs, A, waves)
+                 : launch_path_t<8, uint16_t, 2, PH>(st, jobs, A, waves); break;
+      default: return fail(SGM_ERR_INVALID_ARGUMENT, "unsupported lane group %d", R);
+    }
+    return launch_check("census cost pass");
+  } else {
   if (mode & MODE_PLANE) {
     // path-cost planes (packed lane groups 2, 4, 8): first-phase sweeps store
     // their plane (fused census cost or uint8 cost volume), last sweeps add them
@@ -106,6 +122,7 @@ static int launch_path_phase(hipStream_t st, int R, int sum_bytes, int mode, con
 #undef CASE_WIDE
 #undef CASE_PK
   return launch_check("path kernel");
+  }
 }
 
 }  // namespace sgm

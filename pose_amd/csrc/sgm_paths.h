@@ -63,7 +63,14 @@ struct PathArgs {
   double uniq;
 };
 
-enum { PHASE_FIRST = 0, PHASE_MID = 1, PHASE_LAST = 2 };
+// Row segment of the census cost pass: one wave per kCostSegment pixels of a
+// row, so the pass runs many more waves than rows.
+constexpr int kCostSegment = 128;
+__host__ __device__ inline int cost_segments(int w) { return (w + kCostSegment - 1) / kCostSegment; }
+
+// PHASE_COST: the fused census cost of a left-to-right line, written to the
+// cost volume, with no path recurrence (the cost pass of the plane schedule).
+enum { PHASE_FIRST = 0, PHASE_MID = 1, PHASE_LAST = 2, PHASE_COST = 3 };
 
 // Selection code bits (one byte per pixel, counted by the L-R / count kernels).
 enum { CODE_SELECTED = 1, CODE_BOUNDARY = 2, CODE_AMBIGUOUS = 3, CODE_FULL = 4, CODE_LOWER = 8,
@@ -841,7 +848,9 @@ struct PkSweep {
   static constexpr bool kPlaneIn = PLANE && PHASE == PHASE_LAST;    // reads the planes
   static constexpr bool kPlaneOut = PLANE && PHASE == PHASE_FIRST;  // writes its plane
   static constexpr int NB = (R + 3) / 4;                             // dwords of R bytes
-  static_assert(!PLANE || PHASE != PHASE_MID, "plane sweeps are first-phase or last");
+  static_assert(!PLANE || (PHASE != PHASE_MID && PHASE != PHASE_COST),
+                "plane sweeps are first-phase or last");
+  static_assert(PHASE != PHASE_COST || CM == CM_FUSED, "the cost pass computes the census cost");
 
   const PathJob &J;
   const PathArgs &A;
@@ -880,6 +889,13 @@ struct PkSweep {
     loff = kbase < J.D ? kbase * 2 : kOutOfRange;
     boff = kbase < J.D ? kbase : kOutOfRange;  // uint8 cost volume
     line_start(line, A.dx, A.dy, J.w, J.h, x0, y0, len);
+    if constexpr (PHASE == PHASE_COST) {
+      // a left-to-right row segment: line = row * segments + segment
+      const int nseg = cost_segments(J.w);
+      y0 = line / nseg;
+      x0 = (line - y0 * nseg) * kCostSegment;
+      len = min(kCostSegment, J.w - x0);
+    }
     step_off = (ptrdiff_t)A.dy * J.w + A.dx;
     p0 = (ptrdiff_t)y0 * J.w + x0;
     row_bytes = (uint32_t)J.Dp * 2u;
@@ -970,7 +986,7 @@ struct PkSweep {
       const __amdgpu_buffer_rsrc_t r = make_rsrc(ld_s, in ? (uint32_t)(kPlanes * J.Dp) : 0u);
 #pragma unroll
       for (int k = 0; k < kPlanes; k++) bload_bytes<R, POL>(pl[slot][k], r, poff + k * J.Dp);
-    } else if constexpr (PHASE != PHASE_FIRST) {
+    } else if constexpr (PHASE == PHASE_MID || PHASE == PHASE_LAST) {
       bload_pairs<NP, POL>(sb[slot], make_rsrc(ld_s, in ? row_bytes : 0u), loff);
     }
     if constexpr (SUM2) {
@@ -1044,6 +1060,7 @@ struct PkSweep {
         bstore_bytes<R, POL>(raw, make_rsrc(st_c, cost_row_bytes), boff);
         st_c += cstep;
       }
+      if constexpr (PHASE == PHASE_COST) return;  // the cost pass: no recurrence
     } else if constexpr (CM == CM_LOAD8) {
       widen_costs<NP>(cb[slot], c);
     } else {
@@ -1340,15 +1357,16 @@ __device__ __forceinline__ void sweep_dispatch(const PathJob &J, const PathArgs 
 // [begin[i], begin[i + 1]) sweep direction i; the first split[i] of them are
 // matching direction 0's.  The directions are independent (each reads only
 // the cost volume), so one launch carries all their lines.
-constexpr int kMultiDirs = 6;
+constexpr int kMultiDirs = 7;
 struct MultiDir {
   int ndirs, total_lines;
+  int plane0;  // table jobs: plane index of direction 0 of the launch
   int dx[kMultiDirs], dy[kMultiDirs], cls[kMultiDirs], split[kMultiDirs];
   int begin[kMultiDirs];
   void *plane[kMultiDirs][2];  // jobs passed by value: each direction's plane per job
   // jobs in a device table (batches): the first line of each job per line
-  // class, and the job count; a job's `sum` points at its plane block, plane
-  // 1 + i of direction i lies i + 1 rows of Dp bytes further
+  // class, and the job count; a job's `sum` points at its plane block, the
+  // plane of direction i lies plane0 + i rows of Dp bytes further
   const int *line_begin[3];
   int njobs;
 };
@@ -1386,7 +1404,7 @@ __global__ __launch_bounds__(256) void path_multi_kernel(PathJobs jobs, PathArgs
     const int *lb = B.cls == 0 ? M.line_begin[0] : B.cls == 1 ? M.line_begin[1] : M.line_begin[2];
     const int ji = __builtin_amdgcn_readfirstlane(find_job(lb, M.njobs, local));
     PathJob J = jobs.table[ji];
-    J.sum = (uint8_t *)J.sum + (size_t)(1 + dir) * J.Dp;
+    J.sum = (uint8_t *)J.sum + (size_t)(M.plane0 + dir) * J.Dp;
     sweep_dispatch<R, uint16_t, PHASE_FIRST, MODE>(J, B, local - lb[ji], p2_table);
   } else if (A.njobs > 1 && local >= split) {
     PathJob J = jobs.local[1];
