@@ -1349,7 +1349,10 @@ __device__ __forceinline__ void sweep_dispatch(const PathJob &J, const PathArgs 
   else if constexpr (M == 2)
     sweep_line_pk<R, PHASE, CM_FUSED, 64, NT, false, PLANE>(J, A, line, p2_table);
   else if constexpr (M == 3) sweep_line_pk<R, PHASE, CM_RECOMPUTE>(J, A, line, p2_table);
-  else sweep_line_pk<R, PHASE, CM_LOAD8, 64, NT, SUM2, PLANE>(J, A, line, p2_table);
+  // the last sweep over the planes prefetches 4 steps (7 plane dwords and a
+  // cost dword per step): fewer VGPRs, more resident waves (measured faster)
+  else sweep_line_pk<R, PHASE, CM_LOAD8, (PLANE && PHASE == PHASE_LAST) ? 4 : 64, NT, SUM2, PLANE>(
+      J, A, line, p2_table);
 }
 
 // Up to kMultiDirs path directions in one launch, each a first-phase sweep
