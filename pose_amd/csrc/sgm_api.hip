@@ -692,13 +692,14 @@ extern "C" int sgm_profile_json(sgm_context *ctx, char *buffer, size_t length) {
   static const char *sweep_names[4] = {"auto", "sequential", "paired", "planes"};
   snprintf(b, sizeof b,
            "},\"allocated_bytes\":%lld,\"budget_bytes\":%lld,\"last_plan\":{\"plan\":\"%s\","
-           "\"storage\":\"%s\",\"block_rows\":%d,\"sweeps\":\"%s\"}}",
+           "\"storage\":\"%s\",\"block_rows\":%d,\"sweeps\":\"%s\",\"verify_groups\":%d}}",
            (long long)ctx->allocated, (long long)ctx->budget,
            ctx->last_compact ? "compact" : ctx->last_streamed ? "streamed" : "resident",
            ctx->last_streamed ? storages[ctx->splan.storage & 3] : "none",
            ctx->last_streamed ? ctx->splan.block_rows : 0,
            ctx->last_compact || ctx->last_streamed ? "sequential"
-                                                   : sweep_names[ctx->last_sweeps & 3]);
+                                                   : sweep_names[ctx->last_sweeps & 3],
+           ctx->last_compact ? ctx->esgm_groups : 0);
   s += b;
   if (s.size() + 1 > length) return fail(SGM_ERR_INVALID_ARGUMENT, "buffer too small (%zu)", s.size() + 1);
   memcpy(buffer, s.c_str(), s.size() + 1);
@@ -1675,6 +1676,7 @@ int run_directions_esgm(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "compact verification count");
   for (int k = 0; k < 4; k++) ctx->esgm_last[k] = 0;
+  ctx->esgm_groups = 0;
   int most = 0;
   for (int i = 0; i < nd; i++) {
     ctx->esgm_last[2 * slots[i]] = (int64_t)n - counts[i];
@@ -1682,10 +1684,14 @@ int run_directions_esgm(sgm_context *ctx, const sgm_params *p, int w, int h, Pai
     most = std::max(most, counts[i]);
   }
   if (most > 0) {
-    // verification groups: the budget left by the compact plan, at least one pixel
+    // verification groups: the budget left by the compact plan, at least one
+    // pixel.  Each group sweeps every path over the whole image again, so the
+    // verification time grows with the group count (reported in the profile
+    // as `verify_groups`); the reference verifies in one streamed pass.
     const int64_t room = ctx->budget - esgm_pair_bytes(p, w, h, rows, pool) + 2 * Dp * 2;
     int64_t group = std::max<int64_t>(1, room / ((int64_t)nd * Dp * 2));
     group = std::min<int64_t>(group, most);
+    ctx->esgm_groups = (int)((most + group - 1) / group);
     const size_t vbytes = (size_t)nd * group * Dp * 2;
     if ((rc = ctx->buffers["esgm_verify"].ensure(ctx, vbytes, "esgm_verify"))) return rc;
     uint16_t *verify = (uint16_t *)ctx->buffers["esgm_verify"].ptr;

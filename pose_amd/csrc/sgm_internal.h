@@ -141,6 +141,7 @@ struct sgm_context {
   bool last_streamed = false;
   int esgm_candidates = 2;            // Options.esgmCandidates default
   int64_t esgm_last[4] = {0, 0, 0, 0};  // certified / verified pixels per matching direction
+  int esgm_groups = 0;  // verification groups of the last compact-plan call (0: none needed)
   // streamed plan: copy stream and the events of the two strip aggregate slots
   // (0-1: slot computed, 2-3: slot copied), created on first use
   hipStream_t copy_stream = nullptr;

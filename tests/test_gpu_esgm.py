@@ -86,6 +86,9 @@ def test_verification_in_small_groups():
         got, counts = esgm(ctx, p, left, right, rows=64, pool=2, what="small groups")
         assert counts["left"][1] > 6 and counts["right"][1] > 6
         assert_same(got, want, "small verification groups")
+        # groups of 6 pixels (the budget's room), reported by the profile
+        most = max(counts["left"][1], counts["right"][1])
+        assert ctx.profile()["last_plan"]["verify_groups"] == -(-most // 6)
         # GpuStereo.limit under the compact plan: the largest range it serves
         ctx.plan_mode(sgm.PLAN_ESGM)
         try:
