@@ -205,10 +205,13 @@ int lanes_group(int D) {
   return R;
 }
 
-int padded_disparities(int D) {
-  const int R = lanes_group(D), a = std::max(R, 4);
-  return (D + a - 1) / a * a;
-}
+// Disparities per pixel row of the volumes: D rounded up to a multiple of 64,
+// so every row is whole 64-byte HBM write granules (a uint8 row) and the sweeps
+// store whole rows, pad cells included.  Rows of D bytes would leave granules
+// part-written, which HBM completes by read-modify-write (measured: 1.23x over
+// rows of D rounded to 4 for the batched-distribution pairs, D 122-356).
+// Never above one wave's 64 * lanes_group(D) disparities.
+int padded_disparities(int D) { return (D + 63) / 64 * 64; }
 
 int words_of(const sgm_params *p) { return (p->window_w * p->window_h - 1 + 63) / 64; }
 
@@ -1754,10 +1757,10 @@ int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuff
   const bool fused = !recompute && fused_cost(p);
   int mode = recompute ? 3 : sb == 2 ? (cost_u8(p) ? 4 : 1) : 0;
   // stream the uint8 cost and uint16 aggregate volumes past the caches when
-  // their rows are whole 64-byte lines and they far exceed the 256 MB MALL
-  // (measured: +5% at 1600x1067 D=256, -28% at 2000x1500 D=100)
+  // they far exceed the 256 MB MALL (measured: +5% at 1600x1067 D=256); rows
+  // are whole 64-byte granules (padded disparities), which streaming needs
   const int vol_bytes = mode == 4 ? 3 : 4;  // cost + aggregate bytes per cell
-  const bool nt = (mode == 4 || mode == 1) && Dp % 64 == 0 && R <= 8 &&
+  const bool nt = (mode == 4 || mode == 1) && R <= 8 &&
                   (int64_t)nd * w * h * Dp * vol_bytes > (int64_t)768 << 20;
   if (nt) mode |= MODE_NT;
   if (!fused && !recompute) {

@@ -544,7 +544,7 @@ __device__ __forceinline__ void sweep_line(const PathJob &J, const PathArgs &A, 
 #pragma unroll
         for (int j = 0; j < R; j++) S[j] = SumVec<R, SumT>::wrap(sb[jj].get(j) + L[j]);
       }
-      if (lane_mem && (PHASE != PHASE_LAST || A.keep_sum))
+      if (kbase < Dp && (PHASE != PHASE_LAST || A.keep_sum))  // whole rows, pads included
         SumVec<R, SumT>::store(sum + pp * Dp + kbase, S);
       if (s + PF < len && lane_mem) {
         const ptrdiff_t pn = pp + (ptrdiff_t)PF * step_off;
@@ -856,6 +856,7 @@ struct PkSweep {
   const PathArgs &A;
   const int *p2_table;
   int lane, kbase, loff, boff, x0, y0, len;
+  int sloff, sboff;  // store offsets: every lane inside the padded row
   bool slide;  // computed cost on a horizontal line: sliding window
   ptrdiff_t p0, step_off;
   uint32_t row_bytes, cost_row_bytes;
@@ -888,6 +889,10 @@ struct PkSweep {
     kbase = lane * R;
     loff = kbase < J.D ? kbase * 2 : kOutOfRange;
     boff = kbase < J.D ? kbase : kOutOfRange;  // uint8 cost volume
+    // stores cover the pad cells too (saturated values that no load reads):
+    // whole rows keep HBM writes to whole granules
+    sloff = kbase < J.Dp ? kbase * 2 : kOutOfRange;
+    sboff = kbase < J.Dp ? kbase : kOutOfRange;
     line_start(line, A.dx, A.dy, J.w, J.h, x0, y0, len);
     if constexpr (PHASE == PHASE_COST) {
       // a left-to-right row segment: line = row * segments + segment
@@ -1057,7 +1062,7 @@ struct PkSweep {
       if constexpr (CM == CM_FUSED) {
         uint32_t raw[(R + 3) / 4];
         narrow_costs<NP>(c, raw);
-        bstore_bytes<R, POL>(raw, make_rsrc(st_c, cost_row_bytes), boff);
+        bstore_bytes<R, POL>(raw, make_rsrc(st_c, cost_row_bytes), sboff);
         st_c += cstep;
       }
       if constexpr (PHASE == PHASE_COST) return;  // the cost pass: no recurrence
@@ -1112,9 +1117,9 @@ struct PkSweep {
       // host); other cells saturate at 255 and are never selected
       uint32_t raw[NB];
       narrow_costs<NP>(L, raw);
-      bstore_bytes<R, POL>(raw, make_rsrc(st_s, (uint32_t)sum_store_bytes), boff);
+      bstore_bytes<R, POL>(raw, make_rsrc(st_s, (uint32_t)sum_store_bytes), sboff);
     } else if constexpr (!kPlaneIn) {
-      bstore_pairs<NP, POL>(S, make_rsrc(st_s, (uint32_t)sum_store_bytes), loff);
+      bstore_pairs<NP, POL>(S, make_rsrc(st_s, (uint32_t)sum_store_bytes), sloff);
     }
     st_s += sstep;
     if constexpr (PHASE == PHASE_LAST) {

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <deque>
@@ -90,6 +91,12 @@ struct Request {
   int batch_id = -1;
 };
 
+// Offsets of one request's host inputs and outputs in its slot's pinned
+// staging buffer (SIZE_MAX: not requested).
+struct Staged {
+  size_t left, right, vl, vr, checked, dl, dr, reasons, boundary;
+};
+
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
@@ -98,7 +105,33 @@ struct Slot {
   std::vector<Request *> reqs;
   size_t off = 0, bytes = 0;
   bool busy = false;
+  // batching mode: pinned staging of the batch's host inputs and outputs, so
+  // the batch's copies run asynchronously on the slot's stream
+  sgm::HostBuffer staging;
+  std::vector<Staged> staged;
 };
+
+// Lays out a request's host buffers in a staging buffer from offset `at`.
+static Staged stage_layout(const sgm_pair &pr, size_t &at) {
+  const size_t n = (size_t)pr.width * pr.height;
+  auto take = [&](bool used, size_t bytes) {
+    if (!used) return SIZE_MAX;
+    const size_t o = at;
+    at = (at + bytes + 255) / 256 * 256;
+    return o;
+  };
+  Staged st;
+  st.left = take(true, n);
+  st.right = take(true, n);
+  st.vl = take(pr.valid_left != nullptr, n);
+  st.vr = take(pr.valid_right != nullptr, n);
+  st.checked = take(true, 4 * n);
+  st.dl = take(pr.dl != nullptr, 4 * n);
+  st.dr = take(pr.dr != nullptr, 4 * n);
+  st.reasons = take(pr.reasons != nullptr, n);
+  st.boundary = take(pr.boundary != nullptr, n);
+  return st;
+}
 
 }  // namespace
 
@@ -151,6 +184,18 @@ void sgm_server::run() {
       } else {
         rc = batch_finish(s.batch);
         sgm_batch_destroy(s.batch);
+        // outputs from the pinned staging to the callers' buffers
+        const uint8_t *sb = (const uint8_t *)s.staging.ptr;
+        for (size_t i = 0; rc == SGM_OK && i < s.reqs.size(); i++) {
+          const sgm_pair &pr = s.reqs[i]->pair;
+          const Staged &st = s.staged[i];
+          const size_t n = (size_t)pr.width * pr.height;
+          memcpy(pr.checked, sb + st.checked, 4 * n);
+          if (pr.dl) memcpy(pr.dl, sb + st.dl, 4 * n);
+          if (pr.dr) memcpy(pr.dr, sb + st.dr, 4 * n);
+          if (pr.reasons) memcpy(pr.reasons, sb + st.reasons, n);
+          if (pr.boundary) memcpy(pr.boundary, sb + st.boundary, n);
+        }
       }
       lk.lock();
       const double t = now();
@@ -245,9 +290,39 @@ void sgm_server::run() {
       const int id = next_batch++;
       (void)id;
       lk.unlock();
+      // host inputs into the slot's pinned staging; the batch copies from and
+      // to it, so its transfers do not block this thread
+      size_t staged_bytes = 0;
+      s.staged.clear();
+      for (const sgm_pair &pr : prs) s.staged.push_back(stage_layout(pr, staged_bytes));
+      int rc = s.staging.ensure(staged_bytes, "serving staging");
+      if (rc == SGM_OK) {
+        uint8_t *sb = (uint8_t *)s.staging.ptr;
+        for (size_t i = 0; i < prs.size(); i++) {
+          sgm_pair &pr = prs[i];
+          const Staged &st = s.staged[i];
+          const size_t n = (size_t)pr.width * pr.height;
+          memcpy(sb + st.left, pr.left, n);
+          memcpy(sb + st.right, pr.right, n);
+          pr.left = sb + st.left;
+          pr.right = sb + st.right;
+          if (pr.valid_left) {
+            memcpy(sb + st.vl, pr.valid_left, n);
+            memcpy(sb + st.vr, pr.valid_right, n);
+            pr.valid_left = sb + st.vl;
+            pr.valid_right = sb + st.vr;
+          }
+          pr.checked = (float *)(sb + st.checked);
+          if (pr.dl) pr.dl = (float *)(sb + st.dl);
+          if (pr.dr) pr.dr = (float *)(sb + st.dr);
+          if (pr.reasons) pr.reasons = sb + st.reasons;
+          if (pr.boundary) pr.boundary = sb + st.boundary;
+        }
+      }
       sgm_batch *b = nullptr;
-      int rc = batch_create(ctx, &params, prs.data(), (int)prs.size(), SGM_BATCH_PHASED,
-                            (uint8_t *)region + off, bytes, s.stream, &b, recompute, planes);
+      if (rc == SGM_OK)
+        rc = batch_create(ctx, &params, prs.data(), (int)prs.size(), SGM_BATCH_PHASED,
+                          (uint8_t *)region + off, bytes, s.stream, &b, recompute, planes);
       if (rc == SGM_OK) rc = batch_launch(b);
       if (rc == SGM_OK) {
         hipError_t e = hipEventRecord(s.done, s.stream);
@@ -532,6 +607,7 @@ extern "C" int sgm_server_destroy(sgm_server *srv) {
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.lane) (void)sgm_close(sl.lane);
     else if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    sl.staging.release();
   }
   for (auto &kv : srv->all) delete kv.second;
   if (srv->region) {

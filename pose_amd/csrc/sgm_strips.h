@@ -193,7 +193,8 @@ __device__ __forceinline__ void strip_segment(const StripJob &J, const PathArgs 
 #pragma unroll
         for (int j = 0; j < R; j++) S[j] = SumVec<R, SumT>::wrap(sb[jj].get(j) + L[j]);
       }
-      if (lane_mem && PHASE != PHASE_LAST) SumVec<R, SumT>::store(sum + pp * Dp + kbase, S);
+      // whole rows, pads included
+      if (kbase < Dp && PHASE != PHASE_LAST) SumVec<R, SumT>::store(sum + pp * Dp + kbase, S);
       if (s + PF < len && lane_mem) {
         const ptrdiff_t pn = pp + (ptrdiff_t)PF * step_off;
         cb[jj].load(cost + pn * Dp + kbase);

@@ -398,8 +398,7 @@ def plane_scratch_bytes(pr):
     plane rows per pixel, both matching directions, 256-byte aligned."""
     h, w = pr["left"].shape
     D = pr["disparities"]
-    R = 1 << max(0, (D - 1) // 64).bit_length()  # lane group
-    Dp = -(-D // max(R, 4)) * max(R, 4)           # padded disparities
+    Dp = -(-D // 64) * 64  # padded disparities: whole 64-byte rows
     a = lambda v: -(-v // 256) * 256
     return 2 * (a(w * h * Dp) + a(w * h * Dp * 7))
 

@@ -216,7 +216,8 @@ def test_gpu_stereo_report_keys_and_storage():
             seen[mode] = r
         assert seen["auto"]["storage"] == "resident"
         assert seen["streamed"]["storage"] == "streamed_ram"
-        assert seen["streamed"]["aggregate_ram_bytes"] == 2 * 640 * 480 * 96 * 2
+        # both directions' uint16 aggregates, rows of 96 disparities padded to 128
+        assert seen["streamed"]["aggregate_ram_bytes"] == 2 * 640 * 480 * 128 * 2
         assert seen["esgm"]["storage"] == "compact_with_sparse_exact_verification"
         assert seen["esgm"]["candidates_per_path"] == 2
         assert seen["esgm"]["estimated_device_bytes"] < seen["auto"]["estimated_device_bytes"]
