@@ -177,15 +177,37 @@ static int queue_launch(sgm_batch *b, int lo, int hi, unsigned *head, const char
   return e == hipSuccess ? SGM_OK : hip_fail(e, "batch kernel launch");
 }
 
-// Phase-major run: census (and cost) items, the sweeps direction by direction
-// and lane group by lane group with the specialised sweep kernel, the L-R items.
+// Census (lr = false) or L-R (lr = true) phase of a phase-major run as one
+// regular grid over every image or pair: 64-column tiles of row bands, four
+// per block (a queue item is a whole row band, too few waves for a few pairs).
+static int grid_launch(sgm_batch *b, bool lr) {
+  int w = 1, h = 1;
+  for (const sgm_pair &pr : b->pairs) {
+    w = std::max(w, pr.width);
+    h = std::max(h, pr.height);
+  }
+  const int rows = lr ? kGridLrRows : kBatchCensusRows;
+  const unsigned n = (unsigned)b->pairs.size() * (lr ? 1u : 2u);
+  dim3 grid((unsigned)((w + 64 * kGridWaves - 1) / (64 * kGridWaves)), (unsigned)((h + rows - 1) / rows), n);
+  ProfileScope ps(b->ctx, lr ? "batch lr" : "batch census");
+  if (lr) batch_lr_grid<<<grid, 64 * kGridWaves, 0, b->stream>>>(b->args);
+  else if (b->words == 1) batch_census_grid<1><<<grid, 64 * kGridWaves, 0, b->stream>>>(b->args);
+  else batch_census_grid<2><<<grid, 64 * kGridWaves, 0, b->stream>>>(b->args);
+  return launch_check(lr ? "batch lr grid" : "batch census grid");
+}
+
+// Phase-major run: the census grid (and the cost items), the sweeps direction
+// by direction and lane group by lane group with the specialised sweep kernel,
+// the L-R grid.
 static int phased_enqueue(sgm_batch *b) {
   sgm_context *ctx = b->ctx;
   const sgm_params *p = &b->params;
   const std::vector<int> &pb = b->phase_begin;
-  const int last_path = BPH_PATH0 + p->directions - 1;
   int rc;
-  if ((rc = queue_launch(b, 0, pb[BPH_PATH0], b->args.head, "batch census"))) return rc;
+  if ((rc = grid_launch(b, false))) return rc;
+  // cost items (non-fused costs) after the census grid: their waits on the
+  // census phase pass at once, an earlier launch having run it
+  if ((rc = queue_launch(b, pb[BPH_COST], pb[BPH_PATH0], b->args.head, "batch cost"))) return rc;
   const int sb = b->sum_bytes;
   if (b->planes) {
     // path-cost planes, sub-batch after sub-batch (one scratch region): per
@@ -230,7 +252,7 @@ static int phased_enqueue(sgm_batch *b) {
                             path_waves(ctx, g.R), b->stream)))
         return rc;
     }
-    return queue_launch(b, pb[last_path + 1], pb[b->args.nph], b->head2, "batch lr");
+    return grid_launch(b, true);
   }
   const int base_mode = b->recompute ? 3 : sb == 2 ? (b->cost8 ? 4 : 1) : 0;
   for (int dir = 0; dir < p->directions; dir++) {
@@ -248,7 +270,7 @@ static int phased_enqueue(sgm_batch *b) {
         return rc;
     }
   }
-  return queue_launch(b, pb[last_path + 1], pb[b->args.nph], b->head2, "batch lr");
+  return grid_launch(b, true);
 }
 
 static int batch_enqueue(sgm_batch *b) {
@@ -269,6 +291,10 @@ static int batch_enqueue(sgm_batch *b) {
       if (e != hipSuccess) return hip_fail(e, "batch upload");
     }
   }
+  if (b->gate_wait) {
+    hipError_t e = hipStreamWaitEvent(st, b->gate_wait, 0);
+    if (e != hipSuccess) return hip_fail(e, "batch gate wait");
+  }
   {
     const size_t words = b->control_words;
     fill_u32_kernel<<<(unsigned)((words + 255) / 256), 256, 0, st>>>(b->d_control, 0u, words);
@@ -284,6 +310,10 @@ static int batch_enqueue(sgm_batch *b) {
     hipError_t e = hipLaunchKernel(batch_kernel_of(b), dim3(blocks), dim3(threads), kargs, 0, st);
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
+  }
+  if (b->gate_done) {
+    hipError_t e = hipEventRecord(b->gate_done, st);
+    if (e != hipSuccess) return hip_fail(e, "batch gate record");
   }
   if (!(b->flags & SGM_BATCH_DEVICE)) {
     for (size_t i = 0; i < b->pairs.size(); i++) {
@@ -357,7 +387,6 @@ size_t batch_arena_bytes(const sgm_params *params, const sgm_pair *pairs, int co
   const bool zncc = params->cost == SGM_COST_ZNCC || params->cost == SGM_COST_ZNCC_FP32;
   Carver cv;
   const int nph = BPH_PATH0 + params->directions + 1, stride = 2 * count + 1;
-  cv.take(4);
   cv.take(4);
   cv.take(4);
   cv.take((size_t)nph * stride * 4);
@@ -513,7 +542,7 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
   const int nph = BPH_PATH0 + params->directions + 1;
   const int stride = 2 * count + 1;
   const size_t done_words = (size_t)nph * stride;
-  const size_t ctrl_head = cv.take(4), ctrl_error = cv.take(4), ctrl_head2 = cv.take(4);
+  const size_t ctrl_head = cv.take(4), ctrl_error = cv.take(4);
   const size_t ctrl_done = cv.take(done_words * 4);
   const size_t ctrl_stats = cv.take((size_t)count * 16 * 8);
   const size_t ctrl_end = cv.off;
@@ -585,7 +614,6 @@ int batch_create(sgm_context *ctx, const sgm_params *params, const sgm_pair *pai
   }
   uint8_t *base = (uint8_t *)b->arena;
   b->d_control = (unsigned *)(base + ctrl_head);
-  b->head2 = (unsigned *)(base + ctrl_head2);
   b->control_words = (ctrl_end - ctrl_head) / 4;
   b->d_stats = (unsigned long long *)(base + ctrl_stats);
 
@@ -847,12 +875,16 @@ int batch_finish(sgm_batch *b) {
   if (e != hipSuccess) return hip_fail(e, "batch synchronize");
   collect_profile(ctx);
   int err = 0;
-  e = hipMemcpy(&err, b->args.error, sizeof err, hipMemcpyDeviceToHost);
+  // copies on the batch's own stream: a copy on the null stream can queue
+  // behind other streams' kernels on a shared hardware queue
+  e = hipMemcpyAsync(&err, b->args.error, sizeof err, hipMemcpyDeviceToHost, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
   if (e != hipSuccess) return hip_fail(e, "batch status read");
   if (err) return fail(SGM_ERR_TIMEOUT, "batch work queue wait expired (dependency never completed)");
   const size_t count = b->pairs.size();
   std::vector<unsigned long long> st(count * 16);
-  e = hipMemcpy(st.data(), b->d_stats, st.size() * 8, hipMemcpyDeviceToHost);
+  e = hipMemcpyAsync(st.data(), b->d_stats, st.size() * 8, hipMemcpyDeviceToHost, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
   if (e != hipSuccess) return hip_fail(e, "batch counters read");
   for (size_t i = 0; i < count; i++) {
     sgm_stats *out = b->pairs[i].stats;
@@ -869,7 +901,8 @@ int batch_finish(sgm_batch *b) {
     s.lr_rejected = (int64_t)c[6];
     s.valid = (int64_t)c[10];
     if (b->flags & SGM_BATCH_DEVICE) {
-      e = hipMemcpy(out, &s, sizeof s, hipMemcpyHostToDevice);
+      e = hipMemcpyAsync(out, &s, sizeof s, hipMemcpyHostToDevice, b->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
       if (e != hipSuccess) return hip_fail(e, "batch counters write");
     } else {
       *out = s;

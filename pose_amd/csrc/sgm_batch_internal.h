@@ -60,6 +60,10 @@ struct sgm_batch {
   int total_items = 0;
   sgm::BatchArgs args{};
   hipGraphExec_t graph_exec = nullptr;
+  // host-buffer batches of a server: the kernels wait for `gate_wait` (the
+  // previous batch's kernels) after the uploads and record `gate_done` before
+  // the downloads, so batches overlap their transfers, not their kernels
+  hipEvent_t gate_wait = nullptr, gate_done = nullptr;
   // phase-major runs (SGM_BATCH_PHASED): per lane group R the pairs' path jobs
   // and, per line class (horizontal, vertical, diagonal), the first line of
   // each job in the group's launch
@@ -80,7 +84,6 @@ struct sgm_batch {
   int nsub = 1;
   std::vector<Group> groups;
   std::vector<int> phase_begin;  // host copy of the queue's phase boundaries
-  unsigned *head2 = nullptr;     // queue head of the L-R launch
   void *group_tables = nullptr;
   size_t group_tables_bytes = 0;
 };

@@ -112,13 +112,14 @@ __device__ __forceinline__ bool batch_wait(const BatchArgs &B, int ph, int job) 
 constexpr int kBatchCensusRows = 8;
 constexpr int kBatchTileW = 64 + 10, kBatchTileH = kBatchCensusRows + 10;
 
+// One 64-column chunk [x0, x0 + 64) of the row band.
 template <int WORDS>
-__device__ __forceinline__ void batch_census(const BatchArgs &B, const BatchImage &I, int r0,
-                                             int r1, uint8_t *tile) {
+__device__ __forceinline__ void batch_census_chunk(const BatchArgs &B, const BatchImage &I, int x0,
+                                                   int r0, int r1, uint8_t *tile) {
   const int lane = lane_id();
   const int rx = B.ww >> 1, ry = B.wh >> 1;
   const size_t n = (size_t)I.w * I.h;
-  for (int x0 = 0; x0 < I.w; x0 += 64) {
+  {
     const int tw = 64 + 2 * rx, th = (r1 - r0) + 2 * ry;
     for (int i = lane; i < tw * th; i += 64) {
       const int ty = i / tw, tx = i - ty * tw;
@@ -171,6 +172,12 @@ __device__ __forceinline__ void batch_census(const BatchArgs &B, const BatchImag
   }
 }
 
+template <int WORDS>
+__device__ __forceinline__ void batch_census(const BatchArgs &B, const BatchImage &I, int r0,
+                                             int r1, uint8_t *tile) {
+  for (int x0 = 0; x0 < I.w; x0 += 64) batch_census_chunk<WORDS>(B, I, x0, r0, r1, tile);
+}
+
 // Cost volume rows [r0, r1) of one matching direction (non-fused costs).
 template <bool ZNCC>
 __device__ __forceinline__ void batch_cost(const BatchArgs &B, const CostJob &J, int r0, int r1) {
@@ -203,13 +210,15 @@ __device__ __forceinline__ void batch_cost(const BatchArgs &B, const CostJob &J,
   }
 }
 
-// Left-right check + counters of rows [r0, r1) of one pair (as lr_kernel).
-__device__ __forceinline__ void batch_lr(const BatchArgs &B, const BatchPairDev &P, int r0, int r1) {
+// Left-right check + counters (as lr_kernel) of columns x0 + lane,
+// x0 + lane + 64, ... below x1 of rows [r0, r1) of one pair: outputs written,
+// the lane's eight counts added to c (stats fields 0-6 and 10).
+__device__ __forceinline__ void batch_lr_span(const BatchArgs &B, const BatchPairDev &P, int r0,
+                                              int r1, int x0, int x1, unsigned *c) {
   const int lane = lane_id();
   const int rx = B.ww >> 1, ry = B.wh >> 1;
-  unsigned c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int y = r0; y < r1; y++)
-    for (int x = lane; x < P.w; x += 64) {
+    for (int x = x0 + lane; x < x1; x += 64) {
       const size_t p = (size_t)y * P.w + x;
       const int code = P.code[p];
       c[0] += (code & CODE_LOWER) ? 1 : 0;
@@ -240,7 +249,16 @@ __device__ __forceinline__ void batch_lr(const BatchArgs &B, const BatchPairDev 
       if (P.reasons) P.reasons[p] = (uint8_t)reason;
       if (P.boundary) P.boundary[p] = P.boundary_tmp[p];
     }
-  const int field[8] = {0, 1, 2, 3, 4, 5, 6, 10};
+}
+
+// Stats field of count i of batch_lr_span.
+__device__ __forceinline__ int lr_field(int i) { return i < 7 ? i : 10; }
+
+// Rows [r0, r1) of one pair, every column.
+__device__ __forceinline__ void batch_lr(const BatchArgs &B, const BatchPairDev &P, int r0, int r1) {
+  const int lane = lane_id();
+  unsigned c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  batch_lr_span(B, P, r0, r1, 0, P.w, c);
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     int v = (int)c[i];
@@ -251,7 +269,7 @@ __device__ __forceinline__ void batch_lr(const BatchArgs &B, const BatchPairDev 
     v += __shfl_xor(v, 8);
     v += __shfl_xor(v, 16);
     v += __shfl_xor(v, 32);
-    if (lane == 0 && v) atomicAdd(P.stats + field[i], (unsigned long long)v);
+    if (lane == 0 && v) atomicAdd(P.stats + lr_field(i), (unsigned long long)v);
   }
 }
 
@@ -402,6 +420,48 @@ void batch_kernel(BatchArgs B) {
     if (lane == 0)
       __hip_atomic_fetch_add(B.done + (size_t)cur_ph * B.stride + cur_job, pending,
                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Phase-major runs: the census and L-R phases as regular grids over every
+// image / pair of the batch (blockIdx.z), four waves per block, each wave one
+// 64-column tile of a row band; the same per-tile work as the queue items.
+constexpr int kGridWaves = 4, kGridLrRows = 8;
+
+template <int WORDS>
+static __global__ __launch_bounds__(256) void batch_census_grid(BatchArgs B) {
+  __shared__ uint8_t tiles[kGridWaves][kBatchTileH * kBatchTileW];
+  const BatchImage I = B.images[blockIdx.z];
+  const int wave = threadIdx.x >> 6;
+  const int x0 = (blockIdx.x * kGridWaves + wave) * 64, r0 = blockIdx.y * kBatchCensusRows;
+  if (x0 >= I.w || r0 >= I.h) return;  // wave-uniform: the tile lies outside this image
+  batch_census_chunk<WORDS>(B, I, x0, r0, min(I.h, r0 + kBatchCensusRows), tiles[wave]);
+}
+
+static __global__ __launch_bounds__(256) void batch_lr_grid(BatchArgs B) {
+  __shared__ unsigned sums[kGridWaves][8];
+  const BatchPairDev P = B.pairs[blockIdx.z];
+  const int lane = lane_id(), wave = threadIdx.x >> 6;
+  const int x0 = (blockIdx.x * kGridWaves + wave) * 64, r0 = blockIdx.y * kGridLrRows;
+  unsigned c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (x0 < P.w && r0 < P.h)
+    batch_lr_span(B, P, r0, min(P.h, r0 + kGridLrRows), x0, min(P.w, x0 + 64), c);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    int v = (int)c[i];
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 4);
+    v += __shfl_xor(v, 8);
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    if (lane == 0) sums[wave][i] = (unsigned)v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    unsigned v = 0;
+    for (int w = 0; w < kGridWaves; w++) v += sums[w][threadIdx.x];
+    if (v) atomicAdd(P.stats + lr_field((int)threadIdx.x), (unsigned long long)v);
   }
 }
 

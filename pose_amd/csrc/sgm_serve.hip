@@ -100,6 +100,7 @@ struct Staged {
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
+  hipEvent_t kernels_done = nullptr;  // batching mode: the batch's kernels completed
   sgm_batch *batch = nullptr;
   sgm_context *lane = nullptr;  // lane mode: the slot's own context (and stream)
   std::vector<Request *> reqs;
@@ -109,6 +110,9 @@ struct Slot {
   // the batch's copies run asynchronously on the slot's stream
   sgm::HostBuffer staging;
   std::vector<Staged> staged;
+  // outgrown staging buffers, freed when the server closes: freeing pinned
+  // memory waits for the whole device, which would stall the other streams
+  std::vector<sgm::HostBuffer> retired;
 };
 
 // Lays out a request's host buffers in a staging buffer from offset `at`.
@@ -152,6 +156,10 @@ struct sgm_server {
   std::map<uint64_t, Request *> all;
   uint64_t next_ticket = 1;
   int next_batch = 0;
+  // batching mode: the slot whose batch was launched last; the next batch's
+  // kernels wait for its kernels (batches overlap transfers, not kernels:
+  // concurrent phase-major batches slow each other down, measured)
+  Slot *last_launched = nullptr;
   bool stop = false;
   std::thread worker;
   std::chrono::steady_clock::time_point t0;
@@ -295,7 +303,14 @@ void sgm_server::run() {
       size_t staged_bytes = 0;
       s.staged.clear();
       for (const sgm_pair &pr : prs) s.staged.push_back(stage_layout(pr, staged_bytes));
-      int rc = s.staging.ensure(staged_bytes, "serving staging");
+      int rc = SGM_OK;
+      if (staged_bytes > s.staging.cap) {
+        // grow by at least half, keeping the old buffer until the server closes
+        const size_t grow = std::max(staged_bytes, s.staging.cap + s.staging.cap / 2);
+        if (s.staging.ptr) s.retired.push_back(s.staging);
+        s.staging = sgm::HostBuffer{};
+        rc = s.staging.ensure(grow, "serving staging");
+      }
       if (rc == SGM_OK) {
         uint8_t *sb = (uint8_t *)s.staging.ptr;
         for (size_t i = 0; i < prs.size(); i++) {
@@ -323,7 +338,12 @@ void sgm_server::run() {
       if (rc == SGM_OK)
         rc = batch_create(ctx, &params, prs.data(), (int)prs.size(), SGM_BATCH_PHASED,
                           (uint8_t *)region + off, bytes, s.stream, &b, recompute, planes);
-      if (rc == SGM_OK) rc = batch_launch(b);
+      if (rc == SGM_OK) {
+        b->gate_wait = last_launched && last_launched != &s ? last_launched->kernels_done : nullptr;
+        b->gate_done = s.kernels_done;
+        rc = batch_launch(b);
+        if (rc == SGM_OK) last_launched = &s;
+      }
       if (rc == SGM_OK) {
         hipError_t e = hipEventRecord(s.done, s.stream);
         if (e != hipSuccess) rc = hip_fail(e, "hipEventRecord");
@@ -505,6 +525,8 @@ extern "C" int sgm_server_create(sgm_context *ctx, const sgm_params *params, int
       e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
     }
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+    if (e == hipSuccess && !s->lanes)
+      e = hipEventCreateWithFlags(&sl.kernels_done, hipEventDisableTiming);
     if (e != hipSuccess) {
       rc = hip_fail(e, "server stream");
       break;
@@ -513,6 +535,7 @@ extern "C" int sgm_server_create(sgm_context *ctx, const sgm_params *params, int
   if (rc) {
     for (Slot &sl : s->slots) {
       if (sl.done) (void)hipEventDestroy(sl.done);
+      if (sl.kernels_done) (void)hipEventDestroy(sl.kernels_done);
       if (sl.lane) (void)sgm_close(sl.lane);
       else if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
@@ -605,9 +628,11 @@ extern "C" int sgm_server_destroy(sgm_server *srv) {
   (void)hipSetDevice(srv->ctx->device);
   for (Slot &sl : srv->slots) {
     if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.kernels_done) (void)hipEventDestroy(sl.kernels_done);
     if (sl.lane) (void)sgm_close(sl.lane);
     else if (sl.stream) (void)hipStreamDestroy(sl.stream);
     sl.staging.release();
+    for (sgm::HostBuffer &hb : sl.retired) hb.release();
   }
   for (auto &kv : srv->all) delete kv.second;
   if (srv->region) {
