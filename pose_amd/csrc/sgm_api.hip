@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -15,6 +16,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <fcntl.h>
@@ -713,6 +715,32 @@ extern "C" int sgm_profile_json(sgm_context *ctx, char *buffer, size_t length) {
 // ---------------------------------------------------------------------------
 // Launchers
 namespace sgm {
+
+void parallel_copy(const std::vector<HostCopy> &copies) {
+  constexpr size_t kChunk = size_t(4) << 20;
+  std::vector<HostCopy> chunks;
+  size_t total = 0;
+  for (const HostCopy &c : copies) {
+    for (size_t o = 0; o < c.bytes; o += kChunk)
+      chunks.push_back({(uint8_t *)c.dst + o, (const uint8_t *)c.src + o, std::min(kChunk, c.bytes - o)});
+    total += c.bytes;
+  }
+  const unsigned hw = std::max(2u, std::thread::hardware_concurrency());
+  const size_t threads = std::min<size_t>({(size_t)8, hw / 2, chunks.size()});
+  if (threads <= 1 || total < (size_t(16) << 20)) {
+    for (const HostCopy &c : chunks) memcpy(c.dst, c.src, c.bytes);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  auto work = [&] {
+    for (size_t i; (i = next.fetch_add(1, std::memory_order_relaxed)) < chunks.size();)
+      memcpy(chunks[i].dst, chunks[i].src, chunks[i].bytes);
+  };
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < threads; t++) pool.emplace_back(work);
+  work();
+  for (std::thread &t : pool) t.join();
+}
 
 int launch_check(const char *what) {
   hipError_t e = hipGetLastError();
@@ -2714,11 +2742,12 @@ int lane_collect(sgm_context *ctx, const sgm_pair &pr, sgm_stats *stats) {
   if ((rc = finish(ctx))) return rc;  // the stream has already passed the work
   const size_t n = (size_t)pr.width * pr.height;
   const uint8_t *out = (const uint8_t *)ctx->host_buffers["lane_out"].ptr;
-  memcpy(pr.checked, out, n * 4);
-  if (pr.dl) memcpy(pr.dl, out + 4 * n, n * 4);
-  if (pr.dr) memcpy(pr.dr, out + 8 * n, n * 4);
-  if (pr.reasons) memcpy(pr.reasons, out + 12 * n, n);
-  if (pr.boundary) memcpy(pr.boundary, out + 13 * n, n);
+  std::vector<HostCopy> copies = {{pr.checked, out, n * 4}};
+  if (pr.dl) copies.push_back({pr.dl, out + 4 * n, n * 4});
+  if (pr.dr) copies.push_back({pr.dr, out + 8 * n, n * 4});
+  if (pr.reasons) copies.push_back({pr.reasons, out + 12 * n, n});
+  if (pr.boundary) copies.push_back({pr.boundary, out + 13 * n, n});
+  parallel_copy(copies);
   unsigned long long counters[16];
   memcpy(counters, out + 14 * n, sizeof counters);
   if (stats) stats_from_device(counters, stats);

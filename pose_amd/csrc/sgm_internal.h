@@ -70,6 +70,15 @@ int64_t schedule_bytes(const sgm_params *p, int w, int h, int schedule);
 int sweep_schedule(const sgm_context *ctx, const sgm_params *p, int w, int h, bool cost_volume,
                    int64_t budget);
 int lines_of(int cls, int w, int h);
+struct HostCopy {
+  void *dst;
+  const void *src;
+  size_t bytes;
+};
+// Host-to-host copies split in 4 MiB chunks over up to eight threads: copies
+// into fresh caller buffers are bound by first-touch page faults, which
+// proceed in parallel.
+void parallel_copy(const std::vector<HostCopy> &copies);
 void collect_profile(sgm_context *ctx);
 // plain pair on the context's stream without waiting (serving lanes)
 int lane_enqueue(sgm_context *ctx, const sgm_params *base, const sgm_pair &pr);

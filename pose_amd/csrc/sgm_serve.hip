@@ -194,16 +194,18 @@ void sgm_server::run() {
         sgm_batch_destroy(s.batch);
         // outputs from the pinned staging to the callers' buffers
         const uint8_t *sb = (const uint8_t *)s.staging.ptr;
+        std::vector<sgm::HostCopy> copies;
         for (size_t i = 0; rc == SGM_OK && i < s.reqs.size(); i++) {
           const sgm_pair &pr = s.reqs[i]->pair;
           const Staged &st = s.staged[i];
           const size_t n = (size_t)pr.width * pr.height;
-          memcpy(pr.checked, sb + st.checked, 4 * n);
-          if (pr.dl) memcpy(pr.dl, sb + st.dl, 4 * n);
-          if (pr.dr) memcpy(pr.dr, sb + st.dr, 4 * n);
-          if (pr.reasons) memcpy(pr.reasons, sb + st.reasons, n);
-          if (pr.boundary) memcpy(pr.boundary, sb + st.boundary, n);
+          copies.push_back({pr.checked, sb + st.checked, 4 * n});
+          if (pr.dl) copies.push_back({pr.dl, sb + st.dl, 4 * n});
+          if (pr.dr) copies.push_back({pr.dr, sb + st.dr, 4 * n});
+          if (pr.reasons) copies.push_back({pr.reasons, sb + st.reasons, n});
+          if (pr.boundary) copies.push_back({pr.boundary, sb + st.boundary, n});
         }
+        sgm::parallel_copy(copies);
       }
       lk.lock();
       const double t = now();
