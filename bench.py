@@ -275,6 +275,40 @@ def run_streamed(sgm, W, H, D, P, d_left, d_right, rows=64, steps=2):
             "pcie_bytes_per_pair": 2 * plan["host_bytes"]}
 
 
+def run_aggregate32(sgm, W, H, D, P, d_left, d_right, steps=3):
+    """The benchmark pair with 32-bit aggregates (--gpu-aggregate-bits 32 in the
+    reference; also what a P2 beyond the 16-bit bound selects): uint16 cost
+    volume, int32 aggregate volume, one direction per launch with the generic
+    recurrence; inputs in HBM."""
+    params = workload_params(sgm)
+    params.aggregate_bytes = 4
+    ctx = sgm.Context(0)
+    n = W * H
+    out = ctx.alloc(4 * n)
+    ctx.match_pair_device(params, d_left, d_right, W, H, out)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.match_pair_device(params, d_left, d_right, W, H, out)
+    ctx.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    ctx.set_profiling(True)
+    ctx.profile()
+    ctx.match_pair_device(params, d_left, d_right, W, H, out)
+    ctx.synchronize()
+    prof = {k: round(v["total_ms"], 3) for k, v in ctx.profile()["device_kernel_milliseconds"].items()}
+    ctx.set_profiling(False)
+    plan = ctx.plan(params, W, H)
+    out.free()
+    ctx.close()
+    upd = W * H * D * P * 2
+    return {"workload": "plain pair 1600x1067 D=256 with 32-bit aggregates (uint16 cost volume, "
+                        "int32 aggregate volume, generic recurrence)",
+            "ms_per_pair": round(t * 1e3, 3), "gupdates_per_s": round(upd / t / 1e9, 3),
+            "kernel_ms": prof, "device_bytes": plan["pair_bytes"],
+            "aggregate_bytes": plan["aggregate_bytes"]}
+
+
 def run_esgm(sgm, W, H, D, P, d_left, d_right, rows=64, pool=2, steps=2):
     """The benchmark pair under the compact plan (proposal and sum sweeps per
     path over strips of `rows` rows, certification, exact verification of the
@@ -596,6 +630,7 @@ def main():
     ap.add_argument("--no-streamed", action="store_true", help="skip the strip-streamed leg")
     ap.add_argument("--no-esgm", action="store_true", help="skip the compact (esgm) leg")
     ap.add_argument("--no-filtered", action="store_true", help="skip the post-L-R filter leg")
+    ap.add_argument("--no-agg32", action="store_true", help="skip the 32-bit aggregate leg")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals, pairs/s")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--serve-streams", type=int, default=3, help="server streams / lanes")
@@ -701,6 +736,7 @@ def main():
 
     streamed = None if args.no_streamed else run_streamed(sgm, W, H, D, P, d_left, d_right)
     esgm = None if args.no_esgm else run_esgm(sgm, W, H, D, P, d_left, d_right)
+    agg32 = None if args.no_agg32 else run_aggregate32(sgm, W, H, D, P, d_left, d_right)
     for b in (d_left, d_right, d_out, d_stats):
         b.free()
     ctx.close()
@@ -783,6 +819,7 @@ def main():
         "filtered": filtered,
         "streamed": streamed,
         "esgm": esgm,
+        "aggregate32": agg32,
         "zncc": zncc,
         "batched": batched,
         "batched_small": batched_small,
