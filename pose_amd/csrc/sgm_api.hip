@@ -737,7 +737,11 @@ void parallel_copy(const std::vector<HostCopy> &copies) {
       memcpy(chunks[i].dst, chunks[i].src, chunks[i].bytes);
   };
   std::vector<std::thread> pool;
-  for (size_t t = 1; t < threads; t++) pool.emplace_back(work);
+  try {
+    for (size_t t = 1; t < threads; t++) pool.emplace_back(work);
+  } catch (...) {
+    // no more threads: the ones started and this one share the chunks
+  }
   work();
   for (std::thread &t : pool) t.join();
 }
