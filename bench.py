@@ -275,6 +275,55 @@ def run_streamed(sgm, W, H, D, P, d_left, d_right, rows=64, steps=2):
             "pcie_bytes_per_pair": 2 * plan["host_bytes"]}
 
 
+def run_small_pair(sgm, steps=50):
+    """One 640x480 pair, D=64, census 9x7, P1=10, P2=120, 8 paths, L+R + L-R,
+    inputs in HBM: the volumes (C + S about 79 MB per direction) fit the
+    256 MB MALL, so the pair is latency- and launch-bound, not HBM-bound."""
+    import numpy as np
+    from pose_amd import synth
+    W, H, D = 640, 480, 64
+    p = sgm.SgmParams(window_w=9, window_h=7, disparities=D, min_disparity=0, cost=0, p1=10,
+                      p2=120, edge_scale=32, directions=8, aggregate_bytes=0, census_weight=500,
+                      ad_truncation=30, gradient_truncation=60, reserved=0, uniqueness=0.03,
+                      lr_threshold=1.0)
+    left, right, _ = synth.stereo_pair(W, H, D, 0, seed=1)
+    ctx = sgm.Context(0)
+    n = W * H
+    dl, dr, out = ctx.alloc(n).upload(left), ctx.alloc(n).upload(right), ctx.alloc(4 * n)
+    for _ in range(3):
+        ctx.match_pair_device(p, dl, dr, W, H, out)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.match_pair_device(p, dl, dr, W, H, out)
+    ctx.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    lat = []  # one call and its completion at a time
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        ctx.match_pair_device(p, dl, dr, W, H, out)
+        ctx.synchronize()
+        lat.append(time.perf_counter() - t1)
+    ctx.set_profiling(True)
+    ctx.profile()
+    ctx.match_pair_device(p, dl, dr, W, H, out)
+    ctx.synchronize()
+    prof = {k: round(v["total_ms"], 4) for k, v in ctx.profile()["device_kernel_milliseconds"].items()}
+    ctx.set_profiling(False)
+    valid = int(np.isfinite(out.download(np.float32, n)).sum())
+    for b in (dl, dr, out):
+        b.free()
+    ctx.close()
+    upd = W * H * D * 8 * 2
+    lat.sort()
+    return {"workload": "plain pair 640x480, census 9x7, D=64, P1=10 P2=120, 8 paths, L+R + L-R, "
+                        "inputs in HBM; ms_per_pair back to back, latency one call at a time",
+            "ms_per_pair": round(t * 1e3, 4), "gupdates_per_s": round(upd / t / 1e9, 3),
+            "latency_ms_p50": round(lat[len(lat) // 2] * 1e3, 4),
+            "kernel_ms": prof, "kernel_ms_total": round(sum(prof.values()), 4),
+            "valid_pixels": valid}
+
+
 def run_aggregate32(sgm, W, H, D, P, d_left, d_right, steps=3):
     """The benchmark pair with 32-bit aggregates (--gpu-aggregate-bits 32 in the
     reference; also what a P2 beyond the 16-bit bound selects): uint16 cost
@@ -631,6 +680,7 @@ def main():
     ap.add_argument("--no-esgm", action="store_true", help="skip the compact (esgm) leg")
     ap.add_argument("--no-filtered", action="store_true", help="skip the post-L-R filter leg")
     ap.add_argument("--no-agg32", action="store_true", help="skip the 32-bit aggregate leg")
+    ap.add_argument("--no-small", action="store_true", help="skip the 640x480 D=64 pair leg")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals, pairs/s")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--serve-streams", type=int, default=3, help="server streams / lanes")
@@ -737,6 +787,7 @@ def main():
     streamed = None if args.no_streamed else run_streamed(sgm, W, H, D, P, d_left, d_right)
     esgm = None if args.no_esgm else run_esgm(sgm, W, H, D, P, d_left, d_right)
     agg32 = None if args.no_agg32 else run_aggregate32(sgm, W, H, D, P, d_left, d_right)
+    small_pair = None if args.no_small else run_small_pair(sgm)
     for b in (d_left, d_right, d_out, d_stats):
         b.free()
     ctx.close()
@@ -820,6 +871,7 @@ def main():
         "streamed": streamed,
         "esgm": esgm,
         "aggregate32": agg32,
+        "small_pair": small_pair,
         "zncc": zncc,
         "batched": batched,
         "batched_small": batched_small,
