@@ -125,6 +125,55 @@ def cpu_baseline(w, h, D, rows=256, trials=3):
                                "trials_s": [round(t, 3) for t in runs1]})
 
 
+def cpu_baseline_configs(trials=3):
+    """The C oracle over every host core on the other workloads, rates in
+    updates of the sample: the 640x480 D=64 pair whole and a quarter-height
+    strip of the FP32 ZNCC pair (its FP64 oracle), one warm-up and `trials`
+    timed runs, median; the batched distribution on its two smallest pairs of
+    the 48 (one warm-up, one timed run)."""
+    from oracle import oracle as O
+    from pose_amd import synth
+    threads = len(os.sched_getaffinity(0))
+    out = {}
+    left, right, _ = synth.stereo_pair(640, 480, 64, 0, seed=1)
+    op = O.params(window_w=9, window_h=7, disparities=64, p1=10, p2=120, lr_threshold=1.0)
+    t, runs = timed_trials(lambda: O.compute_parallel(op, left, right, 640, 480, threads=threads),
+                           trials)
+    out["640x480_d64"] = {"value": round(640 * 480 * 64 * 16 / t / 1e9, 4), "unit": "Gupdates/s",
+                          "cores": threads, "sample": "whole pair, median of %d" % trials,
+                          "trials_s": [round(x, 3) for x in runs]}
+    left, right, _ = synth.stereo_pair(1600, 1067, 192, 0, seed=3, gain=True)
+    rows = 267  # a quarter of the pair's rows bounds the FP64 oracle's time
+    op = O.params(window_w=7, window_h=7, disparities=192, cost=4, p1=12, p2=150,
+                  lr_threshold=1.0)
+    t, runs = timed_trials(lambda: O.compute_parallel(op, left[:rows], right[:rows], 1600, rows,
+                                                      threads=threads), trials)
+    out["zncc_1600x1067_d192"] = {"value": round(1600 * rows * 192 * 16 / t / 1e9, 4),
+                                  "unit": "Gupdates/s", "cores": threads,
+                                  "sample": "1600x%d strip of the pair, FP64 oracle, median of %d"
+                                            % (rows, trials),
+                                  "trials_s": [round(x, 3) for x in runs]}
+    spec = sorted(batch_workload(), key=lambda s: s[0] * s[1] * s[2])[:2]
+    pairs = []
+    for (w, h, D, md, seed) in spec:
+        left, right, _ = synth.stereo_pair(w, h, D, md, seed=seed)
+        pairs.append((O.params(window_w=9, window_h=7, disparities=D, min_disparity=md, p1=10,
+                               p2=120, lr_threshold=1.0), left, right, w, h))
+
+    def run_subset():
+        for (op, left, right, w, h) in pairs:
+            O.compute_parallel(op, left, right, w, h, threads=threads)
+    t, runs = timed_trials(run_subset, 1)
+    upd = sum(w * h * op.disparities * 16 for (op, _, _, w, h) in pairs)
+    out["batched_distribution"] = {"value": round(upd / t / 1e9, 4), "unit": "Gupdates/s",
+                                   "cores": threads,
+                                   "sample": "the 2 smallest of the 48 pairs (%s), one warm-up "
+                                             "and one timed run, %.2f s" % (
+                                                 ", ".join("%dx%d D=%d" % (w, h, D)
+                                                           for (w, h, D, _, _) in spec), t)}
+    return out
+
+
 def batch_workload(count=48, seed=4):
     """The batched workload: uneven landscape pairs, long side U[800, 3000], aspect
     4:3 or 3:2, D U[64, 384], min disparity U[-32, 32], per-pair seeds."""
@@ -816,6 +865,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(W, H, D)
+        cpu["other_workloads"] = cpu_baseline_configs()
 
     lat_ms = sorted(x * 1e3 for x in lat)
     line = {
