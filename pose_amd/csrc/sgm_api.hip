@@ -184,6 +184,9 @@ int validate_params(const sgm_params *p, int w, int h) {
   // descriptor planes are addressed with 32-bit byte offsets (8 B per pixel)
   if ((int64_t)w * h > (int64_t)1 << 28)
     return fail(SGM_ERR_INVALID_ARGUMENT, "image %dx%d exceeds 2^28 pixels", w, h);
+  // the cost kernels launch one grid row per image row (grid y <= 65535)
+  if (h > 65535)
+    return fail(SGM_ERR_INVALID_ARGUMENT, "image height %d exceeds 65535 rows", h);
   return SGM_OK;
 }
 

@@ -187,13 +187,19 @@ static int grid_launch(sgm_batch *b, bool lr) {
     h = std::max(h, pr.height);
   }
   const int rows = lr ? kGridLrRows : kBatchCensusRows;
-  const unsigned n = (unsigned)b->pairs.size() * (lr ? 1u : 2u);
-  dim3 grid((unsigned)((w + 64 * kGridWaves - 1) / (64 * kGridWaves)), (unsigned)((h + rows - 1) / rows), n);
+  const int n = (int)b->pairs.size() * (lr ? 1 : 2);
+  constexpr int kMaxZ = 65535;  // grid z limit: images / pairs in slices
   ProfileScope ps(b->ctx, lr ? "batch lr" : "batch census");
-  if (lr) batch_lr_grid<<<grid, 64 * kGridWaves, 0, b->stream>>>(b->args);
-  else if (b->words == 1) batch_census_grid<1><<<grid, 64 * kGridWaves, 0, b->stream>>>(b->args);
-  else batch_census_grid<2><<<grid, 64 * kGridWaves, 0, b->stream>>>(b->args);
-  return launch_check(lr ? "batch lr grid" : "batch census grid");
+  for (int z0 = 0; z0 < n; z0 += kMaxZ) {
+    dim3 grid((unsigned)((w + 64 * kGridWaves - 1) / (64 * kGridWaves)),
+              (unsigned)((h + rows - 1) / rows), (unsigned)std::min(kMaxZ, n - z0));
+    if (lr) batch_lr_grid<<<grid, 64 * kGridWaves, 0, b->stream>>>(b->args, z0);
+    else if (b->words == 1) batch_census_grid<1><<<grid, 64 * kGridWaves, 0, b->stream>>>(b->args, z0);
+    else batch_census_grid<2><<<grid, 64 * kGridWaves, 0, b->stream>>>(b->args, z0);
+    int rc = launch_check(lr ? "batch lr grid" : "batch census grid");
+    if (rc) return rc;
+  }
+  return SGM_OK;
 }
 
 // Phase-major run: the census grid (and the cost items), the sweeps direction

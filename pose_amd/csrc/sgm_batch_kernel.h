@@ -429,18 +429,18 @@ void batch_kernel(BatchArgs B) {
 constexpr int kGridWaves = 4, kGridLrRows = 8;
 
 template <int WORDS>
-static __global__ __launch_bounds__(256) void batch_census_grid(BatchArgs B) {
+static __global__ __launch_bounds__(256) void batch_census_grid(BatchArgs B, int z0) {
   __shared__ uint8_t tiles[kGridWaves][kBatchTileH * kBatchTileW];
-  const BatchImage I = B.images[blockIdx.z];
+  const BatchImage I = B.images[z0 + (int)blockIdx.z];
   const int wave = threadIdx.x >> 6;
   const int x0 = (blockIdx.x * kGridWaves + wave) * 64, r0 = blockIdx.y * kBatchCensusRows;
   if (x0 >= I.w || r0 >= I.h) return;  // wave-uniform: the tile lies outside this image
   batch_census_chunk<WORDS>(B, I, x0, r0, min(I.h, r0 + kBatchCensusRows), tiles[wave]);
 }
 
-static __global__ __launch_bounds__(256) void batch_lr_grid(BatchArgs B) {
+static __global__ __launch_bounds__(256) void batch_lr_grid(BatchArgs B, int z0) {
   __shared__ unsigned sums[kGridWaves][8];
-  const BatchPairDev P = B.pairs[blockIdx.z];
+  const BatchPairDev P = B.pairs[z0 + (int)blockIdx.z];
   const int lane = lane_id(), wave = threadIdx.x >> 6;
   const int x0 = (blockIdx.x * kGridWaves + wave) * 64, r0 = blockIdx.y * kGridLrRows;
   unsigned c[8] = {0, 0, 0, 0, 0, 0, 0, 0};

@@ -348,6 +348,18 @@ def test_repeatable_and_recovers_from_allocation_failure():
         ctx.close()
 
 
+def test_image_limits_are_validated(gpu_context):
+    """Sizes the kernels cannot launch are refused before any work: images
+    taller than 65535 rows (one grid row per image row in the cost kernels),
+    above 2^28 pixels, or smaller than the census window plus a border."""
+    p = to_sgm(O.params(window_w=9, window_h=7, disparities=16))
+    for (w, h) in ((16, 65536), (1 << 15, (1 << 13) + 1), (10, 200), (200, 8)):
+        with pytest.raises(sgm.SgmError) as e:
+            gpu_context.plan(p, w, h)
+        assert e.value.code == -1, (w, h)
+    assert gpu_context.plan(p, 16, 65535)["padded_disparities"] == 64
+
+
 def test_zncc_fp32_against_fp64_oracle(gpu_context):
     """FP32 ZNCC variant: at most 0.1% of pixels may differ by more than 1 disparity
     from the FP64 oracle (tolerance of the ZNCC workload)."""
