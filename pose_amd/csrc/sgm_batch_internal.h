@@ -64,6 +64,11 @@ struct sgm_batch {
   // previous batch's kernels) after the uploads and record `gate_done` before
   // the downloads, so batches overlap their transfers, not their kernels
   hipEvent_t gate_wait = nullptr, gate_done = nullptr;
+  // phase-major plane runs: a second stream for every other lane group of a
+  // sub-batch (the groups' pairs are disjoint), forked from and joined back
+  // into `stream` around each sub-batch
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   // phase-major runs (SGM_BATCH_PHASED): per lane group R the pairs' path jobs
   // and, per line class (horizontal, vertical, diagonal), the first line of
   // each job in the group's launch
