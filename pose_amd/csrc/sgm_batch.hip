@@ -964,9 +964,11 @@ extern "C" int sgm_batch_destroy(sgm_batch *b) {
   (void)hipStreamSynchronize(b->stream);
   if (b->side) (void)hipStreamSynchronize(b->side);
   if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
-  if (b->fork) (void)hipEventDestroy(b->fork);
-  if (b->join) (void)hipEventDestroy(b->join);
-  if (b->side) (void)hipStreamDestroy(b->side);
+  if (b->side_owned) {
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
+    if (b->side) (void)hipStreamDestroy(b->side);
+  }
   if (b->external) {
     delete b;
     return SGM_OK;

@@ -69,6 +69,7 @@ struct sgm_batch {
   // into `stream` around each sub-batch
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
+  bool side_owned = true;  // false: the serving slot's stream and events
   // phase-major runs (SGM_BATCH_PHASED): per lane group R the pairs' path jobs
   // and, per line class (horizontal, vertical, diagonal), the first line of
   // each job in the group's launch

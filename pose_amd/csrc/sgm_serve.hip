@@ -101,6 +101,10 @@ struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   hipEvent_t kernels_done = nullptr;  // batching mode: the batch's kernels completed
+  // batching mode: the second stream of the batches' plane sweeps and its
+  // fork / join events
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   sgm_batch *batch = nullptr;
   sgm_context *lane = nullptr;  // lane mode: the slot's own context (and stream)
   std::vector<Request *> reqs;
@@ -343,6 +347,12 @@ void sgm_server::run() {
       if (rc == SGM_OK) {
         b->gate_wait = last_launched && last_launched != &s ? last_launched->kernels_done : nullptr;
         b->gate_done = s.kernels_done;
+        if (b->planes) {
+          b->side = s.side;
+          b->fork = s.fork;
+          b->join = s.join;
+          b->side_owned = false;
+        }
         rc = batch_launch(b);
         if (rc == SGM_OK) last_launched = &s;
       }
@@ -529,6 +539,9 @@ extern "C" int sgm_server_create(sgm_context *ctx, const sgm_params *params, int
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
     if (e == hipSuccess && !s->lanes)
       e = hipEventCreateWithFlags(&sl.kernels_done, hipEventDisableTiming);
+    if (e == hipSuccess && !s->lanes) e = hipStreamCreateWithFlags(&sl.side, hipStreamNonBlocking);
+    if (e == hipSuccess && !s->lanes) e = hipEventCreateWithFlags(&sl.fork, hipEventDisableTiming);
+    if (e == hipSuccess && !s->lanes) e = hipEventCreateWithFlags(&sl.join, hipEventDisableTiming);
     if (e != hipSuccess) {
       rc = hip_fail(e, "server stream");
       break;
@@ -538,6 +551,9 @@ extern "C" int sgm_server_create(sgm_context *ctx, const sgm_params *params, int
     for (Slot &sl : s->slots) {
       if (sl.done) (void)hipEventDestroy(sl.done);
       if (sl.kernels_done) (void)hipEventDestroy(sl.kernels_done);
+      if (sl.fork) (void)hipEventDestroy(sl.fork);
+      if (sl.join) (void)hipEventDestroy(sl.join);
+      if (sl.side) (void)hipStreamDestroy(sl.side);
       if (sl.lane) (void)sgm_close(sl.lane);
       else if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
@@ -631,6 +647,9 @@ extern "C" int sgm_server_destroy(sgm_server *srv) {
   for (Slot &sl : srv->slots) {
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.kernels_done) (void)hipEventDestroy(sl.kernels_done);
+    if (sl.fork) (void)hipEventDestroy(sl.fork);
+    if (sl.join) (void)hipEventDestroy(sl.join);
+    if (sl.side) (void)hipStreamDestroy(sl.side);
     if (sl.lane) (void)sgm_close(sl.lane);
     else if (sl.stream) (void)hipStreamDestroy(sl.stream);
     sl.staging.release();
