@@ -400,11 +400,12 @@ typedef struct sgm_pair {
 #define SGM_BATCH_DEVICE 1     /* sgm_pair pointers are device pointers */
 #define SGM_BATCH_GRAPH 2      /* capture the launch into a hipGraph and replay it */
 #define SGM_BATCH_NO_SYNC 4    /* do not wait for completion (device mode only) */
-/* Phase-major run: the work-queue kernel takes the census (and cost) items of
- * every pair, then each path direction runs as one launch per lane group of
- * the specialised sweep kernel over a table of all pairs' lines, then the
- * work-queue kernel takes the left-right items; all in one graph with
- * SGM_BATCH_GRAPH.  Same results. */
+/* Phase-major run: one census launch over every image (then the work-queue
+ * kernel for the cost items of non-census costs), the path sweeps as launches
+ * per lane group of the specialised sweep kernels over a table of all pairs'
+ * lines (with path-cost planes: per sub-batch of pairs, lane groups on two
+ * streams), then one left-right launch over every pair; all in one graph
+ * with SGM_BATCH_GRAPH.  Same results. */
 #define SGM_BATCH_PHASED 8
 
 typedef struct sgm_batch sgm_batch;

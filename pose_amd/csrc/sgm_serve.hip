@@ -3,11 +3,11 @@
 // A server thread admits submitted pairs in arrival order as soon as their
 // device footprint fits the free part of a budgeted memory pool and a stream
 // slot is free, launches every admitted pair waiting at that moment as one
-// phase-major batch on that stream (work-queue kernel for the census and L-R
-// items, one sweep launch per direction and lane group), and keeps up to
-// `streams` batches in flight, so requests that arrive while others run
-// overlap with them on the device.  Volumes stay device-resident for the life
-// of their batch.  With max_batch == 1 each slot is a lane instead: a context
+// phase-major batch on that stream (census and L-R grids over the batch, the
+// sweep launches per lane group), and keeps up to `streams` batches in
+// flight.  A batch's host buffers go through the slot's pinned staging, so its
+// copies overlap other batches' kernels; its kernels wait for the previous
+// batch's kernels.  Volumes stay device-resident for the life of their batch.  With max_batch == 1 each slot is a lane instead: a context
 // of its own that runs every admitted pair through the plain per-pair kernels
 // on its stream, the lanes overlapping on the device, admission by the device
 // memory the lanes hold.  Neither mode has a counterpart in the reference,
