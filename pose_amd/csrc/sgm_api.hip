@@ -1553,6 +1553,54 @@ int launch_esgm_verify(sgm_context *ctx, int R, const SummaryJobs &J, int slots)
 // over every path and strip, certification, exact verification of the rest in
 // groups that fit the budget, then selection of every pixel from its summary.
 // Same contract as run_directions.
+// Longest-first dispatch order of the lines of a seven-direction launch over
+// a w x h pair (nd matching directions), in a device buffer cached by shape:
+// the launch has several times more lines than resident waves, and in line
+// order its last waves included long diagonal lines that then ran alone
+// (measured 2.48 -> 2.30 ms at 1600x1067 D=256).  Null, for the natural
+// order, when the buffer does not fit the budget.
+static const int *multi_line_order(sgm_context *ctx, const MultiDir &M, int w, int h, int nd) {
+  const int key[4] = {w, h, nd, M.ndirs};
+  Buffer &buf = ctx->buffers["multi_order"];
+  if (buf.ptr && memcmp(ctx->multi_order_key, key, sizeof key) == 0) return (const int *)buf.ptr;
+  const size_t bytes = (size_t)M.total_lines * sizeof(int);
+  if (bytes > buf.cap && ctx->allocated - (int64_t)buf.cap + (int64_t)bytes > ctx->budget)
+    return nullptr;
+  if (buf.ensure(ctx, bytes, "line order")) return nullptr;
+  // counting sort by length, descending; lines of equal length in line order
+  const int maxlen = std::max(w, h);
+  std::vector<int> len((size_t)M.total_lines), start((size_t)maxlen + 2, 0);
+  for (int k = 0; k < M.ndirs; k++) {
+    const int dx = M.dx[k], dy = M.dy[k];
+    for (int m = 0; m < nd; m++)
+      for (int l = 0; l < M.split[k]; l++) {
+        int n;
+        if (dy == 0) {
+          n = w;
+        } else if (dx == 0) {
+          n = h;
+        } else {  // the diagonal line enumeration of line_start
+          const int x = l < w ? l : (dx > 0 ? 0 : w - 1);
+          const int y = l < w ? (dy > 0 ? 0 : h - 1) : (dy > 0 ? l - w + 1 : h - 2 - (l - w));
+          n = std::min(dx > 0 ? w - x : x + 1, dy > 0 ? h - y : y + 1);
+        }
+        len[(size_t)M.begin[k] + (size_t)m * M.split[k] + l] = n;
+      }
+  }
+  for (int n : len) start[maxlen - n + 1]++;  // bucket maxlen - n: longest first
+  for (int i = 1; i <= maxlen + 1; i++) start[i] += start[i - 1];
+  ctx->multi_order.assign(len.size(), 0);
+  for (size_t g = 0; g < len.size(); g++) ctx->multi_order[start[maxlen - len[g]]++] = (int)g;
+  hipError_t e = hipMemcpyAsync(buf.ptr, ctx->multi_order.data(), bytes, hipMemcpyHostToDevice,
+                                ctx->stream);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  memcpy(ctx->multi_order_key, key, sizeof key);
+  return (const int *)buf.ptr;
+}
+
 int run_directions_esgm(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuffers &B,
                         const int *slots, int nd, int out_slot, uint8_t *reasons,
                         uint8_t *boundary) {
@@ -1892,6 +1940,7 @@ int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuff
       M.total_lines += Ak.total_lines;
       for (int i = 0; i < nd; i++) M.plane[k][i] = B.planes[slots[i]] + (size_t)k * Dp;
     }
+    M.order = multi_line_order(ctx, M, w, h, nd);
     if (interrupted()) return fail(SGM_ERR_INTERRUPTED, "GPU SGM interrupted");
     if ((rc = launch_path_multi(ctx, R, mode | MODE_PLANE, pj, dir_args(0), M, waves))) return rc;
     PathJobs jl = pj;

@@ -1380,6 +1380,10 @@ struct MultiDir {
   // plane of direction i lies plane0 + i rows of Dp bytes further
   const int *line_begin[3];
   int njobs;
+  // dispatch order (optional): wave slot i sweeps line order[i], so the
+  // longest lines start first and the launch does not end on a few long
+  // lines dispatched last
+  const int *order;
 };
 
 template <int R, int MODE>
@@ -1388,8 +1392,9 @@ __global__ __launch_bounds__(256) void path_multi_kernel(PathJobs jobs, PathArgs
   build_p2_table(p2_table, A);  // P1, P2 and the edge scale are shared by every direction
   __syncthreads();
   const int waves = blockDim.x >> 6;
-  const int gline = __builtin_amdgcn_readfirstlane(blockIdx.x * waves + (int)(threadIdx.x >> 6));
-  if (gline >= M.total_lines) return;
+  const int slot = __builtin_amdgcn_readfirstlane(blockIdx.x * waves + (int)(threadIdx.x >> 6));
+  if (slot >= M.total_lines) return;
+  const int gline = M.order ? __builtin_amdgcn_readfirstlane(M.order[slot]) : slot;
   // direction of this line: fields picked with constant indices (uniform)
   PathArgs B = A;
   B.dx = M.dx[0];
