@@ -676,15 +676,18 @@ def write_traffic(fetch_dir, write_dir, copy_bytes=2 << 30, paths=8):
     fetch_scale = copy_bytes / 1024.0 / mean_of(fetch, is_copy)
     write_scale = copy_bytes / 1024.0 / mean_of(write, is_copy)
     cells = 2 * W * H * D
-    planes = any("path_multi_kernel" in k for k, _ in fetch)
+    # the workload's instances only (lane group 4, uint16 aggregates): other
+    # legs of the same run launch other lane groups and 32-bit aggregates
+    sweep = lambda k, phase: "path_kernel<4, unsigned short, %d, " % phase in k
+    planes = any("path_multi_kernel<4, " in k for k, _ in fetch)
     if planes:
-        kinds = (("cost", lambda k: "path_kernel" in k and ", 3, " in k, cells, 1),
-                 ("multi", lambda k: "path_multi_kernel" in k, 2 * (paths - 1) * cells, 1),
-                 ("last", lambda k: "path_kernel" in k and ", 2, " in k, paths * cells, 1))
+        kinds = (("cost", lambda k: sweep(k, 3), cells, 1),
+                 ("multi", lambda k: "path_multi_kernel<4, " in k, 2 * (paths - 1) * cells, 1),
+                 ("last", lambda k: sweep(k, 2), paths * cells, 1))
     else:
-        kinds = (("first", lambda k: "path_kernel" in k and ", 0, " in k, 3 * cells, 1),
-                 ("mid", lambda k: "path_kernel" in k and ", 1, " in k, 5 * cells, paths - 2),
-                 ("last", lambda k: "path_kernel" in k and ", 2, " in k, 3 * cells, 1))
+        kinds = (("first", lambda k: sweep(k, 0), 3 * cells, 1),
+                 ("mid", lambda k: sweep(k, 1), 5 * cells, paths - 2),
+                 ("last", lambda k: sweep(k, 2), 3 * cells, 1))
     out = {"workload": "1600x1067, census 9x7, D=256, 8 paths, both matching directions per launch",
            "sweeps": "planes" if planes else "sequential",
            "source_hash": kernel_source_hash(), "fetch_scale": round(fetch_scale, 4),
