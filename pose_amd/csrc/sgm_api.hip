@@ -845,6 +845,7 @@ PathArgs path_args(const sgm_params *p, int dir, int njobs, int total_lines, int
   A.total_lines = total_lines;
   A.split = 0;
   A.uniq = p->uniqueness;
+  A.order = nullptr;
   return A;
 }
 
@@ -1553,51 +1554,52 @@ int launch_esgm_verify(sgm_context *ctx, int R, const SummaryJobs &J, int slots)
 // over every path and strip, certification, exact verification of the rest in
 // groups that fit the budget, then selection of every pixel from its summary.
 // Same contract as run_directions.
-// Longest-first dispatch order of the lines of a seven-direction launch over
-// a w x h pair (nd matching directions), in a device buffer cached by shape:
-// the launch has several times more lines than resident waves, and in line
-// order its last waves included long diagonal lines that then ran alone
-// (measured 2.48 -> 2.30 ms at 1600x1067 D=256).  Null, for the natural
-// order, when the buffer does not fit the budget.
-static const int *multi_line_order(sgm_context *ctx, const MultiDir &M, int w, int h, int nd) {
-  const int key[4] = {w, h, nd, M.ndirs};
-  Buffer &buf = ctx->buffers["multi_order"];
-  if (buf.ptr && memcmp(ctx->multi_order_key, key, sizeof key) == 0) return (const int *)buf.ptr;
-  const size_t bytes = (size_t)M.total_lines * sizeof(int);
+// Longest-first dispatch order of the lines of a launch over a w x h pair (nd
+// matching directions; lines [begin[k], begin[k] + nd * split[k]) sweep
+// direction (dx[k], dy[k]), matching direction m's from begin[k] + m *
+// split[k]), in a device buffer cached by shape.  A launch with several times
+// more lines than resident waves otherwise ends on long lines dispatched last
+// (the seven-direction launch at 1600x1067 D=256: 2.48 -> 2.30 ms).  Null, for
+// the natural order, when the buffer does not fit the budget.
+static const int *line_order(sgm_context *ctx, const char *name, int ndirs, const int *dx,
+                             const int *dy, const int *split, const int *begin, int total, int w,
+                             int h, int nd) {
+  sgm_context::LineOrder &lo = ctx->line_orders[name];
+  const int key[6] = {w, h, nd, ndirs, dx[0], dy[0]};
+  Buffer &buf = ctx->buffers[name];
+  if (buf.ptr && memcmp(lo.key, key, sizeof key) == 0) return (const int *)buf.ptr;
+  const size_t bytes = (size_t)total * sizeof(int);
   if (bytes > buf.cap && ctx->allocated - (int64_t)buf.cap + (int64_t)bytes > ctx->budget)
     return nullptr;
   if (buf.ensure(ctx, bytes, "line order")) return nullptr;
   // counting sort by length, descending; lines of equal length in line order
   const int maxlen = std::max(w, h);
-  std::vector<int> len((size_t)M.total_lines), start((size_t)maxlen + 2, 0);
-  for (int k = 0; k < M.ndirs; k++) {
-    const int dx = M.dx[k], dy = M.dy[k];
+  std::vector<int> len((size_t)total), start((size_t)maxlen + 2, 0);
+  for (int k = 0; k < ndirs; k++)
     for (int m = 0; m < nd; m++)
-      for (int l = 0; l < M.split[k]; l++) {
+      for (int l = 0; l < split[k]; l++) {
         int n;
-        if (dy == 0) {
+        if (dy[k] == 0) {
           n = w;
-        } else if (dx == 0) {
+        } else if (dx[k] == 0) {
           n = h;
         } else {  // the diagonal line enumeration of line_start
-          const int x = l < w ? l : (dx > 0 ? 0 : w - 1);
-          const int y = l < w ? (dy > 0 ? 0 : h - 1) : (dy > 0 ? l - w + 1 : h - 2 - (l - w));
-          n = std::min(dx > 0 ? w - x : x + 1, dy > 0 ? h - y : y + 1);
+          const int x = l < w ? l : (dx[k] > 0 ? 0 : w - 1);
+          const int y = l < w ? (dy[k] > 0 ? 0 : h - 1) : (dy[k] > 0 ? l - w + 1 : h - 2 - (l - w));
+          n = std::min(dx[k] > 0 ? w - x : x + 1, dy[k] > 0 ? h - y : y + 1);
         }
-        len[(size_t)M.begin[k] + (size_t)m * M.split[k] + l] = n;
+        len[(size_t)begin[k] + (size_t)m * split[k] + l] = n;
       }
-  }
   for (int n : len) start[maxlen - n + 1]++;  // bucket maxlen - n: longest first
   for (int i = 1; i <= maxlen + 1; i++) start[i] += start[i - 1];
-  ctx->multi_order.assign(len.size(), 0);
-  for (size_t g = 0; g < len.size(); g++) ctx->multi_order[start[maxlen - len[g]]++] = (int)g;
-  hipError_t e = hipMemcpyAsync(buf.ptr, ctx->multi_order.data(), bytes, hipMemcpyHostToDevice,
-                                ctx->stream);
+  lo.host.assign(len.size(), 0);
+  for (size_t g = 0; g < len.size(); g++) lo.host[start[maxlen - len[g]]++] = (int)g;
+  hipError_t e = hipMemcpyAsync(buf.ptr, lo.host.data(), bytes, hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return nullptr;
   }
-  memcpy(ctx->multi_order_key, key, sizeof key);
+  memcpy(lo.key, key, sizeof key);
   return (const int *)buf.ptr;
 }
 
@@ -1940,14 +1942,18 @@ int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuff
       M.total_lines += Ak.total_lines;
       for (int i = 0; i < nd; i++) M.plane[k][i] = B.planes[slots[i]] + (size_t)k * Dp;
     }
-    M.order = multi_line_order(ctx, M, w, h, nd);
+    M.order = line_order(ctx, "multi_order", M.ndirs, M.dx, M.dy, M.split, M.begin, M.total_lines,
+                         w, h, nd);
     if (interrupted()) return fail(SGM_ERR_INTERRUPTED, "GPU SGM interrupted");
     if ((rc = launch_path_multi(ctx, R, mode | MODE_PLANE, pj, dir_args(0), M, waves))) return rc;
     PathJobs jl = pj;
     for (int i = 0; i < nd; i++) jl.local[i].sum = B.planes[slots[i]];
     if (interrupted()) return fail(SGM_ERR_INTERRUPTED, "GPU SGM interrupted");
-    return launch_path(ctx, R, sb, mode | MODE_PLANE, jl, dir_args(p->directions - 1),
-                       PHASE_LAST, waves);
+    PathArgs AL = dir_args(p->directions - 1);
+    const int zero = 0;
+    AL.order = line_order(ctx, "last_order", 1, &AL.dx, &AL.dy, &AL.split, &zero, AL.total_lines, w,
+                          h, nd);
+    return launch_path(ctx, R, sb, mode | MODE_PLANE, jl, AL, PHASE_LAST, waves);
   }
   // Paired sweeps: after the first direction, the middle directions run two
   // per launch, the first of each pair adding into the aggregate volume and

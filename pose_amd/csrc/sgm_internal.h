@@ -168,10 +168,13 @@ struct sgm_context {
   std::atomic<int> interrupt{0};
   bool last_compact = false;
   int sweeps = SGM_SWEEPS_AUTO;  // sgm_set_sweeps
-  // longest-first line order of the seven-direction launch ("multi_order"
-  // buffer) and the (w, h, matching directions, path directions) it is for
-  std::vector<int> multi_order;
-  int multi_order_key[4] = {0, 0, 0, 0};
+  // longest-first line orders of the plane sweeps, by buffer name: the host
+  // copy and the (w, h, matching directions, directions, first dx, dy) it is for
+  struct LineOrder {
+    std::vector<int> host;
+    int key[6] = {0, 0, 0, 0, 0, 0};
+  };
+  std::map<std::string, LineOrder> line_orders;
   int last_sweeps = SGM_SWEEPS_SEQUENTIAL;
   sgm::StreamedPlan splan;  // streamed plan picked for the current call
   std::vector<sgm::PendingEvent> pending_events;

@@ -61,6 +61,9 @@ struct PathArgs {
   int njobs, total_lines;
   int split;               // two by-value jobs: first global line of job 1
   double uniq;
+  // dispatch order of the lines of a by-value launch (optional): wave slot i
+  // sweeps line order[i]
+  const int *order;
 };
 
 // Row segment of the census cost pass: one wave per kCostSegment pixels of a
@@ -1475,9 +1478,10 @@ __global__ __launch_bounds__(256) void path_kernel(PathJobs jobs, PathArgs A) {
   build_p2_table(p2_table, A);
   __syncthreads();
   const int waves = blockDim.x >> 6;
-  const int gline =
+  const int slot =
       __builtin_amdgcn_readfirstlane(blockIdx.x * waves + (int)(threadIdx.x >> 6));
-  if (gline >= A.total_lines) return;
+  if (slot >= A.total_lines) return;
+  const int gline = A.order ? __builtin_amdgcn_readfirstlane(A.order[slot]) : slot;
   if (jobs.table) {
     const int ji = __builtin_amdgcn_readfirstlane(find_job(jobs.line_begin, A.njobs, gline));
     const PathJob J = jobs.table[ji];
