@@ -1554,6 +1554,32 @@ int launch_esgm_verify(sgm_context *ctx, int R, const SummaryJobs &J, int slots)
 // over every path and strip, certification, exact verification of the rest in
 // groups that fit the budget, then selection of every pixel from its summary.
 // Same contract as run_directions.
+}  // namespace
+
+namespace sgm {
+
+int line_length(int dx, int dy, int line, int w, int h) {
+  if (dy == 0) return w;
+  if (dx == 0) return h;
+  // the diagonal line enumeration of line_start
+  const int x = line < w ? line : (dx > 0 ? 0 : w - 1);
+  const int y = line < w ? (dy > 0 ? 0 : h - 1) : (dy > 0 ? line - w + 1 : h - 2 - (line - w));
+  return std::min(dx > 0 ? w - x : x + 1, dy > 0 ? h - y : y + 1);
+}
+
+void longest_first(const std::vector<int> &len, int maxlen, std::vector<int> &order) {
+  // counting sort by length, descending; lines of equal length in line order
+  std::vector<int> start((size_t)maxlen + 2, 0);
+  for (int n : len) start[maxlen - n + 1]++;  // bucket maxlen - n: longest first
+  for (int i = 1; i <= maxlen + 1; i++) start[i] += start[i - 1];
+  order.assign(len.size(), 0);
+  for (size_t g = 0; g < len.size(); g++) order[start[maxlen - len[g]]++] = (int)g;
+}
+
+}  // namespace sgm
+
+namespace {
+
 // Longest-first dispatch order of the lines of a launch over a w x h pair (nd
 // matching directions; lines [begin[k], begin[k] + nd * split[k]) sweep
 // direction (dx[k], dy[k]), matching direction m's from begin[k] + m *
@@ -1572,28 +1598,12 @@ static const int *line_order(sgm_context *ctx, const char *name, int ndirs, cons
   if (bytes > buf.cap && ctx->allocated - (int64_t)buf.cap + (int64_t)bytes > ctx->budget)
     return nullptr;
   if (buf.ensure(ctx, bytes, "line order")) return nullptr;
-  // counting sort by length, descending; lines of equal length in line order
-  const int maxlen = std::max(w, h);
-  std::vector<int> len((size_t)total), start((size_t)maxlen + 2, 0);
+  std::vector<int> len((size_t)total);
   for (int k = 0; k < ndirs; k++)
     for (int m = 0; m < nd; m++)
-      for (int l = 0; l < split[k]; l++) {
-        int n;
-        if (dy[k] == 0) {
-          n = w;
-        } else if (dx[k] == 0) {
-          n = h;
-        } else {  // the diagonal line enumeration of line_start
-          const int x = l < w ? l : (dx[k] > 0 ? 0 : w - 1);
-          const int y = l < w ? (dy[k] > 0 ? 0 : h - 1) : (dy[k] > 0 ? l - w + 1 : h - 2 - (l - w));
-          n = std::min(dx[k] > 0 ? w - x : x + 1, dy[k] > 0 ? h - y : y + 1);
-        }
-        len[(size_t)begin[k] + (size_t)m * split[k] + l] = n;
-      }
-  for (int n : len) start[maxlen - n + 1]++;  // bucket maxlen - n: longest first
-  for (int i = 1; i <= maxlen + 1; i++) start[i] += start[i - 1];
-  lo.host.assign(len.size(), 0);
-  for (size_t g = 0; g < len.size(); g++) lo.host[start[maxlen - len[g]]++] = (int)g;
+      for (int l = 0; l < split[k]; l++)
+        len[(size_t)begin[k] + (size_t)m * split[k] + l] = line_length(dx[k], dy[k], l, w, h);
+  longest_first(len, std::max(w, h), lo.host);
   hipError_t e = hipMemcpyAsync(buf.ptr, lo.host.data(), bytes, hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) {
     (void)hipGetLastError();

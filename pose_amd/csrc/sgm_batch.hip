@@ -92,7 +92,7 @@ static int build_groups(sgm_batch *b, const std::vector<PathJob> &jobs, const sg
   const int sb = b->sum_bytes;
   std::vector<uint8_t> host;
   struct Pending {
-    size_t jobs_off, lb_off[4];
+    size_t jobs_off, lb_off[4], multi_off = 0, last_off = 0;
   };
   std::vector<Pending> offs;
   b->groups.clear();
@@ -134,6 +134,33 @@ static int build_groups(sgm_batch *b, const std::vector<PathJob> &jobs, const sg
       host.resize(pd.lb_off[cls] + lb.size() * sizeof(int));
       memcpy(host.data() + pd.lb_off[cls], lb.data(), lb.size() * sizeof(int));
     }
+    if (b->planes && !b->external) {
+      // longest-first orders (as the plain path's): the directions but the
+      // last in one launch, each over every job's lines of its class, then
+      // the last direction alone
+      static const int dirs[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
+      const int P = b->params.directions;
+      int maxlen = 1;
+      for (const PathJob &J : gj) maxlen = std::max(maxlen, std::max(J.w, J.h));
+      auto lengths = [&](int dir, std::vector<int> &len) {
+        const int dx = dirs[dir][0], dy = dirs[dir][1];
+        const int cls = dy == 0 ? 0 : dx == 0 ? 1 : 2;
+        for (const PathJob &J : gj)
+          for (int l = 0; l < lines_of(cls, J.w, J.h); l++) len.push_back(line_length(dx, dy, l, J.w, J.h));
+      };
+      std::vector<int> len, order;
+      for (int d = 0; d < P - 1; d++) lengths(d, len);  // the launch's line numbering
+      longest_first(len, maxlen, order);
+      pd.multi_off = align_up(host.size(), 256);
+      host.resize(pd.multi_off + order.size() * sizeof(int));
+      memcpy(host.data() + pd.multi_off, order.data(), order.size() * sizeof(int));
+      len.clear();
+      lengths(P - 1, len);
+      longest_first(len, maxlen, order);
+      pd.last_off = align_up(host.size(), 256);
+      host.resize(pd.last_off + order.size() * sizeof(int));
+      memcpy(host.data() + pd.last_off, order.data(), order.size() * sizeof(int));
+    }
     offs.push_back(pd);
     b->groups.push_back(g);
   }
@@ -156,6 +183,10 @@ static int build_groups(sgm_batch *b, const std::vector<PathJob> &jobs, const sg
   for (size_t i = 0; i < b->groups.size(); i++) {
     b->groups[i].jobs = (const PathJob *)(base + offs[i].jobs_off);
     for (int cls = 0; cls < 4; cls++) b->groups[i].line_begin[cls] = (const int *)(base + offs[i].lb_off[cls]);
+    if (offs[i].multi_off) {
+      b->groups[i].order_multi = (const int *)(base + offs[i].multi_off);
+      b->groups[i].order_last = (const int *)(base + offs[i].last_off);
+    }
   }
   b->phased = true;
   return SGM_OK;
@@ -270,10 +301,12 @@ static int phased_enqueue(sgm_batch *b) {
       for (int cls = 0; cls < 3; cls++) M.line_begin[cls] = g.line_begin[cls];
       M.njobs = g.njobs;
       M.plane0 = 0;
+      M.order = g.order_multi;
       if ((rc = launch_path_multi(ctx, g.R, 4 | nt | MODE_PLANE, pj, args(0), M,
                                   path_waves(ctx, g.R), st)))
         return rc;
-      const PathArgs AL = args(p->directions - 1);
+      PathArgs AL = args(p->directions - 1);
+      AL.order = g.order_last;
       pj.line_begin = g.line_begin[AL.cls];
       if ((rc = launch_path(ctx, g.R, sb, 4 | nt | MODE_PLANE, pj, AL, PHASE_LAST,
                             path_waves(ctx, g.R), st)))

@@ -80,6 +80,9 @@ struct sgm_batch {
     // per line class (horizontal, vertical, diagonal, cost-pass row segments)
     const int *line_begin[4] = {nullptr, nullptr, nullptr, nullptr};
     int total_lines[4] = {0, 0, 0, 0};
+    // plane runs of non-serving batches: longest-first line orders of the
+    // seven-direction launch and of the last sweep (null: line order)
+    const int *order_multi = nullptr, *order_last = nullptr;
   };
   bool phased = false;
   // path-cost planes: the scratch region of a sub-batch's cost volumes and

@@ -70,6 +70,10 @@ int64_t schedule_bytes(const sgm_params *p, int w, int h, int schedule);
 int sweep_schedule(const sgm_context *ctx, const sgm_params *p, int w, int h, bool cost_volume,
                    int64_t budget);
 int lines_of(int cls, int w, int h);
+// length of line `line` of direction (dx, dy) in a w x h image (line_start)
+int line_length(int dx, int dy, int line, int w, int h);
+// the indices of `len` by length, longest first, equal lengths in index order
+void longest_first(const std::vector<int> &len, int maxlen, std::vector<int> &order);
 struct HostCopy {
   void *dst;
   const void *src;
