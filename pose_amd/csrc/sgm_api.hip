@@ -1979,8 +1979,14 @@ int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuff
     if (paired && dir > 0 && dir + 1 < last) {
       // dir into the aggregate, dir + 1 into the second partial aggregate
       // (its first direction when dir == 1)
-      if ((rc = launch_path_pair(ctx, R, mode, dir == 1 ? PHASE_FIRST : PHASE_MID, pj,
-                                 dir_args(dir), pjb, dir_args(dir + 1), waves)))
+      static const char *pair_names[8] = {"", "pair_order1", "", "pair_order3", "", "pair_order5"};
+      PathArgs Aa = dir_args(dir), Ab = dir_args(dir + 1);
+      const int dxs[2] = {Aa.dx, Ab.dx}, dys[2] = {Aa.dy, Ab.dy}, splits[2] = {Aa.split, Ab.split};
+      const int begins[2] = {0, Aa.total_lines};
+      Aa.order = line_order(ctx, pair_names[dir], 2, dxs, dys, splits, begins,
+                            Aa.total_lines + Ab.total_lines, w, h, nd);
+      if ((rc = launch_path_pair(ctx, R, mode, dir == 1 ? PHASE_FIRST : PHASE_MID, pj, Aa, pjb, Ab,
+                                 waves)))
         return rc;
       dir += 2;
       continue;
@@ -1992,7 +1998,15 @@ int run_directions(sgm_context *ctx, const sgm_params *p, int w, int h, PairBuff
       for (int i = 0; i < nd; i++) jobs.local[i].sum2 = B.sum_b[slots[i]];
       m |= MODE_SUM2;
     }
-    if ((rc = launch_path(ctx, R, sb, m, jobs, dir_args(dir), phase, waves))) return rc;
+    PathArgs Ad = dir_args(dir);
+    if (Ad.cls == 2) {  // diagonal lines differ in length: longest first
+      static const char *dir_names[8] = {"", "", "", "", "dir_order4", "dir_order5", "dir_order6",
+                                         "dir_order7"};
+      const int zero = 0;
+      Ad.order = line_order(ctx, dir_names[dir], 1, &Ad.dx, &Ad.dy, &Ad.split, &zero,
+                            Ad.total_lines, w, h, nd);
+    }
+    if ((rc = launch_path(ctx, R, sb, m, jobs, Ad, phase, waves))) return rc;
     dir++;
   }
   return SGM_OK;

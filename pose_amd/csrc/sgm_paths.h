@@ -1450,7 +1450,10 @@ __global__ __launch_bounds__(256) void path_pair_kernel(PathJobs ja, PathArgs A,
   build_p2_table(p2_table, A);  // P1, P2 and the edge scale are shared by both directions
   __syncthreads();
   const int waves = blockDim.x >> 6;
-  int gline = __builtin_amdgcn_readfirstlane(blockIdx.x * waves + (int)(threadIdx.x >> 6));
+  const int slot = __builtin_amdgcn_readfirstlane(blockIdx.x * waves + (int)(threadIdx.x >> 6));
+  if (slot >= A.total_lines + B.total_lines) return;
+  // A.order: dispatch order over both directions' lines (A's, then B's)
+  int gline = A.order ? __builtin_amdgcn_readfirstlane(A.order[slot]) : slot;
   if (gline < A.total_lines) {
     if (A.njobs > 1 && gline >= A.split) {
       const PathJob J = ja.local[1];
